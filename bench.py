@@ -1,0 +1,257 @@
+#!/usr/bin/env python3
+"""Benchmark of the Yukawa all-pairs + RK4 hot path on one MI355X.
+
+Workload (the headline metric's configuration): one cloud of N = 131072
+particles on a hexagonal lattice (500 um spacing, +-50 um jitter), charge
+6000 e, mass 2.8e-14 kg, lambda = 500 um (kappa = 2000 /m), confinement
+k = 5e-11 N/m, drag gamma = 2 /s, thermal off, RK4 with dt = 1e-3 s and the
+reference's dt check, FP64.  A "step" is one RK4 step (4 all-pairs force
+evaluations + the dt check).
+
+Prints ONE JSON line:
+  value        pair interactions/s = 4 N (N-1) steps / t   (ordered pairs)
+  rk4_steps_per_s, ms_per_step
+  roofline     the all-pairs kernel: algorithmic FLOP per launch
+               (7 per far ordered pair, 20 per near one) / its mean launch
+               time (HIP events around every launch on the kernel's stream),
+               against the FP64 vector peak of 78.6 TFLOP/s
+  fp32_pair    the same run with FP32 pair arithmetic (speed and error)
+  cpu_baseline the reference's own SSE/OpenMP code (compiled from its
+               sources into oracle/_ref) timed on the host cores on a bounded
+               sample (1 warm-up + a few timed steps of the same workload)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK = 78.6e12       # MI355X FP64 vector, FLOP/s
+FLOP_FAR, FLOP_NEAR = 7, 20
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def near_ordered_pairs(x, y, kappa):
+    """Ordered pairs (i != j) with r * kappa < 10 (host pass on a snapshot)."""
+    from scipy.spatial import cKDTree
+
+    t = cKDTree(np.c_[x, y])
+    r = 10.0 / kappa
+    return int(t.count_neighbors(t, r * (1 - 1e-12))) - len(x)
+
+
+def setup(wl):
+    import demon_simulation_code_amd as dm
+
+    x, y = dm.hex_lattice(wl.n, wl.spacing, wl.jitter, wl.seed)
+    q = np.full(wl.n, wl.charge)
+    m = np.full(wl.n, wl.mass)
+    return x, y, q, m
+
+
+def run_gpu(wl, steps, warmup, precision, shape=None):
+    import demon_simulation_code_amd as dm
+
+    x, y, q, m = setup(wl)
+    n = wl.n
+    forces = dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB
+    c = dm.Context(n, shielding=wl.kappa, confine=wl.confine, gamma=wl.gamma, forces=forces,
+                   precision=precision)
+    c.upload(x, y, None, None, q, m)
+    if shape:
+        c.set_pair_shape(*shape)
+    t = 0.0
+    if warmup:
+        t = c.rk4_steps(warmup, wl.dt, t)
+    xs0, ys0, _, _ = c.download()
+    c.synchronize()
+    t0 = time.perf_counter()
+    t = c.rk4_steps(steps, wl.dt, t)
+    c.synchronize()
+    wall = time.perf_counter() - t0
+    pair_ms, total_ms = c.last_timing()
+    xs1, ys1, vx1, vy1 = c.download()
+    st = c.stats()
+    shape_used = c.pair_shape()
+    c.close()
+    return dict(wall=wall, pair_ms=pair_ms, total_ms=total_ms, t=t, stats=st,
+                snap0=(xs0, ys0), final=(xs1, ys1, vx1, vy1), shape=shape_used)
+
+
+def cpu_baseline(wl, budget_s=45.0):
+    """The reference's own code (OpenMP build) on the host cores, bounded sample:
+    one ShieldedCoulombForce::force1 evaluation as a probe, then one full RK4
+    step (dt check, 4 force evaluations, O(N) stages) when that fits the budget."""
+    from oracle.bind import Oracle, OracleParams, Reference
+
+    n = wl.n
+    x, y, q, m = setup(wl)
+    z = np.zeros(n)
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = max(1, min(threads, 16))
+    if Reference.available(openmp=True):
+        ref = Reference(openmp=True)
+        kind = "reference"
+
+        def evaluate():
+            ref.yukawa(x, y, q, wl.kappa, 1, threads=threads)
+
+        def step():
+            ref.rk4_steps(x, y, z, z, q, m, wl.kappa, wl.confine, wl.gamma, 0.0, 7, wl.dt, 1,
+                          threads=threads)
+    else:
+        orc = Oracle()
+        kind = "port"
+        threads = 1
+        op = OracleParams(wl.kappa, wl.confine, wl.gamma, 0.0, 0, 7, 1)
+
+        def evaluate():
+            orc.yukawa(x, y, q, wl.kappa)
+
+        def step():
+            orc.rk4_steps(x, y, z, z, q, m, op, wl.dt, 1)
+
+    t0 = time.perf_counter()
+    evaluate()
+    t_eval = time.perf_counter() - t0
+    out = {"unit": "pair interactions/s", "cores": threads, "kind": kind}
+    if 4.5 * t_eval < budget_s:
+        t0 = time.perf_counter()
+        step()
+        t_step = time.perf_counter() - t0
+        out["value"] = 4.0 * n * (n - 1) / t_step
+        out["rk4_steps_per_s"] = 1.0 / t_step
+        out["sample"] = (f"1 RK4 step of the N={n} FP64 workload ({t_step:.2f} s) after one "
+                         f"force-evaluation probe ({t_eval:.2f} s)")
+    else:
+        out["value"] = 1.0 * n * (n - 1) / t_eval
+        out["rk4_steps_per_s"] = None
+        out["sample"] = f"1 ShieldedCoulombForce::force1 evaluation at N={n} ({t_eval:.2f} s)"
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=131072)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--shape", type=str, default="", help="i_per_lane,j_slices")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import demon_simulation_code_amd as dm
+    from demon_simulation_code_amd.physics import LARGE_131K, Workload
+
+    wl = LARGE_131K if args.n == LARGE_131K.n else Workload(f"hex{args.n}_fp64", n=args.n, steps=args.steps)
+    shape = tuple(int(v) for v in args.shape.split(",")) if args.shape else None
+    n, K, W = wl.n, args.steps, args.warmup
+
+    log(f"FP64 run: N={n} warmup={W} steps={K}")
+    r = run_gpu(wl, K, W, dm.FP64, shape)
+    pairs_total = 4.0 * n * (n - 1) * K
+    value = pairs_total / r["wall"]
+    log(f"FP64 done: {r['wall']:.3f} s, {value:.4g} pair interactions/s, shape={r['shape']}")
+
+    near0 = near_ordered_pairs(*r["snap0"], wl.kappa)
+    near1 = near_ordered_pairs(r["final"][0], r["final"][1], wl.kappa)
+    near = 0.5 * (near0 + near1)
+    ordered = n * (n - 1)
+    flop_launch = FLOP_FAR * (ordered - near) + FLOP_NEAR * near
+    launches = 4 * K
+    mean_launch_s = r["pair_ms"] / 1e3 / launches
+    achieved = flop_launch / mean_launch_s
+    roofline = {
+        "bound": "valu",
+        "achieved": achieved / 1e12,
+        "peak": FP64_PEAK / 1e12,
+        "unit": "TFLOP/s",
+        "frac": achieved / FP64_PEAK,
+        "traffic": None,
+        "kernel": "k_pairs<2,false> (FP64 all-pairs)",
+        "mean_launch_ms": mean_launch_s * 1e3,
+        "flop_per_launch": flop_launch,
+        "near_fraction": near / ordered,
+        "pair_kernel_share": r["pair_ms"] / r["total_ms"] if r["total_ms"] else None,
+    }
+
+    out = {
+        "metric": "pair interactions/s + RK4 steps/s, N=131k, FP64 vs FP32-pair; adjoint grads/s",
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": r["wall"] / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": wl.name + "_rk4",
+            "n": n,
+            "dt": wl.dt,
+            "lambda_m": wl.lam,
+            "charge_e": wl.charge_e,
+            "k_spring": wl.k_spring,
+            "gamma": wl.gamma,
+            "thermal": False,
+            "dt_check": True,
+            "pair_shape": list(r["shape"]),
+        },
+        "rk4_steps_per_s": K / r["wall"],
+        "roofline": roofline,
+        "dt_reductions": r["stats"].dt_reductions,
+    }
+
+    if not args.no_fp32:
+        log("FP32-pair run")
+        r32 = run_gpu(wl, K, W, dm.FP32_PAIR, shape)
+        x64, y64 = r["final"][0], r["final"][1]
+        x32, y32 = r32["final"][0], r32["final"][1]
+        perr = float(np.max(np.hypot(x64 - x32, y64 - y32)) / np.max(np.hypot(x64, y64)))
+        out["fp32_pair"] = {
+            "value": pairs_total / r32["wall"],
+            "unit": "pair interactions/s",
+            "rk4_steps_per_s": K / r32["wall"],
+            "speedup_vs_fp64": r["wall"] / r32["wall"],
+            "pos_rel_err_vs_fp64_after_steps": perr,
+            "steps": K + W,
+            "mean_launch_ms": r32["pair_ms"] / launches,
+        }
+        log(f"FP32-pair done: {r32['wall']:.3f} s")
+
+    if not args.no_cpu:
+        log("CPU baseline (reference code on host cores)")
+        try:
+            out["cpu_baseline"] = cpu_baseline(wl)
+            out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
+        except Exception as e:  # the GPU result stands without it
+            out["cpu_baseline"] = {"error": repr(e)}
+        log("CPU baseline done")
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

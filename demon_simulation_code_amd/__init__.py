@@ -1,0 +1,28 @@
+"""MI355X-native implementation of DEMON's hot path (Yukawa all-pairs force +
+RK4 stepping) behind a C ABI (include/demon_gpu.h).
+
+The compute runs in libdemon_gpu.so (hand-written gfx950 HIP); this package
+only binds it.  Importing fails loudly when the library is not built.
+"""
+from ._lib import (  # noqa: F401
+    CONFINEMENT,
+    DRAG,
+    FP32_PAIR,
+    FP64,
+    LIB_PATH,
+    SHIELDED_COULOMB,
+    THERMAL,
+    Context,
+    DemonError,
+    Params,
+    Stats,
+    hex_lattice,
+    lib,
+)
+from .physics import ELECTRON_CHARGE, EPSILON0, COULOMB, confine_from_k  # noqa: F401
+
+__all__ = [
+    "Context", "DemonError", "Params", "Stats", "hex_lattice", "lib", "LIB_PATH",
+    "CONFINEMENT", "DRAG", "SHIELDED_COULOMB", "THERMAL", "FP64", "FP32_PAIR",
+    "ELECTRON_CHARGE", "EPSILON0", "COULOMB", "confine_from_k",
+]

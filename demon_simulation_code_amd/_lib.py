@@ -1,0 +1,223 @@
+"""ctypes binding of libdemon_gpu.so (the C ABI declared in include/demon_gpu.h).
+
+The library is the product: there is no Python or CPU fallback.  If the
+shared object is missing or fails to load, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("DEMON_GPU_LIB", _PKG / "libdemon_gpu.so"))
+
+# return codes
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_ALLOC = -3
+ERR_STATE = -4
+ERR_UNSUPPORTED = -5
+
+# force flags (Force.h ForceFlag values)
+CONFINEMENT = 1
+DRAG = 2
+SHIELDED_COULOMB = 4
+THERMAL = 16
+
+FP64 = 0
+FP32_PAIR = 1
+
+
+class DemonError(RuntimeError):
+    """A C-ABI call returned a negative code."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("shielding", ctypes.c_double),
+        ("confine", ctypes.c_double),
+        ("drag_gamma", ctypes.c_double),
+        ("heat_val", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("forces", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("dt_check", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int64),
+        ("dt_reductions", ctypes.c_int64),
+        ("force_evals", ctypes.c_int64),
+        ("current_time", ctypes.c_double),
+        ("last_dt", ctypes.c_double),
+    ]
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_CTX = ctypes.c_void_p
+
+# name -> (restype, argtypes); every entry must be exported (checked by tests)
+SIGNATURES = {
+    "demon_version": (ctypes.c_char_p, []),
+    "demon_abi_version": (ctypes.c_int, []),
+    "demon_last_error": (ctypes.c_char_p, [_CTX]),
+    "demon_create": (ctypes.c_int, [ctypes.POINTER(_CTX), _I64, ctypes.POINTER(Params)]),
+    "demon_destroy": (None, [_CTX]),
+    "demon_set_params": (ctypes.c_int, [_CTX, ctypes.POINTER(Params)]),
+    "demon_get_params": (ctypes.c_int, [_CTX, ctypes.POINTER(Params)]),
+    "demon_upload": (ctypes.c_int, [_CTX, _D, _D, _D, _D, _D, _D]),
+    "demon_download": (ctypes.c_int, [_CTX, _D, _D, _D, _D]),
+    "demon_force_stage": (ctypes.c_int, [_CTX, _I32, _I32, ctypes.c_uint64, _D, _D, _D, _D, _D, _D]),
+    "demon_rk4_move": (ctypes.c_int, [_CTX, _D, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_I64)]),
+    "demon_rk4_steps": (ctypes.c_int, [_CTX, _I64, ctypes.c_double, _D]),
+    "demon_use_graph": (ctypes.c_int, [_CTX, _I32]),
+    "demon_get_stats": (ctypes.c_int, [_CTX, ctypes.POINTER(Stats)]),
+    "demon_synchronize": (ctypes.c_int, [_CTX]),
+    "demon_set_pair_shape": (ctypes.c_int, [_CTX, _I32, _I32]),
+    "demon_get_pair_shape": (ctypes.c_int, [_CTX, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+    "demon_last_timing": (ctypes.c_int, [_CTX, _D, _D]),
+    "demon_hex_lattice": (ctypes.c_int, [_I64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _D, _D]),
+}
+
+
+def _load() -> ctypes.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m demon_simulation_code_amd.build` "
+            "(there is no CPU fallback)"
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous float64 numpy array")
+    return a.ctypes.data_as(_D)
+
+
+def _check(rc: int, ctx=None):
+    if rc != OK:
+        msg = lib.demon_last_error(ctx)
+        raise DemonError(rc, msg.decode() if msg else "")
+
+
+def hex_lattice(n: int, spacing: float = 500e-6, jitter: float = 50e-6, seed: int = 0):
+    """Synthetic input: n hexagonal-lattice sites nearest the origin, jittered."""
+    x = np.empty(n)
+    y = np.empty(n)
+    _check(lib.demon_hex_lattice(n, spacing, jitter, seed, _ptr(x), _ptr(y)))
+    return x, y
+
+
+class Context:
+    """One device-resident cloud (demon_ctx)."""
+
+    def __init__(self, n: int, *, shielding: float, confine: float = 0.0, gamma: float = 0.0,
+                 heat: float = 0.0, seed: int = 0, forces: int = CONFINEMENT | DRAG | SHIELDED_COULOMB,
+                 precision: int = FP64, dt_check: bool = True, device: int = 0):
+        self.n = int(n)
+        self.params = Params(shielding, confine, gamma, heat, seed, forces, precision,
+                             1 if dt_check else 0, device)
+        h = _CTX()
+        _check(lib.demon_create(ctypes.byref(h), self.n, ctypes.byref(self.params)))
+        self._h = h
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.demon_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- calls ------------------------------------------------------------
+    def _c(self, rc):
+        _check(rc, self._h)
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        self._c(lib.demon_set_params(self._h, ctypes.byref(self.params)))
+
+    def upload(self, x, y, vx=None, vy=None, charge=None, mass=None):
+        self._c(lib.demon_upload(self._h, _ptr(x), _ptr(y), _ptr(vx), _ptr(vy), _ptr(charge), _ptr(mass)))
+
+    def download(self):
+        out = [np.empty(self.n) for _ in range(4)]
+        self._c(lib.demon_download(self._h, *[_ptr(a) for a in out]))
+        return tuple(out)
+
+    def force_stage(self, forces, stage, step, xs, ys, vxs=None, vys=None, fx=None, fy=None):
+        fx = np.zeros(self.n) if fx is None else fx
+        fy = np.zeros(self.n) if fy is None else fy
+        self._c(lib.demon_force_stage(self._h, forces, stage, step, _ptr(xs), _ptr(ys), _ptr(vxs),
+                                      _ptr(vys), _ptr(fx), _ptr(fy)))
+        return fx, fy
+
+    def rk4_steps(self, steps: int, dt: float, t: float = 0.0) -> float:
+        tt = ctypes.c_double(t)
+        self._c(lib.demon_rk4_steps(self._h, int(steps), dt, ctypes.byref(tt)))
+        return tt.value
+
+    def rk4_move(self, t: float, end_time: float, dt: float):
+        tt = ctypes.c_double(t)
+        st = _I64(0)
+        self._c(lib.demon_rk4_move(self._h, ctypes.byref(tt), end_time, dt, ctypes.byref(st)))
+        return tt.value, st.value
+
+    def use_graph(self, on: bool = True):
+        self._c(lib.demon_use_graph(self._h, 1 if on else 0))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self._c(lib.demon_get_stats(self._h, ctypes.byref(s)))
+        return s
+
+    def synchronize(self):
+        self._c(lib.demon_synchronize(self._h))
+
+    def set_pair_shape(self, i_per_lane: int, j_slices: int = 0):
+        self._c(lib.demon_set_pair_shape(self._h, i_per_lane, j_slices))
+
+    def pair_shape(self):
+        r, s = _I32(0), _I32(0)
+        self._c(lib.demon_get_pair_shape(self._h, ctypes.byref(r), ctypes.byref(s)))
+        return r.value, s.value
+
+    def last_timing(self):
+        a, b = ctypes.c_double(0), ctypes.c_double(0)
+        self._c(lib.demon_last_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value

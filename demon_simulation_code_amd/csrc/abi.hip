@@ -1,0 +1,602 @@
+// abi.hip -- the C ABI of libdemon_gpu.so (declared in include/demon_gpu.h):
+// context lifetime, host<->device transfers in the caller's particle order,
+// and the launch sequence of Runge_Kutta4::moveParticles.
+//
+// Device layout (all FP64 SoA of npad = roundup(n, 1024) entries, in the
+// device order -- a Morton order of the uploaded positions):
+//   x y vx vy q m              base state and per-particle constants
+//   sb[0], sb[1]               ping-pong stage records {xs ys vxs vys nc}
+//   sx                         scratch stage record for demon_force_stage
+//   acc1[4], acc23[4]          RK increments (k l m n) of stage 1, of 2 (+3)
+//   fout[2]                    force output of demon_force_stage
+//   part[slices][2]            all-pairs partial sums per j slice
+// plus orig[npad] (caller index of each device slot) and the Scalars block.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/demon_gpu.h"
+#include "internal.h"
+#include "philox.h"
+
+using namespace demon;
+
+struct demon_ctx {
+    std::string err;
+    demon_params p{};
+    int64_t n = 0, npad = 0;
+    hipStream_t stream = nullptr;
+    double *d_arena = nullptr;   // 31 arrays of npad doubles
+    double *d_part = nullptr;    // kMaxSlices * 2 * npad
+    int32_t *d_orig = nullptr;
+    Scalars *d_sc = nullptr;
+    double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *q = nullptr, *m = nullptr;
+    StageBuf sb[2]{}, sx{};
+    double *acc1 = nullptr, *acc23 = nullptr, *fout = nullptr;
+    std::vector<int64_t> perm;   // perm[k] = caller index stored in device slot k
+    std::vector<double> h_tmp;   // host staging (npad * 4)
+    bool uploaded = false;
+    int32_t i_per_lane = 2;
+    int32_t slices_req = 0;      // 0 = automatic
+    int32_t slices = 1, jslice = 0;
+    bool use_graph = false;
+    hipGraphExec_t graph = nullptr;
+    bool timing = true;
+    std::vector<hipEvent_t> ev;  // pairs of events around each all-pairs launch
+    size_t ev_used = 0;
+    hipEvent_t ev_call[2]{};
+    double last_pair_ms = 0.0, last_total_ms = 0.0;
+    int64_t force_evals = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(demon_ctx *c, int code, const std::string &msg) {
+    if (c)
+        c->err = msg;
+    else
+        g_err = msg;
+    return code;
+}
+
+int hip_fail(demon_ctx *c, hipError_t e, const char *what) {
+    return fail(c, DEMON_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(c, expr)                                        \
+    do {                                                        \
+        hipError_t e_ = (expr);                                 \
+        if (e_ != hipSuccess) return hip_fail((c), e_, #expr);  \
+    } while (0)
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+int check_params(demon_ctx *c, const demon_params *p) {
+    if (!p) return fail(c, DEMON_ERR_ARG, "params is NULL");
+    const int32_t known = DEMON_FORCE_CONFINEMENT | DEMON_FORCE_DRAG |
+                          DEMON_FORCE_SHIELDED_COULOMB | DEMON_FORCE_THERMAL;
+    if (p->forces & ~known)
+        return fail(c, DEMON_ERR_UNSUPPORTED,
+                    "force flags outside Confinement|Drag|ShieldedCoulomb|Thermal are not "
+                    "implemented on the device");
+    if ((p->forces & DEMON_FORCE_SHIELDED_COULOMB) && !(p->shielding > 0.0 && std::isfinite(p->shielding)))
+        return fail(c, DEMON_ERR_ARG, "shielding constant must be positive and finite");
+    if (p->precision != DEMON_PRECISION_FP64 && p->precision != DEMON_PRECISION_FP32_PAIR)
+        return fail(c, DEMON_ERR_ARG, "unknown precision");
+    return DEMON_OK;
+}
+
+// Number of j slices: enough waves to give every SIMD of the 256 CUs about
+// four, while keeping each slice at least 64 j particles long.
+void choose_slices(demon_ctx *c) {
+    const int64_t waves_per_slice = c->npad / (64 * c->i_per_lane);
+    int64_t s = c->slices_req > 0 ? c->slices_req : (4096 + waves_per_slice - 1) / waves_per_slice;
+    const int64_t max_by_len = std::max<int64_t>(1, c->npad / 64);
+    s = std::max<int64_t>(1, std::min<int64_t>({s, (int64_t)kMaxSlices, max_by_len}));
+    int64_t js = round_up((c->npad + s - 1) / s, 8);
+    s = (c->npad + js - 1) / js;
+    c->slices = (int32_t)s;
+    c->jslice = (int32_t)js;
+}
+
+uint64_t morton2(uint32_t a, uint32_t b) {
+    auto spread = [](uint64_t v) {
+        v &= 0xFFFFFFFFull;
+        v = (v | (v << 16)) & 0x0000FFFF0000FFFFull;
+        v = (v | (v << 8)) & 0x00FF00FF00FF00FFull;
+        v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0Full;
+        v = (v | (v << 2)) & 0x3333333333333333ull;
+        v = (v | (v << 1)) & 0x5555555555555555ull;
+        return v;
+    };
+    return spread(a) | (spread(b) << 1);
+}
+
+// Device order: sort by the Morton code of the positions on a 2^16 grid over
+// their bounding box (ties and non-finite positions keep caller order).
+void build_perm(demon_ctx *c, const double *x, const double *y) {
+    const int64_t n = c->n;
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int64_t i = 0; i < n; ++i) {
+        if (std::isfinite(x[i])) { x0 = std::min(x0, x[i]); x1 = std::max(x1, x[i]); }
+        if (std::isfinite(y[i])) { y0 = std::min(y0, y[i]); y1 = std::max(y1, y[i]); }
+    }
+    const double sx = (x1 > x0) ? 65535.0 / (x1 - x0) : 0.0;
+    const double sy = (y1 > y0) ? 65535.0 / (y1 - y0) : 0.0;
+    std::vector<std::pair<uint64_t, int64_t>> key((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t a = 0, b = 0;
+        if (std::isfinite(x[i]) && std::isfinite(y[i])) {
+            a = (uint32_t)std::min(65535.0, std::max(0.0, (x[i] - x0) * sx));
+            b = (uint32_t)std::min(65535.0, std::max(0.0, (y[i] - y0) * sy));
+        }
+        key[(size_t)i] = {morton2(a, b), i};
+    }
+    std::sort(key.begin(), key.end());
+    c->perm.resize((size_t)n);
+    for (int64_t k = 0; k < n; ++k) c->perm[(size_t)k] = key[(size_t)k].second;
+}
+
+// host caller-order array -> device slot order (into h_tmp + off) -> device
+int put(demon_ctx *c, double *dst, const double *src, double fill, size_t slot) {
+    double *h = c->h_tmp.data() + slot * (size_t)c->npad;
+    for (int64_t k = 0; k < c->n; ++k) h[k] = src ? src[c->perm[(size_t)k]] : fill;
+    for (int64_t k = c->n; k < c->npad; ++k) h[k] = 0.0;
+    HIP_TRY(c, hipMemcpyAsync(dst, h, sizeof(double) * c->npad, hipMemcpyHostToDevice, c->stream));
+    return DEMON_OK;
+}
+
+int enqueue_pairs(demon_ctx *c, const StageBuf &st, int gated, int dtcheck) {
+    PairLaunch a{};
+    a.xs = st.xs; a.ys = st.ys; a.nc = st.nc; a.q = c->q; a.orig = c->d_orig;
+    a.part = c->d_part; a.sc = c->d_sc; a.n = c->n; a.npad = c->npad;
+    a.jslice = c->jslice; a.slices = c->slices; a.i_per_lane = c->i_per_lane;
+    a.dtcheck = dtcheck; a.partner = 1; a.gated = gated;
+    a.fp32pair = c->p.precision == DEMON_PRECISION_FP32_PAIR;
+    a.kappa = c->p.shielding;
+    const double rc = 10.0 / c->p.shielding;
+    a.rc2p = rc * rc * (1.0 + kCutoffSlack);
+    const bool ev = c->timing && !c->use_graph;
+    if (ev) {
+        if (c->ev_used + 2 > c->ev.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                HIP_TRY(c, hipEventCreate(&e));
+                c->ev.push_back(e);
+            }
+        }
+        HIP_TRY(c, hipEventRecord(c->ev[c->ev_used], c->stream));
+    }
+    HIP_TRY(c, launch_pairs(a, c->stream));
+    if (ev) {
+        HIP_TRY(c, hipEventRecord(c->ev[c->ev_used + 1], c->stream));
+        c->ev_used += 2;
+    }
+    return DEMON_OK;
+}
+
+UpdateLaunch make_update(demon_ctx *c, int stage) {
+    UpdateLaunch u{};
+    // stage records: 1 reads sb0, 2 reads sb1, 3 reads sb0, 4 reads sb1
+    const int cur = (stage - 1) & 1;
+    u.cur = c->sb[cur];
+    u.nxt = c->sb[cur ^ 1];
+    u.x = c->x; u.y = c->y; u.vx = c->vx; u.vy = c->vy; u.q = c->q; u.m = c->m;
+    u.orig = c->d_orig; u.part = c->d_part; u.slices = c->slices;
+    u.acc1 = c->acc1; u.acc23 = c->acc23; u.sc = c->d_sc;
+    u.n = c->n; u.npad = c->npad; u.forces = c->p.forces; u.dtcheck = c->p.dt_check;
+    u.confine = c->p.confine; u.gamma = c->p.drag_gamma; u.heat = c->p.heat_val; u.seed = c->p.seed;
+    return u;
+}
+
+int enqueue_step(demon_ctx *c) {
+    HIP_TRY(c, launch_boundary(c->d_sc, 0, c->stream));
+    const bool yk = c->p.forces & DEMON_FORCE_SHIELDED_COULOMB;
+    for (int s = 1; s <= 4; ++s) {
+        const StageBuf &st = c->sb[(s - 1) & 1];
+        if (yk) {
+            const int rc = enqueue_pairs(c, st, 1, s == 1 && c->p.dt_check);
+            if (rc) return rc;
+        } else if (s == 1 && c->p.dt_check) {
+            return fail(c, DEMON_ERR_UNSUPPORTED,
+                        "the dt check runs inside the all-pairs pass; enable ShieldedCoulomb");
+        }
+        HIP_TRY(c, launch_update(s, make_update(c, s), c->stream));
+    }
+    return DEMON_OK;
+}
+
+void drop_graph(demon_ctx *c) {
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+}
+
+int ensure_graph(demon_ctx *c) {
+    if (c->graph) return DEMON_OK;
+    hipGraph_t g = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_step(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
+    return DEMON_OK;
+}
+
+__global__ void k_call_setup(Scalars *sc, double t, double t_end, double init_dt,
+                             unsigned long long remaining) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc->t = t;
+        sc->t_end = t_end;
+        sc->init_dt = init_dt;
+        sc->remaining = remaining;
+        sc->active = 0;
+    }
+}
+
+// Queues `steps` steps plus the closing boundary, then reads the scalars back.
+int run_steps(demon_ctx *c, int64_t steps, Scalars *out) {
+    c->ev_used = 0;
+    HIP_TRY(c, hipEventRecord(c->ev_call[0], c->stream));
+    if (c->use_graph) {
+        const int rc = ensure_graph(c);
+        if (rc) return rc;
+        for (int64_t s = 0; s < steps; ++s) HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+    } else {
+        for (int64_t s = 0; s < steps; ++s) {
+            const int rc = enqueue_step(c);
+            if (rc) return rc;
+        }
+    }
+    HIP_TRY(c, launch_boundary(c->d_sc, 1, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev_call[1], c->stream));
+    HIP_TRY(c, hipMemcpyAsync(out, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev_call[0], c->ev_call[1]));
+    c->last_total_ms += ms;
+    for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
+        float pm = 0.0f;
+        HIP_TRY(c, hipEventElapsedTime(&pm, c->ev[k], c->ev[k + 1]));
+        c->last_pair_ms += pm;
+    }
+    c->force_evals += 4 * steps;
+    if (out->err == 2)
+        return fail(c, DEMON_ERR_STATE,
+                    "dt check: two tested particles coincide (distance 0); the reference's "
+                    "modifyTimeStep would divide dt by 10 forever");
+    return DEMON_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *demon_version(void) { return "demon-mi355x 0.1 (gfx950)"; }
+int demon_abi_version(void) { return DEMON_ABI_VERSION; }
+
+const char *demon_last_error(const demon_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_err.c_str();
+}
+
+int demon_create(demon_ctx **out, int64_t n, const demon_params *p) {
+    if (!out) return fail(nullptr, DEMON_ERR_ARG, "ctx out-pointer is NULL");
+    *out = nullptr;
+    if (n < 2 || n > (int64_t)1 << 30) return fail(nullptr, DEMON_ERR_ARG, "n must be in [2, 2^30]");
+    int rc = check_params(nullptr, p);
+    if (rc) return rc;
+    demon_ctx *c = new (std::nothrow) demon_ctx();
+    if (!c) return fail(nullptr, DEMON_ERR_ALLOC, "host allocation failed");
+    c->p = *p;
+    c->n = n;
+    c->npad = round_up(n, kPad);
+    auto bail = [&](int code) {
+        g_err = c->err;
+        demon_destroy(c);
+        return code;
+    };
+    hipError_t e = hipSetDevice(p->device);
+    if (e != hipSuccess) return bail(hip_fail(c, e, "hipSetDevice"));
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(hip_fail(c, e, "hipStreamCreate"));
+    const size_t P = (size_t)c->npad;
+    e = hipMalloc((void **)&c->d_arena, sizeof(double) * P * 31);
+    if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc state arena failed"));
+    e = hipMalloc((void **)&c->d_part, sizeof(double) * P * 2 * kMaxSlices);
+    if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc partial sums failed"));
+    e = hipMalloc((void **)&c->d_orig, sizeof(int32_t) * P);
+    if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc index map failed"));
+    e = hipMalloc((void **)&c->d_sc, sizeof(Scalars));
+    if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc scalars failed"));
+    e = hipMemset(c->d_arena, 0, sizeof(double) * P * 31);
+    if (e == hipSuccess) e = hipMemset(c->d_part, 0, sizeof(double) * P * 2 * kMaxSlices);
+    if (e == hipSuccess) e = hipMemset(c->d_sc, 0, sizeof(Scalars));
+    if (e != hipSuccess) return bail(hip_fail(c, e, "hipMemset"));
+    double *b = c->d_arena;
+    auto take = [&]() { double *r = b; b += P; return r; };
+    c->x = take(); c->y = take(); c->vx = take(); c->vy = take(); c->q = take(); c->m = take();
+    for (StageBuf *s : {&c->sb[0], &c->sb[1], &c->sx}) {
+        s->xs = take(); s->ys = take(); s->vxs = take(); s->vys = take(); s->nc = take();
+    }
+    c->acc1 = b; b += 4 * P;
+    c->acc23 = b; b += 4 * P;
+    c->fout = b; b += 2 * P;
+    for (int k = 0; k < 2; ++k) {
+        e = hipEventCreate(&c->ev_call[k]);
+        if (e != hipSuccess) return bail(hip_fail(c, e, "hipEventCreate"));
+    }
+    c->h_tmp.assign(P * 4, 0.0);
+    choose_slices(c);
+    *out = c;
+    return DEMON_OK;
+}
+
+void demon_destroy(demon_ctx *c) {
+    if (!c) return;
+    drop_graph(c);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->ev_call)
+        if (e) (void)hipEventDestroy(e);
+    if (c->d_arena) (void)hipFree(c->d_arena);
+    if (c->d_part) (void)hipFree(c->d_part);
+    if (c->d_orig) (void)hipFree(c->d_orig);
+    if (c->d_sc) (void)hipFree(c->d_sc);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int demon_set_params(demon_ctx *c, const demon_params *p) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (p->device != c->p.device) return fail(c, DEMON_ERR_ARG, "device cannot change");
+    c->p = *p;
+    drop_graph(c);
+    return DEMON_OK;
+}
+
+int demon_get_params(const demon_ctx *c, demon_params *p) {
+    if (!c || !p) return fail(nullptr, DEMON_ERR_ARG, "NULL argument");
+    *p = c->p;
+    return DEMON_OK;
+}
+
+int demon_upload(demon_ctx *c, const double *x, const double *y, const double *vx,
+                 const double *vy, const double *charge, const double *mass) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!x || !y || !charge || !mass) return fail(c, DEMON_ERR_ARG, "x, y, charge and mass are required");
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    build_perm(c, x, y);
+    std::vector<int32_t> orig((size_t)c->npad);
+    for (int64_t k = 0; k < c->npad; ++k)
+        orig[(size_t)k] = k < c->n ? (int32_t)c->perm[(size_t)k] : (int32_t)(0x40000000 + k);
+    HIP_TRY(c, hipMemcpy(c->d_orig, orig.data(), sizeof(int32_t) * c->npad, hipMemcpyHostToDevice));
+    int rc;
+    // each put uses its own staging slot until the stream is synchronised
+    if ((rc = put(c, c->x, x, 0.0, 0)) || (rc = put(c, c->y, y, 0.0, 1)) ||
+        (rc = put(c, c->vx, vx, 0.0, 2)) || (rc = put(c, c->vy, vy, 0.0, 3)))
+        return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if ((rc = put(c, c->q, charge, 0.0, 0)) || (rc = put(c, c->m, mass, 1.0, 1))) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    // stage-1 record = base state
+    const size_t bytes = sizeof(double) * c->npad;
+    HIP_TRY(c, hipMemcpyAsync(c->sb[0].xs, c->x, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->sb[0].ys, c->y, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->sb[0].vxs, c->vx, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->sb[0].vys, c->vy, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, launch_prepare_stage(c->sb[0], c->n, c->npad, c->stream));
+    HIP_TRY(c, launch_prepare_stage(c->sb[1], c->n, c->npad, c->stream));
+    HIP_TRY(c, launch_prepare_stage(c->sx, 0, c->npad, c->stream));
+    Scalars s{};
+    s.dtmin_bits = 0x7f800000u;
+    HIP_TRY(c, hipMemcpyAsync(c->d_sc, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_rmax(c->sb[0], c->n, c->d_sc, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->uploaded = true;
+    c->force_evals = 0;
+    return DEMON_OK;
+}
+
+int demon_download(demon_ctx *c, double *x, double *y, double *vx, double *vy) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    double *dst[4] = {x, y, vx, vy};
+    const double *src[4] = {c->x, c->y, c->vx, c->vy};
+    for (int a = 0; a < 4; ++a) {
+        if (!dst[a]) continue;
+        HIP_TRY(c, hipMemcpyAsync(c->h_tmp.data(), src[a], sizeof(double) * c->npad,
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (int64_t k = 0; k < c->n; ++k) dst[a][c->perm[(size_t)k]] = c->h_tmp[(size_t)k];
+    }
+    return DEMON_OK;
+}
+
+int demon_force_stage(demon_ctx *c, int32_t forces, int32_t stage, uint64_t step,
+                      const double *xs, const double *ys, const double *vxs,
+                      const double *vys, double *fx, double *fy) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "demon_upload must provide charge and mass first");
+    if (!xs || !ys || !fx || !fy) return fail(c, DEMON_ERR_ARG, "NULL array");
+    if ((forces & DEMON_FORCE_DRAG) && (!vxs || !vys)) return fail(c, DEMON_ERR_ARG, "drag needs velocities");
+    if (forces & ~c->p.forces) return fail(c, DEMON_ERR_ARG, "force not enabled in the context");
+    if (stage < 1 || stage > 4) return fail(c, DEMON_ERR_ARG, "stage must be 1..4");
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    int rc;
+    if ((rc = put(c, c->sx.xs, xs, 0.0, 0)) || (rc = put(c, c->sx.ys, ys, 0.0, 1)) ||
+        (rc = put(c, c->sx.vxs, vxs, 0.0, 2)) || (rc = put(c, c->sx.vys, vys, 0.0, 3)))
+        return rc;
+    HIP_TRY(c, launch_prepare_stage(c->sx, c->n, c->npad, c->stream));
+    const bool yk = forces & DEMON_FORCE_SHIELDED_COULOMB;
+    c->ev_used = 0;
+    if (yk && (rc = enqueue_pairs(c, c->sx, 0, 0))) return rc;
+    ForceOutLaunch f{};
+    f.st = c->sx; f.q = c->q; f.m = c->m; f.orig = c->d_orig; f.part = c->d_part;
+    f.slices = c->slices; f.fx = c->fout; f.fy = c->fout + c->npad; f.n = c->n; f.npad = c->npad;
+    f.forces = forces; f.confine = c->p.confine; f.gamma = c->p.drag_gamma; f.heat = c->p.heat_val;
+    f.seed = c->p.seed; f.step = step; f.stage = stage;
+    HIP_TRY(c, launch_force_out(f, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->h_tmp.data(), c->fout, sizeof(double) * 2 * c->npad,
+                              hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const double *gx = c->h_tmp.data(), *gy = gx + c->npad;
+    for (int64_t k = 0; k < c->n; ++k) {
+        const int64_t o = c->perm[(size_t)k];
+        fx[o] += gx[k];
+        fy[o] += gy[k];
+    }
+    return DEMON_OK;
+}
+
+int demon_rk4_steps(demon_ctx *c, int64_t steps, double init_dt, double *current_time) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
+    if (!current_time || steps < 0 || !(init_dt > 0.0)) return fail(c, DEMON_ERR_ARG, "bad steps/dt/time");
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    c->last_pair_ms = c->last_total_ms = 0.0;
+    hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, *current_time,
+                       std::numeric_limits<double>::infinity(), init_dt,
+                       (unsigned long long)steps);
+    HIP_TRY(c, hipGetLastError());
+    Scalars s{};
+    const int rc = run_steps(c, steps, &s);
+    if (rc) return rc;
+    *current_time = s.t;
+    return DEMON_OK;
+}
+
+int demon_rk4_move(demon_ctx *c, double *current_time, double end_time, double init_dt,
+                   int64_t *steps_done) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
+    if (!current_time || !(init_dt > 0.0)) return fail(c, DEMON_ERR_ARG, "bad dt/time");
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    c->last_pair_ms = c->last_total_ms = 0.0;
+    Scalars s{};
+    HIP_TRY(c, hipMemcpy(&s, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    const unsigned long long before = s.steps_done;
+    hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, *current_time,
+                       end_time, init_dt, ~0ull);
+    HIP_TRY(c, hipGetLastError());
+    double t = *current_time;
+    for (;;) {
+        // steps the while loop needs if dt stays init_dt (same accumulation)
+        int64_t est = 0;
+        for (double tt = t; tt < end_time && est < 4096; tt += init_dt) ++est;
+        if (est == 0) break;
+        const int rc = run_steps(c, est, &s);
+        if (rc) return rc;
+        if (!(s.t > t)) return fail(c, DEMON_ERR_STATE, "time does not advance (dt underflow)");
+        t = s.t;
+    }
+    *current_time = t;
+    if (steps_done) *steps_done = (int64_t)(s.steps_done - before);
+    return DEMON_OK;
+}
+
+int demon_use_graph(demon_ctx *c, int32_t enable) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    c->use_graph = enable != 0;
+    if (!c->use_graph) drop_graph(c);
+    return DEMON_OK;
+}
+
+int demon_get_stats(const demon_ctx *c, demon_stats *out) {
+    if (!c || !out) return fail(nullptr, DEMON_ERR_ARG, "NULL argument");
+    Scalars s{};
+    hipError_t e = hipMemcpy(&s, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(const_cast<demon_ctx *>(c), e, "hipMemcpy stats");
+    out->steps = (int64_t)s.steps_done;
+    out->dt_reductions = (int64_t)s.dt_reductions;
+    out->force_evals = c->force_evals;
+    out->current_time = s.t;
+    out->last_dt = s.dt_step;
+    return DEMON_OK;
+}
+
+int demon_synchronize(demon_ctx *c) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DEMON_OK;
+}
+
+int demon_set_pair_shape(demon_ctx *c, int32_t i_per_lane, int32_t j_slices) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (i_per_lane != 1 && i_per_lane != 2 && i_per_lane != 4)
+        return fail(c, DEMON_ERR_ARG, "i_per_lane must be 1, 2 or 4");
+    if (j_slices < 0 || j_slices > kMaxSlices) return fail(c, DEMON_ERR_ARG, "j_slices out of range");
+    c->i_per_lane = i_per_lane;
+    c->slices_req = j_slices;
+    choose_slices(c);
+    drop_graph(c);
+    return DEMON_OK;
+}
+
+int demon_get_pair_shape(const demon_ctx *c, int32_t *i_per_lane, int32_t *j_slices) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (i_per_lane) *i_per_lane = c->i_per_lane;
+    if (j_slices) *j_slices = c->slices;
+    return DEMON_OK;
+}
+
+int demon_last_timing(const demon_ctx *c, double *pair_ms, double *total_ms) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (pair_ms) *pair_ms = c->last_pair_ms;
+    if (total_ms) *total_ms = c->last_total_ms;
+    return DEMON_OK;
+}
+
+int demon_hex_lattice(int64_t n, double spacing, double jitter, uint64_t seed, double *x,
+                      double *y) {
+    if (n < 1 || !x || !y || !(spacing > 0.0)) return fail(nullptr, DEMON_ERR_ARG, "bad lattice arguments");
+    // radius (in spacings) of a disc holding n sites of area sqrt(3)/2 each, plus margin
+    const double rho = std::sqrt((double)n * std::sqrt(3.0) / (2.0 * M_PI)) + 3.0;
+    const int64_t L = (int64_t)std::ceil(2.0 * rho) + 2;
+    struct Site { int64_t r2, j, i; };
+    std::vector<Site> sites;
+    sites.reserve((size_t)((2 * L + 1) * (2 * L + 1)));
+    for (int64_t j = -L; j <= L; ++j)
+        for (int64_t i = -L; i <= L; ++i) sites.push_back({i * i + i * j + j * j, j, i});
+    if ((int64_t)sites.size() < n) return fail(nullptr, DEMON_ERR_ARG, "lattice too small");
+    std::partial_sort(sites.begin(), sites.begin() + n, sites.end(), [](const Site &a, const Site &b) {
+        if (a.r2 != b.r2) return a.r2 < b.r2;
+        if (a.j != b.j) return a.j < b.j;
+        return a.i < b.i;
+    });
+    const double h = 0.8660254037844386;  // sqrt(3)/2
+    double cx = 0.0, cy = 0.0;
+    for (int64_t k = 0; k < n; ++k) {
+        x[k] = spacing * ((double)sites[(size_t)k].i + 0.5 * (double)sites[(size_t)k].j);
+        y[k] = spacing * (h * (double)sites[(size_t)k].j);
+        cx += x[k];
+        cy += y[k];
+    }
+    cx /= (double)n;
+    cy /= (double)n;
+    for (int64_t k = 0; k < n; ++k) {
+        const U32x4 o = draw(seed, (uint64_t)k, 0, kDomainLattice, 0);
+        x[k] = (x[k] - cx) + jitter * (2.0 * u01(o.v[0], o.v[1]) - 1.0);
+        y[k] = (y[k] - cy) + jitter * (2.0 * u01(o.v[2], o.v[3]) - 1.0);
+    }
+    return DEMON_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// internal.h -- device data layout and kernel launchers shared by the
+// translation units of libdemon_gpu.so.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace demon {
+
+// 1/(4*pi*eps0) with eps0 = 8.8541878e-12, as ShieldedCoulombForce.cpp:21.
+constexpr double kCoulomb = 1.0 / (4.0 * 3.14159265358979323846 * 8.8541878E-12);
+// Relative widening of the cutoff radius^2 in the squared-distance prefilter.
+constexpr double kCutoffSlack = 1.0e-9;
+// |r|^2 is stored scaled by (1 - 2^-46) so that the expanded-form
+// squared distance |ri|^2 + |rj|^2 - 2 ri.rj errs on the near side by more
+// than its rounding error (<= 16 u (|ri|^2 + |rj|^2), u = 2^-53).
+constexpr double kNormShrink = 1.0 - 0x1.0p-46;
+// Particles are padded to a multiple of this (covers every i tile shape).
+constexpr int kPad = 1024;
+// Maximum number of j slices of the all-pairs kernel.
+constexpr int kMaxSlices = 64;
+
+// One RK stage's state, in the device (tile) order: positions, velocities and
+// the prefilter norm nc = -(xs^2 + ys^2) * kNormShrink (-inf in padding).
+struct StageBuf {
+    double *xs, *ys, *vxs, *vys, *nc;
+};
+
+// Device scalars of the stepping loop.
+struct Scalars {
+    double t;                 // integrator currentTime
+    double t_end;             // moveParticles(endTime)
+    double init_dt;           // Integrator::init_dt
+    double dt_step;           // dt of the current step (after the dt check)
+    unsigned long long step;        // RNG step counter
+    unsigned long long steps_done;  // steps taken since upload
+    unsigned long long remaining;   // steps still allowed in this call
+    unsigned long long dt_reductions;
+    unsigned long long rmax2_bits;  // max |r|^2 of the stage-1 positions (double bits)
+    unsigned int dtmin_bits;        // min FP32 squared distance of tested pairs (float bits)
+    int active;                     // 1 while the current step runs
+    int err;                        // sticky device-side error code
+    int pad_;
+};
+
+struct PairLaunch {
+    const double *xs, *ys, *nc, *q;
+    const int32_t *orig;
+    double *part;            // [slices][2][npad] partial force sums
+    Scalars *sc;
+    int64_t n, npad;
+    int32_t jslice;          // j extent per slice (multiple of the chunk)
+    int32_t slices;
+    int32_t i_per_lane;      // 1, 2 or 4
+    int32_t dtcheck;         // fold the FP32 dt check into the near path
+    int32_t partner;         // intra-pair formula for (2k, 2k+1) in caller order
+    int32_t gated;           // skip when sc->active == 0
+    int32_t fp32pair;        // FP32 pair arithmetic
+    double kappa, rc2p;      // shielding and (10/kappa)^2 (1 + kCutoffSlack)
+};
+
+hipError_t launch_pairs(const PairLaunch &a, hipStream_t s);
+
+struct UpdateLaunch {
+    StageBuf cur, nxt;
+    double *x, *y, *vx, *vy;        // base state
+    const double *q, *m;
+    const int32_t *orig;
+    const double *part;
+    int32_t slices;
+    double *acc1, *acc23;           // [4][npad]: (k, l, m, n) of stage 1; stage 2 (+3)
+    Scalars *sc;
+    int64_t n, npad;
+    int32_t forces;
+    int32_t dtcheck;
+    double confine, gamma, heat;
+    uint64_t seed;
+};
+
+hipError_t launch_update(int stage, const UpdateLaunch &a, hipStream_t s);
+hipError_t launch_boundary(Scalars *sc, int closing, hipStream_t s);
+
+// Force::forceS evaluation on a stage state: out = forces(xs, ys, vxs, vys).
+struct ForceOutLaunch {
+    StageBuf st;
+    const double *q, *m;
+    const int32_t *orig;
+    const double *part;
+    int32_t slices;
+    double *fx, *fy;
+    int64_t n, npad;
+    int32_t forces;
+    double confine, gamma, heat;
+    uint64_t seed, step;
+    int32_t stage;
+};
+hipError_t launch_force_out(const ForceOutLaunch &a, hipStream_t s);
+
+// nc = -(xs^2+ys^2)*kNormShrink for i < n; padding gets xs=ys=0, nc=-inf.
+hipError_t launch_prepare_stage(StageBuf st, int64_t n, int64_t npad, hipStream_t s);
+// rmax2 = max |r|^2 over the stage positions (into sc->rmax2_bits, after reset).
+hipError_t launch_rmax(StageBuf st, int64_t n, Scalars *sc, hipStream_t s);
+
+}  // namespace demon

@@ -1,0 +1,265 @@
+// stage.hip -- the O(N) per-stage work of Runge_Kutta4::moveParticles, fused
+// into one pass per stage.
+//
+// For stage s the update kernel, per particle:
+//   * sums the stage force in the driver's order (driver.cpp:372-392):
+//       Confinement  F  = (q C) r_s            ConfinementForce.cpp:48-54
+//       Drag         F += (m (-gamma)) v_s     DragForce.cpp:143-149
+//       Yukawa       F += sum of the all-pairs partial sums (fixed order)
+//       Thermal      F += heat U (cos t, sin t) ThermalForce.cpp:118-122
+//   * forms the RK increments k = (dt F)/m, l = dt v_s (Runge_Kutta4.cpp:32-104),
+//   * builds the next stage state (CacheOperator.cpp:28-73: x + l/2, x + l)
+//     or, after stage 4, the combine x += (a1 + ((a2 + a3) 2 + a4)) / 6
+//     (Runge_Kutta4.cpp:107-151),
+//   * writes the next stage's pair records (x, y, v, nc).
+// Stage 1 also turns the device-wide FP32 minimum distance into the step's dt
+// (Integrator::tryToReduceTimeStep, Integrator.cpp:144-155).
+// The force arrays of the reference are never materialised between stages,
+// so nothing needs zeroing (Runge_Kutta4.cpp:44-46).
+#include "internal.h"
+#include "philox.h"
+
+namespace demon {
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+__device__ __forceinline__ double da(double a1, double a23, double a4) {
+    // Runge_Kutta4::da with a2 + a3 already summed (same rounding: a2 + a3 first).
+    return (a1 + (a23 * 2.0 + a4)) / 6.0;
+}
+
+// Stage force on particle i (caller index oi) in the reference's summation order.
+__device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys, double vxs,
+                                            double vys, double qi, double mi,
+                                            const double *part, int32_t slices, int64_t npad,
+                                            int64_t i, double confine, double gamma,
+                                            double heat, uint64_t seed, uint64_t step,
+                                            int32_t stage, int32_t oi, double &fx, double &fy) {
+    fx = 0.0;
+    fy = 0.0;
+    if (forces & 1) {
+        const double cv = qi * confine;
+        fx = fx + cv * xs;
+        fy = fy + cv * ys;
+    }
+    if (forces & 2) {
+        const double dr = mi * (-gamma);
+        fx = fx + dr * vxs;
+        fy = fy + dr * vys;
+    }
+    if (forces & 4) {
+        for (int32_t s = 0; s < slices; ++s) {
+            fx = fx + part[(size_t)(2 * s) * npad + i];
+            fy = fy + part[(size_t)(2 * s + 1) * npad + i];
+        }
+    }
+    if (forces & 16) {
+        const U32x4 o = draw(seed, (uint64_t)oi, step, kDomainThermal, (uint32_t)stage);
+        const double uu = u01(o.v[0], o.v[1]);
+        const double th = u01(o.v[2], o.v[3]) * kTwoPi;
+        const double tv = uu * heat;
+        fx = fx + tv * cos(th);
+        fy = fy + tv * sin(th);
+    }
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
+    __shared__ double red[4];
+    Scalars *sc = a.sc;
+    if (!sc->active) return;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+
+    double dt;
+    if (STAGE == 1) {
+        // Integrator::modifyTimeStep: while any tested pair has
+        // sqrtf(s) <= dist: dist /= 10, dt /= 10 (Integrator.cpp:144-155).  The
+        // outcome depends only on the smallest s, so every thread replays the
+        // loop on the device-wide minimum.
+        dt = sc->init_dt;
+        int err = 0;
+        if (a.dtcheck) {
+            const float d = sqrtf(__uint_as_float(sc->dtmin_bits));
+            float dist = 1.0e-4f;
+            int k = 0;
+            while (d <= dist) {
+                dist = dist / 10.0f;
+                dt = dt / 10.0;
+                if (++k > 60) {  // coincident particles: the reference loops forever
+                    err = 2;
+                    break;
+                }
+            }
+        }
+        if (i == 0) {
+            sc->dt_step = dt;
+            if (dt != sc->init_dt) sc->dt_reductions += 1;
+            if (err) sc->err = err;
+            sc->rmax2_bits = 0;  // refilled by this step's stage 4
+        }
+    } else {
+        dt = sc->dt_step;
+    }
+
+    double r2new = 0.0;
+    if (i < a.n) {
+        const double xs = a.cur.xs[i], ys = a.cur.ys[i];
+        const double vxs = a.cur.vxs[i], vys = a.cur.vys[i];
+        const double qi = a.q[i], mi = a.m[i];
+        double fx, fy;
+        stage_force(a.forces, xs, ys, vxs, vys, qi, mi, a.part, a.slices, a.npad, i, a.confine,
+                    a.gamma, a.heat, a.seed, sc->step, STAGE, a.orig[i], fx, fy);
+        const double k = (dt * fx) / mi;
+        const double l = dt * vxs;
+        const double mm = (dt * fy) / mi;
+        const double nn = dt * vys;
+        const double x = a.x[i], y = a.y[i], vx = a.vx[i], vy = a.vy[i];
+        double *A1 = a.acc1, *A23 = a.acc23;
+        const int64_t P = a.npad;
+        double nx, ny, nvx, nvy;
+        if (STAGE == 1 || STAGE == 2) {
+            double *A = (STAGE == 1) ? A1 : A23;
+            A[i] = k;
+            A[P + i] = l;
+            A[2 * P + i] = mm;
+            A[3 * P + i] = nn;
+            nx = 0.5 * l + x;
+            ny = 0.5 * nn + y;
+            nvx = 0.5 * k + vx;
+            nvy = 0.5 * mm + vy;
+        } else if (STAGE == 3) {
+            A23[i] = A23[i] + k;
+            A23[P + i] = A23[P + i] + l;
+            A23[2 * P + i] = A23[2 * P + i] + mm;
+            A23[3 * P + i] = A23[3 * P + i] + nn;
+            nx = x + l;
+            ny = y + nn;
+            nvx = vx + k;
+            nvy = vy + mm;
+        } else {
+            nvx = vx + da(A1[i], A23[i], k);
+            nx = x + da(A1[P + i], A23[P + i], l);
+            nvy = vy + da(A1[2 * P + i], A23[2 * P + i], mm);
+            ny = y + da(A1[3 * P + i], A23[3 * P + i], nn);
+            a.x[i] = nx;
+            a.y[i] = ny;
+            a.vx[i] = nvx;
+            a.vy[i] = nvy;
+        }
+        const double r2 = nx * nx + ny * ny;
+        a.nxt.xs[i] = nx;
+        a.nxt.ys[i] = ny;
+        a.nxt.vxs[i] = nvx;
+        a.nxt.vys[i] = nvy;
+        a.nxt.nc[i] = -r2 * kNormShrink;
+        r2new = r2;
+    }
+    if (STAGE == 4) {
+        // block max of |r|^2 for the next stage-1 candidate radius
+        double v = r2new;
+        for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double b = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+            if (b > 0.0) atomicMax(&sc->rmax2_bits, (unsigned long long)__double_as_longlong(b));
+        }
+    }
+}
+
+// Step boundary: close the previous step (t += dt, Runge_Kutta4.cpp:135) and
+// decide whether the next one runs (the while test, :24).
+// The closing launch of a call only finishes the last step.
+__global__ void k_boundary(Scalars *sc, int closing) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (sc->active) {
+        sc->t = sc->t + sc->dt_step;
+        sc->steps_done += 1;
+        sc->step += 1;
+    }
+    const int act = !closing && (sc->t < sc->t_end) && (sc->remaining > 0) && (sc->err == 0);
+    sc->active = act;
+    if (act) sc->remaining -= 1;
+    sc->dtmin_bits = 0x7f800000u;  // +inf
+}
+
+__global__ __launch_bounds__(256) void k_force_out(ForceOutLaunch a) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    double fx, fy;
+    stage_force(a.forces, a.st.xs[i], a.st.ys[i], a.st.vxs[i], a.st.vys[i], a.q[i], a.m[i],
+                a.part, a.slices, a.npad, i, a.confine,
+                a.gamma, a.heat, a.seed, a.step, a.stage, a.orig[i], fx, fy);
+    a.fx[i] = fx;
+    a.fy[i] = fy;
+}
+
+__global__ __launch_bounds__(256) void k_prepare(StageBuf st, int64_t n, int64_t npad) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= npad) return;
+    if (i < n) {
+        const double x = st.xs[i], y = st.ys[i];
+        st.nc[i] = -(x * x + y * y) * kNormShrink;
+    } else {
+        st.xs[i] = 0.0;
+        st.ys[i] = 0.0;
+        st.vxs[i] = 0.0;
+        st.vys[i] = 0.0;
+        st.nc[i] = -__builtin_inf();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rmax(StageBuf st, int64_t n, Scalars *sc) {
+    __shared__ double red[4];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (i < n) v = st.xs[i] * st.xs[i] + st.ys[i] * st.ys[i];
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double b = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        if (b > 0.0) atomicMax(&sc->rmax2_bits, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+hipError_t launch_update(int stage, const UpdateLaunch &a, hipStream_t s) {
+    const dim3 grid(blocks_for(a.n)), block(256);
+    switch (stage) {
+        case 1: hipLaunchKernelGGL(k_update<1>, grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_update<2>, grid, block, 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_update<3>, grid, block, 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_update<4>, grid, block, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_boundary(Scalars *sc, int closing, hipStream_t s) {
+    hipLaunchKernelGGL(k_boundary, dim3(1), dim3(64), 0, s, sc, closing);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_out(const ForceOutLaunch &a, hipStream_t s) {
+    if (a.n < 1) return hipSuccess;
+    hipLaunchKernelGGL(k_force_out, dim3(blocks_for(a.n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_prepare_stage(StageBuf st, int64_t n, int64_t npad, hipStream_t s) {
+    hipLaunchKernelGGL(k_prepare, dim3(blocks_for(npad)), dim3(256), 0, s, st, n, npad);
+    return hipGetLastError();
+}
+
+hipError_t launch_rmax(StageBuf st, int64_t n, Scalars *sc, hipStream_t s) {
+    hipLaunchKernelGGL(k_rmax, dim3(blocks_for(n)), dim3(256), 0, s, st, n, sc);
+    return hipGetLastError();
+}
+
+}  // namespace demon

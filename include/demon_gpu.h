@@ -1,0 +1,152 @@
+/*
+ * demon_gpu.h -- C ABI of the MI355X (gfx950) implementation of DEMON's hot
+ * path: the O(N^2) shielded-Coulomb (Yukawa) force with its 10-lambda cutoff,
+ * confinement, drag and thermal forces, and Runge-Kutta-4 stepping with the
+ * reference's adaptive time-step check.
+ *
+ * Plain C types only: host arrays are double* of length n in the caller's
+ * particle order; nothing here exposes HIP or torch types.  Every call returns
+ * DEMON_OK (0) or a negative DEMON_ERR_* code; demon_last_error() gives the
+ * message.  A context is not thread-safe; use one per thread / per cloud.
+ *
+ * Reference interfaces replaced (AU-PSL/demon_simulation_code):
+ *   Cloud (Cloud.h:16-87)                  -> demon_create / demon_upload /
+ *                                             demon_download
+ *   Force::force1..force4 (Force.h:15-69)  -> demon_force_stage
+ *     ShieldedCoulombForce (ShieldedCoulombForce.cpp:37-237)
+ *     ConfinementForce     (ConfinementForce.cpp:13-54)
+ *     DragForce            (DragForce.cpp:13-55)
+ *     ThermalForce         (ThermalForce.cpp:42-122)
+ *   Force::readForce / constructor args    -> demon_set_params
+ *   Runge_Kutta4::moveParticles (Runge_Kutta4.cpp:23-137), including
+ *     Integrator::modifyTimeStep (Integrator.cpp:68-182)
+ *                                          -> demon_rk4_move / demon_rk4_steps
+ */
+#ifndef DEMON_GPU_H
+#define DEMON_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DEMON_ABI_VERSION 1
+
+/* Return codes. */
+#define DEMON_OK 0
+#define DEMON_ERR_ARG (-1)         /* bad argument (null pointer, bad n, bad flag) */
+#define DEMON_ERR_HIP (-2)         /* HIP runtime error (message has the HIP string) */
+#define DEMON_ERR_ALLOC (-3)       /* device allocation failed */
+#define DEMON_ERR_STATE (-4)       /* state the reference cannot step (see message) */
+#define DEMON_ERR_UNSUPPORTED (-5) /* option not available in this build */
+
+/* Force selection: the reference's ForceFlag bit values (Force.h:75-91). */
+#define DEMON_FORCE_CONFINEMENT 1
+#define DEMON_FORCE_DRAG 2
+#define DEMON_FORCE_SHIELDED_COULOMB 4
+#define DEMON_FORCE_THERMAL 16
+
+/* Pair-force arithmetic. */
+#define DEMON_PRECISION_FP64 0      /* FP64 throughout */
+#define DEMON_PRECISION_FP32_PAIR 1 /* FP32 distance/exp/force per pair, FP64 sums and state */
+
+typedef struct demon_params {
+    double shielding;  /* kappa = 1/lambda [1/m]; ShieldedCoulombForce(cloud, shieldingConstant) */
+    double confine;    /* C [V/m^2]; ConfinementForce(cloud, confineConst): F += q*C*r */
+    double drag_gamma; /* gamma [1/s]; DragForce(cloud, gamma): F += m*(-gamma)*v */
+    double heat_val;   /* [N]; ThermalForce(cloud, heatVal): F += heatVal*U*(cos t, sin t) */
+    uint64_t seed;     /* key of the counter-based (Philox4x32-10) thermal RNG */
+    int32_t forces;    /* OR of DEMON_FORCE_* ; the reference always adds the dt check */
+    int32_t precision; /* DEMON_PRECISION_* */
+    int32_t dt_check;  /* 1: Integrator::modifyTimeStep each step (reference behaviour) */
+    int32_t device;    /* HIP device ordinal */
+} demon_params;
+
+/* Counters of the work done since demon_upload (or the last reset). */
+typedef struct demon_stats {
+    int64_t steps;           /* RK4 steps taken */
+    int64_t dt_reductions;   /* steps whose dt was divided by 10 at least once */
+    int64_t force_evals;     /* all-pairs force evaluations (4 per step) */
+    double current_time;     /* integrator time */
+    double last_dt;          /* dt of the last step */
+} demon_stats;
+
+typedef struct demon_ctx demon_ctx;
+
+/* Library version string and ABI version. */
+const char *demon_version(void);
+int demon_abi_version(void);
+
+/* Message of the last failed call on ctx (or of the calling thread when ctx is
+ * NULL, e.g. after a failed demon_create).  Never NULL. */
+const char *demon_last_error(const demon_ctx *ctx);
+
+/* Allocates an n-particle cloud on device p->device (Cloud::Cloud, Cloud.cpp:23-37).
+ * n >= 2.  With dt_check the reference requires n % 4 == 0 (Integrator.cpp:90);
+ * this build accepts any n and tests the partial last group's pairs. */
+int demon_create(demon_ctx **ctx, int64_t n, const demon_params *p);
+void demon_destroy(demon_ctx *ctx);
+
+/* Replaces the force parameters (ShieldedCoulombForce/ConfinementForce/
+ * DragForce/ThermalForce::readForce, e.g. ShieldedCoulombForce.cpp:268-277). */
+int demon_set_params(demon_ctx *ctx, const demon_params *p);
+int demon_get_params(const demon_ctx *ctx, demon_params *p);
+
+/* Copies the host Cloud arrays (x, y, Vx, Vy, charge, mass; Cloud.h:21-23) to
+ * the device and resets the step counters.  Velocities may be NULL (zero). */
+int demon_upload(demon_ctx *ctx, const double *x, const double *y, const double *vx,
+                 const double *vy, const double *charge, const double *mass);
+
+/* Copies the current state back (any pointer may be NULL). */
+int demon_download(demon_ctx *ctx, double *x, double *y, double *vx, double *vy);
+
+/* Force::force<stage>(t) for the forces in `forces` (a subset of the
+ * context's flags) evaluated on the given stage state, ADDED into fx/fy
+ * (host arrays, caller's order), as Force::forceS adds into Cloud::forceX/Y.
+ * stage (1..4) only keys the thermal RNG (with `step`).  vxs/vys may be NULL
+ * when DRAG is not requested.  Uses the context's charge and mass. */
+int demon_force_stage(demon_ctx *ctx, int32_t forces, int32_t stage, uint64_t step,
+                      const double *xs, const double *ys, const double *vxs,
+                      const double *vys, double *fx, double *fy);
+
+/* Runge_Kutta4::moveParticles(end_time): while (*current_time < end_time)
+ * take a step of dt = modifyTimeStep(1e-4f, init_dt) (Runge_Kutta4.cpp:24-25).
+ * *current_time is read on entry and updated.  *steps_done may be NULL. */
+int demon_rk4_move(demon_ctx *ctx, double *current_time, double end_time, double init_dt,
+                   int64_t *steps_done);
+
+/* Exactly `steps` iterations of the same loop body (no end-time test). */
+int demon_rk4_steps(demon_ctx *ctx, int64_t steps, double init_dt, double *current_time);
+
+/* Captures one RK4 step into a HIP graph and replays it for later
+ * demon_rk4_* calls (1 = on, 0 = plain launches). */
+int demon_use_graph(demon_ctx *ctx, int32_t enable);
+
+int demon_get_stats(const demon_ctx *ctx, demon_stats *s);
+
+/* Waits for all work queued on the context's stream. */
+int demon_synchronize(demon_ctx *ctx);
+
+/* Kernel shape of the all-pairs kernel: i particles per lane (1, 2 or 4) and
+ * number of j slices (0 = choose from n).  Exposed for tuning. */
+int demon_set_pair_shape(demon_ctx *ctx, int32_t i_per_lane, int32_t j_slices);
+int demon_get_pair_shape(const demon_ctx *ctx, int32_t *i_per_lane, int32_t *j_slices);
+
+/* Milliseconds of device time of the last demon_rk4_* call, split into the
+ * all-pairs kernels and everything else (HIP events on the context stream). */
+int demon_last_timing(const demon_ctx *ctx, double *pair_ms, double *total_ms);
+
+/* Host-only helper (no device needed): the synthetic hexagonal-lattice input
+ * used by the benchmarks -- the n sites of a triangular lattice of spacing
+ * `spacing` nearest the origin (ordered by lattice radius, then row, then
+ * column), centred, each coordinate jittered by U(-jitter, +jitter) drawn from
+ * Philox4x32-10 keyed by `seed`. */
+int demon_hex_lattice(int64_t n, double spacing, double jitter, uint64_t seed, double *x,
+                      double *y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DEMON_GPU_H */

@@ -1,0 +1,227 @@
+"""ctypes binding of the CPU oracle (liboracle.so) and, where it was built, of
+the reference's own hot path compiled in place (_ref/libdemon_ref_*.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py -- never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "liboracle.so"
+REF_SERIAL_SO = HERE / "_ref" / "libdemon_ref_serial.so"
+REF_OMP_SO = HERE / "_ref" / "libdemon_ref_omp.so"
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I64 = ctypes.c_int64
+
+
+class OracleParams(ctypes.Structure):
+    _fields_ = [
+        ("shielding", ctypes.c_double),
+        ("confine", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("heat_val", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("forces", ctypes.c_int32),
+        ("dt_check", ctypes.c_int32),
+    ]
+
+
+def build(ref: bool = True) -> None:
+    """make -C oracle (the _ref targets only where the reference sources exist)."""
+    targets = ["oracle"]
+    if ref and Path("/root/reference/Cloud.cpp").exists():
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", str(HERE), *targets], check=True)
+
+
+def p(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+class Oracle:
+    def __init__(self, path: Path = ORACLE_SO):
+        if not path.exists():
+            build(ref=False)
+        L = ctypes.CDLL(str(path))
+        self.L = L
+        L.oracle_coulomb_const.restype = ctypes.c_double
+        L.oracle_yukawa_add.argtypes = [_I64, _D, _D, _D, ctypes.c_double, _D, _D]
+        L.oracle_yukawa_add.restype = ctypes.c_int
+        for name in ("oracle_yukawa_gather_add", "oracle_yukawa_longdouble"):
+            getattr(L, name).argtypes = [_I64, _D, _D, _D, ctypes.c_double, _D, _D]
+            getattr(L, name).restype = None
+        L.oracle_yukawa_rows.argtypes = [_I64, _D, _D, _D, ctypes.c_double,
+                                         ctypes.POINTER(ctypes.c_int64), _I64, _D, _D]
+        L.oracle_yukawa_rows.restype = None
+        L.oracle_near_pairs.argtypes = [_I64, _D, _D, ctypes.c_double]
+        L.oracle_near_pairs.restype = ctypes.c_int64
+        L.oracle_confinement_add.argtypes = [_I64, _D, _D, _D, ctypes.c_double, _D, _D]
+        L.oracle_drag_add.argtypes = [_I64, _D, _D, _D, ctypes.c_double, _D, _D]
+        L.oracle_philox4x32.argtypes = [ctypes.POINTER(ctypes.c_uint32)] * 3
+        L.oracle_thermal_add.argtypes = [_I64, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64,
+                                         ctypes.c_int, _D, _D]
+        L.oracle_modify_time_step.argtypes = [_I64, _D, _D, ctypes.c_float, ctypes.c_double]
+        L.oracle_modify_time_step.restype = ctypes.c_double
+        L.oracle_dtcheck_min_s.argtypes = [_I64, _D, _D]
+        L.oracle_dtcheck_min_s.restype = ctypes.c_float
+        L.oracle_dtcheck_covers.argtypes = [_I64, _I64]
+        L.oracle_dtcheck_covers.restype = ctypes.c_int
+        L.oracle_rk4_steps.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.POINTER(OracleParams),
+                                       ctypes.c_double, _I64, _D, ctypes.POINTER(ctypes.c_uint64)]
+        L.oracle_rk4_move.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.POINTER(OracleParams),
+                                      ctypes.c_double, ctypes.c_double, _D,
+                                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_I64)]
+        L.oracle_stage_force.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.POINTER(OracleParams),
+                                         ctypes.c_uint64, ctypes.c_int, _D, _D]
+        self.coulomb = L.oracle_coulomb_const()
+
+    # --- forces -----------------------------------------------------------
+    def yukawa(self, x, y, q, kappa, fx=None, fy=None):
+        n = len(x)
+        fx = np.zeros(n) if fx is None else fx
+        fy = np.zeros(n) if fy is None else fy
+        x, y, q = _f64(x), _f64(y), _f64(q)
+        if n % 2 == 0:
+            rc = self.L.oracle_yukawa_add(n, p(x), p(y), p(q), kappa, p(fx), p(fy))
+            assert rc == 0
+        else:
+            self.L.oracle_yukawa_gather_add(n, p(x), p(y), p(q), kappa, p(fx), p(fy))
+        return fx, fy
+
+    def yukawa_gather(self, x, y, q, kappa):
+        n = len(x)
+        fx, fy = np.zeros(n), np.zeros(n)
+        self.L.oracle_yukawa_gather_add(n, p(_f64(x)), p(_f64(y)), p(_f64(q)), kappa, p(fx), p(fy))
+        return fx, fy
+
+    def yukawa_longdouble(self, x, y, q, kappa):
+        n = len(x)
+        fx, fy = np.zeros(n), np.zeros(n)
+        self.L.oracle_yukawa_longdouble(n, p(_f64(x)), p(_f64(y)), p(_f64(q)), kappa, p(fx), p(fy))
+        return fx, fy
+
+    def yukawa_rows(self, x, y, q, kappa, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        fx, fy = np.zeros(len(rows)), np.zeros(len(rows))
+        self.L.oracle_yukawa_rows(len(x), p(_f64(x)), p(_f64(y)), p(_f64(q)), kappa,
+                                  rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(rows),
+                                  p(fx), p(fy))
+        return fx, fy
+
+    def near_pairs(self, x, y, kappa) -> int:
+        return int(self.L.oracle_near_pairs(len(x), p(_f64(x)), p(_f64(y)), kappa))
+
+    def stage_force(self, xs, ys, vxs, vys, q, m, params: OracleParams, step=0, stage=1):
+        n = len(xs)
+        fx, fy = np.zeros(n), np.zeros(n)
+        rc = self.L.oracle_stage_force(n, p(_f64(xs)), p(_f64(ys)), p(_f64(vxs)), p(_f64(vys)),
+                                       p(_f64(q)), p(_f64(m)), ctypes.byref(params), step, stage,
+                                       p(fx), p(fy))
+        assert rc == 0
+        return fx, fy
+
+    def thermal(self, n, heat, seed, step, stage):
+        fx, fy = np.zeros(n), np.zeros(n)
+        self.L.oracle_thermal_add(n, heat, seed, step, stage, p(fx), p(fy))
+        return fx, fy
+
+    def philox(self, ctr, key):
+        c = (ctypes.c_uint32 * 4)(*ctr)
+        k = (ctypes.c_uint32 * 2)(*key)
+        o = (ctypes.c_uint32 * 4)()
+        self.L.oracle_philox4x32(c, k, o)
+        return list(o)
+
+    # --- dt check ---------------------------------------------------------
+    def modify_time_step(self, x, y, dist=1.0e-4, dt=1.0e-3) -> float:
+        return self.L.oracle_modify_time_step(len(x), p(_f64(x)), p(_f64(y)), dist, dt)
+
+    def dtcheck_min_s(self, x, y) -> float:
+        return self.L.oracle_dtcheck_min_s(len(x), p(_f64(x)), p(_f64(y)))
+
+    def covers(self, i, j) -> bool:
+        return bool(self.L.oracle_dtcheck_covers(i, j))
+
+    # --- RK4 ----------------------------------------------------------------
+    def rk4_steps(self, x, y, vx, vy, q, m, params: OracleParams, dt, steps, t=0.0, step0=0):
+        """Returns (x, y, vx, vy, t, step) after `steps` steps (copies)."""
+        s = [_f64(a).copy() for a in (x, y, vx, vy)]
+        tt = np.array([t])
+        sc = ctypes.c_uint64(step0)
+        rc = self.L.oracle_rk4_steps(len(s[0]), *[p(a) for a in s], p(_f64(q)), p(_f64(m)),
+                                     ctypes.byref(params), dt, steps, p(tt), ctypes.byref(sc))
+        assert rc == 0, "oracle_rk4_steps rejected the input"
+        return (*s, float(tt[0]), sc.value)
+
+    def rk4_move(self, x, y, vx, vy, q, m, params: OracleParams, dt, end_time, t=0.0):
+        s = [_f64(a).copy() for a in (x, y, vx, vy)]
+        tt = np.array([t])
+        sc = ctypes.c_uint64(0)
+        nst = _I64(0)
+        rc = self.L.oracle_rk4_move(len(s[0]), *[p(a) for a in s], p(_f64(q)), p(_f64(m)),
+                                    ctypes.byref(params), dt, end_time, p(tt), ctypes.byref(sc),
+                                    ctypes.byref(nst))
+        assert rc == 0
+        return (*s, float(tt[0]), nst.value)
+
+
+class Reference:
+    """The reference's own classes (compiled in place) behind ref_harness.cpp."""
+
+    def __init__(self, openmp: bool = False):
+        path = REF_OMP_SO if openmp else REF_SERIAL_SO
+        if not path.exists():
+            raise FileNotFoundError(f"{path} not built (needs the reference sources)")
+        L = ctypes.CDLL(str(path))
+        self.L = L
+        self.openmp = bool(L.ref_is_openmp())
+        L.ref_yukawa_stage.argtypes = [ctypes.c_int, _I64, _D, _D, _D, ctypes.c_double, _D, _D, ctypes.c_int]
+        L.ref_modify_time_step.argtypes = [_I64, _D, _D, ctypes.c_float, ctypes.c_double]
+        L.ref_modify_time_step.restype = ctypes.c_double
+        L.ref_rk4.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                              ctypes.c_int, _I64, ctypes.c_double, _D, ctypes.c_int]
+
+    @staticmethod
+    def available(openmp: bool = False) -> bool:
+        return (REF_OMP_SO if openmp else REF_SERIAL_SO).exists()
+
+    def yukawa(self, x, y, q, kappa, stage=1, threads=0):
+        n = len(x)
+        fx, fy = np.zeros(n), np.zeros(n)
+        rc = self.L.ref_yukawa_stage(stage, n, p(_f64(x)), p(_f64(y)), p(_f64(q)), kappa, p(fx), p(fy), threads)
+        assert rc == 0
+        return fx, fy
+
+    def modify_time_step(self, x, y, dist=1.0e-4, dt=1.0e-3):
+        return self.L.ref_modify_time_step(len(x), p(_f64(x)), p(_f64(y)), dist, dt)
+
+    def rk4_steps(self, x, y, vx, vy, q, m, kappa, confine, gamma, heat, forces, dt, steps,
+                  t=0.0, threads=0):
+        s = [_f64(a).copy() for a in (x, y, vx, vy)]
+        tt = np.array([t])
+        rc = self.L.ref_rk4(len(s[0]), *[p(a) for a in s], p(_f64(q)), p(_f64(m)), kappa, confine,
+                            gamma, heat, forces, dt, 0, steps, 0.0, p(tt), threads)
+        assert rc == 0
+        return (*s, float(tt[0]))
+
+    def rk4_move(self, x, y, vx, vy, q, m, kappa, confine, gamma, heat, forces, dt, end_time,
+                 t=0.0, threads=0):
+        s = [_f64(a).copy() for a in (x, y, vx, vy)]
+        tt = np.array([t])
+        rc = self.L.ref_rk4(len(s[0]), *[p(a) for a in s], p(_f64(q)), p(_f64(m)), kappa, confine,
+                            gamma, heat, forces, dt, 1, 0, end_time, p(tt), threads)
+        assert rc == 0
+        return (*s, float(tt[0]))

@@ -1,0 +1,532 @@
+/*
+ * demon_oracle.c -- CPU oracle for the DEMON Yukawa + RK4 hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see demon_oracle.h).  Built with
+ * -O2 -ffp-contract=off -fno-fast-math so that every + - * / is a single
+ * IEEE binary64/binary32 operation in source order, as in the reference's
+ * SSE2 code (VectorCompatibility.h:99-105: fmadd_pd is mul then add).
+ *
+ * All citations are to AU-PSL/demon_simulation_code.
+ */
+#define _GNU_SOURCE 1 /* M_PI */
+#include "demon_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+double oracle_coulomb_const(void)
+{
+    /* ShieldedCoulombForce.cpp:21 with Cloud.cpp:16 */
+    return 1.0 / (4.0 * M_PI * 8.8541878E-12);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Yukawa (shielded Coulomb) pair force                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Intra-pair term for particles (p, p+1): ShieldedCoulombForce.cpp:138-159.
+ * forceC = ((C*qq)*(1+u)) / (((d*d)*d)*exp(u)); F_p += forceC*d, F_p+1 -= it. */
+static void yk_intra(int64_t p, const double *xs, const double *ys, const double *q,
+                     double kappa, double coul, double *fx, double *fy)
+{
+    const double dX = xs[p] - xs[p + 1];          /* _mm_hsub_pd lane 0 */
+    const double dY = ys[p] - ys[p + 1];          /* _mm_hsub_pd lane 1 */
+    const double d = sqrt(dX * dX + dY * dY);
+    const double u = d * kappa;
+    if (u < 10.0) {
+        const double qq = q[p] * q[p + 1];
+        const double fc = coul * qq * (1.0 + u) / (d * d * d * exp(u));
+        const double fX = fc * dX;
+        const double fY = fc * dY;
+        fx[p] += fX;       fx[p + 1] += -fX;      /* _mm_set_pd(-fX, fX) */
+        fy[p] += fY;       fy[p + 1] += -fY;
+    }
+}
+
+/* One 2-lane cross evaluation (ShieldedCoulombForce.cpp:172-198 and 211-237).
+ * Lane l pairs outer particle o[l] with inner particle in[l].  If no lane is
+ * inside the cutoff nothing is added; otherwise both lanes are added, a lane
+ * outside the cutoff carrying exp = 0 (exp_pd, :279-292). */
+static void yk_cross2(const int64_t o[2], const int64_t in[2],
+                      const double *xs, const double *ys, const double *q,
+                      double kappa, double coul, double *fx, double *fy)
+{
+    double dX[2], dY[2], d[2], u[2];
+    int live[2];
+    for (int l = 0; l < 2; ++l) {
+        dX[l] = xs[o[l]] - xs[in[l]];
+        dY[l] = ys[o[l]] - ys[in[l]];
+        d[l] = sqrt(dX[l] * dX[l] + dY[l] * dY[l]);
+        u[l] = d[l] * kappa;
+        live[l] = u[l] < 10.0;
+    }
+    if (!live[0] && !live[1])
+        return;
+    for (int l = 0; l < 2; ++l) {
+        const double e = live[l] ? exp(-u[l]) : 0.0;
+        const double qq = q[o[l]] * q[in[l]];
+        const double fc = coul * qq * (1.0 + u[l]) * e / (d[l] * d[l] * d[l]);
+        const double fX = fc * dX[l];
+        const double fY = fc * dY[l];
+        fx[o[l]] += fX;
+        fy[o[l]] += fY;
+        fx[in[l]] -= fX;
+        fy[in[l]] -= fY;
+    }
+}
+
+int oracle_yukawa_add(int64_t n, const double *xs, const double *ys,
+                      const double *q, double kappa, double *fx, double *fy)
+{
+    if (n < 2 || (n & 1))
+        return -1;
+    const double coul = oracle_coulomb_const();
+    /* Outer loop over particle pairs (ShieldedCoulombForce.cpp:43). */
+    for (int64_t p = 0; p < n - 1; p += 2) {
+        yk_intra(p, xs, ys, q, kappa, coul, fx, fy);                    /* :51 */
+        for (int64_t i = p + 2; i < n; i += 2) {                         /* :52 */
+            const int64_t o[2] = {p, p + 1};
+            const int64_t straight[2] = {i, i + 1};                      /* force  :54 */
+            const int64_t swapped[2] = {i + 1, i};                       /* forcer :55 */
+            yk_cross2(o, straight, xs, ys, q, kappa, coul, fx, fy);
+            yk_cross2(o, swapped, xs, ys, q, kappa, coul, fx, fy);
+        }
+    }
+    return 0;
+}
+
+void oracle_yukawa_gather_add(int64_t n, const double *xs, const double *ys,
+                              const double *q, double kappa, double *fx, double *fy)
+{
+    const double coul = oracle_coulomb_const();
+    for (int64_t i = 0; i < n; ++i) {
+        double ax = fx[i], ay = fy[i];
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i)
+                continue;
+            const double dX = xs[i] - xs[j];
+            const double dY = ys[i] - ys[j];
+            const double d = sqrt(dX * dX + dY * dY);
+            const double u = d * kappa;
+            if (u < 10.0) {
+                const double fc = coul * (q[i] * q[j]) * (1.0 + u) * exp(-u) / (d * d * d);
+                ax += fc * dX;
+                ay += fc * dY;
+            }
+        }
+        fx[i] = ax;
+        fy[i] = ay;
+    }
+}
+
+void oracle_yukawa_rows(int64_t n, const double *xs, const double *ys, const double *q,
+                        double kappa, const int64_t *rows, int64_t nrows, double *fx, double *fy)
+{
+    const double coul = oracle_coulomb_const();
+    for (int64_t k = 0; k < nrows; ++k) {
+        const int64_t i = rows[k];
+        double ax = 0.0, ay = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i)
+                continue;
+            const double dX = xs[i] - xs[j];
+            const double dY = ys[i] - ys[j];
+            const double d = sqrt(dX * dX + dY * dY);
+            const double u = d * kappa;
+            if (u < 10.0) {
+                const double fc = coul * (q[i] * q[j]) * (1.0 + u) * exp(-u) / (d * d * d);
+                ax += fc * dX;
+                ay += fc * dY;
+            }
+        }
+        fx[k] = ax;
+        fy[k] = ay;
+    }
+}
+
+void oracle_yukawa_longdouble(int64_t n, const double *xs, const double *ys,
+                              const double *q, double kappa, double *fx, double *fy)
+{
+    const long double coul = 1.0L / (4.0L * 3.14159265358979323846264338327950288L * 8.8541878E-12L);
+    for (int64_t i = 0; i < n; ++i) {
+        long double ax = 0.0L, ay = 0.0L;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i)
+                continue;
+            const long double dX = (long double)xs[i] - (long double)xs[j];
+            const long double dY = (long double)ys[i] - (long double)ys[j];
+            const long double d = sqrtl(dX * dX + dY * dY);
+            const long double u = d * (long double)kappa;
+            /* classify with the reference's double rounding so that the
+             * set of contributing pairs is identical */
+            const double dxd = xs[i] - xs[j], dyd = ys[i] - ys[j];
+            const double ud = sqrt(dxd * dxd + dyd * dyd) * kappa;
+            if (ud < 10.0) {
+                const long double fc = coul * (long double)q[i] * (long double)q[j] *
+                                       (1.0L + u) * expl(-u) / (d * d * d);
+                ax += fc * dX;
+                ay += fc * dY;
+            }
+        }
+        fx[i] = (double)ax;
+        fy[i] = (double)ay;
+    }
+}
+
+int64_t oracle_near_pairs(int64_t n, const double *xs, const double *ys, double kappa)
+{
+    int64_t c = 0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j) {
+            if (i == j)
+                continue;
+            const double dX = xs[i] - xs[j], dY = ys[i] - ys[j];
+            if (sqrt(dX * dX + dY * dY) * kappa < 10.0)
+                ++c;
+        }
+    return c;
+}
+
+/* ------------------------------------------------------------------------ */
+/* O(N) forces                                                              */
+/* ------------------------------------------------------------------------ */
+
+void oracle_confinement_add(int64_t n, const double *xs, const double *ys,
+                            const double *q, double confine, double *fx, double *fy)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const double cv = q[i] * confine;       /* ConfinementForce.cpp:50 */
+        fx[i] += cv * xs[i];                    /* :52 */
+        fy[i] += cv * ys[i];                    /* :53 */
+    }
+}
+
+void oracle_drag_add(int64_t n, const double *vxs, const double *vys,
+                     const double *m, double gamma, double *fx, double *fy)
+{
+    const double drag_const = -gamma;           /* DragForce.h:17 */
+    for (int64_t i = 0; i < n; ++i) {
+        const double dr = m[i] * drag_const;    /* DragForce.cpp:144 */
+        fx[i] += dr * vxs[i];                   /* :146 */
+        fy[i] += dr * vys[i];                   /* :147 */
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Counter-based RNG and the thermal force                                  */
+/* ------------------------------------------------------------------------ */
+
+static inline void philox_mulhilo(uint32_t a, uint32_t b, uint32_t *hi, uint32_t *lo)
+{
+    const uint64_t p = (uint64_t)a * (uint64_t)b;
+    *hi = (uint32_t)(p >> 32);
+    *lo = (uint32_t)p;
+}
+
+void oracle_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    /* Philox4x32 with 10 rounds: multipliers 0xD2511F53 / 0xCD9E8D57,
+     * Weyl key increments 0x9E3779B9 / 0xBB67AE85 (Salmon et al. 2011). */
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0, lo0, hi1, lo1;
+        philox_mulhilo(0xD2511F53u, c0, &hi0, &lo0);
+        philox_mulhilo(0xCD9E8D57u, c2, &hi1, &lo1);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline double u53(uint32_t lo, uint32_t hi)
+{
+    const uint64_t v = (((uint64_t)hi << 32) | lo) >> 11;
+    return (double)v * 0x1.0p-53;
+}
+
+void oracle_thermal_add(int64_t n, double heat, uint64_t seed, uint64_t step,
+                        int stage, double *fx, double *fy)
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int64_t i = 0; i < n; ++i) {
+        /* counter: particle, step, (domain 1 = thermal, stage) */
+        const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), (uint32_t)step,
+                                 (1u << 24) | ((uint32_t)stage << 16) |
+                                     ((uint32_t)(step >> 32) & 0xFFFFu)};
+        uint32_t o[4];
+        oracle_philox4x32(ctr, key, o);
+        const double uu = u53(o[0], o[1]);                  /* RandCache::r  in [0,1) */
+        const double th = u53(o[2], o[3]) * (2.0 * M_PI);   /* RandCache::r1 in [0,2pi) */
+        const double tv = uu * heat;                        /* ThermalForce.cpp:119 */
+        fx[i] += tv * cos(th);                              /* :120 */
+        fy[i] += tv * sin(th);                              /* :121 */
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Integrator::modifyTimeStep                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Lanes of loadFloatVector (Integrator.cpp:162-168): _mm_set_ps puts x[o] in
+ * lane 3, so lane k holds particle o+3-k. */
+static inline int64_t lane_particle(int64_t o, int k) { return o + 3 - k; }
+
+/* tryToReduceTimeStep (Integrator.cpp:144-155) on a 4-lane vector of pairs. */
+static void try_reduce(const int64_t a[4], const int64_t b[4], const double *x,
+                       const double *y, float *dist, double *dt)
+{
+    float s[4];
+    for (int k = 0; k < 4; ++k) {
+        const float sx = (float)x[a[k]] - (float)x[b[k]];
+        const float sy = (float)y[a[k]] - (float)y[b[k]];
+        s[k] = sx * sx + sy * sy;                  /* add_ps(mul_ps, mul_ps) */
+    }
+    for (;;) {
+        int within = 0;
+        for (int k = 0; k < 4; ++k)
+            within |= sqrtf(s[k]) <= *dist;         /* isWithInDistance :181 */
+        if (!within)
+            break;
+        *dist /= 10.0f;
+        *dt /= 10.0f;
+    }
+}
+
+double oracle_modify_time_step(int64_t n, const double *x, const double *y,
+                               float dist, double dt)
+{
+    for (int64_t o = 0; o < n - 1; o += 4) {                       /* :90 */
+        int64_t A[4];
+        for (int k = 0; k < 4; ++k)
+            A[k] = lane_particle(o, k);
+        /* hsub(A, shuffle(A,A,(1,2,0,3))) = (A0-A1, A2-A3, A3-A0, A2-A1)  :96-98 */
+        {
+            const int64_t a[4] = {A[0], A[2], A[3], A[2]};
+            const int64_t b[4] = {A[1], A[3], A[0], A[1]};
+            try_reduce(a, b, x, y, &dist, &dt);
+        }
+        /* hsub(A, shuffle(A,A,(1,3,0,2))) = (A0-A1, A2-A3, A2-A0, A3-A1)  :102-104 */
+        {
+            const int64_t a[4] = {A[0], A[2], A[2], A[3]};
+            const int64_t b[4] = {A[1], A[3], A[0], A[1]};
+            try_reduce(a, b, x, y, &dist, &dt);
+        }
+        for (int64_t in = o + 4; in < n; in += 4) {                  /* :107 */
+            int64_t B[4];
+            for (int k = 0; k < 4; ++k)
+                B[k] = lane_particle(in, k);
+            /* A - B                                                     :112-114 */
+            {
+                const int64_t b[4] = {B[0], B[1], B[2], B[3]};
+                try_reduce(A, b, x, y, &dist, &dt);
+            }
+            /* A - shuffle(B,(3,2,1,0)) : identity shuffle, same pairs    :117-119 */
+            {
+                const int64_t b[4] = {B[0], B[1], B[2], B[3]};
+                try_reduce(A, b, x, y, &dist, &dt);
+            }
+            /* A - shuffle(B,(0,3,2,1)) = A - (B1,B2,B3,B0)               :122-124 */
+            {
+                const int64_t b[4] = {B[1], B[2], B[3], B[0]};
+                try_reduce(A, b, x, y, &dist, &dt);
+            }
+            /* A - shuffle(B,(1,0,3,2)) = A - (B2,B3,B0,B1)               :127-129 */
+            {
+                const int64_t b[4] = {B[2], B[3], B[0], B[1]};
+                try_reduce(A, b, x, y, &dist, &dt);
+            }
+        }
+    }
+    return dt;
+}
+
+int oracle_dtcheck_covers(int64_t i, int64_t j)
+{
+    if (i == j)
+        return 0;
+    if (i > j) {
+        const int64_t t = i; i = j; j = t;
+    }
+    const int64_t gi = i >> 2, gj = j >> 2;
+    if (gi == gj)
+        return 1;                       /* all 6 intra-group pairs are tested */
+    /* cross-group: outer lane i%4, inner lane j%4; the pattern j = i+1 (mod 4)
+     * is never tested (derived from the three distinct shuffles above) */
+    return ((j & 3) != ((i + 1) & 3));
+}
+
+float oracle_dtcheck_min_s(int64_t n, const double *x, const double *y)
+{
+    float best = INFINITY;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = i + 1; j < n; ++j) {
+            if (!oracle_dtcheck_covers(i, j))
+                continue;
+            const float sx = (float)x[i] - (float)x[j];
+            const float sy = (float)y[i] - (float)y[j];
+            const float s = sx * sx + sy * sy;
+            if (s < best)
+                best = s;
+        }
+    return best;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Runge_Kutta4::moveParticles                                              */
+/* ------------------------------------------------------------------------ */
+
+int oracle_stage_force(int64_t n, const double *xs, const double *ys,
+                       const double *vxs, const double *vys, const double *q,
+                       const double *m, const oracle_params *p, uint64_t step,
+                       int stage, double *fx, double *fy)
+{
+    /* driver.cpp:372-392 construction order = summation order */
+    if (p->forces & ORACLE_CONFINEMENT)
+        oracle_confinement_add(n, xs, ys, q, p->confine, fx, fy);
+    if (p->forces & ORACLE_DRAG)
+        oracle_drag_add(n, vxs, vys, m, p->gamma, fx, fy);
+    if (p->forces & ORACLE_SHIELDED_COULOMB) {
+        if (n & 1)
+            oracle_yukawa_gather_add(n, xs, ys, q, p->shielding, fx, fy);
+        else if (oracle_yukawa_add(n, xs, ys, q, p->shielding, fx, fy))
+            return -1;
+    }
+    if (p->forces & ORACLE_THERMAL)
+        oracle_thermal_add(n, p->heat_val, p->seed, step, stage, fx, fy);
+    return 0;
+}
+
+typedef struct rk_work {
+    double *k[4], *l[4], *mm[4], *nn[4];
+    double *fx, *fy, *xc, *yc, *vxc, *vyc;
+    double *block;
+} rk_work;
+
+static int rk_alloc(rk_work *w, int64_t n)
+{
+    w->block = (double *)calloc((size_t)(n > 0 ? n : 1) * 22, sizeof(double));
+    if (!w->block)
+        return -1;
+    double *b = w->block;
+    for (int s = 0; s < 4; ++s) {
+        w->k[s] = b; b += n;
+        w->l[s] = b; b += n;
+        w->mm[s] = b; b += n;
+        w->nn[s] = b; b += n;
+    }
+    w->fx = b; b += n;
+    w->fy = b; b += n;
+    w->xc = b; b += n;
+    w->yc = b; b += n;
+    w->vxc = b; b += n;
+    w->vyc = b;
+    return 0;
+}
+
+/* (a1 + ((a2 + a3)*2 + a4)) / 6   Runge_Kutta4.cpp:147-151 */
+static inline double rk_da(double a1, double a2, double a3, double a4)
+{
+    return (a1 + ((a2 + a3) * 2.0 + a4)) / 6.0;
+}
+
+static int rk4_one_step(int64_t n, double *x, double *y, double *vx, double *vy,
+                        const double *q, const double *m, const oracle_params *p,
+                        double init_dt, double *t, uint64_t *step_counter, rk_work *w)
+{
+    /* Runge_Kutta4.cpp:25 */
+    const double dt = p->dt_check ? oracle_modify_time_step(n, x, y, 1.0e-4f, init_dt) : init_dt;
+    const uint64_t step = *step_counter;
+    for (int s = 0; s < 4; ++s) {
+        const double *xs, *ys, *vxs, *vys;
+        if (s == 0) {
+            /* operation1 is a no-op; stage-1 getters read x, Vx (Cloud.cpp:277,415) */
+            xs = x; ys = y; vxs = vx; vys = vy;
+        } else {
+            /* CacheOperator::operation2..4 (CacheOperator.cpp:28-73) */
+            const double *pl = w->l[s - 1], *pn = w->nn[s - 1];
+            const double *pk = w->k[s - 1], *pm = w->mm[s - 1];
+            for (int64_t i = 0; i < n; ++i) {
+                if (s < 3) {
+                    w->xc[i] = 0.5 * pl[i] + x[i];
+                    w->yc[i] = 0.5 * pn[i] + y[i];
+                    w->vxc[i] = 0.5 * pk[i] + vx[i];
+                    w->vyc[i] = 0.5 * pm[i] + vy[i];
+                } else {
+                    w->xc[i] = x[i] + pl[i];
+                    w->yc[i] = y[i] + pn[i];
+                    w->vxc[i] = vx[i] + pk[i];
+                    w->vyc[i] = vy[i] + pm[i];
+                }
+            }
+            xs = w->xc; ys = w->yc; vxs = w->vxc; vys = w->vyc;
+        }
+        if (oracle_stage_force(n, xs, ys, vxs, vys, q, m, p, step, s + 1, w->fx, w->fy))
+            return -1;
+        /* stage loop, Runge_Kutta4.cpp:32-47 */
+        for (int64_t i = 0; i < n; ++i) {
+            w->k[s][i] = (dt * w->fx[i]) / m[i];
+            w->l[s][i] = dt * vxs[i];
+            w->mm[s][i] = (dt * w->fy[i]) / m[i];
+            w->nn[s][i] = dt * vys[i];
+            w->fx[i] = 0.0;
+            w->fy[i] = 0.0;
+        }
+    }
+    /* combine, Runge_Kutta4.cpp:107-133 */
+    for (int64_t i = 0; i < n; ++i) {
+        vx[i] += rk_da(w->k[0][i], w->k[1][i], w->k[2][i], w->k[3][i]);
+        x[i] += rk_da(w->l[0][i], w->l[1][i], w->l[2][i], w->l[3][i]);
+        vy[i] += rk_da(w->mm[0][i], w->mm[1][i], w->mm[2][i], w->mm[3][i]);
+        y[i] += rk_da(w->nn[0][i], w->nn[1][i], w->nn[2][i], w->nn[3][i]);
+    }
+    *t += dt;                                   /* :135 */
+    *step_counter = step + 1;
+    return 0;
+}
+
+int oracle_rk4_steps(int64_t n, double *x, double *y, double *vx, double *vy,
+                     const double *q, const double *m, const oracle_params *p,
+                     double init_dt, int64_t steps, double *t, uint64_t *step_counter)
+{
+    if (n < 1)
+        return -1;
+    if (p->dt_check && (n & 3))
+        return -1;
+    rk_work w;
+    if (rk_alloc(&w, n))
+        return -1;
+    int rc = 0;
+    for (int64_t s = 0; s < steps && !rc; ++s)
+        rc = rk4_one_step(n, x, y, vx, vy, q, m, p, init_dt, t, step_counter, &w);
+    free(w.block);
+    return rc;
+}
+
+int oracle_rk4_move(int64_t n, double *x, double *y, double *vx, double *vy,
+                    const double *q, const double *m, const oracle_params *p,
+                    double init_dt, double end_time, double *t,
+                    uint64_t *step_counter, int64_t *steps_done)
+{
+    if (n < 1)
+        return -1;
+    if (p->dt_check && (n & 3))
+        return -1;
+    rk_work w;
+    if (rk_alloc(&w, n))
+        return -1;
+    int rc = 0;
+    int64_t c = 0;
+    while (*t < end_time && !rc) {               /* Runge_Kutta4.cpp:24 */
+        rc = rk4_one_step(n, x, y, vx, vy, q, m, p, init_dt, t, step_counter, &w);
+        ++c;
+    }
+    free(w.block);
+    if (steps_done)
+        *steps_done = c;
+    return rc;
+}

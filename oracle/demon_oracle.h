@@ -1,0 +1,142 @@
+/*
+ * demon_oracle.h -- CPU oracle for the DEMON Yukawa + RK4 hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP product is
+ * compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product (libdemon_gpu.so and the
+ * demon_simulation_code_amd package) never links, imports or calls it.
+ *
+ * Each function restates one piece of the reference algorithm
+ * (AU-PSL/demon_simulation_code) in plain C, citing the file:line it follows.
+ * Arithmetic is kept unfused (the build uses -ffp-contract=off) and in the
+ * reference's evaluation order, so that a serial build of the reference and
+ * this oracle agree bit for bit where libm agrees.
+ *
+ * How each function is pinned (beyond "retyped formula"):
+ *   oracle_yukawa_add        bitwise vs the compiled reference (serial build),
+ *                            two-body closed form, Newton-3, cutoff at u=10,
+ *                            long-double brute force (tests/test_oracle.py)
+ *   oracle_yukawa_gather_add long-double brute force, equality with
+ *                            oracle_yukawa_add to rounding
+ *   oracle_confinement_add,
+ *   oracle_drag_add          linear closed forms; bitwise vs reference
+ *   oracle_rk4_steps         bitwise vs compiled reference trajectories;
+ *                            RK4 amplification polynomial on the linear
+ *                            (confinement + drag) system
+ *   oracle_modify_time_step  constructed cases incl. the coverage hole;
+ *                            bitwise vs compiled reference
+ *   oracle_dtcheck_min_s     brute force over the coverage predicate
+ *   oracle_philox4x32        published Random123 known-answer vectors
+ *   oracle_thermal_add       moments of the published uniform/angle model
+ */
+#ifndef DEMON_ORACLE_H
+#define DEMON_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 1/(4*pi*eps0) with eps0 = 8.8541878e-12 (Cloud.cpp:16, ShieldedCoulombForce.cpp:21). */
+double oracle_coulomb_const(void);
+
+/* Force flags, same bit values as the reference's ForceFlag (Force.h:75-91). */
+enum {
+    ORACLE_CONFINEMENT = 1,
+    ORACLE_DRAG = 2,
+    ORACLE_SHIELDED_COULOMB = 4,
+    ORACLE_THERMAL = 16
+};
+
+typedef struct oracle_params {
+    double shielding;   /* kappa = 1/lambda [1/m]            (ShieldedCoulombForce.h:29) */
+    double confine;     /* confinement constant C [V/m^2]    (ConfinementForce.h:20) */
+    double gamma;       /* drag frequency; dragConst = -gamma (DragForce.h:17) */
+    double heat_val;    /* thermal force scale [N]           (ThermalForce.h:25) */
+    uint64_t seed;      /* key of the counter-based thermal RNG */
+    int32_t forces;     /* bitmask of ORACLE_* flags */
+    int32_t dt_check;   /* 1: run Integrator::modifyTimeStep each step */
+} oracle_params;
+
+/* ShieldedCoulombForce::forceS (ShieldedCoulombForce.cpp:37-237), serial
+ * evaluation order.  Adds into fx/fy.  n must be even (the reference loads
+ * particles in pairs).  Returns 0, or -1 if n is odd. */
+int oracle_yukawa_add(int64_t n, const double *xs, const double *ys,
+                      const double *q, double kappa, double *fx, double *fy);
+
+/* Same physics for any n as a per-particle gather, j ascending, with the
+ * cross-pair formula (ShieldedCoulombForce.cpp:183-184) for every pair. */
+void oracle_yukawa_gather_add(int64_t n, const double *xs, const double *ys,
+                              const double *q, double kappa, double *fx, double *fy);
+
+/* Gather form restricted to the particles listed in rows[0..nrows): writes
+ * (not adds) their Yukawa force into fx[k], fy[k]; cross-pair formula,
+ * j ascending.  For sampled checks at sizes where the full loop is slow. */
+void oracle_yukawa_rows(int64_t n, const double *xs, const double *ys, const double *q,
+                        double kappa, const int64_t *rows, int64_t nrows, double *fx, double *fy);
+
+/* Same, evaluated in long double (reference-free brute force for pins). */
+void oracle_yukawa_longdouble(int64_t n, const double *xs, const double *ys,
+                              const double *q, double kappa, double *fx, double *fy);
+
+/* Number of ordered pairs (i != j) with sqrt(dx^2+dy^2)*kappa < 10 under the
+ * reference's rounding (ShieldedCoulombForce.cpp:175-178). */
+int64_t oracle_near_pairs(int64_t n, const double *xs, const double *ys, double kappa);
+
+/* ConfinementForce::force (ConfinementForce.cpp:48-54): F += (q*C)*r_s. */
+void oracle_confinement_add(int64_t n, const double *xs, const double *ys,
+                            const double *q, double confine, double *fx, double *fy);
+
+/* DragForce::force (DragForce.cpp:143-149): F += (m*(-gamma))*v_s. */
+void oracle_drag_add(int64_t n, const double *vxs, const double *vys,
+                     const double *m, double gamma, double *fx, double *fy);
+
+/* Philox4x32-10 (Salmon et al., SC'11).  ctr[4], key[2] -> out[4]. */
+void oracle_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ThermalForce::force (ThermalForce.cpp:118-122) with the random draws of
+ * RandCache (RandomNumbers.h:34-61) replaced by Philox: for particle i at
+ * (step, stage), U in [0,1) and theta in [0,2pi); F += heat*U*(cos, sin). */
+void oracle_thermal_add(int64_t n, double heat, uint64_t seed, uint64_t step,
+                        int stage, double *fx, double *fy);
+
+/* Integrator::modifyTimeStep (Integrator.cpp:68-134, 144-182), serial, with
+ * the reference's exact pair coverage.  n must be a multiple of 4. */
+double oracle_modify_time_step(int64_t n, const double *x, const double *y,
+                               float dist, double dt);
+
+/* Minimum over the pairs the reference tests of the FP32 squared distance
+ * RN(RN(dx*dx)+RN(dy*dy)), +inf if none.  Any n (groups of 4, partial last
+ * group).  Used to check the GPU's reduction. */
+float oracle_dtcheck_min_s(int64_t n, const double *x, const double *y);
+
+/* 1 if the reference's modifyTimeStep tests the pair (i, j), else 0. */
+int oracle_dtcheck_covers(int64_t i, int64_t j);
+
+/* Runge_Kutta4::moveParticles (Runge_Kutta4.cpp:23-137) driven for exactly
+ * `steps` steps (the while-loop body, with the dt check when enabled).
+ * State arrays are updated in place; *t is advanced by each step's dt.
+ * Forces are summed in the driver's order: Confinement, Drag,
+ * ShieldedCoulomb, Thermal (driver.cpp:372-392).  Returns 0 or -1 (bad n). */
+int oracle_rk4_steps(int64_t n, double *x, double *y, double *vx, double *vy,
+                     const double *q, const double *m, const oracle_params *p,
+                     double init_dt, int64_t steps, double *t, uint64_t *step_counter);
+
+/* Runge_Kutta4::moveParticles(endTime) with its while (t < end) loop. */
+int oracle_rk4_move(int64_t n, double *x, double *y, double *vx, double *vy,
+                    const double *q, const double *m, const oracle_params *p,
+                    double init_dt, double end_time, double *t,
+                    uint64_t *step_counter, int64_t *steps_done);
+
+/* The stage force of the RK4 loop on a given stage state (all enabled forces). */
+int oracle_stage_force(int64_t n, const double *xs, const double *ys,
+                       const double *vxs, const double *vys, const double *q,
+                       const double *m, const oracle_params *p, uint64_t step,
+                       int stage, double *fx, double *fy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DEMON_ORACLE_H */

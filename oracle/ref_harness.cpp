@@ -1,0 +1,161 @@
+// ref_harness.cpp -- extern "C" entry points around the reference's own hot-path
+// classes, compiled together with the reference sources where they lie
+// (AU-PSL/demon_simulation_code) into oracle/_ref/.  TEST INFRASTRUCTURE ONLY:
+// used to pin the oracle and as the CPU baseline; never linked by the product.
+//
+// The reference sources are not copied: oracle/Makefile compiles them in place.
+#include "Cloud.h"
+#include "ConfinementForce.h"
+#include "DragForce.h"
+#include "Runge_Kutta4.h"
+#include "ShieldedCoulombForce.h"
+#include "ThermalForce.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+// Static Cloud members are defined by the reference's driver (driver.cpp:122-127),
+// which is not part of this harness; give them the driver's defaults.
+double Cloud::interParticleSpacing = 0.003;
+double Cloud::dustParticleMassDensity = 2200;
+double Cloud::justX = 0.0;
+double Cloud::justY = 0.0;
+double Cloud::velX = 0.0;
+double Cloud::velY = 0.0;
+
+namespace {
+
+void set_threads(int nthreads) {
+#ifdef _OPENMP
+    // Cloud::Cloud forces omp_get_num_procs() threads (Cloud.cpp:34-36); the
+    // harness re-applies the caller's count afterwards.
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+
+Cloud *make_cloud(int64_t n, const double *x, const double *y, const double *vx,
+                  const double *vy, const double *q, const double *m) {
+    Cloud *c = new Cloud((cloud_index)n);
+    for (int64_t i = 0; i < n; ++i) {
+        c->x[i] = x ? x[i] : 0.0;
+        c->y[i] = y ? y[i] : 0.0;
+        c->Vx[i] = vx ? vx[i] : 0.0;
+        c->Vy[i] = vy ? vy[i] : 0.0;
+        c->charge[i] = q ? q[i] : 0.0;
+        c->mass[i] = m ? m[i] : 1.0;
+        // Cloud never zeroes the force arrays (Cloud.cpp:31); the harness does.
+        c->forceX[i] = 0.0;
+        c->forceY[i] = 0.0;
+    }
+    return c;
+}
+
+// Exposes the protected Integrator::modifyTimeStep.
+struct RK4Probe : public Runge_Kutta4 {
+    RK4Probe(Cloud *C, const ForceArray &FA, double dt, double t0) : Runge_Kutta4(C, FA, dt, t0) {}
+    double check(float dist, double dt) const { return modifyTimeStep(dist, dt); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ref_is_openmp(void) {
+#ifdef _OPENMP
+    return 1;
+#else
+    return 0;
+#endif
+}
+
+// ShieldedCoulombForce::force<stage>(t) on the given stage positions; adds the
+// result into fx/fy.  Stage 1 reads x/y, stages 2-4 read the stage caches.
+int ref_yukawa_stage(int stage, int64_t n, const double *xs, const double *ys, const double *q,
+                     double kappa, double *fx, double *fy, int nthreads) {
+    if (n < 2 || (n & 1) || stage < 1 || stage > 4) return -1;
+    Cloud *c = make_cloud(n, xs, ys, nullptr, nullptr, q, nullptr);
+    set_threads(nthreads);
+    if (stage > 1) {
+        for (int64_t i = 0; i < n; i += 2) {
+            c->xCache[i / 2] = _mm_set_pd(xs[i + 1], xs[i]);
+            c->yCache[i / 2] = _mm_set_pd(ys[i + 1], ys[i]);
+        }
+    }
+    {
+        ShieldedCoulombForce f(c, kappa);
+        switch (stage) {
+            case 1: f.force1(0.0); break;
+            case 2: f.force2(0.0); break;
+            case 3: f.force3(0.0); break;
+            default: f.force4(0.0); break;
+        }
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        fx[i] += c->forceX[i];
+        fy[i] += c->forceY[i];
+    }
+    delete c;
+    return 0;
+}
+
+// Integrator::modifyTimeStep(dist, dt) on positions x/y.
+double ref_modify_time_step(int64_t n, const double *x, const double *y, float dist, double dt) {
+    if (n < 4 || (n & 3)) return -1.0;
+    Cloud *c = make_cloud(n, x, y, nullptr, nullptr, nullptr, nullptr);
+    ForceArray none;
+    double r;
+    {
+        RK4Probe probe(c, none, dt, 0.0);
+        r = probe.check(dist, dt);
+    }
+    delete c;
+    return r;
+}
+
+// Runge_Kutta4 with the driver's force set and order (driver.cpp:322-326,372-392):
+// Confinement, Drag, ShieldedCoulomb, Thermal (flags as in Force.h:75-91).
+// mode 0: run exactly `steps` steps (each moveParticles call is given an end
+//         time just above the current time, so the while loop runs once);
+// mode 1: a single moveParticles(end_time) call.
+// State is updated in place; *t receives the integrator's currentTime.
+int ref_rk4(int64_t n, double *x, double *y, double *vx, double *vy, const double *q,
+            const double *m, double kappa, double confine, double gamma, double heat,
+            int forces, double dt, int mode, int64_t steps, double end_time, double *t,
+            int nthreads) {
+    if (n < 4 || (n & 3)) return -1;
+    Cloud *c = make_cloud(n, x, y, vx, vy, q, m);
+    set_threads(nthreads);
+    ForceArray fa;
+    if (forces & ConfinementForceFlag) fa.push_back(new ConfinementForce(c, confine));
+    if (forces & DragForceFlag) fa.push_back(new DragForce(c, gamma));
+    if (forces & ShieldedCoulombForceFlag) fa.push_back(new ShieldedCoulombForce(c, kappa));
+    if (forces & ThermalForceFlag) fa.push_back(new ThermalForce(c, heat));
+    {
+        Runge_Kutta4 I(c, fa, dt, *t);
+        if (mode == 0) {
+            for (int64_t s = 0; s < steps; ++s)
+                I.moveParticles(std::nextafter(I.currentTime, INFINITY));
+        } else {
+            I.moveParticles(end_time);
+        }
+        *t = I.currentTime;
+    }
+    for (Force *f : fa) delete f;
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = c->x[i];
+        y[i] = c->y[i];
+        vx[i] = c->Vx[i];
+        vy[i] = c->Vy[i];
+    }
+    delete c;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+"""GPU parity of the per-stage forces (Force::force1..4) against the oracle.
+
+Bar (north star): FP64 forces within 1e-12 relative per evaluation, measured
+condition-aware: |dF_i| / (sum_j |F_ij| + |F_conf,i| + |F_drag,i|).
+FP32-pair variant: within 1e-5 relative on the same scale.
+"""
+import numpy as np
+import pytest
+
+import demon_simulation_code_amd as dm
+from oracle.bind import OracleParams
+from tests.helpers import CONFINE, GAMMA, KAPPA, M0, Q0, charges, force_rel_err, lattice, pair_abs_sum
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def _ctx(n, q, forces=dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB, precision=dm.FP64,
+         heat=0.0, x=None, y=None, shape=None):
+    c = dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, heat=heat, seed=11,
+                   forces=forces, precision=precision)
+    if x is None:
+        x, y = lattice(n)
+    c.upload(x, y, None, None, q, np.full(n, M0))
+    if shape:
+        c.set_pair_shape(*shape)
+    return c
+
+
+@pytest.mark.parametrize("n", [2, 3, 64, 255, 256, 1000, 1024, 4096])
+@pytest.mark.parametrize("stage", [1, 4])
+def test_yukawa_stage_fp64(oracle, n, stage):
+    x, y = lattice(n, seed=n)
+    q = charges(n)
+    with _ctx(n, q, x=x, y=y) as c:
+        gx, gy = c.force_stage(dm.SHIELDED_COULOMB, stage, 0, x, y)
+    fx, fy = oracle.yukawa(x, y, q, KAPPA)
+    scale = pair_abs_sum(x, y, q, KAPPA) + 1e-300
+    assert force_rel_err(fx, fy, gx, gy, scale) <= FP64_TOL
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 7), (2, 3), (4, 1), (4, 64)])
+def test_yukawa_pair_shapes(oracle, shape):
+    n = 3000
+    x, y = lattice(n, seed=5)
+    q = charges(n)
+    with _ctx(n, q, x=x, y=y, shape=shape) as c:
+        gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 2, 0, x, y)
+    fx, fy = oracle.yukawa(x, y, q, KAPPA)
+    assert force_rel_err(fx, fy, gx, gy, pair_abs_sum(x, y, q, KAPPA)) <= FP64_TOL
+
+
+def test_all_stage_forces_fp64(oracle):
+    n = 1024
+    x, y = lattice(n, seed=3)
+    rng = np.random.default_rng(4)
+    vx, vy = rng.normal(0, 1e-2, n), rng.normal(0, 1e-2, n)
+    q = charges(n)
+    m = np.full(n, M0) * (1 + 0.05 * rng.uniform(-1, 1, n))
+    forces = dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB | dm.THERMAL
+    heat = 3.0e-15
+    c = dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, heat=heat, seed=99, forces=forces)
+    with c:
+        c.upload(x, y, vx, vy, q, m)
+        for stage in (1, 2, 3, 4):
+            gx, gy = c.force_stage(forces, stage, 17, x, y, vx, vy)
+            op = OracleParams(KAPPA, CONFINE, GAMMA, heat, 99, forces, 1)
+            fx, fy = oracle.stage_force(x, y, vx, vy, q, m, op, step=17, stage=stage)
+            scale = (pair_abs_sum(x, y, q, KAPPA) + np.abs(q * CONFINE) * np.hypot(x, y)
+                     + m * GAMMA * np.hypot(vx, vy) + heat)
+            assert force_rel_err(fx, fy, gx, gy, scale) <= FP64_TOL, stage
+
+
+def test_force_stage_adds_into_arrays(oracle):
+    """Force::forceS adds into Cloud::forceX/Y; it never overwrites."""
+    n = 256
+    x, y = lattice(n)
+    q = charges(n)
+    base_x = np.linspace(-1e-12, 1e-12, n)
+    base_y = np.linspace(1e-12, -1e-12, n)
+    with _ctx(n, q, x=x, y=y) as c:
+        gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y, fx=base_x.copy(), fy=base_y.copy())
+    fx, fy = oracle.yukawa(x, y, q, KAPPA, fx=base_x.copy(), fy=base_y.copy())
+    scale = pair_abs_sum(x, y, q, KAPPA) + np.abs(base_x) + np.abs(base_y)
+    assert force_rel_err(fx, fy, gx, gy, scale) <= FP64_TOL
+
+
+def test_cutoff_boundary_exact(oracle):
+    """Pairs placed 1 ulp either side of u = 10 are classified as the reference does."""
+    lam = 1.0 / KAPPA
+    r_cut = 10.0 * lam
+    xs, ys = [], []
+    for k in range(16):
+        base = 0.05 * k
+        r = r_cut
+        for _ in range(k % 4):
+            r = np.nextafter(r, np.inf if k % 2 else -np.inf)
+        xs += [base, base + r]
+        ys += [0.0, 0.0]
+    x, y = np.array(xs), np.array(ys)
+    n = len(x)
+    q = np.full(n, Q0)
+    with _ctx(n, q, x=x, y=y) as c:
+        gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y)
+    fx, fy = oracle.yukawa(x, y, q, KAPPA)
+    np.testing.assert_array_equal(fx == 0.0, gx == 0.0)
+    scale = pair_abs_sum(x, y, q, KAPPA) + 1e-300
+    assert force_rel_err(fx, fy, gx, gy, scale) <= FP64_TOL
+
+
+def test_newton3_full_size():
+    """N = 131072: sum of Yukawa forces vanishes (Newton's third law) to rounding."""
+    n = 131072
+    x, y = lattice(n)
+    q = charges(n)
+    with _ctx(n, q, x=x, y=y) as c:
+        gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y)
+    rows = np.random.default_rng(0).choice(n, 512, replace=False)
+    scale = pair_abs_sum(x, y, q, KAPPA, rows).mean() * n
+    assert abs(gx.sum()) / scale < 1e-12 and abs(gy.sum()) / scale < 1e-12
+
+
+def test_sampled_rows_full_size(oracle):
+    """N = 131072: 512 sampled particles against the oracle's gather."""
+    n = 131072
+    x, y = lattice(n)
+    q = charges(n)
+    rows = np.random.default_rng(1).choice(n, 512, replace=False)
+    with _ctx(n, q, x=x, y=y) as c:
+        gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y)
+    fx, fy = oracle.yukawa_rows(x, y, q, KAPPA, rows)
+    scale = pair_abs_sum(x, y, q, KAPPA, rows)
+    assert force_rel_err(fx, fy, gx[rows], gy[rows], scale) <= FP64_TOL
+
+
+@pytest.mark.parametrize("n", [1024, 4096])
+def test_yukawa_fp32_pair(oracle, n):
+    x, y = lattice(n, seed=2)
+    q = charges(n)
+    with _ctx(n, q, x=x, y=y, precision=dm.FP32_PAIR) as c:
+        gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y)
+    fx, fy = oracle.yukawa(x, y, q, KAPPA)
+    err = force_rel_err(fx, fy, gx, gy, pair_abs_sum(x, y, q, KAPPA))
+    assert err <= FP32_TOL
+    assert err > 0.0  # it really is the FP32 path
