@@ -49,8 +49,49 @@ def sources() -> list[Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
+HOST = PKG / "host"
+HOST_LIB = "libdemon_host.so"
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra"]
+
+
+def cxx() -> str:
+    return os.environ.get("CXX") or shutil.which("g++") or "g++"
+
+
+def build_host(verbose: bool = False) -> list[Path]:
+    """The host C++ mirror (libdemon_host.so) and the `demon` driver, linked
+    against libdemon_gpu.so with an $ORIGIN rpath."""
+    inc = [f"-I{INCLUDE}", f"-I{HOST}"]
+    rpath = "-Wl,-rpath,$ORIGIN"
+    lib = PKG / HOST_LIB
+    steps = [
+        [cxx(), *CXX_FLAGS, *inc, "-shared", "-o", str(lib), str(HOST / "demon_host.cpp"),
+         f"-L{PKG}", "-ldemon_gpu", rpath],
+        [cxx(), *CXX_FLAGS, *inc, "-o", str(PKG / "demon"), str(HOST / "driver.cpp"),
+         f"-L{PKG}", "-ldemon_host", "-ldemon_gpu", rpath],
+    ]
+    test_src = PKG.parent / "tests" / "host" / "test_host.cpp"
+    if test_src.exists():  # the host mirror's test driver (tests/test_host_mirror.py)
+        steps.append([cxx(), *CXX_FLAGS, *inc, "-o", str(test_src.with_suffix("")), str(test_src),
+                      f"-L{PKG}", "-ldemon_host", "-ldemon_gpu", f"-Wl,-rpath,{PKG}"])
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed:\n{r.stdout}")
+    return [lib, PKG / "demon"]
+
+
 def build(verbose: bool = False, jobs: int = 4) -> Path:
-    """Compiles every csrc/*.hip and links PKG/libdemon_gpu.so.  Always rebuilds."""
+    """Compiles every csrc/*.hip, links PKG/libdemon_gpu.so, then the host
+    mirror and driver.  Always rebuilds."""
+    out = build_hip(verbose, jobs)
+    build_host(verbose)
+    return out
+
+
+def build_hip(verbose: bool = False, jobs: int = 4) -> Path:
     out = PKG / LIB_NAME
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)

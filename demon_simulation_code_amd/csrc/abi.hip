@@ -52,6 +52,7 @@ struct demon_ctx {
     hipEvent_t ev_call[2]{};
     double last_pair_ms = 0.0, last_total_ms = 0.0;
     int64_t force_evals = 0;
+    uint64_t rng_step = 0;       // thermal RNG step counter; survives re-uploads
 };
 
 namespace {
@@ -275,6 +276,7 @@ int run_steps(demon_ctx *c, int64_t steps, Scalars *out) {
         c->last_pair_ms += pm;
     }
     c->force_evals += 4 * steps;
+    c->rng_step = out->step;
     if (out->err == 2)
         return fail(c, DEMON_ERR_STATE,
                     "dt check: two tested particles coincide (distance 0); the reference's "
@@ -406,6 +408,7 @@ int demon_upload(demon_ctx *c, const double *x, const double *y, const double *v
     HIP_TRY(c, launch_prepare_stage(c->sx, 0, c->npad, c->stream));
     Scalars s{};
     s.dtmin_bits = 0x7f800000u;
+    s.step = c->rng_step;  // a re-upload never replays earlier thermal draws
     HIP_TRY(c, hipMemcpyAsync(c->d_sc, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, launch_rmax(c->sb[0], c->n, c->d_sc, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));

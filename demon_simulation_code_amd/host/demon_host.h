@@ -1,0 +1,248 @@
+// demon_host.h -- host C++ mirror of the reference's plugin surface for the
+// hot path, backed by the C ABI of libdemon_gpu.so.
+//
+// Same class names, constructor arguments and call contracts as the
+// reference (AU-PSL/demon_simulation_code), in namespace demon_amd:
+//   Cloud                      Cloud.h:16-87 (public SoA arrays, stage caches)
+//   Force / ForceArray         Force.h:15-91 (force1..force4 add into
+//                              Cloud::forceX/forceY; ForceFlag bit values)
+//   ShieldedCoulombForce       ShieldedCoulombForce.h:15-39
+//   ConfinementForce           ConfinementForce.h:13-30
+//   DragForce                  DragForce.h:13-30
+//   ThermalForce               ThermalForce.h:13-30
+//   Integrator, Runge_Kutta4   Integrator.h:17-38, Runge_Kutta4.h:14-31
+//   checkParams                driver.cpp:626-824 (Params.cfg)
+// The forces evaluate on the device; Runge_Kutta4::moveParticles runs the
+// whole while (t < end) loop on the device and copies the state back.
+// There is no CPU code path: a Force that is not one of the device forces
+// makes Runge_Kutta4 throw.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/demon_gpu.h"
+
+namespace demon_amd {
+
+using cloud_index = int64_t;
+constexpr double kPi = 3.14159265358979323846;
+
+// ForceFlag values of the reference (Force.h:75-91).
+enum ForceFlag : long {
+    ConfinementForceFlag = 1,
+    DragForceFlag = 2,
+    ShieldedCoulombForceFlag = 4,
+    GravitationalForceFlag = 8,
+    ThermalForceFlag = 16,
+    ThermalForceLocalizedFlag = 32,
+    DrivingForceFlag = 64,
+    RotationalForceFlag = 128,
+    TimeVaryingDragForceFlag = 256,
+    TimeVaryingThermalForceFlag = 512,
+    MagneticForceFlag = 1024,
+    ConfinementForceVoidFlag = 2048,
+    ElectricForceFlag = 4096,
+    RectConfinementForceFlag = 8192,
+    VertElectricForceFlag = 16384
+};
+using force_flags = long;
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string &msg) : std::runtime_error(msg), code(code) {}
+    int code;
+};
+
+// Throws Error when a C-ABI call fails.
+void check(int rc, const demon_ctx *ctx);
+
+class Cloud {
+public:
+    explicit Cloud(cloud_index numPar, int device = 0);
+    ~Cloud();
+    Cloud(const Cloud &) = delete;
+    Cloud &operator=(const Cloud &) = delete;
+
+    const cloud_index n;
+    // current positions, velocities, charges and masses (Cloud.h:21-22)
+    std::vector<double> x, y, Vx, Vy, charge, mass;
+    // force accumulators used by force1..force4 (Cloud.h:27); zeroed here,
+    // unlike the reference (Cloud.cpp:31 leaves them uninitialised)
+    std::vector<double> forceX, forceY;
+    // stage states read by force2..force4 (Cloud.h:28, CacheOperator.cpp)
+    std::vector<double> xCache, yCache, VxCache, VyCache;
+
+    static double interParticleSpacing;   // Cloud.h:31
+    static const double electronCharge;   // -1.602e-19 C (Cloud.cpp:15)
+    static const double epsilon0;         // 8.8541878e-12 F/m (Cloud.cpp:16)
+    static double dustParticleMassDensity;
+    static double justX, justY, velX, velY;
+
+    // Cloud::initializeGrid (Cloud.cpp:95-135): square grid, Gaussian charge
+    // (qMean, qSigma in units of e) and radius (rMean, rSigma).  The draws use
+    // std::mt19937_64 with an explicit seed instead of the wall clock.
+    static std::unique_ptr<Cloud> initializeGrid(cloud_index numParticles, cloud_index rowX,
+                                                 cloud_index rowY, double rMean, double rSigma,
+                                                 double qMean, double qSigma, uint64_t seed = 0,
+                                                 int device = 0);
+    // Hexagonal-lattice patch (demon_hex_lattice), uniform charge and mass.
+    static std::unique_ptr<Cloud> initializeHexLattice(cloud_index numParticles, double spacing,
+                                                       double jitter, uint64_t seed, double charge,
+                                                       double mass, int device = 0);
+
+    // Stage getters of the reference (Cloud.cpp:277-475), element-wise:
+    // stage 1 reads x/Vx, stages 2-4 the caches.
+    const double *stageX(int stage) const { return stage == 1 ? x.data() : xCache.data(); }
+    const double *stageY(int stage) const { return stage == 1 ? y.data() : yCache.data(); }
+    const double *stageVx(int stage) const { return stage == 1 ? Vx.data() : VxCache.data(); }
+    const double *stageVy(int stage) const { return stage == 1 ? Vy.data() : VyCache.data(); }
+
+    // Device mirror.  The context is created on first use with the forces'
+    // parameters; syncToDevice uploads x, y, Vx, Vy, charge, mass.
+    demon_ctx *context(const demon_params &p);
+    void syncToDevice();
+    void syncFromDevice();
+    int device() const { return device_; }
+
+private:
+    demon_ctx *ctx_ = nullptr;
+    demon_params ctxParams_{};
+    int device_;
+};
+
+class Force {
+public:
+    Cloud *const cloud;
+    explicit Force(Cloud *C) : cloud(C) {}
+    virtual ~Force() {}
+    virtual void force1(double currentTime) = 0;
+    virtual void force2(double currentTime) = 0;
+    virtual void force3(double currentTime) = 0;
+    virtual void force4(double currentTime) = 0;
+    // The device parameters this force contributes and its ForceFlag bit;
+    // Runge_Kutta4 builds one parameter block from all forces.
+    virtual force_flags flag() const = 0;
+    virtual void contribute(demon_params &p) const = 0;
+
+protected:
+    // Evaluates this force on the cloud's stage-S state on the device and adds
+    // it into cloud->forceX / forceY (the contract of Force::forceS).
+    void evaluate(int stage, uint64_t step);
+};
+
+typedef std::vector<Force *> ForceArray;
+
+class ShieldedCoulombForce : public Force {
+public:
+    ShieldedCoulombForce(Cloud *C, double shieldingConstant) : Force(C), shielding(shieldingConstant) {}
+    void force1(double) override { evaluate(1, 0); }
+    void force2(double) override { evaluate(2, 0); }
+    void force3(double) override { evaluate(3, 0); }
+    void force4(double) override { evaluate(4, 0); }
+    force_flags flag() const override { return ShieldedCoulombForceFlag; }
+    void contribute(demon_params &p) const override { p.shielding = shielding; }
+    double shielding;  // inverse shielding length [1/m]
+};
+
+class ConfinementForce : public Force {
+public:
+    ConfinementForce(Cloud *C, double confineConst) : Force(C), confine(confineConst) {}
+    void force1(double) override { evaluate(1, 0); }
+    void force2(double) override { evaluate(2, 0); }
+    void force3(double) override { evaluate(3, 0); }
+    void force4(double) override { evaluate(4, 0); }
+    force_flags flag() const override { return ConfinementForceFlag; }
+    void contribute(demon_params &p) const override { p.confine = confine; }
+    double confine;  // [V/m^2]
+};
+
+class DragForce : public Force {
+public:
+    DragForce(Cloud *C, double gamma) : Force(C), dragConst(-gamma) {}
+    void force1(double) override { evaluate(1, 0); }
+    void force2(double) override { evaluate(2, 0); }
+    void force3(double) override { evaluate(3, 0); }
+    void force4(double) override { evaluate(4, 0); }
+    force_flags flag() const override { return DragForceFlag; }
+    void contribute(demon_params &p) const override { p.drag_gamma = -dragConst; }
+    double dragConst;  // -gamma [1/s] (DragForce.h:17)
+};
+
+class ThermalForce : public Force {
+public:
+    ThermalForce(Cloud *C, double redFactor, uint64_t seed = 0) : Force(C), heatVal(redFactor), seed(seed) {}
+    void force1(double) override { evaluate(1, step_++); }
+    void force2(double) override { evaluate(2, step_); }
+    void force3(double) override { evaluate(3, step_); }
+    void force4(double) override { evaluate(4, step_); }
+    force_flags flag() const override { return ThermalForceFlag; }
+    void contribute(demon_params &p) const override {
+        p.heat_val = heatVal;
+        p.seed = seed;
+    }
+    double heatVal;  // [N]
+    uint64_t seed;
+
+private:
+    uint64_t step_ = 0;
+};
+
+class Integrator {
+public:
+    Integrator(Cloud *C, const ForceArray &FA, double timeStep, double startTime)
+        : currentTime(startTime), cloud(C), forces(FA), init_dt(timeStep) {}
+    virtual ~Integrator() {}
+    double currentTime;
+    virtual void moveParticles(double endTime) = 0;
+
+protected:
+    Cloud *const cloud;
+    const ForceArray &forces;
+    const double init_dt;
+};
+
+class Runge_Kutta4 : public Integrator {
+public:
+    Runge_Kutta4(Cloud *C, const ForceArray &FA, double timeStep, double startTime)
+        : Integrator(C, FA, timeStep, startTime) {}
+    // Runge_Kutta4::moveParticles (Runge_Kutta4.cpp:23-137) with the dt check
+    // of Integrator::modifyTimeStep, entirely on the device.
+    void moveParticles(double endTime) override;
+    // Steps taken by the last moveParticles call.
+    int64_t lastSteps() const { return lastSteps_; }
+
+private:
+    int64_t lastSteps_ = 0;
+};
+
+// The driver's parameter set (driver.cpp:66-127 defaults) and Params.cfg
+// parsing (checkParams, driver.cpp:626-824).
+struct Params {
+    double voidDecay = 0.4, magneticFieldStrength = 1.0;
+    cloud_index numParticles = 4;
+    double startTime = 0.0, dataTimeStep = 0.01, simTimeStep = 0.01 / 100.0, spacing = 0.003;
+    double endTime = 5.0, confinementConst = 100, confinementConstX = 100, confinementConstY = 1000;
+    double shieldingConstant = 2E4, dragGamma = 10.0, thermRed = 1E-14, thermRed1 = 1E-14;
+    double thermScale = 1E-14, thermOffset = 0.0, heatRadius = 0.001, driveConst = 0.00001;
+    double waveAmplitude = 1E-13, waveShift = 0.007, machSpeed = 0.2, massFactor = 100;
+    double qMean = 6000.0, qSigma = 100.0, rMean = 1.45E-6, rSigma = 0.0;
+    double rmin = 0.003 * 5.0, rmax = 0.003 * 10.0, rotConst = 1E-15, dragScale = -1.0;
+    double electricFieldStrength = 0, plasmaRadius = 1, vertElectricFieldStrength = 0;
+    double verticalDecay = 1, gravitationalFieldStrength = 0, justifyX = 0, justifyY = 0;
+    double massDensity = 2200, velocityX = 0.0, velocityY = 0.0;
+    force_flags usedForces = 0;
+};
+
+// Reads whitespace-separated `key value` pairs; unknown keys are ignored and
+// a later duplicate wins.  `forceFlags` is a comma-terminated list of letters
+// (B D L R S T v V w E F G); a letter after the last comma is dropped, as in
+// the reference.  Conflicting flags (e.g. T with L) and a flag given twice
+// throw Error (the reference prints the message and exits).
+void checkParams(const std::string &path, Params &params);
+void checkParamsText(const std::string &text, Params &params);
+
+}  // namespace demon_amd

@@ -1,0 +1,196 @@
+// driver.cpp -- the `demon` command: the reference driver's main loop
+// (driver.cpp:315-475) over the device integrator.
+//
+//   demon [-P Params.cfg] [-n rows [cols]] [-t dt] [-e end] [-o dataTimeStep]
+//         [-s shielding] [-C confine] [-g gamma] [-T heat] [-q mean sigma]
+//         [-r mean sigma] [-i spacing] [-p x y] [-k vx vy] [-d density]
+//         [-O snapshots.csv] [--hex N] [--seed S] [--device D] [-h]
+//
+// Options are applied left to right, so options after -P override the file
+// (driver.cpp:834-1030).  Always-on forces: Drag, Confinement, ShieldedCoulomb
+// (driver.cpp:322-326); -T adds the thermal force.  The reference's other
+// force plugins and -I/-M/-c/-f are not part of this build and are rejected.
+// Snapshots (t, index, x, y, vx, vy) go to a CSV file every dataTimeStep.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+
+#include "demon_host.h"
+
+using namespace demon_amd;
+
+namespace {
+
+void help() {
+    std::cout << "demon: Yukawa all-pairs + RK4 on an MI355X\n"
+                 "  -P file        read Params.cfg\n"
+                 "  -n R [C]       R*C particles on a grid (rounded up to a multiple of 4)\n"
+                 "  --hex N        N particles on a jittered hexagonal lattice instead\n"
+                 "  -t dt -e end -o dataTimeStep\n"
+                 "  -s kappa -C confine -g gamma -T heat\n"
+                 "  -q qMean qSigma -r rMean rSigma -i spacing -p x y -k vx vy -d density\n"
+                 "  -O file.csv    write snapshots; --seed S; --device D; -h help\n";
+}
+
+bool is_number(const char *s) {
+    char *end = nullptr;
+    std::strtod(s, &end);
+    return end && *end == '\0' && end != s;
+}
+
+struct Cli {
+    Params P;
+    cloud_index rowX = 4, rowY = 0;
+    int64_t hex = 0;
+    uint64_t seed = 0;
+    int device = 0;
+    std::string out;
+};
+
+double take(int argc, char **argv, int &i, double fallback, const char *what) {
+    if (i + 1 < argc && is_number(argv[i + 1])) return std::atof(argv[++i]);
+    std::cout << "Warning: option incomplete. Using default " << what << " (" << fallback << ")." << std::endl;
+    return fallback;
+}
+
+// setParticleRows (driver.cpp:274-306): at least 4, a multiple of 4.
+void set_rows(Cli &c) {
+    cloud_index n = c.rowY == 0 ? c.rowX : c.rowX * c.rowY;
+    if (n < 4) {
+        c.rowX = 2;
+        c.rowY = 2;
+        n = 4;
+    }
+    while (n % 4) {
+        if (c.rowY == 0) {
+            ++c.rowX;
+            n = c.rowX;
+        } else if (c.rowY < c.rowX) {
+            ++c.rowY;
+            n = c.rowX * c.rowY;
+        } else {
+            ++c.rowX;
+            n = c.rowX * c.rowY;
+        }
+    }
+    c.P.numParticles = n;
+}
+
+Cli parse(int argc, char **argv) {
+    Cli c;
+    c.P.numParticles = 4;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        Params &P = c.P;
+        if (a == "-h") { help(); std::exit(0); }
+        else if (a == "-P") {
+            if (i + 1 >= argc) throw Error(DEMON_ERR_ARG, "-P needs a file");
+            checkParams(argv[++i], P);
+            c.rowX = P.numParticles;
+            c.rowY = 0;
+        } else if (a == "-n") {
+            c.rowX = (cloud_index)take(argc, argv, i, (double)c.rowX, "number of row x particles");
+            if (i + 1 < argc && is_number(argv[i + 1])) c.rowY = (cloud_index)std::atof(argv[++i]);
+            set_rows(c);
+        } else if (a == "--hex") c.hex = (int64_t)take(argc, argv, i, 1024, "hex particles");
+        else if (a == "--seed") c.seed = (uint64_t)take(argc, argv, i, 0, "seed");
+        else if (a == "--device") c.device = (int)take(argc, argv, i, 0, "device");
+        else if (a == "-t") P.simTimeStep = take(argc, argv, i, P.simTimeStep, "time step");
+        else if (a == "-e") P.endTime = take(argc, argv, i, P.endTime, "end time");
+        else if (a == "-o") P.dataTimeStep = take(argc, argv, i, P.dataTimeStep, "data time step");
+        else if (a == "-s") P.shieldingConstant = take(argc, argv, i, P.shieldingConstant, "shielding constant");
+        else if (a == "-C") P.confinementConst = take(argc, argv, i, P.confinementConst, "confinementConst");
+        else if (a == "-g") P.dragGamma = take(argc, argv, i, P.dragGamma, "dragGamma");
+        else if (a == "-T") {
+            if (P.usedForces & (ThermalForceFlag | ThermalForceLocalizedFlag | TimeVaryingThermalForceFlag))
+                throw Error(DEMON_ERR_ARG, "option -T already set or conflicts with -L/-v");
+            P.usedForces |= ThermalForceFlag;
+            P.thermRed = take(argc, argv, i, P.thermRed, "heat factor");
+        } else if (a == "-q") {
+            P.qMean = take(argc, argv, i, P.qMean, "mean number of charges");
+            P.qSigma = take(argc, argv, i, P.qSigma, "number of charges sigma");
+        } else if (a == "-r") {
+            P.rMean = take(argc, argv, i, P.rMean, "mean dust radius");
+            P.rSigma = take(argc, argv, i, P.rSigma, "dust radius sigma");
+        } else if (a == "-i") Cloud::interParticleSpacing = take(argc, argv, i, Cloud::interParticleSpacing, "spacing");
+        else if (a == "-p") {
+            Cloud::justX = take(argc, argv, i, Cloud::justX, "justify x");
+            Cloud::justY = take(argc, argv, i, Cloud::justY, "justify y");
+        } else if (a == "-k") {
+            Cloud::velX = take(argc, argv, i, Cloud::velX, "velocity x");
+            Cloud::velY = take(argc, argv, i, Cloud::velY, "velocity y");
+        } else if (a == "-d") Cloud::dustParticleMassDensity = take(argc, argv, i, Cloud::dustParticleMassDensity, "massDensity");
+        else if (a == "-O") {
+            if (i + 1 >= argc) throw Error(DEMON_ERR_ARG, "-O needs a file");
+            c.out = argv[++i];
+        } else {
+            throw Error(DEMON_ERR_UNSUPPORTED, "Unknown or unsupported option " + a);
+        }
+    }
+    const force_flags device_forces = ConfinementForceFlag | DragForceFlag | ShieldedCoulombForceFlag | ThermalForceFlag;
+    if (c.P.usedForces & ~device_forces)
+        throw Error(DEMON_ERR_UNSUPPORTED, "forceFlags select forces that are not part of this build "
+                                           "(only Confinement, Drag, ShieldedCoulomb and Thermal run on the device)");
+    return c;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    const auto start = std::chrono::steady_clock::now();
+    try {
+        Cli c = parse(argc, argv);
+        Params &P = c.P;
+        std::unique_ptr<Cloud> cloud;
+        if (c.hex > 0)
+            cloud = Cloud::initializeHexLattice(c.hex, Cloud::interParticleSpacing, 0.1 * Cloud::interParticleSpacing,
+                                                c.seed, P.qMean * Cloud::electronCharge,
+                                                (4.0 / 3.0) * kPi * Cloud::dustParticleMassDensity * P.rMean * P.rMean * P.rMean,
+                                                c.device);
+        else
+            cloud = Cloud::initializeGrid(P.numParticles, c.rowX, c.rowY, P.rMean, P.rSigma, P.qMean,
+                                          P.qSigma, c.seed, c.device);
+        // driver.cpp:372-392: construction order is summation order
+        ForceArray forces;
+        ConfinementForce conf(cloud.get(), P.confinementConst);
+        DragForce drag(cloud.get(), P.dragGamma);
+        ShieldedCoulombForce yuk(cloud.get(), P.shieldingConstant);
+        ThermalForce therm(cloud.get(), P.thermRed, c.seed);
+        forces.push_back(&conf);
+        forces.push_back(&drag);
+        forces.push_back(&yuk);
+        if (P.usedForces & ThermalForceFlag) forces.push_back(&therm);
+
+        FILE *out = c.out.empty() ? nullptr : std::fopen(c.out.c_str(), "w");
+        if (!c.out.empty() && !out) throw Error(DEMON_ERR_ARG, "cannot open " + c.out);
+        auto snapshot = [&](double t) {
+            if (!out) return;
+            for (cloud_index i = 0; i < cloud->n; ++i)
+                std::fprintf(out, "%.17g,%lld,%.17g,%.17g,%.17g,%.17g\n", t, (long long)i, cloud->x[i],
+                             cloud->y[i], cloud->Vx[i], cloud->Vy[i]);
+            std::fflush(out);
+        };
+        snapshot(P.startTime);
+        Runge_Kutta4 I(cloud.get(), forces, P.simTimeStep, P.startTime);
+        double startTime = P.startTime;
+        int64_t steps = 0;
+        while (startTime < P.endTime) {  // driver.cpp:443-450
+            std::cout << "\rCurrent Time: " << I.currentTime << "s (" << I.currentTime / P.endTime * 100.0
+                      << "% Complete)" << std::flush;
+            I.moveParticles(startTime += P.dataTimeStep);
+            steps += I.lastSteps();
+            snapshot(I.currentTime);
+        }
+        if (out) std::fclose(out);
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        std::cout << "\nN = " << cloud->n << ", " << steps << " RK4 steps, t = " << I.currentTime
+                  << " s, elapsed " << secs << " s" << std::endl;
+    } catch (const Error &e) {
+        std::cerr << "Error: " << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}

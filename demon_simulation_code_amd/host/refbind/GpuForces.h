@@ -1,0 +1,200 @@
+// GpuForces.h -- the binding a maintainer of AU-PSL/demon_simulation_code adds
+// to the reference tree to run its hot path on an MI355X through the C ABI of
+// libdemon_gpu.so (include/demon_gpu.h).  Header-only; it is compiled against
+// the reference's own headers (Force.h, Integrator.h, Cloud.h, cfitsio).
+//
+//   GpuShieldedCoulombForce  drop-in for ShieldedCoulombForce
+//                            (ShieldedCoulombForce.h:15-39): force1..force4
+//                            read the same stage getters' data (x / xCache)
+//                            and add into Cloud::forceX/forceY.
+//   GpuConfinementForce, GpuDragForce, GpuThermalForce
+//                            the same for ConfinementForce, DragForce and
+//                            ThermalForce, carrying their parameters.
+//   GpuRungeKutta4           drop-in for Runge_Kutta4 (Runge_Kutta4.h:14-31):
+//                            moveParticles(endTime) uploads the Cloud, runs
+//                            the whole while (t < end) loop -- dt check,
+//                            four force stages, RK4 combine -- on the device
+//                            and downloads x, y, Vx, Vy.  Every force in its
+//                            ForceArray must be one of the Gpu* forces.
+// Errors of the C ABI are reported on stderr and end the program with
+// exit(1), the reference's convention for FITS errors (driver.cpp:202-213).
+#ifndef DEMON_GPU_FORCES_H
+#define DEMON_GPU_FORCES_H
+
+#include <cstdio>
+#include <cstdlib>
+
+#include "Force.h"
+#include "Integrator.h"
+#include "demon_gpu.h"
+
+namespace demon_gpu_binding {
+
+inline void require(int rc, const demon_ctx *ctx, const char *what) {
+    if (rc != DEMON_OK) {
+        std::fprintf(stderr, "Error: %s: %s\n", what, demon_last_error(ctx));
+        std::exit(1);
+    }
+}
+
+// Stage arrays of the reference's Cloud as contiguous doubles: stage 1 is
+// x / Vx; stages 2-4 are the doubleV caches (Cloud.cpp:277-475), whose
+// memory holds the same particle order.
+inline const double *stage_x(const Cloud *c, int s) { return s == 1 ? c->x : reinterpret_cast<const double *>(c->xCache); }
+inline const double *stage_y(const Cloud *c, int s) { return s == 1 ? c->y : reinterpret_cast<const double *>(c->yCache); }
+inline const double *stage_vx(const Cloud *c, int s) { return s == 1 ? c->Vx : reinterpret_cast<const double *>(c->VxCache); }
+inline const double *stage_vy(const Cloud *c, int s) { return s == 1 ? c->Vy : reinterpret_cast<const double *>(c->VyCache); }
+
+// Common part of the Gpu* forces: a device context of the cloud's size.
+class GpuForceBase : public Force {
+public:
+    GpuForceBase(Cloud *C, int flag) : Force(C), flag_(flag) {}
+    ~GpuForceBase() { demon_destroy(ctx_); }
+    int flag() const { return flag_; }
+    virtual void contribute(demon_params &p) const = 0;
+
+protected:
+    void stage(int s) {
+        demon_params p{};
+        p.shielding = 1.0;
+        p.forces = flag_;
+        contribute(p);
+        if (!ctx_) require(demon_create(&ctx_, (int64_t)cloud->n, &p), nullptr, "demon_create");
+        else require(demon_set_params(ctx_, &p), ctx_, "demon_set_params");
+        require(demon_upload(ctx_, cloud->x, cloud->y, cloud->Vx, cloud->Vy, cloud->charge, cloud->mass),
+                ctx_, "demon_upload");
+        require(demon_force_stage(ctx_, flag_, s, step_, stage_x(cloud, s), stage_y(cloud, s),
+                                  stage_vx(cloud, s), stage_vy(cloud, s), cloud->forceX, cloud->forceY),
+                ctx_, "demon_force_stage");
+        if (s == 4) ++step_;
+    }
+
+private:
+    int flag_;
+    demon_ctx *ctx_ = nullptr;
+    uint64_t step_ = 0;
+};
+
+#define DEMON_GPU_FORCE_STAGES                        \
+    void force1(const double) { stage(1); }           \
+    void force2(const double) { stage(2); }           \
+    void force3(const double) { stage(3); }           \
+    void force4(const double) { stage(4); }
+
+// FITS keywords as the CPU classes write them (e.g. ShieldedCoulombForce.cpp:239-277).
+inline void write_key(fitsfile *file, int *error, long bit, const char *key, double value, const char *comment) {
+    if (!*error) fits_movabs_hdu(file, 1, IMAGE_HDU, error);
+    if (!*error) {
+        long flags = 0;
+        fits_read_key_lng(file, const_cast<char *>("FORCES"), &flags, NULL, error);
+        flags |= bit;
+        if (*error == KEY_NO_EXIST || *error == VALUE_UNDEFINED) *error = 0;
+        if (!*error)
+            fits_update_key(file, TLONG, const_cast<char *>("FORCES"), &flags,
+                            const_cast<char *>("Force configuration."), error);
+    }
+    if (!*error) fits_write_key_dbl(file, const_cast<char *>(key), value, 6, const_cast<char *>(comment), error);
+}
+
+inline void read_key(fitsfile *file, int *error, const char *key, double *value) {
+    if (!*error) fits_movabs_hdu(file, 1, IMAGE_HDU, error);
+    if (!*error) fits_read_key_dbl(file, const_cast<char *>(key), value, NULL, error);
+}
+
+}  // namespace demon_gpu_binding
+
+class GpuShieldedCoulombForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuShieldedCoulombForce(Cloud *C, double shieldingConstant)
+        : GpuForceBase(C, DEMON_FORCE_SHIELDED_COULOMB), shielding(shieldingConstant) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const { p.shielding = shielding; }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, ShieldedCoulombForceFlag, "shieldingConstant", shielding,
+                                     "[m^-1] (ShieldedCoulombForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) { demon_gpu_binding::read_key(f, e, "shieldingConstant", &shielding); }
+    double shielding;
+};
+
+class GpuConfinementForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuConfinementForce(Cloud *C, double confineConst) : GpuForceBase(C, DEMON_FORCE_CONFINEMENT), confine(confineConst) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const { p.confine = confine; }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, ConfinementForceFlag, "confineConst", confine, "[V/m^2] (ConfinementForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) { demon_gpu_binding::read_key(f, e, "confineConst", &confine); }
+    double confine;
+};
+
+class GpuDragForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuDragForce(Cloud *C, double gamma) : GpuForceBase(C, DEMON_FORCE_DRAG), dragConst(-gamma) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const { p.drag_gamma = -dragConst; }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, DragForceFlag, "dragConst", dragConst, "[Hz] (DragForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) { demon_gpu_binding::read_key(f, e, "dragConst", &dragConst); }
+    double dragConst;
+};
+
+class GpuThermalForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuThermalForce(Cloud *C, double redFactor, uint64_t seed = 0)
+        : GpuForceBase(C, DEMON_FORCE_THERMAL), heatVal(redFactor), seed(seed) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const {
+        p.heat_val = heatVal;
+        p.seed = seed;
+    }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, ThermalForceFlag, "heatingValue", heatVal, "[N] (ThermalForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) { demon_gpu_binding::read_key(f, e, "heatingValue", &heatVal); }
+    double heatVal;
+    uint64_t seed;
+};
+
+#undef DEMON_GPU_FORCE_STAGES
+
+class GpuRungeKutta4 : public Integrator {
+public:
+    GpuRungeKutta4(Cloud *const C, const ForceArray &FA, const double timeStep, const double startTime,
+                   int device = 0)
+        : Integrator(C, FA, timeStep, startTime), device_(device) {}
+    ~GpuRungeKutta4() { demon_destroy(ctx_); }
+
+    void moveParticles(const double endTime) {
+        demon_params p{};
+        p.shielding = 1.0;
+        p.dt_check = 1;  // Runge_Kutta4.cpp:25 always calls modifyTimeStep
+        p.device = device_;
+        for (Force *const F : forces) {
+            demon_gpu_binding::GpuForceBase *g = dynamic_cast<demon_gpu_binding::GpuForceBase *>(F);
+            if (!g) {
+                std::fprintf(stderr, "Error: GpuRungeKutta4 needs Gpu* forces only\n");
+                std::exit(1);
+            }
+            p.forces |= g->flag();
+            g->contribute(p);
+        }
+        if (!ctx_) demon_gpu_binding::require(demon_create(&ctx_, (int64_t)cloud->n, &p), nullptr, "demon_create");
+        else demon_gpu_binding::require(demon_set_params(ctx_, &p), ctx_, "demon_set_params");
+        demon_gpu_binding::require(
+            demon_upload(ctx_, cloud->x, cloud->y, cloud->Vx, cloud->Vy, cloud->charge, cloud->mass), ctx_,
+            "demon_upload");
+        demon_gpu_binding::require(demon_rk4_move(ctx_, &currentTime, endTime, init_dt, nullptr), ctx_,
+                                   "demon_rk4_move");
+        demon_gpu_binding::require(demon_download(ctx_, cloud->x, cloud->y, cloud->Vx, cloud->Vy), ctx_,
+                                   "demon_download");
+    }
+
+private:
+    int device_;
+    demon_ctx *ctx_ = nullptr;
+};
+
+#endif  // DEMON_GPU_FORCES_H

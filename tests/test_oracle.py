@@ -1,0 +1,279 @@
+"""Pins of the CPU oracle (no GPU needed).
+
+Each oracle function is checked against something other than its own
+formula: the reference's own outputs (golden fixtures made by
+tests/golden/make_golden.py from the reference's sources; live when
+oracle/_ref is built), closed forms, invariants and long-double brute force.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle.bind import OracleParams, Reference
+
+GOLD = Path(__file__).resolve().parent / "golden"
+E = -1.602e-19
+Q0 = 6000 * E
+M0 = 2.8e-14
+KAPPA = 2000.0
+COUL = 1.0 / (4.0 * np.pi * 8.8541878e-12)
+CONFINE = 5e-11 / abs(Q0)
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.int64), np.asarray(b).view(np.int64))
+
+
+# --- golden vectors from the reference ------------------------------------
+
+@pytest.mark.parametrize("n", [64, 256, 1024])
+def test_yukawa_matches_reference_golden_bitwise(oracle, n):
+    d = np.load(GOLD / f"forces_n{n}.npz")
+    fx, fy = oracle.yukawa(d["x"], d["y"], d["q"], float(d["kappa"]))
+    assert bits_equal(fx, d["f1x"]) and bits_equal(fy, d["f1y"])
+    # force3 reads the stage caches; same arithmetic
+    assert bits_equal(fx, d["f3x"]) and bits_equal(fy, d["f3y"])
+
+
+@pytest.mark.parametrize("n", [64, 256, 1024])
+def test_rk4_matches_reference_golden_bitwise(oracle, n):
+    d = np.load(GOLD / f"rk4_n{n}.npz")
+    op = OracleParams(float(d["kappa"]), float(d["confine"]), float(d["gamma"]), 0.0, 0, 7, 1)
+    state = (d["x"], d["y"], np.zeros(n), np.zeros(n))
+    done = 0
+    for target in (1, 10, 100):
+        if f"x_{target}" not in d:
+            continue
+        *state, t, _ = oracle.rk4_steps(*state, d["q"], d["m"], op, float(d["dt"]), target - done,
+                                        t=done * float(d["dt"]))
+        done = target
+        for name, arr in zip(("x", "y", "vx", "vy"), state):
+            assert bits_equal(arr, d[f"{name}_{target}"]), (name, target)
+
+
+def test_dtcheck_matches_reference_golden(oracle):
+    d = np.load(GOLD / "dtcheck.npz")
+    for a, b, sep, want in d["cases"]:
+        xx, yy = d["x"].copy(), d["y"].copy()
+        xx[int(b)] = xx[int(a)] + sep
+        yy[int(b)] = yy[int(a)]
+        assert oracle.modify_time_step(xx, yy, 1e-4, 1e-3) == want
+
+
+# --- live comparison with the reference build (skipped where not built) ---
+
+@pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_random_configs_bitwise_vs_live_reference(oracle):
+    ref = Reference()
+    rng = np.random.default_rng(0)
+    for trial in range(6):
+        n = 4 * int(rng.integers(2, 40))
+        x = rng.uniform(-2e-3, 2e-3, n)
+        y = rng.uniform(-2e-3, 2e-3, n)
+        q = Q0 * rng.uniform(0.5, 1.5, n)
+        kappa = float(rng.uniform(500, 8000))
+        fx, fy = oracle.yukawa(x, y, q, kappa)
+        rx, ry = ref.yukawa(x, y, q, kappa, stage=1 + trial % 4)
+        assert bits_equal(fx, rx) and bits_equal(fy, ry)
+        assert oracle.modify_time_step(x, y, 1e-4, 1e-3) == ref.modify_time_step(x, y, 1e-4, 1e-3)
+        m = M0 * rng.uniform(0.8, 1.2, n)
+        op = OracleParams(kappa, CONFINE, 2.0, 0.0, 0, 7, 1)
+        z = np.zeros(n)
+        o = oracle.rk4_steps(x, y, z, z, q, m, op, 1e-4, 5)
+        r = ref.rk4_steps(x, y, z, z, q, m, kappa, CONFINE, 2.0, 0.0, 7, 1e-4, 5)
+        for a, b in zip(o[:4], r[:4]):
+            assert bits_equal(a, b)
+        assert o[4] == r[4]
+
+
+@pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_move_loop_step_count_vs_live_reference(oracle):
+    """moveParticles(end) drives by `while (t < end)`; the oracle's count is the reference's."""
+    ref = Reference()
+    n = 16
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-3e-3, 3e-3, n)
+    y = rng.uniform(-3e-3, 3e-3, n)
+    q, m, z = np.full(n, Q0), np.full(n, M0), np.zeros(n)
+    op = OracleParams(KAPPA, CONFINE, 2.0, 0.0, 0, 7, 1)
+    for end in (0.01, 0.1, 0.37):
+        o = oracle.rk4_move(x, y, z, z, q, m, op, 1e-3, end)
+        r = ref.rk4_move(x, y, z, z, q, m, KAPPA, CONFINE, 2.0, 0.0, 7, 1e-3, end)
+        assert o[4] == r[4]
+        for a, b in zip(o[:4], r[:4]):
+            assert bits_equal(a, b)
+
+
+# --- closed forms and invariants -------------------------------------------
+
+def test_two_body_closed_form(oracle):
+    for r in (100e-6, 500e-6, 2e-3, 4.9e-3):
+        x = np.array([0.0, r])
+        y = np.array([0.0, 0.0])
+        q = np.array([Q0, 1.3 * Q0])
+        fx, fy = oracle.yukawa(x, y, q, KAPPA)
+        u = r * KAPPA
+        f = COUL * Q0 * 1.3 * Q0 * (1 + u) * np.exp(-u) / r**2
+        assert fx[0] == pytest.approx(-f, rel=1e-14)
+        assert fx[1] == pytest.approx(f, rel=1e-14)
+        assert fy[0] == 0.0 and fy[1] == 0.0
+
+
+def test_cutoff_is_hard(oracle):
+    r_cut = 10.0 / KAPPA
+    for r, live in ((np.nextafter(r_cut, 0), True), (r_cut, False), (np.nextafter(r_cut, 1), False),
+                    (2 * r_cut, False)):
+        x = np.array([0.0, r])
+        fx, _ = oracle.yukawa(x, np.zeros(2), np.array([Q0, Q0]), KAPPA)
+        assert (fx[0] != 0.0) == live
+
+
+def test_newton3_and_long_double(oracle):
+    rng = np.random.default_rng(5)
+    n = 64
+    x = rng.uniform(-2e-3, 2e-3, n)
+    y = rng.uniform(-2e-3, 2e-3, n)
+    q = Q0 * rng.uniform(0.8, 1.2, n)
+    fx, fy = oracle.yukawa(x, y, q, KAPPA)
+    gx, gy = oracle.yukawa_gather(x, y, q, KAPPA)
+    lx, ly = oracle.yukawa_longdouble(x, y, q, KAPPA)
+    scale = np.abs(lx).max() + np.abs(ly).max()
+    assert abs(fx.sum()) / scale < 1e-13 and abs(fy.sum()) / scale < 1e-13
+    for ax, ay in ((fx, fy), (gx, gy)):
+        assert np.max(np.hypot(ax - lx, ay - ly)) / scale < 1e-13
+
+
+def test_rows_equal_gather(oracle):
+    rng = np.random.default_rng(6)
+    n = 200
+    x, y = rng.uniform(-3e-3, 3e-3, (2, n))
+    q = Q0 * rng.uniform(0.8, 1.2, n)
+    gx, gy = oracle.yukawa_gather(x, y, q, KAPPA)
+    rows = np.array([0, 7, 199, 100])
+    rx, ry = oracle.yukawa_rows(x, y, q, KAPPA, rows)
+    assert bits_equal(rx, gx[rows]) and bits_equal(ry, gy[rows])
+
+
+def test_symmetric_hexagon_centre_force_vanishes(oracle):
+    a = 500e-6
+    ang = np.arange(6) * np.pi / 3
+    x = np.r_[0.0, a * np.cos(ang)]
+    y = np.r_[0.0, a * np.sin(ang)]
+    fx, fy = oracle.yukawa_gather(x, y, np.full(7, Q0), KAPPA)
+    single = COUL * Q0 * Q0 * (1 + a * KAPPA) * np.exp(-a * KAPPA) / a**2
+    assert abs(fx[0]) < 1e-14 * single and abs(fy[0]) < 1e-14 * single
+
+
+def test_confinement_drag_closed_form(oracle):
+    rng = np.random.default_rng(7)
+    n = 10
+    x, y, vx, vy = rng.normal(size=(4, n)) * 1e-3
+    q = Q0 * rng.uniform(0.5, 1.5, n)
+    m = M0 * rng.uniform(0.5, 1.5, n)
+    op = OracleParams(KAPPA, CONFINE, 2.0, 0.0, 0, 1 | 2, 0)
+    fx, fy = oracle.stage_force(x, y, vx, vy, q, m, op)
+    np.testing.assert_allclose(fx, q * CONFINE * x - 2.0 * m * vx, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(fy, q * CONFINE * y - 2.0 * m * vy, rtol=1e-15, atol=0)
+
+
+def test_rk4_linear_system_amplification_polynomial(oracle):
+    """Confinement + drag only: one RK4 step is T4(hA) applied to (x, v)."""
+    n = 4
+    rng = np.random.default_rng(8)
+    x, y, vx, vy = rng.normal(size=(4, n)) * 1e-3
+    q = np.full(n, Q0)
+    m = np.full(n, M0)
+    gamma, h = 2.0, 1e-3
+    op = OracleParams(KAPPA, CONFINE, gamma, 0.0, 0, 1 | 2, 0)
+    o = oracle.rk4_steps(x, y, vx, vy, q, m, op, h, 1)
+    A = np.array([[0.0, 1.0], [Q0 * CONFINE / M0, -gamma]])
+    hA = h * A
+    T = np.eye(2) + hA + hA @ hA / 2 + hA @ hA @ hA / 6 + hA @ hA @ hA @ hA / 24
+    for i in range(n):
+        ex = T @ np.array([x[i], vx[i]])
+        ey = T @ np.array([y[i], vy[i]])
+        assert o[0][i] == pytest.approx(ex[0], rel=1e-13)
+        assert o[2][i] == pytest.approx(ex[1], rel=1e-12)
+        assert o[1][i] == pytest.approx(ey[0], rel=1e-13)
+        assert o[3][i] == pytest.approx(ey[1], rel=1e-12)
+
+
+def test_two_body_energy_drift_is_fourth_order(oracle):
+    """gamma = 0, confinement + Yukawa: energy error after a fixed time scales ~ h^4."""
+    x0 = np.array([-0.4e-3, 0.4e-3])
+    y0 = np.zeros(2)
+    vx0 = np.zeros(2)
+    vy0 = np.array([0.01, -0.01])
+    q, m = np.full(2, Q0), np.full(2, M0)
+
+    def energy(x, y, vx, vy):
+        r = np.hypot(x[0] - x[1], y[0] - y[1])
+        u = r * KAPPA
+        pot = COUL * Q0 * Q0 * np.exp(-u) / r - (Q0 * CONFINE / 2) * np.sum(x**2 + y**2)
+        return 0.5 * M0 * np.sum(vx**2 + vy**2) + pot
+
+    e0 = energy(x0, y0, vx0, vy0)
+    errs = []
+    for h in (4e-4, 2e-4):
+        op = OracleParams(KAPPA, CONFINE, 0.0, 0.0, 0, 1 | 2 | 4, 0)
+        o = oracle.rk4_steps(x0, y0, vx0, vy0, q, m, op, h, int(round(0.04 / h)))
+        errs.append(abs(energy(*o[:4]) - e0))
+    ratio = errs[0] / errs[1]
+    assert 10 < ratio < 40, ratio  # 2^4 = 16
+
+
+# --- dt check ----------------------------------------------------------------
+
+def test_dtcheck_coverage_predicate(oracle):
+    """Literal lane-shuffle replay vs the closed-form predicate, pair by pair."""
+    rng = np.random.default_rng(9)
+    n = 24
+    base_x = np.arange(n) * 1e-3
+    base_y = np.zeros(n)
+    for i in range(n):
+        for j in range(i + 1, n):
+            x, y = base_x.copy(), base_y.copy()
+            x[j] = x[i] + 30e-6
+            y[j] = 0.0 if rng.random() < 0.5 else 1e-6
+            reduced = oracle.modify_time_step(x, y, 1e-4, 1e-3) < 1e-3
+            assert reduced == oracle.covers(i, j), (i, j)
+    # hole examples named in the analysis of Integrator.cpp:112-129
+    assert not oracle.covers(3, 4) and not oracle.covers(0, 5) and oracle.covers(0, 4)
+
+
+def test_dtcheck_min_matches_replay(oracle):
+    rng = np.random.default_rng(10)
+    for _ in range(20):
+        n = 4 * int(rng.integers(2, 12))
+        x = rng.uniform(0, 2e-3, n)
+        y = rng.uniform(0, 2e-3, n)
+        s = oracle.dtcheck_min_s(x, y)
+        d = np.float32(np.sqrt(np.float32(s)))
+        dist, dt = np.float32(1e-4), 1e-3
+        while d <= dist:
+            dist = np.float32(dist / np.float32(10))
+            dt = dt / 10.0
+        assert oracle.modify_time_step(x, y, 1e-4, 1e-3) == dt
+
+
+# --- RNG ---------------------------------------------------------------------
+
+def test_philox_known_answers(oracle):
+    """Random123 known-answer vectors for philox4x32-10."""
+    assert oracle.philox([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert oracle.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert oracle.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_thermal_moments(oracle):
+    n, heat = 200000, 1.0
+    fx, fy = oracle.thermal(n, heat, 123, 7, 2)
+    mag = np.hypot(fx, fy)
+    assert np.mean(mag) == pytest.approx(0.5, abs=3e-3)          # E[U] = 1/2
+    assert np.mean(mag**2) == pytest.approx(1 / 3, abs=3e-3)     # E[U^2] = 1/3
+    assert abs(np.mean(fx)) < 3e-3 and abs(np.mean(fy)) < 3e-3   # isotropy
+    assert np.mean(fx**2) == pytest.approx(1 / 6, abs=3e-3)
+    gx, _ = oracle.thermal(n, heat, 123, 7, 3)
+    assert abs(np.corrcoef(fx, gx)[0, 1]) < 0.01                  # stages independent
