@@ -5,6 +5,7 @@ The compute runs in libdemon_gpu.so (hand-written gfx950 HIP); this package
 only binds it.  Importing fails loudly when the library is not built.
 """
 from ._lib import (  # noqa: F401
+    Batch,
     CONFINEMENT,
     DRAG,
     FP32_PAIR,
@@ -22,7 +23,7 @@ from ._lib import (  # noqa: F401
 from .physics import ELECTRON_CHARGE, EPSILON0, COULOMB, confine_from_k  # noqa: F401
 
 __all__ = [
-    "Context", "DemonError", "Params", "Stats", "hex_lattice", "lib", "LIB_PATH",
+    "Batch", "Context", "DemonError", "Params", "Stats", "hex_lattice", "lib", "LIB_PATH",
     "CONFINEMENT", "DRAG", "SHIELDED_COULOMB", "THERMAL", "FP64", "FP32_PAIR",
     "ELECTRON_CHARGE", "EPSILON0", "COULOMB", "confine_from_k",
 ]

@@ -90,13 +90,29 @@ SIGNATURES = {
     "demon_get_pair_shape": (ctypes.c_int, [_CTX, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "demon_last_timing": (ctypes.c_int, [_CTX, _D, _D]),
     "demon_hex_lattice": (ctypes.c_int, [_I64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _D, _D]),
+    "demon_fit_loss_grad": (ctypes.c_int, [_CTX, _I64, ctypes.c_double, _I64, _I64, _D, _D, _D, _D, _D,
+                                           _D, _D, _D, _D, _D, _D]),
+    "demon_fit_timing": (ctypes.c_int, [_CTX, _D, _D]),
+    "demon_batch_last_error": (ctypes.c_char_p, [_CTX]),
+    "demon_batch_create": (ctypes.c_int, [ctypes.POINTER(_CTX), _I32, ctypes.POINTER(_I64), _D, _D, _D,
+                                          ctypes.POINTER(Params)]),
+    "demon_batch_destroy": (None, [_CTX]),
+    "demon_batch_upload": (ctypes.c_int, [_CTX, _I32, _D, _D, _D, _D, _D, _D]),
+    "demon_batch_download": (ctypes.c_int, [_CTX, _I32, _D, _D, _D, _D]),
+    "demon_batch_steps": (ctypes.c_int, [_CTX, _I64, ctypes.c_double]),
+    "demon_batch_cloud_stats": (ctypes.c_int, [_CTX, _I32, ctypes.POINTER(Stats)]),
+    "demon_batch_use_graph": (ctypes.c_int, [_CTX, _I32]),
+    "demon_batch_last_ms": (ctypes.c_int, [_CTX, _D]),
+    "demon_batch_set_shape": (ctypes.c_int, [_CTX, _I32, _I32, _I32]),
+    "demon_batch_get_shape": (ctypes.c_int, [_CTX, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _D]),
+    "demon_batch_tune": (ctypes.c_int, [_CTX, _I32]),
 }
 
 
 def _load() -> ctypes.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(
-            f"{LIB_PATH} is missing: build it with `python -m demon_simulation_code_amd.build` "
+            f"{LIB_PATH} is missing: build it with `python demon_simulation_code_amd/build.py` "
             "(there is no CPU fallback)"
         )
     lib = ctypes.CDLL(str(LIB_PATH))
@@ -217,7 +233,113 @@ class Context:
         self._c(lib.demon_get_pair_shape(self._h, ctypes.byref(r), ctypes.byref(s)))
         return r.value, s.value
 
+    def fit_loss_grad(self, steps, dt, obs_every, ckpt_every, target_x, target_y, grad=True,
+                      state_grad=False):
+        """Loss of the trajectory fit and (optionally) its gradient.
+
+        Returns (loss, grads) with grads = dict(q, kappa, confine, gamma[, x0, y0, vx0, vy0]).
+        target_x/target_y: arrays [steps // obs_every, n] in caller order."""
+        tx = np.ascontiguousarray(target_x, dtype=np.float64)
+        ty = np.ascontiguousarray(target_y, dtype=np.float64)
+        loss = ctypes.c_double(0)
+        if not grad:
+            self._c(lib.demon_fit_loss_grad(self._h, int(steps), dt, int(obs_every), int(ckpt_every),
+                                            _ptr(tx), _ptr(ty), ctypes.byref(loss), *([None] * 8)))
+            return loss.value, {}
+        gq = np.zeros(self.n)
+        g3 = [ctypes.c_double(0) for _ in range(3)]
+        gs = [np.zeros(self.n) for _ in range(4)] if state_grad else [None] * 4
+        self._c(lib.demon_fit_loss_grad(self._h, int(steps), dt, int(obs_every), int(ckpt_every),
+                                        _ptr(tx), _ptr(ty), ctypes.byref(loss), _ptr(gq),
+                                        ctypes.byref(g3[0]), ctypes.byref(g3[1]), ctypes.byref(g3[2]),
+                                        *[_ptr(a) for a in gs]))
+        out = dict(q=gq, kappa=g3[0].value, confine=g3[1].value, gamma=g3[2].value)
+        if state_grad:
+            out.update(x0=gs[0], y0=gs[1], vx0=gs[2], vy0=gs[3])
+        return loss.value, out
+
+    def fit_timing(self):
+        a, b = ctypes.c_double(0), ctypes.c_double(0)
+        self._c(lib.demon_fit_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def last_timing(self):
         a, b = ctypes.c_double(0), ctypes.c_double(0)
         self._c(lib.demon_last_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+class Batch:
+    """Many independent clouds stepped together (demon_batch)."""
+
+    def __init__(self, sizes, kappa, confine, gamma, *, forces=CONFINEMENT | DRAG | SHIELDED_COULOMB,
+                 precision=FP64, dt_check=True, heat=0.0, seed=0, device=0):
+        self.sizes = [int(v) for v in sizes]
+        k = len(self.sizes)
+        n_arr = (_I64 * k)(*self.sizes)
+        kap = np.ascontiguousarray(np.broadcast_to(kappa, (k,)), dtype=np.float64)
+        con = np.ascontiguousarray(np.broadcast_to(confine, (k,)), dtype=np.float64)
+        gam = np.ascontiguousarray(np.broadcast_to(gamma, (k,)), dtype=np.float64)
+        p = Params(1.0, 0.0, 0.0, heat, seed, forces, precision, 1 if dt_check else 0, device)
+        h = _CTX()
+        rc = lib.demon_batch_create(ctypes.byref(h), k, n_arr, _ptr(kap), _ptr(con), _ptr(gam), ctypes.byref(p))
+        if rc != OK:
+            raise DemonError(rc, lib.demon_batch_last_error(None).decode())
+        self._h = h
+
+    def _c(self, rc):
+        if rc != OK:
+            raise DemonError(rc, lib.demon_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.demon_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, cloud, x, y, vx=None, vy=None, charge=None, mass=None):
+        self._c(lib.demon_batch_upload(self._h, cloud, _ptr(x), _ptr(y), _ptr(vx), _ptr(vy), _ptr(charge), _ptr(mass)))
+
+    def download(self, cloud):
+        n = self.sizes[cloud]
+        out = [np.empty(n) for _ in range(4)]
+        self._c(lib.demon_batch_download(self._h, cloud, *[_ptr(a) for a in out]))
+        return tuple(out)
+
+    def steps(self, steps, dt):
+        self._c(lib.demon_batch_steps(self._h, int(steps), dt))
+
+    def stats(self, cloud) -> Stats:
+        s = Stats()
+        self._c(lib.demon_batch_cloud_stats(self._h, cloud, ctypes.byref(s)))
+        return s
+
+    def use_graph(self, on=True):
+        self._c(lib.demon_batch_use_graph(self._h, 1 if on else 0))
+
+    def last_ms(self):
+        v = ctypes.c_double(0)
+        self._c(lib.demon_batch_last_ms(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_shape(self, bucket, i_per_lane, jslice):
+        self._c(lib.demon_batch_set_shape(self._h, bucket, i_per_lane, jslice))
+
+    def shape(self, bucket):
+        r, j, us = _I32(0), _I32(0), ctypes.c_double(0)
+        self._c(lib.demon_batch_get_shape(self._h, bucket, ctypes.byref(r), ctypes.byref(j), ctypes.byref(us)))
+        return r.value, j.value, us.value
+
+    def tune(self, reps=3):
+        self._c(lib.demon_batch_tune(self._h, reps))

@@ -1,6 +1,7 @@
 """Builds libdemon_gpu.so (the HIP/C-ABI product) in-tree for gfx950.
 
-Usage: python -m demon_simulation_code_amd.build [--verbose]
+Usage: python demon_simulation_code_amd/build.py [--verbose]
+(run by path: importing the package would load the library being rebuilt)
 
 The library is compiled with hipcc directly (no torch extension machinery):
 every .hip file under csrc/ is one translation unit of a single shared

@@ -53,6 +53,11 @@ struct demon_ctx {
     double last_pair_ms = 0.0, last_total_ms = 0.0;
     int64_t force_evals = 0;
     uint64_t rng_step = 0;       // thermal RNG step counter; survives re-uploads
+    // trajectory fit (demon_fit_loss_grad): one arena, grown on demand
+    double *d_fit = nullptr;
+    size_t fit_doubles = 0;
+    double last_fwd_ms = 0.0, last_bwd_ms = 0.0;
+    hipEvent_t ev_fit[3]{};
 };
 
 namespace {
@@ -183,7 +188,7 @@ int enqueue_pairs(demon_ctx *c, const StageBuf &st, int gated, int dtcheck) {
     return DEMON_OK;
 }
 
-UpdateLaunch make_update(demon_ctx *c, int stage) {
+UpdateLaunch make_update(demon_ctx *c, int stage, int dtcheck, double *store) {
     UpdateLaunch u{};
     // stage records: 1 reads sb0, 2 reads sb1, 3 reads sb0, 4 reads sb1
     const int cur = (stage - 1) & 1;
@@ -192,27 +197,32 @@ UpdateLaunch make_update(demon_ctx *c, int stage) {
     u.x = c->x; u.y = c->y; u.vx = c->vx; u.vy = c->vy; u.q = c->q; u.m = c->m;
     u.orig = c->d_orig; u.part = c->d_part; u.slices = c->slices;
     u.acc1 = c->acc1; u.acc23 = c->acc23; u.sc = c->d_sc;
-    u.n = c->n; u.npad = c->npad; u.forces = c->p.forces; u.dtcheck = c->p.dt_check;
+    u.n = c->n; u.npad = c->npad; u.forces = c->p.forces; u.dtcheck = dtcheck;
     u.confine = c->p.confine; u.gamma = c->p.drag_gamma; u.heat = c->p.heat_val; u.seed = c->p.seed;
+    u.store = store;
     return u;
 }
 
-int enqueue_step(demon_ctx *c) {
+// One RK4 step.  store (optional): 4 consecutive stage slots of 5*npad doubles.
+int enqueue_step_ex(demon_ctx *c, int dtcheck, double *store) {
     HIP_TRY(c, launch_boundary(c->d_sc, 0, c->stream));
     const bool yk = c->p.forces & DEMON_FORCE_SHIELDED_COULOMB;
     for (int s = 1; s <= 4; ++s) {
         const StageBuf &st = c->sb[(s - 1) & 1];
         if (yk) {
-            const int rc = enqueue_pairs(c, st, 1, s == 1 && c->p.dt_check);
+            const int rc = enqueue_pairs(c, st, 1, s == 1 && dtcheck);
             if (rc) return rc;
-        } else if (s == 1 && c->p.dt_check) {
+        } else if (s == 1 && dtcheck) {
             return fail(c, DEMON_ERR_UNSUPPORTED,
                         "the dt check runs inside the all-pairs pass; enable ShieldedCoulomb");
         }
-        HIP_TRY(c, launch_update(s, make_update(c, s), c->stream));
+        double *slot = store ? store + (size_t)(s - 1) * 5 * c->npad : nullptr;
+        HIP_TRY(c, launch_update(s, make_update(c, s, dtcheck, slot), c->stream));
     }
     return DEMON_OK;
 }
+
+int enqueue_step(demon_ctx *c) { return enqueue_step_ex(c, c->p.dt_check, nullptr); }
 
 void drop_graph(demon_ctx *c) {
     if (c->graph) {
@@ -282,6 +292,54 @@ int run_steps(demon_ctx *c, int64_t steps, Scalars *out) {
                     "dt check: two tested particles coincide (distance 0); the reference's "
                     "modifyTimeStep would divide dt by 10 forever");
     return DEMON_OK;
+}
+
+
+// Device copies of the base state into / out of a 4*npad checkpoint slot.
+int save_state(demon_ctx *c, double *slot) {
+    const size_t b = sizeof(double) * c->npad;
+    const double *src[4] = {c->x, c->y, c->vx, c->vy};
+    for (int a = 0; a < 4; ++a)
+        HIP_TRY(c, hipMemcpyAsync(slot + (size_t)a * c->npad, src[a], b, hipMemcpyDeviceToDevice, c->stream));
+    return DEMON_OK;
+}
+
+// base state <- slot, and the stage-1 record rebuilt from it.
+int restore_state(demon_ctx *c, const double *slot) {
+    const size_t b = sizeof(double) * c->npad;
+    double *dst[4] = {c->x, c->y, c->vx, c->vy};
+    double *rec[4] = {c->sb[0].xs, c->sb[0].ys, c->sb[0].vxs, c->sb[0].vys};
+    for (int a = 0; a < 4; ++a) {
+        HIP_TRY(c, hipMemcpyAsync(dst[a], slot + (size_t)a * c->npad, b, hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(rec[a], slot + (size_t)a * c->npad, b, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIP_TRY(c, launch_prepare_stage(c->sb[0], c->n, c->npad, c->stream));
+    return DEMON_OK;
+}
+
+int fit_arena(demon_ctx *c, size_t doubles) {
+    if (doubles <= c->fit_doubles) return DEMON_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->d_fit) (void)hipFree(c->d_fit);
+    c->d_fit = nullptr;
+    c->fit_doubles = 0;
+    if (hipMalloc((void **)&c->d_fit, sizeof(double) * doubles) != hipSuccess)
+        return fail(c, DEMON_ERR_ALLOC, "hipMalloc of the fit buffers failed");
+    c->fit_doubles = doubles;
+    return DEMON_OK;
+}
+
+double host_sum(demon_ctx *c, const double *dev, std::vector<double> &tmp, int *rc) {
+    tmp.resize((size_t)c->npad);
+    hipError_t e = hipMemcpyAsync(tmp.data(), dev, sizeof(double) * c->npad, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        *rc = hip_fail(c, e, "hipMemcpy");
+        return 0.0;
+    }
+    double acc = 0.0;
+    for (int64_t k = 0; k < c->n; ++k) acc += tmp[(size_t)k];
+    return acc;
 }
 
 }  // namespace
@@ -358,6 +416,9 @@ void demon_destroy(demon_ctx *c) {
     if (c->d_part) (void)hipFree(c->d_part);
     if (c->d_orig) (void)hipFree(c->d_orig);
     if (c->d_sc) (void)hipFree(c->d_sc);
+    if (c->d_fit) (void)hipFree(c->d_fit);
+    for (hipEvent_t e : c->ev_fit)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -564,6 +625,148 @@ int demon_last_timing(const demon_ctx *c, double *pair_ms, double *total_ms) {
     if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
     if (pair_ms) *pair_ms = c->last_pair_ms;
     if (total_ms) *total_ms = c->last_total_ms;
+    return DEMON_OK;
+}
+
+int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_every,
+                        int64_t ckpt_every, const double *target_x, const double *target_y,
+                        double *loss, double *grad_q, double *grad_kappa, double *grad_confine,
+                        double *grad_gamma, double *grad_x0, double *grad_y0, double *grad_vx0,
+                        double *grad_vy0) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
+    if (steps < 1 || obs_every < 1 || ckpt_every < 1 || !(dt > 0.0) || !loss)
+        return fail(c, DEMON_ERR_ARG, "bad steps/obs_every/ckpt_every/dt/loss");
+    const int32_t allowed = DEMON_FORCE_CONFINEMENT | DEMON_FORCE_DRAG | DEMON_FORCE_SHIELDED_COULOMB;
+    if ((c->p.forces & ~allowed) || !(c->p.forces & DEMON_FORCE_SHIELDED_COULOMB))
+        return fail(c, DEMON_ERR_UNSUPPORTED, "the adjoint covers Confinement, Drag and ShieldedCoulomb only");
+    if (c->p.precision != DEMON_PRECISION_FP64)
+        return fail(c, DEMON_ERR_UNSUPPORTED, "the adjoint is FP64 only");
+    const int64_t n_obs = steps / obs_every;
+    if (n_obs > 0 && (!target_x || !target_y)) return fail(c, DEMON_ERR_ARG, "targets required");
+    const bool want = grad_q || grad_kappa || grad_confine || grad_gamma || grad_x0 || grad_y0 ||
+                      grad_vx0 || grad_vy0;
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    const size_t P = (size_t)c->npad;
+    const int64_t n_ckpt = (steps + ckpt_every - 1) / ckpt_every;
+    // arena: ckpt[n_ckpt][4] | targets[n_obs][2] | seeds[n_obs][2] | lossp | store[K][4][5]
+    //        | bar[4] | ibar[16] | u[2] | g[4] | apart[slices][4]
+    const size_t off_ck = 0, off_tg = off_ck + (size_t)n_ckpt * 4 * P;
+    const size_t off_sd = off_tg + (size_t)n_obs * 2 * P, off_lp = off_sd + (size_t)n_obs * 2 * P;
+    const size_t off_st = off_lp + P, off_bar = off_st + (size_t)ckpt_every * 20 * P;
+    const size_t off_ib = off_bar + 4 * P, off_u = off_ib + 16 * P, off_g = off_u + 2 * P;
+    const size_t off_ap = off_g + 4 * P, total = off_ap + (size_t)c->slices * 4 * P;
+    int rc = fit_arena(c, total);
+    if (rc) return rc;
+    double *A = c->d_fit;
+    for (int k = 0; k < 3; ++k)
+        if (!c->ev_fit[k]) HIP_TRY(c, hipEventCreate(&c->ev_fit[k]));
+    // targets in device order
+    for (int64_t o = 0; o < n_obs; ++o) {
+        if ((rc = put(c, A + off_tg + (size_t)(2 * o) * P, target_x + o * c->n, 0.0, 0)) ||
+            (rc = put(c, A + off_tg + (size_t)(2 * o + 1) * P, target_y + o * c->n, 0.0, 1)))
+            return rc;
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    HIP_TRY(c, hipMemsetAsync(A + off_lp, 0, sizeof(double) * P, c->stream));
+    // store slots need valid padding (nc = -inf) for the adjoint prefilter
+    for (int64_t k = 0; k < ckpt_every * 4; ++k) {
+        double *slot = A + off_st + (size_t)k * 5 * P;
+        StageBuf sbk{slot, slot + P, slot + 2 * P, slot + 3 * P, slot + 4 * P};
+        HIP_TRY(c, launch_prepare_stage(sbk, 0, c->npad, c->stream));
+    }
+    const double norm = 1.0 / (2.0 * (double)c->n * (double)(n_obs > 0 ? n_obs : 1));
+    const double inf = std::numeric_limits<double>::infinity();
+
+    // forward sweep with checkpoints and observations
+    HIP_TRY(c, hipEventRecord(c->ev_fit[0], c->stream));
+    hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, 0.0, inf, dt,
+                       (unsigned long long)steps);
+    HIP_TRY(c, hipGetLastError());
+    c->ev_used = 0;
+    const bool timing = c->timing;
+    c->timing = false;
+    for (int64_t s = 0; s < steps; ++s) {
+        if (s % ckpt_every == 0 && (rc = save_state(c, A + off_ck + (size_t)(s / ckpt_every) * 4 * P))) break;
+        if ((rc = enqueue_step_ex(c, 0, nullptr))) break;
+        if ((s + 1) % obs_every == 0 && (s + 1) / obs_every <= n_obs) {
+            // the step's last update wrote the new state; close it before observing
+            const int64_t o = (s + 1) / obs_every - 1;
+            HIP_TRY(c, launch_observe(c->x, c->y, A + off_tg + (size_t)(2 * o) * P,
+                                      want ? A + off_sd + (size_t)(2 * o) * P : nullptr, A + off_lp,
+                                      norm, c->n, c->npad, c->stream));
+        }
+    }
+    if (!rc) rc = launch_boundary(c->d_sc, 1, c->stream) == hipSuccess ? 0 : fail(c, DEMON_ERR_HIP, "boundary");
+    HIP_TRY(c, hipEventRecord(c->ev_fit[1], c->stream));
+    std::vector<double> tmp;
+    if (!rc) *loss = host_sum(c, A + off_lp, tmp, &rc) * norm;
+    if (!rc && want) {
+        double *bar = A + off_bar, *ibar = A + off_ib, *u = A + off_u, *g = A + off_g, *ap = A + off_ap;
+        HIP_TRY(c, hipMemsetAsync(bar, 0, sizeof(double) * 4 * P, c->stream));
+        HIP_TRY(c, hipMemsetAsync(g, 0, sizeof(double) * 4 * P, c->stream));
+        const double rc2p = (10.0 / c->p.shielding) * (10.0 / c->p.shielding) * (1.0 + kCutoffSlack);
+        for (int64_t seg = n_ckpt - 1; seg >= 0 && !rc; --seg) {
+            const int64_t s0 = seg * ckpt_every, s1 = std::min(steps, s0 + ckpt_every);
+            if ((rc = restore_state(c, A + off_ck + (size_t)seg * 4 * P))) break;
+            hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, 0.0, inf, dt,
+                               (unsigned long long)(s1 - s0));
+            for (int64_t s = s0; s < s1 && !rc; ++s)
+                rc = enqueue_step_ex(c, 0, A + off_st + (size_t)(s - s0) * 20 * P);
+            if (rc) break;
+            HIP_TRY(c, launch_boundary(c->d_sc, 1, c->stream));
+            for (int64_t s = s1 - 1; s >= s0; --s) {
+                const bool obs = (s + 1) % obs_every == 0 && (s + 1) / obs_every <= n_obs;
+                const double *seed = obs ? A + off_sd + (size_t)(2 * ((s + 1) / obs_every - 1)) * P : nullptr;
+                HIP_TRY(c, launch_adj_begin(bar, ibar, u, g, seed, c->m, c->n, c->npad, c->stream));
+                for (int k = 4; k >= 1; --k) {
+                    double *slot = A + off_st + ((size_t)(s - s0) * 4 + (k - 1)) * 5 * P;
+                    StageBuf st{slot, slot + P, slot + 2 * P, slot + 3 * P, slot + 4 * P};
+                    HIP_TRY(c, launch_adj_pairs(st.xs, st.ys, st.nc, c->q, u, u + P, ap, c->npad,
+                                                c->jslice, c->slices, c->i_per_lane, c->p.shielding,
+                                                rc2p, c->stream));
+                    HIP_TRY(c, launch_adj_stage(bar, ibar, u, g, k, st, c->q, c->m, ap, c->slices,
+                                                c->p.forces, dt, c->p.confine, c->p.drag_gamma, c->n,
+                                                c->npad, c->stream));
+                }
+            }
+        }
+        if (!rc) {
+            double gk = host_sum(c, g + P, tmp, &rc);
+            double gc = rc ? 0 : host_sum(c, g + 2 * P, tmp, &rc);
+            double gg = rc ? 0 : host_sum(c, g + 3 * P, tmp, &rc);
+            if (grad_kappa) *grad_kappa = gk;
+            if (grad_confine) *grad_confine = gc;
+            if (grad_gamma) *grad_gamma = gg;
+            double *outs[5] = {grad_q, grad_x0, grad_y0, grad_vx0, grad_vy0};
+            const double *srcs[5] = {g, bar, bar + P, bar + 2 * P, bar + 3 * P};
+            for (int a = 0; a < 5 && !rc; ++a) {
+                if (!outs[a]) continue;
+                tmp.resize(P);
+                hipError_t e = hipMemcpyAsync(tmp.data(), srcs[a], sizeof(double) * P, hipMemcpyDeviceToHost, c->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+                if (e != hipSuccess) rc = hip_fail(c, e, "hipMemcpy gradient");
+                for (int64_t k = 0; k < c->n && !rc; ++k) outs[a][c->perm[(size_t)k]] = tmp[(size_t)k];
+            }
+        }
+    }
+    // the context's state is the initial state again
+    if (!rc) rc = restore_state(c, A + off_ck);
+    HIP_TRY(c, hipEventRecord(c->ev_fit[2], c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->timing = timing;
+    float f0 = 0, f1 = 0;
+    HIP_TRY(c, hipEventElapsedTime(&f0, c->ev_fit[0], c->ev_fit[1]));
+    HIP_TRY(c, hipEventElapsedTime(&f1, c->ev_fit[1], c->ev_fit[2]));
+    c->last_fwd_ms = f0;
+    c->last_bwd_ms = f1;
+    return rc;
+}
+
+int demon_fit_timing(const demon_ctx *c, double *forward_ms, double *backward_ms) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (forward_ms) *forward_ms = c->last_fwd_ms;
+    if (backward_ms) *backward_ms = c->last_bwd_ms;
     return DEMON_OK;
 }
 

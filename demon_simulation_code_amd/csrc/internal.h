@@ -74,6 +74,7 @@ struct UpdateLaunch {
     int32_t dtcheck;
     double confine, gamma, heat;
     uint64_t seed;
+    double *store;                  // optional copy of the current stage record {xs ys vxs vys nc}[npad]
 };
 
 hipError_t launch_update(int stage, const UpdateLaunch &a, hipStream_t s);
@@ -99,5 +100,19 @@ hipError_t launch_force_out(const ForceOutLaunch &a, hipStream_t s);
 hipError_t launch_prepare_stage(StageBuf st, int64_t n, int64_t npad, hipStream_t s);
 // rmax2 = max |r|^2 over the stage positions (into sc->rmax2_bits, after reset).
 hipError_t launch_rmax(StageBuf st, int64_t n, Scalars *sc, hipStream_t s);
+
+// Discrete adjoint (adjoint.hip).
+hipError_t launch_adj_pairs(const double *xs, const double *ys, const double *nc, const double *q,
+                            const double *ux, const double *uy, double *part, int64_t npad,
+                            int32_t jslice, int32_t slices, int32_t i_per_lane, double kappa,
+                            double rc2p, hipStream_t s);
+hipError_t launch_adj_begin(double *bar, double *ibar, double *u, double *g, const double *seed,
+                            const double *m, int64_t n, int64_t npad, hipStream_t s);
+hipError_t launch_adj_stage(double *bar, double *ibar, double *u, double *g, int stage,
+                            const StageBuf &st, const double *q, const double *m, const double *part,
+                            int32_t slices, int32_t forces, double dt, double confine, double gamma,
+                            int64_t n, int64_t npad, hipStream_t s);
+hipError_t launch_observe(const double *x, const double *y, const double *tgt, double *seed,
+                          double *lossp, double norm, int64_t n, int64_t npad, hipStream_t s);
 
 }  // namespace demon

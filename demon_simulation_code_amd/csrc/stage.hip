@@ -108,6 +108,13 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
         const double xs = a.cur.xs[i], ys = a.cur.ys[i];
         const double vxs = a.cur.vxs[i], vys = a.cur.vys[i];
         const double qi = a.q[i], mi = a.m[i];
+        if (a.store) {
+            a.store[i] = xs;
+            a.store[a.npad + i] = ys;
+            a.store[2 * a.npad + i] = vxs;
+            a.store[3 * a.npad + i] = vys;
+            a.store[4 * a.npad + i] = a.cur.nc[i];
+        }
         double fx, fy;
         stage_force(a.forces, xs, ys, vxs, vys, qi, mi, a.part, a.slices, a.npad, i, a.confine,
                     a.gamma, a.heat, a.seed, sc->step, STAGE, a.orig[i], fx, fy);

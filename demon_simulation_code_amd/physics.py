@@ -59,3 +59,23 @@ PLAIN_1K = Workload("hex1024_fp64", n=1024, steps=1000)
 THERMAL_16K = Workload("hex16384_fp64", n=16384, steps=5000)
 LARGE_131K = Workload("hex131072_fp64", n=131072, steps=200)
 ADJOINT_4K = Workload("hex4096_fp64", n=4096, steps=200)
+
+
+def ensemble(clouds: int = 256, seed: int = 4, n_min: int = 64, n_max: int = 8192,
+             multiple_of: int = 1):
+    """The batched workload: per cloud N log-uniform in [n_min, n_max],
+    lambda uniform in [250, 1000] um, charge uniform in [3000, 9000] e,
+    k uniform in [2e-11, 1e-10] N/m, gamma = 2 /s.  Returns a list of dicts
+    (n, kappa, charge, confine, gamma)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(clouds):
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        n = max(n_min, (n + multiple_of - 1) // multiple_of * multiple_of)
+        lam = rng.uniform(250e-6, 1000e-6)
+        q = rng.uniform(3000.0, 9000.0) * ELECTRON_CHARGE
+        k = rng.uniform(2e-11, 1e-10)
+        out.append(dict(n=n, kappa=1.0 / lam, charge=q, confine=confine_from_k(k, q), gamma=2.0))
+    return out

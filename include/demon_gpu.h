@@ -137,6 +137,57 @@ int demon_get_pair_shape(const demon_ctx *ctx, int32_t *i_per_lane, int32_t *j_s
  * all-pairs kernels and everything else (HIP events on the context stream). */
 int demon_last_timing(const demon_ctx *ctx, double *pair_ms, double *total_ms);
 
+/* Differentiable trajectory fit (no counterpart in the reference, which has
+ * no differentiation; it differentiates Runge_Kutta4::moveParticles with
+ * Confinement, Drag and ShieldedCoulomb forces).
+ * From the uploaded state, runs `steps` RK4 steps of fixed dt (no dt check)
+ * and returns the loss
+ *   L = sum_{o=1..steps/obs_every} sum_i |r_i(o obs_every dt) - r*_{o,i}|^2 / (2 n n_obs)
+ * (target_x/target_y: [steps/obs_every][n] host arrays in caller order) and,
+ * by the discrete adjoint of the RK4 map with checkpoints every ckpt_every
+ * steps and recompute, its gradient with respect to the charges (n values),
+ * the shielding constant, the confinement constant, gamma and the initial
+ * state.  Gradient pointers may be NULL (all NULL: loss only).  The context's
+ * state is the initial state again on return.  FP64 only. */
+int demon_fit_loss_grad(demon_ctx *ctx, int64_t steps, double dt, int64_t obs_every,
+                        int64_t ckpt_every, const double *target_x, const double *target_y,
+                        double *loss, double *grad_q, double *grad_kappa, double *grad_confine,
+                        double *grad_gamma, double *grad_x0, double *grad_y0, double *grad_vx0,
+                        double *grad_vy0);
+
+/* Device milliseconds of the last demon_fit_loss_grad: forward sweep, and
+ * recompute + adjoint sweep. */
+int demon_fit_timing(const demon_ctx *ctx, double *forward_ms, double *backward_ms);
+
+/* ---- Batched ensemble: many independent clouds in one device arena ----
+ * Each cloud is its own Cloud + Runge_Kutta4 system (own kappa, C, gamma,
+ * dt check and time); all clouds step together: per stage one persistent
+ * work-queue all-pairs launch per tile shape and one update launch, a step
+ * captured as a HIP graph.  p supplies forces, precision, dt_check, heat_val,
+ * seed (cloud c uses seed + c * 0x9E3779B97F4A7C15) and device. */
+typedef struct demon_batch demon_batch;
+const char *demon_batch_last_error(const demon_batch *b);
+int demon_batch_create(demon_batch **b, int32_t nclouds, const int64_t *n, const double *kappa,
+                       const double *confine, const double *gamma, const demon_params *p);
+void demon_batch_destroy(demon_batch *b);
+int demon_batch_upload(demon_batch *b, int32_t cloud, const double *x, const double *y,
+                       const double *vx, const double *vy, const double *charge,
+                       const double *mass);
+int demon_batch_download(demon_batch *b, int32_t cloud, double *x, double *y, double *vx,
+                         double *vy);
+/* `steps` RK4 steps of every cloud (each with its own dt check). */
+int demon_batch_steps(demon_batch *b, int64_t steps, double init_dt);
+int demon_batch_cloud_stats(const demon_batch *b, int32_t cloud, demon_stats *s);
+int demon_batch_use_graph(demon_batch *b, int32_t enable);
+int demon_batch_last_ms(const demon_batch *b, double *ms);
+/* Tile shape of the clouds whose padded size falls in [2^bucket, 2^(bucket+1)). */
+int demon_batch_set_shape(demon_batch *b, int32_t bucket, int32_t i_per_lane, int32_t jslice);
+int demon_batch_get_shape(const demon_batch *b, int32_t bucket, int32_t *i_per_lane,
+                          int32_t *jslice, double *tuned_us);
+/* Autotuner: times every candidate shape per N bucket present and keeps the
+ * fastest (reps timed runs each, after one warm-up). */
+int demon_batch_tune(demon_batch *b, int32_t reps);
+
 /* Host-only helper (no device needed): the synthetic hexagonal-lattice input
  * used by the benchmarks -- the n sites of a triangular lattice of spacing
  * `spacing` nearest the origin (ordered by lattice radius, then row, then

@@ -87,6 +87,10 @@ class Oracle:
                                       ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_I64)]
         L.oracle_stage_force.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.POINTER(OracleParams),
                                          ctypes.c_uint64, ctypes.c_int, _D, _D]
+        L.oracle_fit_loss_grad.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, _I64,
+                                           _I64, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
+        L.oracle_fit_loss_grad.restype = ctypes.c_int
         self.coulomb = L.oracle_coulomb_const()
 
     # --- forces -----------------------------------------------------------
@@ -145,6 +149,24 @@ class Oracle:
         o = (ctypes.c_uint32 * 4)()
         self.L.oracle_philox4x32(c, k, o)
         return list(o)
+
+    def fit_loss_grad(self, x0, y0, vx0, vy0, q, m, kappa, confine, gamma, dt, steps, obs_every,
+                      tx, ty, state_grad=False):
+        """(loss, dq, dkappa, dconfine, dgamma[, (dx0, dy0, dvx0, dvy0)])"""
+        n = len(x0)
+        loss = np.zeros(1)
+        gq = np.zeros(n)
+        g3 = np.zeros(3)
+        gs = [np.zeros(n) for _ in range(4)] if state_grad else [None] * 4
+        tx = np.ascontiguousarray(tx, dtype=np.float64)
+        ty = np.ascontiguousarray(ty, dtype=np.float64)
+        rc = self.L.oracle_fit_loss_grad(n, p(_f64(x0)), p(_f64(y0)), p(_f64(vx0)), p(_f64(vy0)),
+                                         p(_f64(q)), p(_f64(m)), kappa, confine, gamma, dt, steps,
+                                         obs_every, p(tx), p(ty), p(loss), p(gq), p(g3[0:]),
+                                         p(g3[1:]), p(g3[2:]), *[p(a) for a in gs])
+        assert rc == 0
+        out = (float(loss[0]), gq, float(g3[0]), float(g3[1]), float(g3[2]))
+        return out + (tuple(gs),) if state_grad else out
 
     # --- dt check ---------------------------------------------------------
     def modify_time_step(self, x, y, dist=1.0e-4, dt=1.0e-3) -> float:

@@ -530,3 +530,170 @@ int oracle_rk4_move(int64_t n, double *x, double *y, double *vx, double *vy,
         *steps_done = c;
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Discrete adjoint of the RK4 map (CPU reference gradient)                 */
+/* ------------------------------------------------------------------------ */
+
+/* Stage force F(S) = q C r - gamma m v + sum_j phi_ij d_ij (gather, cross form). */
+static void fit_force(int64_t n, const double *xs, const double *ys, const double *vxs,
+                      const double *vys, const double *q, const double *m, double kappa,
+                      double confine, double gamma, double *fx, double *fy)
+{
+    const double coul = oracle_coulomb_const();
+    for (int64_t i = 0; i < n; ++i) {
+        double ax = q[i] * confine * xs[i] - gamma * m[i] * vxs[i];
+        double ay = q[i] * confine * ys[i] - gamma * m[i] * vys[i];
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i)
+                continue;
+            const double dX = xs[i] - xs[j], dY = ys[i] - ys[j];
+            const double r = sqrt(dX * dX + dY * dY);
+            const double u = r * kappa;
+            if (u < 10.0) {
+                const double phi = coul * (q[i] * q[j]) * (1.0 + u) * exp(-u) / (r * r * r);
+                ax += phi * dX;
+                ay += phi * dY;
+            }
+        }
+        fx[i] = ax;
+        fy[i] = ay;
+    }
+}
+
+int oracle_fit_loss_grad(int64_t n, const double *x0, const double *y0, const double *vx0,
+                         const double *vy0, const double *q, const double *m, double kappa,
+                         double confine, double gamma, double dt, int64_t steps,
+                         int64_t obs_every, const double *tx, const double *ty, double *loss,
+                         double *gq, double *gkappa, double *gconfine, double *ggamma,
+                         double *gx0, double *gy0, double *gvx0, double *gvy0)
+{
+    if (n < 2 || steps < 1 || obs_every < 1)
+        return -1;
+    const int64_t n_obs = steps / obs_every;
+    const double coul = oracle_coulomb_const();
+    /* S[step][stage][4][n]: x, y, vx, vy of each stage state */
+    const size_t per_stage = (size_t)4 * n;
+    double *S = (double *)malloc(sizeof(double) * per_stage * 4 * (size_t)steps);
+    double *st = (double *)malloc(sizeof(double) * per_stage * 2);     /* state, next */
+    double *F = (double *)malloc(sizeof(double) * 2 * n);
+    double *inc = (double *)malloc(sizeof(double) * 4 * 4 * n);         /* l,k,mm,nn per stage */
+    double *seed = (double *)calloc((size_t)(n_obs > 0 ? n_obs : 1) * 2 * n, sizeof(double));
+    double *bar = (double *)calloc((size_t)4 * n, sizeof(double));      /* adjoint of the state */
+    double *ibar = (double *)calloc((size_t)4 * 4 * n, sizeof(double)); /* adjoint of increments */
+    double *sbar = (double *)calloc((size_t)4 * n, sizeof(double));     /* adjoint of a stage state */
+    if (!S || !st || !F || !inc || !seed || !bar || !ibar || !sbar)
+        return -1;
+    memcpy(st, x0, sizeof(double) * n);
+    memcpy(st + n, y0, sizeof(double) * n);
+    memcpy(st + 2 * n, vx0, sizeof(double) * n);
+    memcpy(st + 3 * n, vy0, sizeof(double) * n);
+    double L = 0.0;
+    const double norm = 1.0 / (2.0 * (double)n * (double)(n_obs > 0 ? n_obs : 1));
+    for (int64_t s = 0; s < steps; ++s) {
+        double *Ss = S + per_stage * 4 * (size_t)s;
+        for (int k = 0; k < 4; ++k) {
+            double *Sk = Ss + per_stage * k;
+            const double c = (k == 3) ? 1.0 : 0.5;
+            for (int64_t i = 0; i < n; ++i)
+                for (int a = 0; a < 4; ++a)
+                    Sk[a * n + i] = k == 0 ? st[a * n + i] : st[a * n + i] + c * inc[((k - 1) * 4 + a) * n + i];
+            fit_force(n, Sk, Sk + n, Sk + 2 * n, Sk + 3 * n, q, m, kappa, confine, gamma, F, F + n);
+            for (int64_t i = 0; i < n; ++i) {
+                inc[(k * 4 + 0) * n + i] = dt * Sk[2 * n + i];    /* l  -> x  */
+                inc[(k * 4 + 1) * n + i] = dt * Sk[3 * n + i];    /* n  -> y  */
+                inc[(k * 4 + 2) * n + i] = dt * F[i] / m[i];      /* k  -> vx */
+                inc[(k * 4 + 3) * n + i] = dt * F[n + i] / m[i];  /* mm -> vy */
+            }
+        }
+        for (int64_t i = 0; i < n; ++i)
+            for (int a = 0; a < 4; ++a)
+                st[a * n + i] += rk_da(inc[(0 * 4 + a) * n + i], inc[(1 * 4 + a) * n + i],
+                                       inc[(2 * 4 + a) * n + i], inc[(3 * 4 + a) * n + i]);
+        if ((s + 1) % obs_every == 0 && (s + 1) / obs_every <= n_obs) {
+            const int64_t o = (s + 1) / obs_every - 1;
+            for (int64_t i = 0; i < n; ++i) {
+                const double ex = st[i] - tx[o * n + i], ey = st[n + i] - ty[o * n + i];
+                L += ex * ex + ey * ey;
+                seed[(o * 2) * n + i] = 2.0 * ex * norm;
+                seed[(o * 2 + 1) * n + i] = 2.0 * ey * norm;
+            }
+        }
+    }
+    *loss = L * norm;
+    for (int64_t i = 0; i < n; ++i)
+        gq[i] = 0.0;
+    double gk = 0.0, gc = 0.0, gg = 0.0;
+    const double wk[4] = {1.0 / 6.0, 2.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0};
+    for (int64_t s = steps - 1; s >= 0; --s) {
+        if ((s + 1) % obs_every == 0 && (s + 1) / obs_every <= n_obs) {
+            const int64_t o = (s + 1) / obs_every - 1;
+            for (int64_t i = 0; i < n; ++i) {
+                bar[i] += seed[(o * 2) * n + i];
+                bar[n + i] += seed[(o * 2 + 1) * n + i];
+            }
+        }
+        /* combine: x' = x + sum_k w_k inc_k */
+        for (int k = 0; k < 4; ++k)
+            for (int64_t e = 0; e < 4 * n; ++e)
+                ibar[k * 4 * n + e] = wk[k] * bar[e];
+        const double *Ss = S + per_stage * 4 * (size_t)s;
+        for (int k = 3; k >= 0; --k) {
+            const double *Sk = Ss + per_stage * k;
+            const double *xs = Sk, *ys = Sk + n, *vxs = Sk + 2 * n, *vys = Sk + 3 * n;
+            const double *lb = ibar + (k * 4 + 0) * n, *nb = ibar + (k * 4 + 1) * n;
+            const double *kb = ibar + (k * 4 + 2) * n, *mb = ibar + (k * 4 + 3) * n;
+            /* inc = dt f(S_k):  l = dt vx, k = dt F_x / m  ->  stage adjoint */
+            for (int64_t i = 0; i < n; ++i) {
+                const double ux = kb[i] / m[i], uy = mb[i] / m[i];
+                double rx = confine * q[i] * ux, ry = confine * q[i] * uy;
+                double dq = confine * (ux * xs[i] + uy * ys[i]);
+                double dk = 0.0;
+                for (int64_t j = 0; j < n; ++j) {
+                    if (j == i)
+                        continue;
+                    const double dX = xs[i] - xs[j], dY = ys[i] - ys[j];
+                    const double r = sqrt(dX * dX + dY * dY);
+                    const double u = r * kappa;
+                    if (!(u < 10.0))
+                        continue;
+                    const double e = exp(-u);
+                    const double g = coul * q[j] * (1.0 + u) * e / (r * r * r);   /* phi / q_i */
+                    const double phi = q[i] * g;
+                    const double dphi_r = -coul * q[i] * q[j] * e * (u * u + 3.0 * u + 3.0) / (r * r * r * r * r);
+                    const double dux = ux - kb[j] / m[j], duy = uy - mb[j] / m[j];
+                    const double dot = dX * dux + dY * duy;
+                    rx += phi * dux + dphi_r * dot * dX;
+                    ry += phi * duy + dphi_r * dot * dY;
+                    dq += g * dot;
+                    dk += (-coul * q[i] * q[j] * kappa * e / r) * (ux * dX + uy * dY);
+                }
+                sbar[i] = dt * rx;
+                sbar[n + i] = dt * ry;
+                sbar[2 * n + i] = dt * (lb[i] - gamma * kb[i]);
+                sbar[3 * n + i] = dt * (nb[i] - gamma * mb[i]);
+                gq[i] += dt * dq;
+                gk += dt * dk;
+                gc += dt * q[i] * (ux * xs[i] + uy * ys[i]);
+                gg += dt * (-m[i]) * (ux * vxs[i] + uy * vys[i]);
+            }
+            /* S_k = state + c_k inc_{k-1} */
+            for (int64_t e = 0; e < 4 * n; ++e)
+                bar[e] += sbar[e];
+            if (k > 0) {
+                const double c = (k == 3) ? 1.0 : 0.5;
+                for (int64_t e = 0; e < 4 * n; ++e)
+                    ibar[(k - 1) * 4 * n + e] += c * sbar[e];
+            }
+        }
+    }
+    *gkappa = gk;
+    *gconfine = gc;
+    *ggamma = gg;
+    if (gx0) memcpy(gx0, bar, sizeof(double) * n);
+    if (gy0) memcpy(gy0, bar + n, sizeof(double) * n);
+    if (gvx0) memcpy(gvx0, bar + 2 * n, sizeof(double) * n);
+    if (gvy0) memcpy(gvy0, bar + 3 * n, sizeof(double) * n);
+    free(S); free(st); free(F); free(inc); free(seed); free(bar); free(ibar); free(sbar);
+    return 0;
+}

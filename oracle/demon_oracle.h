@@ -135,6 +135,21 @@ int oracle_stage_force(int64_t n, const double *xs, const double *ys,
                        const double *m, const oracle_params *p, uint64_t step,
                        int stage, double *fx, double *fy);
 
+/* Trajectory-fit loss and its exact gradient (discrete adjoint of the RK4
+ * map of oracle_rk4_steps with Confinement + Drag + ShieldedCoulomb, fixed dt,
+ * gather-form Yukawa with the cross formula).  The reference has no
+ * differentiation; this is the CPU reference gradient for the device adjoint.
+ *   L = sum_{o=1..steps/obs_every} sum_i |r_i(o*obs_every*dt) - r*_{o,i}|^2 / (2 n n_obs)
+ * tx, ty: [n_obs][n] targets.  Outputs: *loss, dL/dq (n), dL/dkappa,
+ * dL/dconfine, dL/dgamma; dL/d(initial state) into gx0..gvy0 when non-NULL.
+ * Stores every stage state: memory 16 n steps doubles.  Returns 0 or -1. */
+int oracle_fit_loss_grad(int64_t n, const double *x0, const double *y0, const double *vx0,
+                         const double *vy0, const double *q, const double *m, double kappa,
+                         double confine, double gamma, double dt, int64_t steps,
+                         int64_t obs_every, const double *tx, const double *ty, double *loss,
+                         double *gq, double *gkappa, double *gconfine, double *ggamma,
+                         double *gx0, double *gy0, double *gvx0, double *gvy0);
+
 #ifdef __cplusplus
 }
 #endif
