@@ -164,7 +164,7 @@ int enqueue_pairs(demon_ctx *c, const StageBuf &st, int gated, int dtcheck) {
     a.xs = st.xs; a.ys = st.ys; a.nc = st.nc; a.q = c->q; a.orig = c->d_orig;
     a.part = c->d_part; a.sc = c->d_sc; a.n = c->n; a.npad = c->npad;
     a.jslice = c->jslice; a.slices = c->slices; a.i_per_lane = c->i_per_lane;
-    a.dtcheck = dtcheck; a.partner = 1; a.gated = gated;
+    a.dtcheck = dtcheck; a.gated = gated;
     a.fp32pair = c->p.precision == DEMON_PRECISION_FP32_PAIR;
     a.kappa = c->p.shielding;
     const double rc = 10.0 / c->p.shielding;
@@ -314,6 +314,8 @@ int restore_state(demon_ctx *c, const double *slot) {
         HIP_TRY(c, hipMemcpyAsync(rec[a], slot + (size_t)a * c->npad, b, hipMemcpyDeviceToDevice, c->stream));
     }
     HIP_TRY(c, launch_prepare_stage(c->sb[0], c->n, c->npad, c->stream));
+    HIP_TRY(c, hipMemsetAsync(&c->d_sc->rmax2_bits, 0, sizeof(unsigned long long), c->stream));
+    HIP_TRY(c, launch_rmax(c->sb[0], c->n, c->d_sc, c->stream));
     return DEMON_OK;
 }
 
@@ -655,7 +657,8 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     const size_t off_sd = off_tg + (size_t)n_obs * 2 * P, off_lp = off_sd + (size_t)n_obs * 2 * P;
     const size_t off_st = off_lp + P, off_bar = off_st + (size_t)ckpt_every * 20 * P;
     const size_t off_ib = off_bar + 4 * P, off_u = off_ib + 16 * P, off_g = off_u + 2 * P;
-    const size_t off_ap = off_g + 4 * P, total = off_ap + (size_t)c->slices * 4 * P;
+    const size_t off_ap = off_g + 4 * P, off_dt = off_ap + (size_t)c->slices * 4 * P;
+    const size_t total = off_dt + (size_t)steps;
     int rc = fit_arena(c, total);
     if (rc) return rc;
     double *A = c->d_fit;
@@ -688,7 +691,10 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     c->timing = false;
     for (int64_t s = 0; s < steps; ++s) {
         if (s % ckpt_every == 0 && (rc = save_state(c, A + off_ck + (size_t)(s / ckpt_every) * 4 * P))) break;
-        if ((rc = enqueue_step_ex(c, 0, nullptr))) break;
+        if ((rc = enqueue_step_ex(c, c->p.dt_check, nullptr))) break;
+        // the step's dt (after the dt check), replayed by the adjoint sweep
+        HIP_TRY(c, hipMemcpyAsync(A + off_dt + s, &c->d_sc->dt_step, sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->stream));
         if ((s + 1) % obs_every == 0 && (s + 1) / obs_every <= n_obs) {
             // the step's last update wrote the new state; close it before observing
             const int64_t o = (s + 1) / obs_every - 1;
@@ -701,6 +707,15 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     HIP_TRY(c, hipEventRecord(c->ev_fit[1], c->stream));
     std::vector<double> tmp;
     if (!rc) *loss = host_sum(c, A + off_lp, tmp, &rc) * norm;
+    std::vector<double> dts((size_t)steps);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(dts.data(), A + off_dt, sizeof(double) * steps, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, e, "hipMemcpy dt");
+        Scalars sc{};
+        if (!rc && hipMemcpy(&sc, c->d_sc, sizeof(sc), hipMemcpyDeviceToHost) == hipSuccess && sc.err == 2)
+            rc = fail(c, DEMON_ERR_STATE, "dt check: two tested particles coincide");
+    }
     if (!rc && want) {
         double *bar = A + off_bar, *ibar = A + off_ib, *u = A + off_u, *g = A + off_g, *ap = A + off_ap;
         HIP_TRY(c, hipMemsetAsync(bar, 0, sizeof(double) * 4 * P, c->stream));
@@ -712,7 +727,7 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
             hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, 0.0, inf, dt,
                                (unsigned long long)(s1 - s0));
             for (int64_t s = s0; s < s1 && !rc; ++s)
-                rc = enqueue_step_ex(c, 0, A + off_st + (size_t)(s - s0) * 20 * P);
+                rc = enqueue_step_ex(c, c->p.dt_check, A + off_st + (size_t)(s - s0) * 20 * P);
             if (rc) break;
             HIP_TRY(c, launch_boundary(c->d_sc, 1, c->stream));
             for (int64_t s = s1 - 1; s >= s0; --s) {
@@ -726,8 +741,8 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
                                                 c->jslice, c->slices, c->i_per_lane, c->p.shielding,
                                                 rc2p, c->stream));
                     HIP_TRY(c, launch_adj_stage(bar, ibar, u, g, k, st, c->q, c->m, ap, c->slices,
-                                                c->p.forces, dt, c->p.confine, c->p.drag_gamma, c->n,
-                                                c->npad, c->stream));
+                                                c->p.forces, dts[(size_t)s], c->p.confine,
+                                                c->p.drag_gamma, c->n, c->npad, c->stream));
                 }
             }
         }

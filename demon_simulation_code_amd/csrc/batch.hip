@@ -435,7 +435,7 @@ int enqueue_bstep(demon_batch *b) {
     BHIP(b, hipGetLastError());
     const dim3 grid((unsigned)(b->P / 256)), blk(256);
     for (int s = 1; s <= 4; ++s) {
-        int32_t flags = pair::kFlagGated | pair::kFlagPartner;
+        int32_t flags = pair::kFlagGated;
         if (s == 1 && b->dt_check) flags |= pair::kFlagDtCheck;
         if (b->forces & DEMON_FORCE_SHIELDED_COULOMB) {
             const int rc = enqueue_bpairs(b, b->sb[(s - 1) & 1], flags);
@@ -782,7 +782,7 @@ int demon_batch_tune(demon_batch *b, int32_t reps) {
                 for (int r = 0; r <= reps && !ok; ++r) {
                     hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * kShapes, b->stream);
                     hipEventRecord(b->ev[0], b->stream);
-                    ok = enqueue_bpairs(b, b->sb[0], pair::kFlagPartner, shape_index(R), d_it, (int32_t)items.size());
+                    ok = enqueue_bpairs(b, b->sb[0], 0, shape_index(R), d_it, (int32_t)items.size());
                     hipEventRecord(b->ev[1], b->stream);
                     if (hipStreamSynchronize(b->stream) != hipSuccess) ok = bfail(b, DEMON_ERR_HIP, "tune sync");
                     float ms = 0.0f;

@@ -52,7 +52,6 @@ struct PairLaunch {
     int32_t slices;
     int32_t i_per_lane;      // 1, 2 or 4
     int32_t dtcheck;         // fold the FP32 dt check into the near path
-    int32_t partner;         // intra-pair formula for (2k, 2k+1) in caller order
     int32_t gated;           // skip when sc->active == 0
     int32_t fp32pair;        // FP32 pair arithmetic
     double kappa, rc2p;      // shielding and (10/kappa)^2 (1 + kCutoffSlack)

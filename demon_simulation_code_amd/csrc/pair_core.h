@@ -9,7 +9,12 @@ namespace pair {
 
 constexpr int kChunk = 8;  // j particles per fast-path chunk
 
-enum : int32_t { kFlagDtCheck = 1, kFlagPartner = 2, kFlagGated = 4 };
+enum : int32_t { kFlagDtCheck = 1, kFlagGated = 4 };
+
+// Relative half-width of the band around the cutoff radius^2 inside which
+// the contribution test falls back to the reference's exact form
+// RN(sqrt(r2)) * kappa < 10 (ShieldedCoulombForce.cpp:175-178).
+constexpr double kBand = 1.0e-12;
 
 // Pairs tested by the reference's modifyTimeStep, in caller indices: all six
 // pairs inside a group of four, and cross-group pairs except j = i+1 (mod 4)
@@ -34,42 +39,71 @@ __device__ __forceinline__ void dtcheck_pair(double xi, double yi, double xj, do
     if (s <= 1.0001e-8f) atomicMin(dtmin, __float_as_uint(s));
 }
 
-// Exact FP64 pair term (i <- j); the candidate test has already passed.
+// Cutoff radius^2 (10/kappa)^2 of a launch and the band around it.
+struct Cutoff {
+    double lo2, hi2, kappa;
+};
+
+__device__ __forceinline__ Cutoff make_cutoff(double kappa, double rc2p) {
+    const double rc2 = rc2p / (1.0 + kCutoffSlack);
+    return Cutoff{rc2 * (1.0 - kBand), rc2 * (1.0 + kBand), kappa};
+}
+
+// Does the pair contribute, exactly as the reference decides?  Outside the
+// band r2 alone decides; inside it the reference's own rounding is replayed.
+__device__ __forceinline__ bool contributes(double r2, const Cutoff &c) {
+    if (r2 < c.lo2) return true;
+    if (r2 >= c.hi2) return false;
+    return sqrt(r2) * c.kappa < 10.0;
+}
+
+// 1/sqrt(r2) to about 1 ulp: the hardware estimate refined by two Newton
+// steps y <- y + y (1 - r2 y^2) / 2 (each doubles the correct bits).
+__device__ __forceinline__ double rsqrt_fp64(double r2) {
+    double y = __builtin_amdgcn_rsq(r2);
+    double e = fma(-r2 * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-r2 * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// FP64 pair term (i <- j) for a candidate pair:
+//   F_i += C q_i q_j (1 + u) e^-u / r^3 * (r_i - r_j),  u = kappa r,
+// the reference's cross-pair form (ShieldedCoulombForce.cpp:183-184).  The
+// distance is the reference's (dx = xi - xj, unfused dx*dx + dy*dy); r and
+// 1/r^3 come from one refined reciprocal square root instead of a square
+// root and a division.
 __device__ __forceinline__ void near_fp64(double xi, double yi, double qi, int32_t oi, double xj,
-                                          double yj, double qj, int32_t oj, double kappa,
+                                          double yj, double qj, int32_t oj, const Cutoff &cut,
                                           int32_t flags, unsigned int *dtmin, double &ax,
                                           double &ay) {
     const double dx = xi - xj;
     const double dy = yi - yj;
-    const double r = sqrt(dx * dx + dy * dy);  // unfused (-ffp-contract=off), IEEE sqrt
-    const double u = r * kappa;
-    if (u < 10.0) {
-        const double cq = kCoulomb * (qi * qj) * (1.0 + u);
-        double fc;
-        if ((flags & kFlagPartner) && oj == (oi ^ 1))
-            fc = cq / (r * r * r * exp(u));      // intra-pair form, ShieldedCoulombForce.cpp:149-150
-        else
-            fc = cq * exp(-u) / (r * r * r);     // cross form, :183-184
+    const double r2 = dx * dx + dy * dy;  // unfused (-ffp-contract=off)
+    if (contributes(r2, cut)) {
+        const double y = rsqrt_fp64(r2);
+        const double u = (r2 * y) * cut.kappa;
+        const double fc = kCoulomb * (qi * qj) * (1.0 + u) * exp(-u) * (y * y * y);
         ax = ax + fc * dx;
         ay = ay + fc * dy;
     }
     if (flags & kFlagDtCheck) dtcheck_pair(xi, yi, xj, yj, oi, oj, dtmin);
 }
 
-// FP32 pair term: distance, exp and force in FP32; the classification and the
-// accumulation stay FP64.
+// FP32 pair term: distance, exp and force in FP32; the contribution test and
+// the accumulation stay FP64.
 __device__ __forceinline__ void near_fp32(double xi, double yi, double qi, int32_t oi, double xj,
-                                          double yj, double qj, int32_t oj, double kappa,
-                                          double rc2, int32_t flags, unsigned int *dtmin,
-                                          double &ax, double &ay) {
+                                          double yj, double qj, int32_t oj, const Cutoff &cut,
+                                          int32_t flags, unsigned int *dtmin, double &ax,
+                                          double &ay) {
     const double dx = xi - xj;
     const double dy = yi - yj;
     const double r2 = dx * dx + dy * dy;
-    if (r2 < rc2) {
+    if (contributes(r2, cut)) {
         const float fxd = (float)dx, fyd = (float)dy;
         const float r2f = fxd * fxd + fyd * fyd;
         const float rinv = rsqrtf(r2f);
-        const float uf = r2f * rinv * (float)kappa;
+        const float uf = r2f * rinv * (float)cut.kappa;
         const float cqf = (float)(kCoulomb * (qi * qj));
         const float fc = cqf * (1.0f + uf) * expf(-uf) * (rinv * rinv * rinv);
         ax = ax + (double)(fc * fxd);
@@ -101,7 +135,7 @@ __device__ __forceinline__ void wave_tile(const double *__restrict__ xs, const d
                                           int32_t j_begin, int32_t j_end, int32_t flags, double kappa,
                                           double rc2p, double rf2, unsigned int *dtmin, double (&ax)[R],
                                           double (&ay)[R]) {
-    const double rc2 = rc2p / (1.0 + kCutoffSlack);  // FP32 path: r^2 < (10/kappa)^2
+    const Cutoff cut = make_cutoff(kappa, rc2p);
     double xi[R], yi[R], xm[R], ym[R], nT[R], qi[R];
     int32_t oi[R];
 #pragma unroll
@@ -115,6 +149,9 @@ __device__ __forceinline__ void wave_tile(const double *__restrict__ xs, const d
         qi[r] = q[i];
         oi[r] = orig[i];
     }
+    // the wave's own i range [wlo, whi): only chunks overlapping it can hold j == i
+    const int32_t wlo = ibase - (int32_t)(threadIdx.x & 63);
+    const int32_t whi = wlo + 64 * R;
     for (int32_t j0 = j_begin; j0 < j_end; j0 += kChunk) {
         double cx[kChunk], cy[kChunk], cn[kChunk];
 #pragma unroll
@@ -123,31 +160,31 @@ __device__ __forceinline__ void wave_tile(const double *__restrict__ xs, const d
             cy[u] = ys[j0 + u];
             cn[u] = nc[j0 + u];
         }
-        // wave-wide candidate mask of the chunk (each compare's lane mask is
-        // OR-ed on the scalar unit)
+        // candidate masks; their OR is formed on the scalar unit
+        bool cand[kChunk][R];
         uint64_t hit = 0;
 #pragma unroll
         for (int u = 0; u < kChunk; ++u) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const double w = fma(xm[r], cx[u], fma(ym[r], cy[u], nT[r]));
-                hit |= __ballot(w < cn[u]);
+                cand[u][r] = w < cn[u];
+                hit |= __ballot(cand[u][r]);
             }
         }
         if (hit) {
-#pragma unroll 1
+            const bool own = j0 < whi && j0 + kChunk > wlo;
+#pragma unroll
             for (int u = 0; u < kChunk; ++u) {
                 const int32_t j = j0 + u;
-                const double xj = xs[j], yj = ys[j], nj = nc[j];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const double w = fma(xm[r], xj, fma(ym[r], yj, nT[r]));
-                    if (w < nj && j != ibase + 64 * r) {
+                    if (cand[u][r] && (!own || j != ibase + 64 * r)) {
                         if (F32)
-                            near_fp32(xi[r], yi[r], qi[r], oi[r], xj, yj, q[j], orig[j], kappa,
-                                      rc2, flags, dtmin, ax[r], ay[r]);
+                            near_fp32(xi[r], yi[r], qi[r], oi[r], cx[u], cy[u], q[j], orig[j], cut,
+                                      flags, dtmin, ax[r], ay[r]);
                         else
-                            near_fp64(xi[r], yi[r], qi[r], oi[r], xj, yj, q[j], orig[j], kappa,
+                            near_fp64(xi[r], yi[r], qi[r], oi[r], cx[u], cy[u], q[j], orig[j], cut,
                                       flags, dtmin, ax[r], ay[r]);
                     }
                 }

@@ -72,7 +72,6 @@ hipError_t launch_shape(const PairLaunch &a, hipStream_t s) {
     const dim3 grid((unsigned)(a.npad / (256 * R)), (unsigned)a.slices);
     int32_t flags = 0;
     if (a.dtcheck) flags |= kFlagDtCheck;
-    if (a.partner) flags |= kFlagPartner;
     if (a.gated) flags |= kFlagGated;
     hipLaunchKernelGGL((k_pairs<R, F32>), grid, dim3(256), 0, s, a.xs, a.ys, a.nc, a.q, a.orig,
                        a.part, a.sc, a.npad, a.jslice, flags, a.kappa, a.rc2p);

@@ -140,8 +140,9 @@ int demon_last_timing(const demon_ctx *ctx, double *pair_ms, double *total_ms);
 /* Differentiable trajectory fit (no counterpart in the reference, which has
  * no differentiation; it differentiates Runge_Kutta4::moveParticles with
  * Confinement, Drag and ShieldedCoulomb forces).
- * From the uploaded state, runs `steps` RK4 steps of fixed dt (no dt check)
- * and returns the loss
+ * From the uploaded state, runs `steps` RK4 steps of dt (each reduced by
+ * the dt check when the context's dt_check is on, as moveParticles does;
+ * the adjoint replays every step's dt) and returns the loss
  *   L = sum_{o=1..steps/obs_every} sum_i |r_i(o obs_every dt) - r*_{o,i}|^2 / (2 n n_obs)
  * (target_x/target_y: [steps/obs_every][n] host arrays in caller order) and,
  * by the discrete adjoint of the RK4 map with checkpoints every ckpt_every

@@ -30,6 +30,7 @@ class OracleParams(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("forces", ctypes.c_int32),
         ("dt_check", ctypes.c_int32),
+        ("gather", ctypes.c_int32),
     ]
 
 
@@ -88,8 +89,9 @@ class Oracle:
         L.oracle_stage_force.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.POINTER(OracleParams),
                                          ctypes.c_uint64, ctypes.c_int, _D, _D]
         L.oracle_fit_loss_grad.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.c_double,
-                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, _I64,
-                                           _I64, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, _I64, _I64, _D, _D, _D, _D, _D, _D, _D,
+                                           _D, _D, _D, _D]
         L.oracle_fit_loss_grad.restype = ctypes.c_int
         self.coulomb = L.oracle_coulomb_const()
 
@@ -151,7 +153,7 @@ class Oracle:
         return list(o)
 
     def fit_loss_grad(self, x0, y0, vx0, vy0, q, m, kappa, confine, gamma, dt, steps, obs_every,
-                      tx, ty, state_grad=False):
+                      tx, ty, state_grad=False, dt_check=False):
         """(loss, dq, dkappa, dconfine, dgamma[, (dx0, dy0, dvx0, dvy0)])"""
         n = len(x0)
         loss = np.zeros(1)
@@ -161,7 +163,8 @@ class Oracle:
         tx = np.ascontiguousarray(tx, dtype=np.float64)
         ty = np.ascontiguousarray(ty, dtype=np.float64)
         rc = self.L.oracle_fit_loss_grad(n, p(_f64(x0)), p(_f64(y0)), p(_f64(vx0)), p(_f64(vy0)),
-                                         p(_f64(q)), p(_f64(m)), kappa, confine, gamma, dt, steps,
+                                         p(_f64(q)), p(_f64(m)), kappa, confine, gamma, dt,
+                                         int(bool(dt_check)), steps,
                                          obs_every, p(tx), p(ty), p(loss), p(gq), p(g3[0:]),
                                          p(g3[1:]), p(g3[2:]), *[p(a) for a in gs])
         assert rc == 0
@@ -214,6 +217,9 @@ class Reference:
         L.ref_yukawa_stage.argtypes = [ctypes.c_int, _I64, _D, _D, _D, ctypes.c_double, _D, _D, ctypes.c_int]
         L.ref_modify_time_step.argtypes = [_I64, _D, _D, ctypes.c_float, ctypes.c_double]
         L.ref_modify_time_step.restype = ctypes.c_double
+        L.ref_rk4_record.argtypes = [_I64, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                     _I64, _I64, _D, ctypes.c_int]
         L.ref_rk4.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double,
                               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                               ctypes.c_int, _I64, ctypes.c_double, _D, ctypes.c_int]
@@ -240,6 +246,15 @@ class Reference:
                             gamma, heat, forces, dt, 0, steps, 0.0, p(tt), threads)
         assert rc == 0
         return (*s, float(tt[0]))
+
+    def rk4_record(self, x, y, q, m, kappa, confine, gamma, heat, forces, dt, steps, every, threads=0):
+        """Snapshots [steps//every, 4, n] (x, y, vx, vy) of one integrator run."""
+        n = len(x)
+        rec = np.zeros((steps // every, 4, n))
+        rc = self.L.ref_rk4_record(n, p(_f64(x)), p(_f64(y)), p(_f64(q)), p(_f64(m)), kappa, confine,
+                                   gamma, heat, forces, dt, steps, every, p(rec), threads)
+        assert rc == 0
+        return rec
 
     def rk4_move(self, x, y, vx, vy, q, m, kappa, confine, gamma, heat, forces, dt, end_time,
                  t=0.0, threads=0):

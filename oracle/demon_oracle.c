@@ -100,6 +100,7 @@ void oracle_yukawa_gather_add(int64_t n, const double *xs, const double *ys,
                               const double *q, double kappa, double *fx, double *fy)
 {
     const double coul = oracle_coulomb_const();
+#pragma omp parallel for schedule(dynamic, 64)
     for (int64_t i = 0; i < n; ++i) {
         double ax = fx[i], ay = fy[i];
         for (int64_t j = 0; j < n; ++j) {
@@ -124,6 +125,7 @@ void oracle_yukawa_rows(int64_t n, const double *xs, const double *ys, const dou
                         double kappa, const int64_t *rows, int64_t nrows, double *fx, double *fy)
 {
     const double coul = oracle_coulomb_const();
+#pragma omp parallel for schedule(dynamic, 16)
     for (int64_t k = 0; k < nrows; ++k) {
         const int64_t i = rows[k];
         double ax = 0.0, ay = 0.0;
@@ -391,7 +393,7 @@ int oracle_stage_force(int64_t n, const double *xs, const double *ys,
     if (p->forces & ORACLE_DRAG)
         oracle_drag_add(n, vxs, vys, m, p->gamma, fx, fy);
     if (p->forces & ORACLE_SHIELDED_COULOMB) {
-        if (n & 1)
+        if ((n & 1) || p->gather)
             oracle_yukawa_gather_add(n, xs, ys, q, p->shielding, fx, fy);
         else if (oracle_yukawa_add(n, xs, ys, q, p->shielding, fx, fy))
             return -1;
@@ -536,11 +538,21 @@ int oracle_rk4_move(int64_t n, double *x, double *y, double *vx, double *vy,
 /* ------------------------------------------------------------------------ */
 
 /* Stage force F(S) = q C r - gamma m v + sum_j phi_ij d_ij (gather, cross form). */
+/* r2 beyond which sqrt(r2) * kappa < 10 is certainly false: the skip only
+ * saves the square root and leaves the set of contributing pairs unchanged
+ * (the rounding of sqrt and of the product is 2^-52 relative at most). */
+static double fit_far2(double kappa)
+{
+    return (10.0 / kappa) * (10.0 / kappa) * (1.0 + 1e-9);
+}
+
 static void fit_force(int64_t n, const double *xs, const double *ys, const double *vxs,
                       const double *vys, const double *q, const double *m, double kappa,
                       double confine, double gamma, double *fx, double *fy)
 {
     const double coul = oracle_coulomb_const();
+    const double far2 = fit_far2(kappa);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         double ax = q[i] * confine * xs[i] - gamma * m[i] * vxs[i];
         double ay = q[i] * confine * ys[i] - gamma * m[i] * vys[i];
@@ -548,6 +560,8 @@ static void fit_force(int64_t n, const double *xs, const double *ys, const doubl
             if (j == i)
                 continue;
             const double dX = xs[i] - xs[j], dY = ys[i] - ys[j];
+            if (dX * dX + dY * dY > far2)
+                continue;
             const double r = sqrt(dX * dX + dY * dY);
             const double u = r * kappa;
             if (u < 10.0) {
@@ -563,12 +577,12 @@ static void fit_force(int64_t n, const double *xs, const double *ys, const doubl
 
 int oracle_fit_loss_grad(int64_t n, const double *x0, const double *y0, const double *vx0,
                          const double *vy0, const double *q, const double *m, double kappa,
-                         double confine, double gamma, double dt, int64_t steps,
+                         double confine, double gamma, double dt0, int dt_check, int64_t steps,
                          int64_t obs_every, const double *tx, const double *ty, double *loss,
                          double *gq, double *gkappa, double *gconfine, double *ggamma,
                          double *gx0, double *gy0, double *gvx0, double *gvy0)
 {
-    if (n < 2 || steps < 1 || obs_every < 1)
+    if (n < 2 || steps < 1 || obs_every < 1 || (dt_check && (n & 3)))
         return -1;
     const int64_t n_obs = steps / obs_every;
     const double coul = oracle_coulomb_const();
@@ -582,7 +596,8 @@ int oracle_fit_loss_grad(int64_t n, const double *x0, const double *y0, const do
     double *bar = (double *)calloc((size_t)4 * n, sizeof(double));      /* adjoint of the state */
     double *ibar = (double *)calloc((size_t)4 * 4 * n, sizeof(double)); /* adjoint of increments */
     double *sbar = (double *)calloc((size_t)4 * n, sizeof(double));     /* adjoint of a stage state */
-    if (!S || !st || !F || !inc || !seed || !bar || !ibar || !sbar)
+    double *dts = (double *)malloc(sizeof(double) * (size_t)steps);     /* each step's dt */
+    if (!S || !st || !F || !inc || !seed || !bar || !ibar || !sbar || !dts)
         return -1;
     memcpy(st, x0, sizeof(double) * n);
     memcpy(st + n, y0, sizeof(double) * n);
@@ -592,6 +607,10 @@ int oracle_fit_loss_grad(int64_t n, const double *x0, const double *y0, const do
     const double norm = 1.0 / (2.0 * (double)n * (double)(n_obs > 0 ? n_obs : 1));
     for (int64_t s = 0; s < steps; ++s) {
         double *Ss = S + per_stage * 4 * (size_t)s;
+        /* the step's dt as Runge_Kutta4::moveParticles picks it
+         * (Runge_Kutta4.cpp:25, Integrator.cpp:68-158) */
+        const double dt = dt_check ? oracle_modify_time_step(n, st, st + n, 1.0e-4f, dt0) : dt0;
+        dts[s] = dt;
         for (int k = 0; k < 4; ++k) {
             double *Sk = Ss + per_stage * k;
             const double c = (k == 3) ? 1.0 : 0.5;
@@ -624,6 +643,7 @@ int oracle_fit_loss_grad(int64_t n, const double *x0, const double *y0, const do
     for (int64_t i = 0; i < n; ++i)
         gq[i] = 0.0;
     double gk = 0.0, gc = 0.0, gg = 0.0;
+    const double far2 = fit_far2(kappa);
     const double wk[4] = {1.0 / 6.0, 2.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0};
     for (int64_t s = steps - 1; s >= 0; --s) {
         if ((s + 1) % obs_every == 0 && (s + 1) / obs_every <= n_obs) {
@@ -638,12 +658,14 @@ int oracle_fit_loss_grad(int64_t n, const double *x0, const double *y0, const do
             for (int64_t e = 0; e < 4 * n; ++e)
                 ibar[k * 4 * n + e] = wk[k] * bar[e];
         const double *Ss = S + per_stage * 4 * (size_t)s;
+        const double dt = dts[s];
         for (int k = 3; k >= 0; --k) {
             const double *Sk = Ss + per_stage * k;
             const double *xs = Sk, *ys = Sk + n, *vxs = Sk + 2 * n, *vys = Sk + 3 * n;
             const double *lb = ibar + (k * 4 + 0) * n, *nb = ibar + (k * 4 + 1) * n;
             const double *kb = ibar + (k * 4 + 2) * n, *mb = ibar + (k * 4 + 3) * n;
             /* inc = dt f(S_k):  l = dt vx, k = dt F_x / m  ->  stage adjoint */
+#pragma omp parallel for schedule(static) reduction(+ : gk, gc, gg)
             for (int64_t i = 0; i < n; ++i) {
                 const double ux = kb[i] / m[i], uy = mb[i] / m[i];
                 double rx = confine * q[i] * ux, ry = confine * q[i] * uy;
@@ -653,6 +675,8 @@ int oracle_fit_loss_grad(int64_t n, const double *x0, const double *y0, const do
                     if (j == i)
                         continue;
                     const double dX = xs[i] - xs[j], dY = ys[i] - ys[j];
+                    if (dX * dX + dY * dY > far2)
+                        continue;
                     const double r = sqrt(dX * dX + dY * dY);
                     const double u = r * kappa;
                     if (!(u < 10.0))
@@ -694,6 +718,6 @@ int oracle_fit_loss_grad(int64_t n, const double *x0, const double *y0, const do
     if (gy0) memcpy(gy0, bar + n, sizeof(double) * n);
     if (gvx0) memcpy(gvx0, bar + 2 * n, sizeof(double) * n);
     if (gvy0) memcpy(gvy0, bar + 3 * n, sizeof(double) * n);
-    free(S); free(st); free(F); free(inc); free(seed); free(bar); free(ibar); free(sbar);
+    free(S); free(st); free(F); free(inc); free(seed); free(bar); free(ibar); free(sbar); free(dts);
     return 0;
 }

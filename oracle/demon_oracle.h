@@ -57,6 +57,8 @@ typedef struct oracle_params {
     uint64_t seed;      /* key of the counter-based thermal RNG */
     int32_t forces;     /* bitmask of ORACLE_* flags */
     int32_t dt_check;   /* 1: run Integrator::modifyTimeStep each step */
+    int32_t gather;     /* 1: Yukawa in gather form (j ascending per i), OpenMP over i;
+                           deterministic, differs from the reference order by rounding */
 } oracle_params;
 
 /* ShieldedCoulombForce::forceS (ShieldedCoulombForce.cpp:37-237), serial
@@ -136,8 +138,11 @@ int oracle_stage_force(int64_t n, const double *xs, const double *ys,
                        int stage, double *fx, double *fy);
 
 /* Trajectory-fit loss and its exact gradient (discrete adjoint of the RK4
- * map of oracle_rk4_steps with Confinement + Drag + ShieldedCoulomb, fixed dt,
- * gather-form Yukawa with the cross formula).  The reference has no
+ * map of oracle_rk4_steps with Confinement + Drag + ShieldedCoulomb,
+ * gather-form Yukawa with the cross formula).  With dt_check each step's dt
+ * is dt0 reduced by oracle_modify_time_step on the step's start positions
+ * (n a multiple of 4), and the adjoint replays those dts; dt is piecewise
+ * constant in the state, so it carries no gradient.  The reference has no
  * differentiation; this is the CPU reference gradient for the device adjoint.
  *   L = sum_{o=1..steps/obs_every} sum_i |r_i(o*obs_every*dt) - r*_{o,i}|^2 / (2 n n_obs)
  * tx, ty: [n_obs][n] targets.  Outputs: *loss, dL/dq (n), dL/dkappa,
@@ -145,7 +150,7 @@ int oracle_stage_force(int64_t n, const double *xs, const double *ys,
  * Stores every stage state: memory 16 n steps doubles.  Returns 0 or -1. */
 int oracle_fit_loss_grad(int64_t n, const double *x0, const double *y0, const double *vx0,
                          const double *vy0, const double *q, const double *m, double kappa,
-                         double confine, double gamma, double dt, int64_t steps,
+                         double confine, double gamma, double dt0, int dt_check, int64_t steps,
                          int64_t obs_every, const double *tx, const double *ty, double *loss,
                          double *gq, double *gkappa, double *gconfine, double *ggamma,
                          double *gx0, double *gy0, double *gvx0, double *gvy0);

@@ -158,4 +158,39 @@ int ref_rk4(int64_t n, double *x, double *y, double *vx, double *vy, const doubl
     return 0;
 }
 
+// One Runge_Kutta4 instance (so one RNG stream) run for `steps` steps,
+// recording x, y, Vx, Vy every `every` steps into rec[k][4][n] (k = 0.. steps/every-1).
+int ref_rk4_record(int64_t n, const double *x, const double *y, const double *q, const double *m,
+                   double kappa, double confine, double gamma, double heat, int forces, double dt,
+                   int64_t steps, int64_t every, double *rec, int nthreads) {
+    if (n < 4 || (n & 3) || every < 1) return -1;
+    Cloud *c = make_cloud(n, x, y, nullptr, nullptr, q, m);
+    set_threads(nthreads);
+    ForceArray fa;
+    if (forces & ConfinementForceFlag) fa.push_back(new ConfinementForce(c, confine));
+    if (forces & DragForceFlag) fa.push_back(new DragForce(c, gamma));
+    if (forces & ShieldedCoulombForceFlag) fa.push_back(new ShieldedCoulombForce(c, kappa));
+    if (forces & ThermalForceFlag) fa.push_back(new ThermalForce(c, heat));
+    {
+        Runge_Kutta4 I(c, fa, dt, 0.0);
+        int64_t k = 0;
+        for (int64_t s = 1; s <= steps; ++s) {
+            I.moveParticles(std::nextafter(I.currentTime, INFINITY));
+            if (s % every == 0) {
+                double *r = rec + (size_t)k * 4 * n;
+                for (int64_t i = 0; i < n; ++i) {
+                    r[i] = c->x[i];
+                    r[n + i] = c->y[i];
+                    r[2 * n + i] = c->Vx[i];
+                    r[3 * n + i] = c->Vy[i];
+                }
+                ++k;
+            }
+        }
+    }
+    for (Force *f : fa) delete f;
+    delete c;
+    return 0;
+}
+
 }  // extern "C"

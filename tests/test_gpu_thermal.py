@@ -1,0 +1,93 @@
+"""Thermal workload (N = 16384 in the benchmark configuration).
+
+The reference's ThermalForce draws from a wall-clock-seeded mt19937_64
+(RandomNumbers.cpp:19-21), so runs cannot be compared draw for draw; the
+device keys Philox4x32-10 by (seed, particle, step, stage) and keeps the
+reference's model (uniform magnitude, uniform angle, per stage,
+ThermalForce.cpp:118-122).  Validation is therefore statistical: kinetic
+temperature and the pair-distance distribution against the reference's own
+code, plus the deterministic thermal-off run at full size.
+
+heatVal for 300 K: per step the thermal velocity kick has variance
+(dt/m)^2 (10/36) heatVal^2 / 6 per component, and with drag the stationary
+<v_x^2> = Var / (2 gamma dt) = k_B T / m, so heatVal = sqrt(43.2 gamma k_B T m / dt).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import demon_simulation_code_amd as dm
+from oracle.bind import OracleParams, Reference
+from tests.helpers import CONFINE, GAMMA, KAPPA, M0, Q0, lattice, pos_rel_err
+
+pytestmark = pytest.mark.gpu
+
+KB = 1.380649e-23
+T_TARGET = 300.0
+DT = 1e-3
+HEAT = math.sqrt(43.2 * GAMMA * KB * T_TARGET * M0 / DT)
+FORCES = dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB
+
+
+def kinetic_T(vx, vy):
+    return M0 * (np.var(vx) + np.var(vy)) / (2 * KB)
+
+
+def pair_hist(x, y, rmax=3e-3, bins=60):
+    d = np.hypot(x[:, None] - x[None, :], y[:, None] - y[None, :])
+    h, _ = np.histogram(d[np.triu_indices(len(x), 1)], bins=bins, range=(0, rmax))
+    return h / h.sum()
+
+
+def test_deterministic_16k_100_steps(oracle):
+    n, steps = 16384, 100
+    x, y = lattice(n, seed=2)
+    q, m, z = np.full(n, Q0), np.full(n, M0), np.zeros(n)
+    with dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, forces=FORCES) as c:
+        c.upload(x, y, z, z, q, m)
+        t = c.rk4_steps(steps, DT, 0.0)
+        g = c.download()
+    o = oracle.rk4_steps(x, y, z, z, q, m, OracleParams(KAPPA, CONFINE, GAMMA, 0.0, 0, 7, 1, 1), DT, steps)
+    assert t == o[4]
+    assert pos_rel_err(o[0], o[1], g[0], g[1]) <= 1e-9
+
+
+def _gpu_snapshots(n, steps, every, seed):
+    x, y = lattice(n, seed=seed)
+    q, m = np.full(n, Q0), np.full(n, M0)
+    out = []
+    with dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, heat=HEAT, seed=seed,
+                    forces=FORCES | dm.THERMAL) as c:
+        c.upload(x, y, None, None, q, m)
+        t = 0.0
+        for _ in range(steps // every):
+            t = c.rk4_steps(every, DT, t)
+            out.append(c.download())
+    return x, y, q, m, out
+
+
+@pytest.mark.skipif(not Reference.available(openmp=True), reason="oracle/_ref not built")
+def test_thermal_statistics_vs_reference():
+    n, steps, every = 1024, 3000, 100
+    x, y, q, m, gs = _gpu_snapshots(n, steps, every, seed=3)
+    rec = Reference(openmp=True).rk4_record(x, y, q, m, KAPPA, CONFINE, GAMMA, HEAT, 7 | 16, DT, steps, every)
+    late = slice(len(gs) // 2, None)
+    Tg = np.mean([kinetic_T(s[2], s[3]) for s in gs[late]])
+    Tr = np.mean([kinetic_T(r[2], r[3]) for r in rec[late]])
+    hg = np.mean([pair_hist(s[0], s[1]) for s in gs[late]], axis=0)
+    hr = np.mean([pair_hist(r[0], r[1]) for r in rec[late]], axis=0)
+    l1 = float(np.abs(hg - hr).sum())
+    print(f"N={n}: T_gpu={Tg:.1f} K, T_ref={Tr:.1f} K, g(r) L1 distance {l1:.4f}")
+    assert abs(Tg - Tr) / Tr < 0.10
+    assert l1 < 0.05
+
+
+def test_thermal_16k_temperature():
+    n, steps, every = 16384, 5000, 250
+    _, _, _, _, gs = _gpu_snapshots(n, steps, every, seed=4)
+    T = [kinetic_T(s[2], s[3]) for s in gs]
+    Tlate = float(np.mean(T[len(T) // 2:]))
+    print(f"N={n}, 5000 steps: kinetic T over the second half {Tlate:.1f} K (target {T_TARGET} K)")
+    assert all(np.isfinite(T))
+    assert abs(Tlate - T_TARGET) / T_TARGET < 0.15
