@@ -12,13 +12,15 @@ Prints ONE JSON line:
   value        pair interactions/s = 4 N (N-1) steps / t   (ordered pairs)
   rk4_steps_per_s, ms_per_step
   roofline     the all-pairs kernel: algorithmic FLOP per launch
-               (7 per far ordered pair, 20 per near one) / its mean launch
-               time (HIP events around every launch on the kernel's stream),
-               against the FP64 vector peak of 78.6 TFLOP/s
+               (7 per far ordered pair, 20 per near one; the near count is
+               exact, counted in the kernel) / its mean launch time (HIP
+               events around every launch on the kernel's stream), against
+               the FP64 vector peak of 78.6 TFLOP/s
   fp32_pair    the same run with FP32 pair arithmetic (speed and error)
   cpu_baseline the reference's own SSE/OpenMP code (compiled from its
                sources into oracle/_ref) timed on the host cores on a bounded
-               sample (1 warm-up + a few timed steps of the same workload)
+               sample: one force evaluation at each of four states spread
+               over the same trajectory
 """
 from __future__ import annotations
 
@@ -42,15 +44,6 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def near_ordered_pairs(x, y, kappa):
-    """Ordered pairs (i != j) with r * kappa < 10 (host pass on a snapshot)."""
-    from scipy.spatial import cKDTree
-
-    t = cKDTree(np.c_[x, y])
-    r = 10.0 / kappa
-    return int(t.count_neighbors(t, r * (1 - 1e-12))) - len(x)
-
-
 def setup(wl):
     import demon_simulation_code_amd as dm
 
@@ -60,7 +53,10 @@ def setup(wl):
     return x, y, q, m
 
 
-def run_gpu(wl, steps, warmup, precision, shape=None):
+def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=()):
+    """Warm up on the workload, re-upload the initial state, then time exactly
+    `steps` RK4 steps from it (one demon_rk4_steps call).  With `snapshots`
+    (step numbers), a second, untimed pass records the state at those steps."""
     import demon_simulation_code_amd as dm
 
     x, y, q, m = setup(wl)
@@ -68,80 +64,74 @@ def run_gpu(wl, steps, warmup, precision, shape=None):
     forces = dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB
     c = dm.Context(n, shielding=wl.kappa, confine=wl.confine, gamma=wl.gamma, forces=forces,
                    precision=precision)
-    c.upload(x, y, None, None, q, m)
     if shape:
         c.set_pair_shape(*shape)
-    t = 0.0
+    c.upload(x, y, None, None, q, m)
     if warmup:
-        t = c.rk4_steps(warmup, wl.dt, t)
-    xs0, ys0, _, _ = c.download()
+        c.rk4_steps(warmup, wl.dt, 0.0)
+    c.upload(x, y, None, None, q, m)
     c.synchronize()
     t0 = time.perf_counter()
-    t = c.rk4_steps(steps, wl.dt, t)
+    t = c.rk4_steps(steps, wl.dt, 0.0)
     c.synchronize()
     wall = time.perf_counter() - t0
     pair_ms, total_ms = c.last_timing()
+    near, tested = c.pair_counts()
     xs1, ys1, vx1, vy1 = c.download()
     st = c.stats()
     shape_used = c.pair_shape()
+    snaps = []
+    if snapshots:
+        c.upload(x, y, None, None, q, m)
+        done, tt = 0, 0.0
+        for s in sorted(snapshots):
+            if s > done:
+                tt = c.rk4_steps(s - done, wl.dt, tt)
+                done = s
+            snaps.append(c.download()[:2])
     c.close()
-    return dict(wall=wall, pair_ms=pair_ms, total_ms=total_ms, t=t, stats=st,
-                snap0=(xs0, ys0), final=(xs1, ys1, vx1, vy1), shape=shape_used)
+    return dict(wall=wall, pair_ms=pair_ms, total_ms=total_ms, t=t, stats=st, near=near,
+                tested=tested, final=(xs1, ys1, vx1, vy1), shape=shape_used, snaps=snaps)
 
 
-def cpu_baseline(wl, budget_s=45.0):
-    """The reference's own code (OpenMP build) on the host cores, bounded sample:
-    one ShieldedCoulombForce::force1 evaluation as a probe, then one full RK4
-    step (dt check, 4 force evaluations, O(N) stages) when that fits the budget."""
-    from oracle.bind import Oracle, OracleParams, Reference
+def cpu_baseline(wl, snaps, snap_steps):
+    """The reference's own code (OpenMP build) on the host cores, on a bounded
+    sample of the same workload: one ShieldedCoulombForce::force1 evaluation
+    at each snapshot state of the FP64 trajectory.  The per-evaluation cost
+    grows as the cloud contracts (more pairs inside the cutoff need the exp
+    and the division), so the sample spans the run."""
+    from oracle.bind import Oracle, Reference, default_threads
 
     n = wl.n
-    x, y, q, m = setup(wl)
-    z = np.zeros(n)
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        threads = os.cpu_count() or 1
-    threads = max(1, min(threads, 16))
+    _, _, q, _ = setup(wl)
     if Reference.available(openmp=True):
-        ref = Reference(openmp=True)
-        kind = "reference"
+        ref, kind, threads = Reference(openmp=True), "reference", default_threads()
 
-        def evaluate():
+        def evaluate(x, y):
             ref.yukawa(x, y, q, wl.kappa, 1, threads=threads)
-
-        def step():
-            ref.rk4_steps(x, y, z, z, q, m, wl.kappa, wl.confine, wl.gamma, 0.0, 7, wl.dt, 1,
-                          threads=threads)
     else:
-        orc = Oracle()
-        kind = "port"
-        threads = 1
-        op = OracleParams(wl.kappa, wl.confine, wl.gamma, 0.0, 0, 7, 1)
+        orc, kind, threads = Oracle(), "port", 1
 
-        def evaluate():
+        def evaluate(x, y):
             orc.yukawa(x, y, q, wl.kappa)
 
-        def step():
-            orc.rk4_steps(x, y, z, z, q, m, op, wl.dt, 1)
-
-    t0 = time.perf_counter()
-    evaluate()
-    t_eval = time.perf_counter() - t0
-    out = {"unit": "pair interactions/s", "cores": threads, "kind": kind}
-    if 4.5 * t_eval < budget_s:
+    times = []
+    for (x, y), s in zip(snaps, snap_steps):
         t0 = time.perf_counter()
-        step()
-        t_step = time.perf_counter() - t0
-        out["value"] = 4.0 * n * (n - 1) / t_step
-        out["rk4_steps_per_s"] = 1.0 / t_step
-        out["sample"] = (f"1 RK4 step of the N={n} FP64 workload ({t_step:.2f} s) after one "
-                         f"force-evaluation probe ({t_eval:.2f} s)")
-    else:
-        out["value"] = 1.0 * n * (n - 1) / t_eval
-        out["rk4_steps_per_s"] = None
-        out["sample"] = f"1 ShieldedCoulombForce::force1 evaluation at N={n} ({t_eval:.2f} s)"
-    return out
+        evaluate(x, y)
+        times.append(time.perf_counter() - t0)
+        log(f"CPU force1 at step {s}: {times[-1]:.2f} s")
+    t_eval = sum(times) / len(times)
+    return {
+        "value": n * (n - 1) / t_eval,
+        "unit": "pair interactions/s",
+        "cores": threads,
+        "kind": kind,
+        "rk4_steps_per_s": 1.0 / (4.0 * t_eval),
+        "sample": (f"ShieldedCoulombForce::force1 at N={n} on the FP64 trajectory's states after "
+                   f"steps {list(snap_steps)}: " + ", ".join(f"{t:.2f} s" for t in times)
+                   + "; RK4 steps/s counts 4 evaluations per step"),
+    }
 
 
 def main():
@@ -166,20 +156,20 @@ def main():
     shape = tuple(int(v) for v in args.shape.split(",")) if args.shape else None
     n, K, W = wl.n, args.steps, args.warmup
 
+    snap_steps = [K // 8, 3 * K // 8, 5 * K // 8, 7 * K // 8] if not args.no_cpu else []
     log(f"FP64 run: N={n} warmup={W} steps={K}")
-    r = run_gpu(wl, K, W, dm.FP64, shape)
+    r = run_gpu(wl, K, W, dm.FP64, shape, snapshots=snap_steps)
     pairs_total = 4.0 * n * (n - 1) * K
     value = pairs_total / r["wall"]
     log(f"FP64 done: {r['wall']:.3f} s, {value:.4g} pair interactions/s, shape={r['shape']}")
 
-    near0 = near_ordered_pairs(*r["snap0"], wl.kappa)
-    near1 = near_ordered_pairs(r["final"][0], r["final"][1], wl.kappa)
-    near = 0.5 * (near0 + near1)
     ordered = n * (n - 1)
-    flop_launch = FLOP_FAR * (ordered - near) + FLOP_NEAR * near
     launches = 4 * K
-    mean_launch_s = r["pair_ms"] / 1e3 / launches
-    achieved = flop_launch / mean_launch_s
+    near = r["near"]  # exact count over the timed launches (in-kernel)
+    flop_total = FLOP_FAR * (launches * ordered - near) + FLOP_NEAR * near
+    pair_s = r["pair_ms"] / 1e3
+    achieved = flop_total / pair_s
+    R = r["shape"][0]
     roofline = {
         "bound": "valu",
         "achieved": achieved / 1e12,
@@ -187,11 +177,13 @@ def main():
         "unit": "TFLOP/s",
         "frac": achieved / FP64_PEAK,
         "traffic": None,
-        "kernel": "k_pairs<2,false> (FP64 all-pairs)",
-        "mean_launch_ms": mean_launch_s * 1e3,
-        "flop_per_launch": flop_launch,
-        "near_fraction": near / ordered,
+        "kernel": f"k_pairs<{R},false> (FP64 all-pairs)",
+        "mean_launch_ms": pair_s * 1e3 / launches,
+        "flop_per_launch": flop_total / launches,
+        "near_fraction": near / (launches * ordered),
+        "tested_fraction": r["tested"] / (launches * ordered),
         "pair_kernel_share": r["pair_ms"] / r["total_ms"] if r["total_ms"] else None,
+        "near_pair_rate": near / pair_s,
     }
 
     out = {
@@ -236,7 +228,7 @@ def main():
             "rk4_steps_per_s": K / r32["wall"],
             "speedup_vs_fp64": r["wall"] / r32["wall"],
             "pos_rel_err_vs_fp64_after_steps": perr,
-            "steps": K + W,
+            "steps": K,
             "mean_launch_ms": r32["pair_ms"] / launches,
         }
         log(f"FP32-pair done: {r32['wall']:.3f} s")
@@ -244,7 +236,7 @@ def main():
     if not args.no_cpu:
         log("CPU baseline (reference code on host cores)")
         try:
-            out["cpu_baseline"] = cpu_baseline(wl)
+            out["cpu_baseline"] = cpu_baseline(wl, r["snaps"], snap_steps)
             out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
         except Exception as e:  # the GPU result stands without it
             out["cpu_baseline"] = {"error": repr(e)}

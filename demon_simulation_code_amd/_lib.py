@@ -89,6 +89,7 @@ SIGNATURES = {
     "demon_set_pair_shape": (ctypes.c_int, [_CTX, _I32, _I32]),
     "demon_get_pair_shape": (ctypes.c_int, [_CTX, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "demon_last_timing": (ctypes.c_int, [_CTX, _D, _D]),
+    "demon_pair_counts": (ctypes.c_int, [_CTX, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "demon_hex_lattice": (ctypes.c_int, [_I64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _D, _D]),
     "demon_fit_loss_grad": (ctypes.c_int, [_CTX, _I64, ctypes.c_double, _I64, _I64, _D, _D, _D, _D, _D,
                                            _D, _D, _D, _D, _D, _D]),
@@ -266,6 +267,12 @@ class Context:
     def last_timing(self):
         a, b = ctypes.c_double(0), ctypes.c_double(0)
         self._c(lib.demon_last_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def pair_counts(self):
+        """(ordered pairs inside the cutoff, ordered pairs distance-tested) since upload."""
+        a, b = _I64(0), _I64(0)
+        self._c(lib.demon_pair_counts(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
 

@@ -31,7 +31,7 @@ struct demon_ctx {
     demon_params p{};
     int64_t n = 0, npad = 0;
     hipStream_t stream = nullptr;
-    double *d_arena = nullptr;   // 31 arrays of npad doubles
+    double *d_arena = nullptr;   // 32 arrays of npad doubles
     double *d_part = nullptr;    // kMaxSlices * 2 * npad
     int32_t *d_orig = nullptr;
     Scalars *d_sc = nullptr;
@@ -41,7 +41,7 @@ struct demon_ctx {
     std::vector<int64_t> perm;   // perm[k] = caller index stored in device slot k
     std::vector<double> h_tmp;   // host staging (npad * 4)
     bool uploaded = false;
-    int32_t i_per_lane = 2;
+    int32_t i_per_lane = 1;
     int32_t slices_req = 0;      // 0 = automatic
     int32_t slices = 1, jslice = 0;
     bool use_graph = false;
@@ -99,14 +99,15 @@ int check_params(demon_ctx *c, const demon_params *p) {
     return DEMON_OK;
 }
 
-// Number of j slices: enough waves to give every SIMD of the 256 CUs about
-// four, while keeping each slice at least 64 j particles long.
+// Number of j slices: about 32768 waves (32 per SIMD of the 256 CUs), so
+// that the uneven per-wave work left by the box culling evens out, while
+// keeping each slice at least 64 j particles long.
 void choose_slices(demon_ctx *c) {
     const int64_t waves_per_slice = c->npad / (64 * c->i_per_lane);
-    int64_t s = c->slices_req > 0 ? c->slices_req : (4096 + waves_per_slice - 1) / waves_per_slice;
+    int64_t s = c->slices_req > 0 ? c->slices_req : (32768 + waves_per_slice - 1) / waves_per_slice;
     const int64_t max_by_len = std::max<int64_t>(1, c->npad / 64);
     s = std::max<int64_t>(1, std::min<int64_t>({s, (int64_t)kMaxSlices, max_by_len}));
-    int64_t js = round_up((c->npad + s - 1) / s, 8);
+    int64_t js = round_up((c->npad + s - 1) / s, kBoxBlock);
     s = (c->npad + js - 1) / js;
     c->slices = (int32_t)s;
     c->jslice = (int32_t)js;
@@ -161,7 +162,7 @@ int put(demon_ctx *c, double *dst, const double *src, double fill, size_t slot) 
 
 int enqueue_pairs(demon_ctx *c, const StageBuf &st, int gated, int dtcheck) {
     PairLaunch a{};
-    a.xs = st.xs; a.ys = st.ys; a.nc = st.nc; a.q = c->q; a.orig = c->d_orig;
+    a.xs = st.xs; a.ys = st.ys; a.nc = st.nc; a.q = c->q; a.orig = c->d_orig; a.box = st.box;
     a.part = c->d_part; a.sc = c->d_sc; a.n = c->n; a.npad = c->npad;
     a.jslice = c->jslice; a.slices = c->slices; a.i_per_lane = c->i_per_lane;
     a.dtcheck = dtcheck; a.gated = gated;
@@ -376,7 +377,7 @@ int demon_create(demon_ctx **out, int64_t n, const demon_params *p) {
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(hip_fail(c, e, "hipStreamCreate"));
     const size_t P = (size_t)c->npad;
-    e = hipMalloc((void **)&c->d_arena, sizeof(double) * P * 31);
+    e = hipMalloc((void **)&c->d_arena, sizeof(double) * P * 32);
     if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc state arena failed"));
     e = hipMalloc((void **)&c->d_part, sizeof(double) * P * 2 * kMaxSlices);
     if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc partial sums failed"));
@@ -384,7 +385,7 @@ int demon_create(demon_ctx **out, int64_t n, const demon_params *p) {
     if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc index map failed"));
     e = hipMalloc((void **)&c->d_sc, sizeof(Scalars));
     if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc scalars failed"));
-    e = hipMemset(c->d_arena, 0, sizeof(double) * P * 31);
+    e = hipMemset(c->d_arena, 0, sizeof(double) * P * 32);
     if (e == hipSuccess) e = hipMemset(c->d_part, 0, sizeof(double) * P * 2 * kMaxSlices);
     if (e == hipSuccess) e = hipMemset(c->d_sc, 0, sizeof(Scalars));
     if (e != hipSuccess) return bail(hip_fail(c, e, "hipMemset"));
@@ -397,6 +398,11 @@ int demon_create(demon_ctx **out, int64_t n, const demon_params *p) {
     c->acc1 = b; b += 4 * P;
     c->acc23 = b; b += 4 * P;
     c->fout = b; b += 2 * P;
+    // one more array holds the three stages' box tables (P/16 doubles each)
+    for (StageBuf *s : {&c->sb[0], &c->sb[1], &c->sx}) {
+        s->box = b;
+        b += P / 16;
+    }
     for (int k = 0; k < 2; ++k) {
         e = hipEventCreate(&c->ev_call[k]);
         if (e != hipSuccess) return bail(hip_fail(c, e, "hipEventCreate"));
@@ -627,6 +633,16 @@ int demon_last_timing(const demon_ctx *c, double *pair_ms, double *total_ms) {
     if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
     if (pair_ms) *pair_ms = c->last_pair_ms;
     if (total_ms) *total_ms = c->last_total_ms;
+    return DEMON_OK;
+}
+
+int demon_pair_counts(const demon_ctx *c, int64_t *near_pairs, int64_t *tested_pairs) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    Scalars s{};
+    hipError_t e = hipMemcpy(&s, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(const_cast<demon_ctx *>(c), e, "hipMemcpy pair counts");
+    if (near_pairs) *near_pairs = (int64_t)s.pairs_near;
+    if (tested_pairs) *tested_pairs = (int64_t)s.pairs_tested;
     return DEMON_OK;
 }
 

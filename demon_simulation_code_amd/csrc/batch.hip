@@ -54,7 +54,8 @@ __global__ __launch_bounds__(256) void k_bpairs(const double *__restrict__ xs,
                                                 const BCloud *__restrict__ clouds,
                                                 const BItem *__restrict__ items, int32_t n_items,
                                                 unsigned int *__restrict__ counter,
-                                                Scalars *__restrict__ sc, int32_t base_flags) {
+                                                Scalars *__restrict__ sc, int32_t base_flags,
+                                                pair::ExpCoef ec) {
     const int lane = threadIdx.x & 63;
     for (;;) {
         int32_t it = 0;
@@ -65,15 +66,17 @@ __global__ __launch_bounds__(256) void k_bpairs(const double *__restrict__ xs,
         const BCloud cl = clouds[item.cloud];
         Scalars *s = sc + item.cloud;
         if ((base_flags & pair::kFlagGated) && !s->active) continue;
-        const double rf2 = pair::candidate_r2(cl.rc2p, base_flags, s->rmax2_bits);
+        const pair::Radii rad = pair::launch_radii(cl.rc2p, base_flags, s->rmax2_bits);
         const int32_t j_begin = item.slice * cl.jslice;
         const int32_t j_end = j_begin + cl.jslice < cl.npad ? j_begin + cl.jslice : cl.npad;
         const int32_t ibase = item.ibase + lane;
         double ax[R], ay[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) ax[r] = ay[r] = 0.0;
-        pair::wave_tile<R, F32>(xs + cl.off, ys + cl.off, nc + cl.off, q + cl.off, orig + cl.off, ibase,
-                                j_begin, j_end, base_flags, cl.kappa, cl.rc2p, rf2, &s->dtmin_bits, ax, ay);
+        uint64_t n_near = 0, n_tested = 0;  // not reported for ensembles
+        pair::wave_tile<R, F32>(xs + cl.off, ys + cl.off, nc + cl.off, q + cl.off, orig + cl.off, nullptr,
+                                cl.n, ibase, j_begin, j_end, base_flags, cl.kappa, cl.rc2p, rad,
+                                ec, &s->dtmin_bits, ax, ay, n_near, n_tested);
         double *px = part + cl.part_off + (size_t)(2 * item.slice) * cl.npad;
         double *py = px + cl.npad;
 #pragma unroll
@@ -402,13 +405,14 @@ int enqueue_bpairs(demon_batch *b, const StageBuf &st, int32_t flags, int only_g
         const BItem *it = items ? items : b->d_items + b->group_off[g];
         unsigned int *ctr = b->d_counters + g;
         const dim3 grid((unsigned)grid_for(cnt)), blk(256);
+        const pair::ExpCoef ec = pair::exp_taylor_coef();
 #define BLAUNCH(RR)                                                                                        \
     if (f32)                                                                                               \
         hipLaunchKernelGGL((k_bpairs<RR, true>), grid, blk, 0, b->stream, st.xs, st.ys, st.nc, b->q, b->orig, \
-                           b->part, b->d_clouds, it, cnt, ctr, b->d_sc, flags);                           \
+                           b->part, b->d_clouds, it, cnt, ctr, b->d_sc, flags, ec);                       \
     else                                                                                                   \
         hipLaunchKernelGGL((k_bpairs<RR, false>), grid, blk, 0, b->stream, st.xs, st.ys, st.nc, b->q, b->orig, \
-                           b->part, b->d_clouds, it, cnt, ctr, b->d_sc, flags);
+                           b->part, b->d_clouds, it, cnt, ctr, b->d_sc, flags, ec);
         if (g == 0) { BLAUNCH(1) } else if (g == 1) { BLAUNCH(2) } else { BLAUNCH(4) }
 #undef BLAUNCH
         BHIP(b, hipGetLastError());

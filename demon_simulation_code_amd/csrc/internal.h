@@ -20,10 +20,16 @@ constexpr int kPad = 1024;
 constexpr int kMaxSlices = 64;
 
 // One RK stage's state, in the device (tile) order: positions, velocities and
-// the prefilter norm nc = -(xs^2 + ys^2) * kNormShrink (-inf in padding).
+// the prefilter norm nc = -(xs^2 + ys^2) * kNormShrink (padding: NaN positions, nc = -inf).
+// box (optional): per 64-particle block the bounding box of its particles'
+// positions, (xmin, xmax, ymin, ymax); +inf/-inf for a block of padding only.
 struct StageBuf {
     double *xs, *ys, *vxs, *vys, *nc;
+    double *box = nullptr;
 };
+
+// Particles per bounding-box block.
+constexpr int kBoxBlock = 64;
 
 // Device scalars of the stepping loop.
 struct Scalars {
@@ -36,6 +42,8 @@ struct Scalars {
     unsigned long long remaining;   // steps still allowed in this call
     unsigned long long dt_reductions;
     unsigned long long rmax2_bits;  // max |r|^2 of the stage-1 positions (double bits)
+    unsigned long long pairs_near;    // ordered pairs inside the cutoff, summed over launches
+    unsigned long long pairs_tested;  // ordered pairs given a distance test, summed over launches
     unsigned int dtmin_bits;        // min FP32 squared distance of tested pairs (float bits)
     int active;                     // 1 while the current step runs
     int err;                        // sticky device-side error code
@@ -44,6 +52,7 @@ struct Scalars {
 
 struct PairLaunch {
     const double *xs, *ys, *nc, *q;
+    const double *box;       // per-block bounding boxes of xs/ys, or null
     const int32_t *orig;
     double *part;            // [slices][2][npad] partial force sums
     Scalars *sc;
@@ -95,7 +104,7 @@ struct ForceOutLaunch {
 };
 hipError_t launch_force_out(const ForceOutLaunch &a, hipStream_t s);
 
-// nc = -(xs^2+ys^2)*kNormShrink for i < n; padding gets xs=ys=0, nc=-inf.
+// nc = -(xs^2+ys^2)*kNormShrink for i < n; padding gets xs=ys=NaN, v=0, nc=-inf.
 hipError_t launch_prepare_stage(StageBuf st, int64_t n, int64_t npad, hipStream_t s);
 // rmax2 = max |r|^2 over the stage positions (into sc->rmax2_bits, after reset).
 hipError_t launch_rmax(StageBuf st, int64_t n, Scalars *sc, hipStream_t s);

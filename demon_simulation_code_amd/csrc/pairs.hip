@@ -42,12 +42,13 @@ __global__ __launch_bounds__(256) void k_pairs(const double *__restrict__ xs,
                                                const double *__restrict__ nc,
                                                const double *__restrict__ q,
                                                const int32_t *__restrict__ orig,
+                                               const double *__restrict__ box,
                                                double *__restrict__ part,
-                                               Scalars *__restrict__ sc, int64_t npad,
+                                               Scalars *__restrict__ sc, int64_t n, int64_t npad,
                                                int32_t jslice, int32_t flags, double kappa,
-                                               double rc2p) {
+                                               double rc2p, ExpCoef ec) {
     if ((flags & kFlagGated) && !sc->active) return;
-    const double rf2 = candidate_r2(rc2p, flags, sc->rmax2_bits);
+    const Radii rad = launch_radii(rc2p, flags, sc->rmax2_bits);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int32_t ibase = (int32_t)blockIdx.x * (256 * R) + wave * (64 * R) + lane;
@@ -56,8 +57,13 @@ __global__ __launch_bounds__(256) void k_pairs(const double *__restrict__ xs,
     double ax[R], ay[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) ax[r] = ay[r] = 0.0;
-    wave_tile<R, F32>(xs, ys, nc, q, orig, ibase, j_begin, j_end, flags, kappa, rc2p, rf2,
-                      &sc->dtmin_bits, ax, ay);
+    uint64_t n_near = 0, n_tested = 0;
+    wave_tile<R, F32>(xs, ys, nc, q, orig, box, (int32_t)n, ibase, j_begin, j_end, flags, kappa,
+                      rc2p, rad, ec, &sc->dtmin_bits, ax, ay, n_near, n_tested);
+    if (lane == 0) {
+        atomicAdd(&sc->pairs_near, (unsigned long long)n_near);
+        atomicAdd(&sc->pairs_tested, (unsigned long long)n_tested);
+    }
     double *px = part + (size_t)(2 * blockIdx.y) * npad;
     double *py = px + npad;
 #pragma unroll
@@ -74,14 +80,15 @@ hipError_t launch_shape(const PairLaunch &a, hipStream_t s) {
     if (a.dtcheck) flags |= kFlagDtCheck;
     if (a.gated) flags |= kFlagGated;
     hipLaunchKernelGGL((k_pairs<R, F32>), grid, dim3(256), 0, s, a.xs, a.ys, a.nc, a.q, a.orig,
-                       a.part, a.sc, a.npad, a.jslice, flags, a.kappa, a.rc2p);
+                       a.box, a.part, a.sc, a.n, a.npad, a.jslice, flags, a.kappa, a.rc2p,
+                       exp_taylor_coef());
     return hipGetLastError();
 }
 
 }  // namespace
 
 hipError_t launch_pairs(const PairLaunch &a, hipStream_t s) {
-    if (a.npad % kPad || a.jslice % kChunk || a.slices < 1) return hipErrorInvalidValue;
+    if (a.npad % kPad || a.jslice % kBoxBlock || a.slices < 1 || a.n > a.npad) return hipErrorInvalidValue;
     if ((int64_t)a.slices * a.jslice < a.npad) return hipErrorInvalidValue;
     switch (a.i_per_lane * 2 + (a.fp32pair ? 1 : 0)) {
         case 2: return launch_shape<1, false>(a, s);

@@ -65,6 +65,27 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
     }
 }
 
+// Bounding box of the wave's 64 consecutive particles (one box block), over
+// the lanes with real == true; written by lane 0.  Every lane of the wave
+// must call it.
+__device__ __forceinline__ void wave_box(bool real, double x, double y, double *box) {
+    const double inf = __builtin_inf();
+    double x0 = real ? x : inf, x1 = real ? x : -inf;
+    double y0 = real ? y : inf, y1 = real ? y : -inf;
+    for (int off = 32; off > 0; off >>= 1) {
+        x0 = fmin(x0, __shfl_xor(x0, off));
+        x1 = fmax(x1, __shfl_xor(x1, off));
+        y0 = fmin(y0, __shfl_xor(y0, off));
+        y1 = fmax(y1, __shfl_xor(y1, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        box[0] = x0;
+        box[1] = x1;
+        box[2] = y0;
+        box[3] = y1;
+    }
+}
+
 template <int STAGE>
 __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
     __shared__ double red[4];
@@ -103,7 +124,7 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
         dt = sc->dt_step;
     }
 
-    double r2new = 0.0;
+    double r2new = 0.0, nx = 0.0, ny = 0.0;
     if (i < a.n) {
         const double xs = a.cur.xs[i], ys = a.cur.ys[i];
         const double vxs = a.cur.vxs[i], vys = a.cur.vys[i];
@@ -125,7 +146,7 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
         const double x = a.x[i], y = a.y[i], vx = a.vx[i], vy = a.vy[i];
         double *A1 = a.acc1, *A23 = a.acc23;
         const int64_t P = a.npad;
-        double nx, ny, nvx, nvy;
+        double nvx, nvy;
         if (STAGE == 1 || STAGE == 2) {
             double *A = (STAGE == 1) ? A1 : A23;
             A[i] = k;
@@ -163,6 +184,7 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
         a.nxt.nc[i] = -r2 * kNormShrink;
         r2new = r2;
     }
+    if (a.nxt.box) wave_box(i < a.n, nx, ny, a.nxt.box + 4 * (i / kBoxBlock));
     if (STAGE == 4) {
         // block max of |r|^2 for the next stage-1 candidate radius
         double v = r2new;
@@ -204,18 +226,21 @@ __global__ __launch_bounds__(256) void k_force_out(ForceOutLaunch a) {
 }
 
 __global__ __launch_bounds__(256) void k_prepare(StageBuf st, int64_t n, int64_t npad) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= npad) return;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;  // npad is a multiple of 256
+    double x = 0.0, y = 0.0;
     if (i < n) {
-        const double x = st.xs[i], y = st.ys[i];
+        x = st.xs[i];
+        y = st.ys[i];
         st.nc[i] = -(x * x + y * y) * kNormShrink;
     } else {
-        st.xs[i] = 0.0;
-        st.ys[i] = 0.0;
+        // NaN coordinates fail every distance test of the pair kernels
+        st.xs[i] = __builtin_nan("");
+        st.ys[i] = __builtin_nan("");
         st.vxs[i] = 0.0;
         st.vys[i] = 0.0;
         st.nc[i] = -__builtin_inf();
     }
+    if (st.box) wave_box(i < n, x, y, st.box + 4 * (i / kBoxBlock));
 }
 
 __global__ __launch_bounds__(256) void k_rmax(StageBuf st, int64_t n, Scalars *sc) {
@@ -260,6 +285,7 @@ hipError_t launch_force_out(const ForceOutLaunch &a, hipStream_t s) {
 }
 
 hipError_t launch_prepare_stage(StageBuf st, int64_t n, int64_t npad, hipStream_t s) {
+    if (npad % 256) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_prepare, dim3(blocks_for(npad)), dim3(256), 0, s, st, n, npad);
     return hipGetLastError();
 }

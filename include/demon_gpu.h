@@ -137,6 +137,13 @@ int demon_get_pair_shape(const demon_ctx *ctx, int32_t *i_per_lane, int32_t *j_s
  * all-pairs kernels and everything else (HIP events on the context stream). */
 int demon_last_timing(const demon_ctx *ctx, double *pair_ms, double *total_ms);
 
+/* Pair counts of the all-pairs kernels since the last upload (no counterpart
+ * in the reference; instrumentation for the throughput report): ordered
+ * pairs that contributed a Yukawa term (inside the cutoff), and ordered
+ * pairs that reached a distance test at all (pairs in 64-particle blocks
+ * whose bounding boxes are farther apart than the cutoff are never tested). */
+int demon_pair_counts(const demon_ctx *ctx, int64_t *near_pairs, int64_t *tested_pairs);
+
 /* Differentiable trajectory fit (no counterpart in the reference, which has
  * no differentiation; it differentiates Runge_Kutta4::moveParticles with
  * Confinement, Drag and ShieldedCoulomb forces).

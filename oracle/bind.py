@@ -7,6 +7,7 @@ cpu_baseline leg of bench.py -- never by the product package.
 from __future__ import annotations
 
 import ctypes
+import os
 import subprocess
 from pathlib import Path
 
@@ -204,6 +205,17 @@ class Oracle:
         return (*s, float(tt[0]), nst.value)
 
 
+def default_threads() -> int:
+    """OpenMP threads for the reference: the CPUs this process may run on,
+    at most 16 (the reference itself would start omp_get_num_procs() threads,
+    Cloud.cpp:34-36, which oversubscribes a CPU-limited container)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
 class Reference:
     """The reference's own classes (compiled in place) behind ref_harness.cpp."""
 
@@ -214,6 +226,7 @@ class Reference:
         L = ctypes.CDLL(str(path))
         self.L = L
         self.openmp = bool(L.ref_is_openmp())
+        self.threads = default_threads() if self.openmp else 1
         L.ref_yukawa_stage.argtypes = [ctypes.c_int, _I64, _D, _D, _D, ctypes.c_double, _D, _D, ctypes.c_int]
         L.ref_modify_time_step.argtypes = [_I64, _D, _D, ctypes.c_float, ctypes.c_double]
         L.ref_modify_time_step.restype = ctypes.c_double
@@ -231,7 +244,8 @@ class Reference:
     def yukawa(self, x, y, q, kappa, stage=1, threads=0):
         n = len(x)
         fx, fy = np.zeros(n), np.zeros(n)
-        rc = self.L.ref_yukawa_stage(stage, n, p(_f64(x)), p(_f64(y)), p(_f64(q)), kappa, p(fx), p(fy), threads)
+        rc = self.L.ref_yukawa_stage(stage, n, p(_f64(x)), p(_f64(y)), p(_f64(q)), kappa, p(fx), p(fy),
+                                     threads or self.threads)
         assert rc == 0
         return fx, fy
 
@@ -243,7 +257,7 @@ class Reference:
         s = [_f64(a).copy() for a in (x, y, vx, vy)]
         tt = np.array([t])
         rc = self.L.ref_rk4(len(s[0]), *[p(a) for a in s], p(_f64(q)), p(_f64(m)), kappa, confine,
-                            gamma, heat, forces, dt, 0, steps, 0.0, p(tt), threads)
+                            gamma, heat, forces, dt, 0, steps, 0.0, p(tt), threads or self.threads)
         assert rc == 0
         return (*s, float(tt[0]))
 
@@ -252,7 +266,7 @@ class Reference:
         n = len(x)
         rec = np.zeros((steps // every, 4, n))
         rc = self.L.ref_rk4_record(n, p(_f64(x)), p(_f64(y)), p(_f64(q)), p(_f64(m)), kappa, confine,
-                                   gamma, heat, forces, dt, steps, every, p(rec), threads)
+                                   gamma, heat, forces, dt, steps, every, p(rec), threads or self.threads)
         assert rc == 0
         return rec
 
@@ -261,6 +275,6 @@ class Reference:
         s = [_f64(a).copy() for a in (x, y, vx, vy)]
         tt = np.array([t])
         rc = self.L.ref_rk4(len(s[0]), *[p(a) for a in s], p(_f64(q)), p(_f64(m)), kappa, confine,
-                            gamma, heat, forces, dt, 1, 0, end_time, p(tt), threads)
+                            gamma, heat, forces, dt, 1, 0, end_time, p(tt), threads or self.threads)
         assert rc == 0
         return (*s, float(tt[0]))
