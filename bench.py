@@ -19,8 +19,9 @@ Prints ONE JSON line:
   fp32_pair    the same run with FP32 pair arithmetic (speed and error)
   cpu_baseline the reference's own SSE/OpenMP code (compiled from its
                sources into oracle/_ref) timed on the host cores on a bounded
-               sample: one force evaluation at each of four states spread
-               over the same trajectory
+               sample: one force evaluation on a random 32768-particle subset
+               of each of four states spread over the same trajectory, scaled
+               by the pair count
 """
 from __future__ import annotations
 
@@ -53,10 +54,11 @@ def setup(wl):
     return x, y, q, m
 
 
-def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=()):
+def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mode=None):
     """Warm up on the workload, re-upload the initial state, then time exactly
-    `steps` RK4 steps from it (one demon_rk4_steps call).  With `snapshots`
-    (step numbers), a second, untimed pass records the state at those steps."""
+    `steps` RK4 steps from it (one demon_rk4_steps call); with `start`, from
+    the state after that many untimed steps instead.  With `snapshots` (step
+    numbers), a second, untimed pass records the state at those steps."""
     import demon_simulation_code_amd as dm
 
     x, y, q, m = setup(wl)
@@ -64,19 +66,24 @@ def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=()):
     forces = dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB
     c = dm.Context(n, shielding=wl.kappa, confine=wl.confine, gamma=wl.gamma, forces=forces,
                    precision=precision)
+    if mode is not None:
+        c.set_pair_mode(mode)
     if shape:
         c.set_pair_shape(*shape)
     c.upload(x, y, None, None, q, m)
     if warmup:
         c.rk4_steps(warmup, wl.dt, 0.0)
     c.upload(x, y, None, None, q, m)
+    t = c.rk4_steps(start, wl.dt, 0.0) if start else 0.0
+    n0, t0_ = c.pair_counts()
     c.synchronize()
     t0 = time.perf_counter()
-    t = c.rk4_steps(steps, wl.dt, 0.0)
+    t = c.rk4_steps(steps, wl.dt, t)
     c.synchronize()
     wall = time.perf_counter() - t0
     pair_ms, total_ms = c.last_timing()
     near, tested = c.pair_counts()
+    near, tested = near - n0, tested - t0_
     xs1, ys1, vx1, vy1 = c.download()
     st = c.stats()
     shape_used = c.pair_shape()
@@ -94,33 +101,46 @@ def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=()):
                 tested=tested, final=(xs1, ys1, vx1, vy1), shape=shape_used, snaps=snaps)
 
 
+CPU_SUBSET = 32768
+
+
 def cpu_baseline(wl, snaps, snap_steps):
     """The reference's own code (OpenMP build) on the host cores, on a bounded
-    sample of the same workload: one ShieldedCoulombForce::force1 evaluation
-    at each snapshot state of the FP64 trajectory.  The per-evaluation cost
-    grows as the cloud contracts (more pairs inside the cutoff need the exp
-    and the division), so the sample spans the run."""
+    sample of the same workload: ShieldedCoulombForce::force1 at four states
+    spread over the FP64 trajectory.  The per-pair cost grows as the cloud
+    contracts (pairs inside the cutoff need the exp and the division; on the
+    imploded cloud one full N = 131072 evaluation takes minutes), so each
+    evaluation runs on a random subset of CPU_SUBSET particles of that state:
+    its pairs are a uniform sample of the state's pairs, with the same
+    fraction inside the cutoff, and the time is scaled by the pair count
+    N(N-1) / (M(M-1)).  A smaller working set only helps the CPU's caches."""
     from oracle.bind import Oracle, Reference, default_threads
 
     n = wl.n
     _, _, q, _ = setup(wl)
+    m_sub = min(n, CPU_SUBSET)
     if Reference.available(openmp=True):
         ref, kind, threads = Reference(openmp=True), "reference", default_threads()
 
-        def evaluate(x, y):
-            ref.yukawa(x, y, q, wl.kappa, 1, threads=threads)
+        def evaluate(x, y, qq):
+            ref.yukawa(x, y, qq, wl.kappa, 1, threads=threads)
     else:
         orc, kind, threads = Oracle(), "port", 1
 
-        def evaluate(x, y):
-            orc.yukawa(x, y, q, wl.kappa)
+        def evaluate(x, y, qq):
+            orc.yukawa(x, y, qq, wl.kappa)
 
+    rng = np.random.default_rng(0)
+    scale = n * (n - 1) / (m_sub * (m_sub - 1))
     times = []
     for (x, y), s in zip(snaps, snap_steps):
+        idx = np.sort(rng.choice(n, m_sub, replace=False))
+        xs, ys, qs = np.ascontiguousarray(x[idx]), np.ascontiguousarray(y[idx]), q[idx]
         t0 = time.perf_counter()
-        evaluate(x, y)
-        times.append(time.perf_counter() - t0)
-        log(f"CPU force1 at step {s}: {times[-1]:.2f} s")
+        evaluate(xs, ys, qs)
+        times.append((time.perf_counter() - t0) * scale)
+        log(f"CPU force1 at step {s} on {m_sub} particles: {times[-1] / scale:.2f} s "
+            f"(scaled to N={n}: {times[-1]:.1f} s)")
     t_eval = sum(times) / len(times)
     return {
         "value": n * (n - 1) / t_eval,
@@ -128,8 +148,9 @@ def cpu_baseline(wl, snaps, snap_steps):
         "cores": threads,
         "kind": kind,
         "rk4_steps_per_s": 1.0 / (4.0 * t_eval),
-        "sample": (f"ShieldedCoulombForce::force1 at N={n} on the FP64 trajectory's states after "
-                   f"steps {list(snap_steps)}: " + ", ".join(f"{t:.2f} s" for t in times)
+        "sample": (f"ShieldedCoulombForce::force1 on a random {m_sub}-particle subset of the FP64 "
+                   f"trajectory's state after steps {list(snap_steps)}, each time scaled by the pair "
+                   f"count to N={n}: " + ", ".join(f"{t:.1f} s" for t in times)
                    + "; RK4 steps/s counts 4 evaluations per step"),
     }
 
@@ -143,6 +164,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--shape", type=str, default="", help="i_per_lane,j_slices")
+    ap.add_argument("--start-step", type=int, default=0,
+                    help="time the steps that follow this many untimed ones (to profile a later "
+                         "phase of the trajectory; the headline workload is 0)")
+    ap.add_argument("--mode", type=str, default="newton3", choices=["newton3", "gather"],
+                    help="all-pairs kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -158,7 +184,8 @@ def main():
 
     snap_steps = [K // 8, 3 * K // 8, 5 * K // 8, 7 * K // 8] if not args.no_cpu else []
     log(f"FP64 run: N={n} warmup={W} steps={K}")
-    r = run_gpu(wl, K, W, dm.FP64, shape, snapshots=snap_steps)
+    mode = dm.PAIRS_NEWTON3 if args.mode == "newton3" else dm.PAIRS_GATHER
+    r = run_gpu(wl, K, W, dm.FP64, shape, snapshots=snap_steps, start=args.start_step, mode=mode)
     pairs_total = 4.0 * n * (n - 1) * K
     value = pairs_total / r["wall"]
     log(f"FP64 done: {r['wall']:.3f} s, {value:.4g} pair interactions/s, shape={r['shape']}")
@@ -170,6 +197,7 @@ def main():
     pair_s = r["pair_ms"] / 1e3
     achieved = flop_total / pair_s
     R = r["shape"][0]
+    kernel = "k_pairs_n3<false>" if mode == dm.PAIRS_NEWTON3 else f"k_pairs<{R},false>"
     roofline = {
         "bound": "valu",
         "achieved": achieved / 1e12,
@@ -177,7 +205,7 @@ def main():
         "unit": "TFLOP/s",
         "frac": achieved / FP64_PEAK,
         "traffic": None,
-        "kernel": f"k_pairs<{R},false> (FP64 all-pairs)",
+        "kernel": kernel + " (FP64 all-pairs)",
         "mean_launch_ms": pair_s * 1e3 / launches,
         "flop_per_launch": flop_total / launches,
         "near_fraction": near / (launches * ordered),
@@ -209,7 +237,9 @@ def main():
             "gamma": wl.gamma,
             "thermal": False,
             "dt_check": True,
+            "pair_kernel": args.mode,
             "pair_shape": list(r["shape"]),
+            "start_step": args.start_step,
         },
         "rk4_steps_per_s": K / r["wall"],
         "roofline": roofline,
@@ -218,7 +248,7 @@ def main():
 
     if not args.no_fp32:
         log("FP32-pair run")
-        r32 = run_gpu(wl, K, W, dm.FP32_PAIR, shape)
+        r32 = run_gpu(wl, K, W, dm.FP32_PAIR, shape, start=args.start_step, mode=mode)
         x64, y64 = r["final"][0], r["final"][1]
         x32, y32 = r32["final"][0], r32["final"][1]
         perr = float(np.max(np.hypot(x64 - x32, y64 - y32)) / np.max(np.hypot(x64, y64)))

@@ -11,6 +11,8 @@ from ._lib import (  # noqa: F401
     FP32_PAIR,
     FP64,
     LIB_PATH,
+    PAIRS_GATHER,
+    PAIRS_NEWTON3,
     SHIELDED_COULOMB,
     THERMAL,
     Context,
@@ -25,5 +27,6 @@ from .physics import ELECTRON_CHARGE, EPSILON0, COULOMB, confine_from_k  # noqa:
 __all__ = [
     "Batch", "Context", "DemonError", "Params", "Stats", "hex_lattice", "lib", "LIB_PATH",
     "CONFINEMENT", "DRAG", "SHIELDED_COULOMB", "THERMAL", "FP64", "FP32_PAIR",
+    "PAIRS_GATHER", "PAIRS_NEWTON3",
     "ELECTRON_CHARGE", "EPSILON0", "COULOMB", "confine_from_k",
 ]

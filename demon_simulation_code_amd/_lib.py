@@ -30,6 +30,8 @@ THERMAL = 16
 
 FP64 = 0
 FP32_PAIR = 1
+PAIRS_GATHER = 0
+PAIRS_NEWTON3 = 1
 
 
 class DemonError(RuntimeError):
@@ -90,6 +92,8 @@ SIGNATURES = {
     "demon_get_pair_shape": (ctypes.c_int, [_CTX, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "demon_last_timing": (ctypes.c_int, [_CTX, _D, _D]),
     "demon_pair_counts": (ctypes.c_int, [_CTX, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "demon_set_pair_mode": (ctypes.c_int, [_CTX, _I32]),
+    "demon_get_pair_mode": (ctypes.c_int, [_CTX, ctypes.POINTER(_I32)]),
     "demon_hex_lattice": (ctypes.c_int, [_I64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _D, _D]),
     "demon_fit_loss_grad": (ctypes.c_int, [_CTX, _I64, ctypes.c_double, _I64, _I64, _D, _D, _D, _D, _D,
                                            _D, _D, _D, _D, _D, _D]),
@@ -268,6 +272,15 @@ class Context:
         a, b = ctypes.c_double(0), ctypes.c_double(0)
         self._c(lib.demon_last_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def set_pair_mode(self, mode: int):
+        """PAIRS_NEWTON3 (each unordered pair once) or PAIRS_GATHER (each ordered pair)."""
+        self._c(lib.demon_set_pair_mode(self._h, int(mode)))
+
+    def pair_mode(self) -> int:
+        m = _I32(0)
+        self._c(lib.demon_get_pair_mode(self._h, ctypes.byref(m)))
+        return m.value
 
     def pair_counts(self):
         """(ordered pairs inside the cutoff, ordered pairs distance-tested) since upload."""

@@ -43,7 +43,11 @@ struct demon_ctx {
     bool uploaded = false;
     int32_t i_per_lane = 1;
     int32_t slices_req = 0;      // 0 = automatic
-    int32_t slices = 1, jslice = 0;
+    int32_t slices = 1, jslice = 0;  // gather kernel (and the adjoint pair kernel)
+    int32_t pair_mode = DEMON_PAIRS_GATHER;
+    int32_t n3_slices = 1, n3_span = 1;  // Newton-3 kernel: superblock offsets per slice
+    double *d_react = nullptr;   // Newton-3 reactions [nsb/2][2][npad]
+    uint32_t *d_rflag = nullptr; // Newton-3 tile flags [nsb][nsb/2]
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
     bool timing = true;
@@ -111,6 +115,58 @@ void choose_slices(demon_ctx *c) {
     s = (c->npad + js - 1) / js;
     c->slices = (int32_t)s;
     c->jslice = (int32_t)js;
+    // Newton-3: nsb/2 + 1 superblock offsets (0 = the diagonal) over the
+    // slices, one workgroup of four waves per (superblock row, slice)
+    const int64_t nsb = c->npad / kSuper, offsets = nsb / 2 + 1;
+    int64_t s3 = c->slices_req > 0 ? c->slices_req : (32768 + 4 * nsb - 1) / (4 * nsb);
+    s3 = std::max<int64_t>(1, std::min<int64_t>({s3, (int64_t)kMaxSlices, offsets}));
+    const int64_t span = (offsets + s3 - 1) / s3;
+    c->n3_span = (int32_t)span;
+    c->n3_slices = (int32_t)((offsets + span - 1) / span);
+}
+
+// Device memory of the Newton-3 reaction buffers.
+size_t n3_bytes(int64_t npad) {
+    const size_t nsb = (size_t)(npad / kSuper), half = nsb / 2;
+    return sizeof(double) * half * 2 * (size_t)npad + sizeof(uint32_t) * half * nsb;
+}
+
+// Newton-3 is the default while its reaction buffers stay under this size.
+constexpr size_t kN3DefaultBytes = (size_t)16 << 30;
+
+int ensure_n3(demon_ctx *c) {
+    if (c->d_react) return DEMON_OK;
+    const size_t nsb = (size_t)(c->npad / kSuper), half = nsb / 2;
+    if (half > (size_t)kMaxHalf)
+        return fail(c, DEMON_ERR_UNSUPPORTED, "the Newton-3 kernel supports n <= 524288; use DEMON_PAIRS_GATHER");
+    const size_t rbytes = sizeof(double) * half * 2 * (size_t)c->npad;
+    if (hipMalloc((void **)&c->d_react, rbytes) != hipSuccess) {
+        c->d_react = nullptr;
+        return fail(c, DEMON_ERR_ALLOC, "hipMalloc of the Newton-3 reaction buffers failed");
+    }
+    if (hipMalloc((void **)&c->d_rflag, sizeof(uint32_t) * half * nsb) != hipSuccess) {
+        (void)hipFree(c->d_react);
+        c->d_react = nullptr;
+        c->d_rflag = nullptr;
+        return fail(c, DEMON_ERR_ALLOC, "hipMalloc of the Newton-3 tile flags failed");
+    }
+    // no tile evaluated yet: the stage kernels must read no reaction
+    HIP_TRY(c, hipMemset(c->d_rflag, 0, sizeof(uint32_t) * half * nsb));
+    return DEMON_OK;
+}
+
+PairSums pair_sums(const demon_ctx *c) {
+    PairSums ps{};
+    ps.part = c->d_part;
+    if (c->pair_mode == DEMON_PAIRS_NEWTON3) {
+        ps.slices = c->n3_slices;
+        ps.react = c->d_react;
+        ps.rflag = c->d_rflag;
+        ps.nsb = (int32_t)(c->npad / kSuper);
+    } else {
+        ps.slices = c->slices;
+    }
+    return ps;
 }
 
 uint64_t morton2(uint32_t a, uint32_t b) {
@@ -181,7 +237,15 @@ int enqueue_pairs(demon_ctx *c, const StageBuf &st, int gated, int dtcheck) {
         }
         HIP_TRY(c, hipEventRecord(c->ev[c->ev_used], c->stream));
     }
-    HIP_TRY(c, launch_pairs(a, c->stream));
+    if (c->pair_mode == DEMON_PAIRS_NEWTON3) {
+        a.react = c->d_react;
+        a.rflag = c->d_rflag;
+        a.slices = c->n3_slices;
+        a.span = c->n3_span;
+        HIP_TRY(c, launch_pairs_n3(a, c->stream));
+    } else {
+        HIP_TRY(c, launch_pairs(a, c->stream));
+    }
     if (ev) {
         HIP_TRY(c, hipEventRecord(c->ev[c->ev_used + 1], c->stream));
         c->ev_used += 2;
@@ -196,7 +260,7 @@ UpdateLaunch make_update(demon_ctx *c, int stage, int dtcheck, double *store) {
     u.cur = c->sb[cur];
     u.nxt = c->sb[cur ^ 1];
     u.x = c->x; u.y = c->y; u.vx = c->vx; u.vy = c->vy; u.q = c->q; u.m = c->m;
-    u.orig = c->d_orig; u.part = c->d_part; u.slices = c->slices;
+    u.orig = c->d_orig; u.pairs = pair_sums(c);
     u.acc1 = c->acc1; u.acc23 = c->acc23; u.sc = c->d_sc;
     u.n = c->n; u.npad = c->npad; u.forces = c->p.forces; u.dtcheck = dtcheck;
     u.confine = c->p.confine; u.gamma = c->p.drag_gamma; u.heat = c->p.heat_val; u.seed = c->p.seed;
@@ -217,7 +281,7 @@ int enqueue_step_ex(demon_ctx *c, int dtcheck, double *store) {
             return fail(c, DEMON_ERR_UNSUPPORTED,
                         "the dt check runs inside the all-pairs pass; enable ShieldedCoulomb");
         }
-        double *slot = store ? store + (size_t)(s - 1) * 5 * c->npad : nullptr;
+        double *slot = store ? store + (size_t)(s - 1) * store_doubles(c->npad) : nullptr;
         HIP_TRY(c, launch_update(s, make_update(c, s, dtcheck, slot), c->stream));
     }
     return DEMON_OK;
@@ -409,6 +473,10 @@ int demon_create(demon_ctx **out, int64_t n, const demon_params *p) {
     }
     c->h_tmp.assign(P * 4, 0.0);
     choose_slices(c);
+    if (n3_bytes(c->npad) <= kN3DefaultBytes && c->npad / kSuper / 2 <= kMaxHalf) {
+        if (ensure_n3(c)) return bail(DEMON_ERR_ALLOC);
+        c->pair_mode = DEMON_PAIRS_NEWTON3;
+    }
     *out = c;
     return DEMON_OK;
 }
@@ -425,9 +493,12 @@ void demon_destroy(demon_ctx *c) {
     if (c->d_orig) (void)hipFree(c->d_orig);
     if (c->d_sc) (void)hipFree(c->d_sc);
     if (c->d_fit) (void)hipFree(c->d_fit);
+    if (c->d_react) (void)hipFree(c->d_react);
+    if (c->d_rflag) (void)hipFree(c->d_rflag);
     for (hipEvent_t e : c->ev_fit)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    (void)hipGetLastError();  // the teardown's errors are not the next call's
     delete c;
 }
 
@@ -521,8 +592,8 @@ int demon_force_stage(demon_ctx *c, int32_t forces, int32_t stage, uint64_t step
     c->ev_used = 0;
     if (yk && (rc = enqueue_pairs(c, c->sx, 0, 0))) return rc;
     ForceOutLaunch f{};
-    f.st = c->sx; f.q = c->q; f.m = c->m; f.orig = c->d_orig; f.part = c->d_part;
-    f.slices = c->slices; f.fx = c->fout; f.fy = c->fout + c->npad; f.n = c->n; f.npad = c->npad;
+    f.st = c->sx; f.q = c->q; f.m = c->m; f.orig = c->d_orig; f.pairs = pair_sums(c);
+    f.fx = c->fout; f.fy = c->fout + c->npad; f.n = c->n; f.npad = c->npad;
     f.forces = forces; f.confine = c->p.confine; f.gamma = c->p.drag_gamma; f.heat = c->p.heat_val;
     f.seed = c->p.seed; f.step = step; f.stage = stage;
     HIP_TRY(c, launch_force_out(f, c->stream));
@@ -624,8 +695,30 @@ int demon_set_pair_shape(demon_ctx *c, int32_t i_per_lane, int32_t j_slices) {
 
 int demon_get_pair_shape(const demon_ctx *c, int32_t *i_per_lane, int32_t *j_slices) {
     if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
-    if (i_per_lane) *i_per_lane = c->i_per_lane;
-    if (j_slices) *j_slices = c->slices;
+    const bool n3 = c->pair_mode == DEMON_PAIRS_NEWTON3;
+    if (i_per_lane) *i_per_lane = n3 ? 1 : c->i_per_lane;
+    if (j_slices) *j_slices = n3 ? c->n3_slices : c->slices;
+    return DEMON_OK;
+}
+
+int demon_set_pair_mode(demon_ctx *c, int32_t mode) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (mode != DEMON_PAIRS_GATHER && mode != DEMON_PAIRS_NEWTON3)
+        return fail(c, DEMON_ERR_ARG, "mode must be DEMON_PAIRS_GATHER or DEMON_PAIRS_NEWTON3");
+    if (mode == DEMON_PAIRS_NEWTON3) {
+        HIP_TRY(c, hipSetDevice(c->p.device));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        const int rc = ensure_n3(c);
+        if (rc) return rc;
+    }
+    c->pair_mode = mode;
+    drop_graph(c);
+    return DEMON_OK;
+}
+
+int demon_get_pair_mode(const demon_ctx *c, int32_t *mode) {
+    if (!c || !mode) return fail(nullptr, DEMON_ERR_ARG, "NULL argument");
+    *mode = c->pair_mode;
     return DEMON_OK;
 }
 
@@ -667,11 +760,12 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     HIP_TRY(c, hipSetDevice(c->p.device));
     const size_t P = (size_t)c->npad;
     const int64_t n_ckpt = (steps + ckpt_every - 1) / ckpt_every;
-    // arena: ckpt[n_ckpt][4] | targets[n_obs][2] | seeds[n_obs][2] | lossp | store[K][4][5]
+    // arena: ckpt[n_ckpt][4] | targets[n_obs][2] | seeds[n_obs][2] | lossp | store[K][4] (SD each)
     //        | bar[4] | ibar[16] | u[2] | g[4] | apart[slices][4]
     const size_t off_ck = 0, off_tg = off_ck + (size_t)n_ckpt * 4 * P;
     const size_t off_sd = off_tg + (size_t)n_obs * 2 * P, off_lp = off_sd + (size_t)n_obs * 2 * P;
-    const size_t off_st = off_lp + P, off_bar = off_st + (size_t)ckpt_every * 20 * P;
+    const size_t SD = store_doubles(c->npad);  // one stored stage record
+    const size_t off_st = off_lp + P, off_bar = off_st + (size_t)ckpt_every * 4 * SD;
     const size_t off_ib = off_bar + 4 * P, off_u = off_ib + 16 * P, off_g = off_u + 2 * P;
     const size_t off_ap = off_g + 4 * P, off_dt = off_ap + (size_t)c->slices * 4 * P;
     const size_t total = off_dt + (size_t)steps;
@@ -690,9 +784,8 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     HIP_TRY(c, hipMemsetAsync(A + off_lp, 0, sizeof(double) * P, c->stream));
     // store slots need valid padding (nc = -inf) for the adjoint prefilter
     for (int64_t k = 0; k < ckpt_every * 4; ++k) {
-        double *slot = A + off_st + (size_t)k * 5 * P;
-        StageBuf sbk{slot, slot + P, slot + 2 * P, slot + 3 * P, slot + 4 * P};
-        HIP_TRY(c, launch_prepare_stage(sbk, 0, c->npad, c->stream));
+        HIP_TRY(c, launch_prepare_stage(store_view(A + off_st + (size_t)k * SD, c->npad), 0, c->npad,
+                                        c->stream));
     }
     const double norm = 1.0 / (2.0 * (double)c->n * (double)(n_obs > 0 ? n_obs : 1));
     const double inf = std::numeric_limits<double>::infinity();
@@ -743,7 +836,7 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
             hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, 0.0, inf, dt,
                                (unsigned long long)(s1 - s0));
             for (int64_t s = s0; s < s1 && !rc; ++s)
-                rc = enqueue_step_ex(c, c->p.dt_check, A + off_st + (size_t)(s - s0) * 20 * P);
+                rc = enqueue_step_ex(c, c->p.dt_check, A + off_st + (size_t)(s - s0) * 4 * SD);
             if (rc) break;
             HIP_TRY(c, launch_boundary(c->d_sc, 1, c->stream));
             for (int64_t s = s1 - 1; s >= s0; --s) {
@@ -751,11 +844,9 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
                 const double *seed = obs ? A + off_sd + (size_t)(2 * ((s + 1) / obs_every - 1)) * P : nullptr;
                 HIP_TRY(c, launch_adj_begin(bar, ibar, u, g, seed, c->m, c->n, c->npad, c->stream));
                 for (int k = 4; k >= 1; --k) {
-                    double *slot = A + off_st + ((size_t)(s - s0) * 4 + (k - 1)) * 5 * P;
-                    StageBuf st{slot, slot + P, slot + 2 * P, slot + 3 * P, slot + 4 * P};
-                    HIP_TRY(c, launch_adj_pairs(st.xs, st.ys, st.nc, c->q, u, u + P, ap, c->npad,
-                                                c->jslice, c->slices, c->i_per_lane, c->p.shielding,
-                                                rc2p, c->stream));
+                    const StageBuf st = store_view(A + off_st + ((size_t)(s - s0) * 4 + (k - 1)) * SD, c->npad);
+                    HIP_TRY(c, launch_adj_pairs(st, c->q, u, u + P, ap, c->n, c->npad, c->jslice, c->slices,
+                                                c->p.shielding, rc2p, c->stream));
                     HIP_TRY(c, launch_adj_stage(bar, ibar, u, g, k, st, c->q, c->m, ap, c->slices,
                                                 c->p.forces, dts[(size_t)s], c->p.confine,
                                                 c->p.drag_gamma, c->n, c->npad, c->stream));

@@ -17,99 +17,111 @@
 //   qbar_i     += dt [ C u_i.r_i + sum_j (phi_ij / q_i) d_ij.(u_i - u_j) ],
 //   kappabar   += dt sum_ij (-Cc q_i q_j kappa e^-u / r) u_i.d_ij,
 //   Cbar       += dt sum_i q_i u_i.r_i,      gammabar += -dt sum_i m_i u_i.v_i.
-// The pair sums run in k_adj_pairs with the same candidate prefilter as the
-// forward kernel (the cutoff set is identical; the derivative ignores the
-// jump at u = 10, which has measure zero).
-#include "internal.h"
+// The pair sums run in k_adj_pairs with the forward kernel's box culling,
+// prefilter and contribution test.
+#include "pair_core.h"
 
 namespace demon {
 
 namespace {
 
-constexpr int kChunk = 8;
+using namespace pair;
 
-template <int R>
+constexpr int kChunkAdj = 4;  // j per chunk: x, y, q, ux, uy in 40 SGPRs
+
+// Adjoint pair sums of one chunk for the wave's i (lane), gather form:
+//   rx, ry += phi (u_i - u_j) + (phi'/r) ((u_i - u_j).d) d
+//   dq     += (phi / q_i) (u_i - u_j).d
+//   dk     += (-Cc q_i q_j kappa e^-u / r) u_i.d
+// with phi = Cc q_i q_j (1 + u) e^-u / r^3, phi'/r = -Cc q_i q_j e^-u (u^2 + 3u + 3) / r^5.
+template <bool CHECK, bool SELF>
+__device__ __forceinline__ void chunk_adj(const ITile<1> &t, double uxi, double uyi, double qi,
+                                          const double (&cx)[kChunkAdj], const double (&cy)[kChunkAdj],
+                                          const double (&cq)[kChunkAdj], const double (&cux)[kChunkAdj],
+                                          const double (&cuy)[kChunkAdj], int32_t j0, const PairEnv &e,
+                                          double &rx, double &ry, double &dq, double &dk) {
+#pragma unroll
+    for (int u = 0; u < kChunkAdj; ++u) {
+        const double dx = t.xi[0] - cx[u];
+        const double dy = t.yi[0] - cy[u];
+        const double r2 = fma(dx, dx, dy * dy);
+        bool in = pair_in<CHECK>(dx, dy, r2, e);
+        if (SELF) in = in && j0 + u != t.ibase;
+        if (in) {
+            const double y = rsqrt_fp64(r2);
+            const double y2 = y * y;
+            const double y3 = y2 * y;
+            const double uu = (r2 * y) * e.cut.kappa;
+            const double cqj = kCoulomb * cq[u];
+            const double ex = exp_neg(uu, e.ec);
+            const double g = (cqj * ex) * (1.0 + uu) * y3;  // phi / q_i
+            const double ce = (qi * cqj) * ex;
+            const double phi = qi * g;
+            const double dphi = -ce * ((uu + 3.0) * uu + 3.0) * (y3 * y2);
+            const double dux = uxi - cux[u], duy = uyi - cuy[u];
+            const double dot = fma(dx, dux, dy * duy);
+            const double w = dphi * dot;
+            rx = rx + fma(phi, dux, w * dx);
+            ry = ry + fma(phi, duy, w * dy);
+            dq = fma(g, dot, dq);
+            dk = fma(-ce * e.cut.kappa * y, fma(uxi, dx, uyi * dy), dk);
+        }
+    }
+}
+
+// Pair sums of the stage adjoint: for each i the sums above over all j of
+// slice blockIdx.y, with the forward kernel's box culling and contribution
+// test (the same cutoff set; the derivative ignores the jump at u = 10,
+// which has measure zero).  Gather form, deterministic.
 __global__ __launch_bounds__(256) void k_adj_pairs(const double *__restrict__ xs,
                                                    const double *__restrict__ ys,
                                                    const double *__restrict__ nc,
+                                                   const double *__restrict__ box,
                                                    const double *__restrict__ q,
                                                    const double *__restrict__ ux,
                                                    const double *__restrict__ uy,
-                                                   double *__restrict__ part, int64_t npad,
-                                                   int32_t jslice, double kappa, double rc2p) {
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int64_t ibase = (int64_t)blockIdx.x * (256 * R) + wave * (64 * R) + lane;
-
-    double xi[R], yi[R], xm[R], ym[R], nT[R], qi[R], uxi[R], uyi[R];
-    double rx[R], ry[R], dq[R], dk[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int64_t i = ibase + 64 * r;
-        xi[r] = xs[i];
-        yi[r] = ys[i];
-        xm[r] = -2.0 * xi[r];
-        ym[r] = -2.0 * yi[r];
-        nT[r] = -(rc2p + nc[i]);
-        qi[r] = q[i];
-        uxi[r] = ux[i];
-        uyi[r] = uy[i];
-        rx[r] = ry[r] = dq[r] = dk[r] = 0.0;
-    }
+                                                   double *__restrict__ part, int32_t n, int32_t npad,
+                                                   int32_t jslice, double kappa, double rc2p, ExpCoef ec) {
+    const Radii rad{rc2p, -1.0};
+    const PairEnv e{make_cutoff(kappa, rc2p), rad, (float)kappa, false, nullptr, nullptr, ec};
+    const int32_t ibase = (int32_t)blockIdx.x * 256 + (int32_t)threadIdx.x;
+    const ITile<1> t = make_itile<1>(xs, ys, nc, q, n, ibase, rad.rf2);
+    const double qi = q[ibase], uxi = ux[ibase], uyi = uy[ibase];
+    const Box ib = box_union(empty_box(), box + 4 * (t.wlo / kBoxBlock));
+    const bool ireal = t.whi <= n;
+    double rx = 0.0, ry = 0.0, dq = 0.0, dk = 0.0;
     const int32_t j_begin = (int32_t)blockIdx.y * jslice;
-    const int32_t j_end = j_begin + jslice < (int32_t)npad ? j_begin + jslice : (int32_t)npad;
-    for (int32_t j0 = j_begin; j0 < j_end; j0 += kChunk) {
-        double cx[kChunk], cy[kChunk], cn[kChunk];
+    const int32_t j_end = j_begin + jslice < npad ? j_begin + jslice : npad;
+    for (int32_t jb = j_begin; jb < j_end; jb += kBoxBlock) {
+        const BlockClass cls = classify(ib, box + 4 * (jb / kBoxBlock), ireal, jb + kBoxBlock <= n, rad, e.cut);
+        if (cls == kSkip) continue;
+        for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkAdj) {
+            double cx[kChunkAdj], cy[kChunkAdj], cq[kChunkAdj], cux[kChunkAdj], cuy[kChunkAdj];
 #pragma unroll
-        for (int u = 0; u < kChunk; ++u) {
-            cx[u] = xs[j0 + u];
-            cy[u] = ys[j0 + u];
-            cn[u] = nc[j0 + u];
-        }
-        uint64_t hit = 0;
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u)
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                hit |= __ballot(fma(xm[r], cx[u], fma(ym[r], cy[u], nT[r])) < cn[u]);
-        if (!hit) continue;
-#pragma unroll 1
-        for (int u = 0; u < kChunk; ++u) {
-            const int32_t j = j0 + u;
-            const double xj = xs[j], yj = ys[j], nj = nc[j];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const double w = fma(xm[r], xj, fma(ym[r], yj, nT[r]));
-                if (!(w < nj) || (int64_t)j == ibase + 64 * r) continue;
-                const double dx = xi[r] - xj, dy = yi[r] - yj;
-                const double rr = sqrt(dx * dx + dy * dy);
-                const double uu = rr * kappa;
-                if (!(uu < 10.0)) continue;
-                const double e = exp(-uu);
-                const double qj = q[j];
-                const double r3 = rr * rr * rr;
-                const double g = kCoulomb * qj * (1.0 + uu) * e / r3;     // phi / q_i
-                const double phi = qi[r] * g;
-                const double cqq = kCoulomb * (qi[r] * qj) * e;
-                const double dphi = -cqq * (uu * uu + 3.0 * uu + 3.0) / (r3 * rr * rr);
-                const double dux = uxi[r] - ux[j], duy = uyi[r] - uy[j];
-                const double dot = dx * dux + dy * duy;
-                rx[r] = rx[r] + (phi * dux + dphi * dot * dx);
-                ry[r] = ry[r] + (phi * duy + dphi * dot * dy);
-                dq[r] = dq[r] + g * dot;
-                dk[r] = dk[r] + (-cqq * kappa / rr) * (uxi[r] * dx + uyi[r] * dy);
+            for (int u = 0; u < kChunkAdj; ++u) {
+                cx[u] = xs[j0 + u];
+                cy[u] = ys[j0 + u];
             }
+            if (cls != kDense && !chunk_hit<1, kChunkAdj>(t, cx, cy, nc, j0)) continue;
+#pragma unroll
+            for (int u = 0; u < kChunkAdj; ++u) {
+                cq[u] = q[j0 + u];
+                cux[u] = ux[j0 + u];
+                cuy[u] = uy[j0 + u];
+            }
+            if (j0 < t.whi && j0 + kChunkAdj > t.wlo)
+                chunk_adj<true, true>(t, uxi, uyi, qi, cx, cy, cq, cux, cuy, j0, e, rx, ry, dq, dk);
+            else if (cls == kDense)
+                chunk_adj<false, false>(t, uxi, uyi, qi, cx, cy, cq, cux, cuy, j0, e, rx, ry, dq, dk);
+            else
+                chunk_adj<true, false>(t, uxi, uyi, qi, cx, cy, cq, cux, cuy, j0, e, rx, ry, dq, dk);
         }
     }
     double *p0 = part + (size_t)(4 * blockIdx.y) * npad;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int64_t i = ibase + 64 * r;
-        p0[i] = rx[r];
-        p0[npad + i] = ry[r];
-        p0[2 * npad + i] = dq[r];
-        p0[3 * npad + i] = dk[r];
-    }
+    p0[ibase] = rx;
+    p0[npad + ibase] = ry;
+    p0[2 * npad + ibase] = dq;
+    p0[3 * npad + ibase] = dk;
 }
 
 // Adjoint state of the backward sweep, per particle (device order):
@@ -212,25 +224,14 @@ unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
-hipError_t launch_adj_pairs(const double *xs, const double *ys, const double *nc, const double *q,
-                            const double *ux, const double *uy, double *part, int64_t npad,
-                            int32_t jslice, int32_t slices, int32_t i_per_lane, double kappa,
-                            double rc2p, hipStream_t s) {
-    if (npad % kPad || jslice % kChunk || (int64_t)slices * jslice < npad) return hipErrorInvalidValue;
-    switch (i_per_lane) {
-        case 1:
-            hipLaunchKernelGGL(k_adj_pairs<1>, dim3((unsigned)(npad / 256), slices), dim3(256), 0, s, xs, ys, nc,
-                               q, ux, uy, part, npad, jslice, kappa, rc2p);
-            break;
-        case 2:
-            hipLaunchKernelGGL(k_adj_pairs<2>, dim3((unsigned)(npad / 512), slices), dim3(256), 0, s, xs, ys, nc,
-                               q, ux, uy, part, npad, jslice, kappa, rc2p);
-            break;
-        default:
-            hipLaunchKernelGGL(k_adj_pairs<4>, dim3((unsigned)(npad / 1024), slices), dim3(256), 0, s, xs, ys,
-                               nc, q, ux, uy, part, npad, jslice, kappa, rc2p);
-            break;
-    }
+hipError_t launch_adj_pairs(const StageBuf &st, const double *q, const double *ux, const double *uy,
+                            double *part, int64_t n, int64_t npad, int32_t jslice, int32_t slices,
+                            double kappa, double rc2p, hipStream_t s) {
+    if (npad % kPad || jslice % kBoxBlock || (int64_t)slices * jslice < npad || !st.box || n > npad)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_adj_pairs, dim3((unsigned)(npad / 256), (unsigned)slices), dim3(256), 0, s, st.xs,
+                       st.ys, st.nc, st.box, q, ux, uy, part, (int32_t)n, (int32_t)npad, jslice, kappa, rc2p,
+                       exp_taylor_coef());
     return hipGetLastError();
 }
 

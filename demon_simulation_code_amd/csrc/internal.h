@@ -28,8 +28,26 @@ struct StageBuf {
     double *box = nullptr;
 };
 
-// Particles per bounding-box block.
+// Particles per bounding-box block, and per superblock of the Newton-3
+// all-pairs kernel (four box blocks, one workgroup).
 constexpr int kBoxBlock = 64;
+constexpr int kSuper = 256;
+// Largest nsb/2 of the Newton-3 kernel: the stage kernels hold a
+// superblock's tile flags as a bitmask of this many bits in LDS.
+constexpr int kMaxHalf = 1024;
+
+// Where the all-pairs kernels leave a stage's Yukawa forces (device order):
+// part[s][2][npad] (s < slices), partial sums of the i side; with the
+// Newton-3 kernel also react[d - 1][2][npad] (d = 1 .. nsb/2), reactions
+// from superblock row T - d (mod nsb), valid where rflag[T nsb/2 + d - 1]
+// is nonzero (T = the particle's superblock).
+struct PairSums {
+    const double *part;
+    int32_t slices;
+    const double *react;       // null for the gather kernel
+    const uint32_t *rflag;
+    int32_t nsb;
+};
 
 // Device scalars of the stepping loop.
 struct Scalars {
@@ -53,6 +71,9 @@ struct Scalars {
 struct PairLaunch {
     const double *xs, *ys, *nc, *q;
     const double *box;       // per-block bounding boxes of xs/ys, or null
+    double *react;           // Newton-3 kernel: reactions [nsb/2][2][npad]
+    uint32_t *rflag;         // Newton-3 kernel: tile flags [nsb][nsb/2]
+    int32_t span;            // Newton-3 kernel: superblock offsets per slice
     const int32_t *orig;
     double *part;            // [slices][2][npad] partial force sums
     Scalars *sc;
@@ -67,14 +88,14 @@ struct PairLaunch {
 };
 
 hipError_t launch_pairs(const PairLaunch &a, hipStream_t s);
+hipError_t launch_pairs_n3(const PairLaunch &a, hipStream_t s);
 
 struct UpdateLaunch {
     StageBuf cur, nxt;
     double *x, *y, *vx, *vy;        // base state
     const double *q, *m;
     const int32_t *orig;
-    const double *part;
-    int32_t slices;
+    PairSums pairs;
     double *acc1, *acc23;           // [4][npad]: (k, l, m, n) of stage 1; stage 2 (+3)
     Scalars *sc;
     int64_t n, npad;
@@ -82,10 +103,20 @@ struct UpdateLaunch {
     int32_t dtcheck;
     double confine, gamma, heat;
     uint64_t seed;
-    double *store;                  // optional copy of the current stage record {xs ys vxs vys nc}[npad]
+    double *store;                  // optional copy of the current stage record: {xs ys vxs vys nc}[npad]
+                                    // then the box table [npad/64][4] (store_doubles(npad) in all)
 };
 
 hipError_t launch_update(int stage, const UpdateLaunch &a, hipStream_t s);
+
+// Doubles in one stored stage record (UpdateLaunch::store) and its view.
+constexpr size_t store_doubles(int64_t npad) { return 5 * (size_t)npad + (size_t)npad / 16; }
+inline StageBuf store_view(double *slot, int64_t npad) {
+    const size_t P = (size_t)npad;
+    StageBuf b{slot, slot + P, slot + 2 * P, slot + 3 * P, slot + 4 * P};
+    b.box = slot + 5 * P;
+    return b;
+}
 hipError_t launch_boundary(Scalars *sc, int closing, hipStream_t s);
 
 // Force::forceS evaluation on a stage state: out = forces(xs, ys, vxs, vys).
@@ -93,8 +124,7 @@ struct ForceOutLaunch {
     StageBuf st;
     const double *q, *m;
     const int32_t *orig;
-    const double *part;
-    int32_t slices;
+    PairSums pairs;
     double *fx, *fy;
     int64_t n, npad;
     int32_t forces;
@@ -109,11 +139,10 @@ hipError_t launch_prepare_stage(StageBuf st, int64_t n, int64_t npad, hipStream_
 // rmax2 = max |r|^2 over the stage positions (into sc->rmax2_bits, after reset).
 hipError_t launch_rmax(StageBuf st, int64_t n, Scalars *sc, hipStream_t s);
 
-// Discrete adjoint (adjoint.hip).
-hipError_t launch_adj_pairs(const double *xs, const double *ys, const double *nc, const double *q,
-                            const double *ux, const double *uy, double *part, int64_t npad,
-                            int32_t jslice, int32_t slices, int32_t i_per_lane, double kappa,
-                            double rc2p, hipStream_t s);
+// Discrete adjoint (adjoint.hip).  st: a stored stage record (with boxes).
+hipError_t launch_adj_pairs(const StageBuf &st, const double *q, const double *ux, const double *uy,
+                            double *part, int64_t n, int64_t npad, int32_t jslice, int32_t slices,
+                            double kappa, double rc2p, hipStream_t s);
 hipError_t launch_adj_begin(double *bar, double *ibar, double *u, double *g, const double *seed,
                             const double *m, int64_t n, int64_t npad, hipStream_t s);
 hipError_t launch_adj_stage(double *bar, double *ibar, double *u, double *g, int stage,

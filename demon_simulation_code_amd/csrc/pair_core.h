@@ -7,7 +7,7 @@
 namespace demon {
 namespace pair {
 
-constexpr int kChunk = 8;  // j particles per fast-path chunk
+constexpr int kChunk = 8;  // j particles per prefilter chunk (gather kernels)
 
 enum : int32_t { kFlagDtCheck = 1, kFlagGated = 4 };
 
@@ -109,32 +109,30 @@ __device__ __forceinline__ double exp_neg(double u, const ExpCoef &ec) {
     return __builtin_ldexp(p, (int)n);
 }
 
-// Pair term (i <- j) for a contributing pair, added into (ax, ay):
-//   F_i += C q_i q_j (1 + u) e^-u / r^3 * (r_i - r_j),  u = kappa r,
+// Pair scale for a contributing pair (i <- j):
+//   F_i += s (r_i - r_j),  s = C q_i q_j (1 + u) e^-u / r^3,  u = kappa r,
 // the reference's cross-pair formula (ShieldedCoulombForce.cpp:183-184);
 // cqi = C q_i.  r and 1/r^3 come from one refined reciprocal square root.
-__device__ __forceinline__ void force_fp64(double dx, double dy, double r2, double cqi, double qj,
-                                           double kappa, const ExpCoef &ec, double &ax, double &ay) {
+__device__ __forceinline__ double pair_scale_fp64(double r2, double cqi, double qj, double kappa,
+                                                  const ExpCoef &ec) {
     const double y = rsqrt_fp64(r2);
     const double u = (r2 * y) * kappa;
-    const double s = (cqi * qj) * ((1.0 + u) * exp_neg(u, ec)) * ((y * y) * y);
-    ax = fma(s, dx, ax);
-    ay = fma(s, dy, ay);
+    return (cqi * qj) * ((1.0 + u) * exp_neg(u, ec)) * ((y * y) * y);
 }
 
-// FP32 pair term: distance, exp and force in FP32 (hardware reciprocal
-// square root and 2^x); the FP64 difference is rounded once, the result is
-// accumulated in FP64.
-__device__ __forceinline__ void force_fp32(double dx, double dy, float cqi, float qj, float kappa,
-                                           double &ax, double &ay) {
-    const float fx = (float)dx, fy = (float)dy;
-    const float r2 = fmaf(fx, fx, fy * fy);
+// FP32 pair force: distance, exp and force in FP32 (hardware reciprocal
+// square root and 2^x); the FP64 difference is rounded once and the
+// components are returned in FP64 for FP64 accumulation.
+__device__ __forceinline__ void pair_force_fp32(double dx, double dy, float cqi, float qj, float kappa,
+                                                double &fx, double &fy) {
+    const float gx = (float)dx, gy = (float)dy;
+    const float r2 = fmaf(gx, gx, gy * gy);
     const float y = __builtin_amdgcn_rsqf(r2);
     const float u = (r2 * y) * kappa;
     const float e = __builtin_amdgcn_exp2f(u * -1.44269504f);
     const float s = (cqi * qj) * ((1.0f + u) * e) * ((y * y) * y);
-    ax = ax + (double)(s * fx);
-    ay = ay + (double)(s * fy);
+    fx = (double)(s * gx);
+    fy = (double)(s * gy);
 }
 
 // Radii^2 of a launch: the candidate radius (the cutoff, widened in the
@@ -165,77 +163,11 @@ struct ITile {
     int32_t wlo, whi;  // the wave's i range
 };
 
-// Exact per-pair path over all (u, r) pairs of one chunk: distance,
-// contribution test (CHECK; without it every pair is known to contribute),
-// force, dt check.  SELF: the chunk overlaps the wave's own i range, so
-// j == i is skipped.  Padding slots (j >= n, and the lanes' own i >= n) hold
-// NaN coordinates, which fail every test.
-template <int R, bool F32, bool CHECK, bool SELF>
-__device__ __forceinline__ void chunk_exact(const ITile<R> &t, const double (&cx)[kChunk],
-                                            const double (&cy)[kChunk], const double (&cq)[kChunk],
-                                            int32_t j0, int32_t n, const Cutoff &cut, float kappa_f,
-                                            const ExpCoef &ec, bool dtc, double dt2,
-                                            const int32_t *__restrict__ orig,
-                                            unsigned int *dtmin, uint64_t &n_near, double (&ax)[R],
-                                            double (&ay)[R]) {
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-        const int32_t j = j0 + u;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const double dx = t.xi[r] - cx[u];
-            const double dy = t.yi[r] - cy[u];
-            const double r2 = fma(dx, dx, dy * dy);
-            bool in = true;
-            if (CHECK) {
-                // outside the band r2 decides; inside it, the reference's test
-                in = r2 < cut.lo2;
-                const bool band = !in && r2 < cut.hi2;
-                if (__builtin_expect(__ballot(band) != 0, 0)) {
-                    if (band) in = exact_contributes(dx, dy, cut.kappa);
-                }
-            }
-            if (SELF) in = in && j != t.ibase + 64 * r;
-            n_near += (uint64_t)__builtin_popcountll(__ballot(in));
-            if (in) {
-                if (F32)
-                    force_fp32(dx, dy, t.cqf[r], (float)cq[u], kappa_f, ax[r], ay[r]);
-                else
-                    force_fp64(dx, dy, r2, t.cqi[r], cq[u], cut.kappa, ec, ax[r], ay[r]);
-            }
-            if (dtc) {
-                const bool close = r2 <= dt2;
-                if (__builtin_expect(__ballot(close) != 0, 0)) {
-                    if (close) dtcheck_pair(t.xi[r], t.yi[r], cx[u], cy[u], orig[t.ibase + 64 * r], orig[j], dtmin);
-                }
-            }
-        }
-    }
-}
-
-// One wavefront's tile: lanes own i = ibase + 64 r (r < R), j runs over
-// [j_begin, j_end) (multiples of 64, wave-uniform).  Accumulates into ax/ay
-// and adds the wave's pair counts to n_near (ordered pairs inside the
-// cutoff) and n_tested (ordered pairs not culled by the boxes).  Pointers are
-// a cloud's arrays (device order); n its particle count.
-//
-// With box tables each 64-j block is first classified against the wave's
-// i-box: blocks whose nearest points are beyond the candidate radius are
-// skipped; blocks whose farthest points lie inside the cutoff (less the band)
-// contribute every pair, so their chunks go straight to the force; the rest
-// run the per-chunk prefilter.  Without box tables every block takes the
-// prefilter path.
-template <int R, bool F32>
-__device__ __forceinline__ void wave_tile(const double *__restrict__ xs, const double *__restrict__ ys,
-                                          const double *__restrict__ nc, const double *__restrict__ q,
-                                          const int32_t *__restrict__ orig, const double *__restrict__ box,
-                                          int32_t n, int32_t ibase, int32_t j_begin, int32_t j_end,
-                                          int32_t flags, double kappa, double rc2p, Radii rad,
-                                          const ExpCoef &ec, unsigned int *dtmin, double (&ax)[R],
-                                          double (&ay)[R], uint64_t &n_near, uint64_t &n_tested) {
-    const Cutoff cut = make_cutoff(kappa, rc2p);
-    const bool dtc = (flags & kFlagDtCheck) != 0;
-    const float kappa_f = (float)kappa;
+// Padding slots (i >= n) get NaN coordinates, which fail every test.
+template <int R>
+__device__ __forceinline__ ITile<R> make_itile(const double *__restrict__ xs, const double *__restrict__ ys,
+                                               const double *__restrict__ nc, const double *__restrict__ q,
+                                               int32_t n, int32_t ibase, double rf2) {
     ITile<R> t;
     t.ibase = ibase;
     t.wlo = ibase - (int32_t)(threadIdx.x & 63);
@@ -248,72 +180,209 @@ __device__ __forceinline__ void wave_tile(const double *__restrict__ xs, const d
         t.yi[r] = i < n ? ys[i] : nan;
         t.xm[r] = -2.0 * t.xi[r];
         t.ym[r] = -2.0 * t.yi[r];
-        t.nT[r] = -(rad.rf2 + nc[i]);
+        t.nT[r] = -(rf2 + nc[i]);
         t.cqi[r] = kCoulomb * q[i];
         t.cqf[r] = (float)t.cqi[r];
     }
+    return t;
+}
+
+// Axis-aligned bounding box (xmin, xmax, ymin, ymax); empty: +inf, -inf.
+struct Box {
+    double x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ Box empty_box() {
     const double inf = __builtin_inf();
-    double bx0 = inf, bx1 = -inf, by0 = inf, by1 = -inf;  // the wave's i-box
-    if (box) {
+    return Box{inf, -inf, inf, -inf};
+}
+
+__device__ __forceinline__ Box box_union(Box a, const double *__restrict__ b) {
+    return Box{fmin(a.x0, b[0]), fmax(a.x1, b[1]), fmin(a.y0, b[2]), fmax(a.y1, b[3])};
+}
+
+// Squared gap between two boxes (0 if they overlap; +inf if either is empty).
+__device__ __forceinline__ double box_gap2(const Box &a, const Box &b) {
+    const double gx = fmax(0.0, fmax(b.x0 - a.x1, a.x0 - b.x1));
+    const double gy = fmax(0.0, fmax(b.y0 - a.y1, a.y0 - b.y1));
+    return gx * gx + gy * gy;
+}
+
+// Squared largest distance between points of two boxes.
+__device__ __forceinline__ double box_span2(const Box &a, const Box &b) {
+    const double hx = fmax(b.x1 - a.x0, a.x1 - b.x0);
+    const double hy = fmax(b.y1 - a.y0, a.y1 - b.y0);
+    return hx * hx + hy * hy;
+}
+
+enum BlockClass { kSkip, kMixed, kDense };
+
+// A 64-j block against a wave's i-box: no pair within the candidate radius
+// (skip), every pair inside the cutoff and clear of the band (dense; 1e-9
+// covers the rounding of the box arithmetic; needs no padding on either
+// side), or per-chunk prefiltering (mixed).
+__device__ __forceinline__ BlockClass classify(const Box &ib, const double *__restrict__ jbox, bool ireal,
+                                               bool jreal, const Radii &rad, const Cutoff &cut) {
+    const Box jb{jbox[0], jbox[1], jbox[2], jbox[3]};
+    if (box_gap2(ib, jb) > rad.rf2) return kSkip;
+    if (ireal && jreal && box_span2(ib, jb) * (1.0 + 1e-9) < cut.lo2) return kDense;
+    return kMixed;
+}
+
+// Any lane of the wave with a candidate pair in the chunk?  Per pair two
+// FMAs and a compare on the expanded squared distance
+//   w = (-2 x_i) x_j + (-2 y_i) y_j - (Rf^2 + nc_i) < nc_j,
+//   nc = -|r|^2 (1 - 2^-46): a superset of the pairs within Rf.
+template <int R, int C>
+__device__ __forceinline__ bool chunk_hit(const ITile<R> &t, const double (&cx)[C],
+                                          const double (&cy)[C], const double *__restrict__ nc, int32_t j0) {
+    uint64_t hit = 0;
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+        const double cn = nc[j0 + u];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const double *b = box + 4 * (t.wlo / kBoxBlock + r);
-            bx0 = fmin(bx0, b[0]);
-            bx1 = fmax(bx1, b[1]);
-            by0 = fmin(by0, b[2]);
-            by1 = fmax(by1, b[3]);
+            const double w = fma(t.xm[r], cx[u], fma(t.ym[r], cy[u], t.nT[r]));
+            hit |= __ballot(w < cn);
         }
     }
-    const bool ireal = t.whi <= n;  // no padding among the wave's i
+    return hit != 0;
+}
+
+// Launch-wide constants of the exact pair path.
+struct PairEnv {
+    Cutoff cut;
+    Radii rad;
+    float kappa_f;
+    bool dtc;
+    const int32_t *__restrict__ orig;
+    unsigned int *dtmin;
+    ExpCoef ec;
+};
+
+// Distance and contribution test of one pair: outside the band r2 decides;
+// inside it, the reference's own test (CHECK; without it the pair is known
+// to contribute).  Then the dt check of the stage-1 launch.
+template <bool CHECK>
+__device__ __forceinline__ bool pair_in(double dx, double dy, double r2, const PairEnv &e) {
+    bool in = true;
+    if (CHECK) {
+        in = r2 < e.cut.lo2;
+        const bool band = !in && r2 < e.cut.hi2;
+        if (__builtin_expect(__ballot(band) != 0, 0)) {
+            if (band) in = exact_contributes(dx, dy, e.cut.kappa);
+        }
+    }
+    return in;
+}
+
+__device__ __forceinline__ void pair_dtcheck(double xi, double yi, double xj, double yj, double r2,
+                                             int32_t i, int32_t j, const PairEnv &e) {
+    if (e.dtc) {
+        const bool close = r2 <= e.rad.dt2;
+        if (__builtin_expect(__ballot(close) != 0, 0)) {
+            if (close) dtcheck_pair(xi, yi, xj, yj, e.orig[i], e.orig[j], e.dtmin);
+        }
+    }
+}
+
+// Gather form over all (u, r) pairs of one chunk: accumulates F_i into
+// ax/ay.  SELF: the chunk overlaps the wave's own i range, so j == i is
+// skipped.
+template <int R, int C, bool F32, bool CHECK, bool SELF>
+__device__ __forceinline__ void chunk_gather(const ITile<R> &t, const double (&cx)[C],
+                                             const double (&cy)[C], const double (&cq)[C],
+                                             int32_t j0, const PairEnv &e, uint64_t &n_near,
+                                             double (&ax)[R], double (&ay)[R]) {
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+        const int32_t j = j0 + u;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const double dx = t.xi[r] - cx[u];
+            const double dy = t.yi[r] - cy[u];
+            const double r2 = fma(dx, dx, dy * dy);
+            bool in = pair_in<CHECK>(dx, dy, r2, e);
+            if (SELF) in = in && j != t.ibase + 64 * r;
+            n_near += (uint64_t)__builtin_popcountll(__ballot(in));
+            if (in) {
+                if (F32) {
+                    double fx, fy;
+                    pair_force_fp32(dx, dy, t.cqf[r], (float)cq[u], e.kappa_f, fx, fy);
+                    ax[r] = ax[r] + fx;
+                    ay[r] = ay[r] + fy;
+                } else {
+                    const double s = pair_scale_fp64(r2, t.cqi[r], cq[u], e.cut.kappa, e.ec);
+                    ax[r] = fma(s, dx, ax[r]);
+                    ay[r] = fma(s, dy, ay[r]);
+                }
+            }
+            pair_dtcheck(t.xi[r], t.yi[r], cx[u], cy[u], r2, t.ibase + 64 * r, j, e);
+        }
+    }
+}
+
+// Gather form over the 64-j blocks [j_begin, j_end) (multiples of 64,
+// wave-uniform) in chunks of C, with box classification when box tables are
+// given.
+// Adds the wave's counts to n_near (ordered pairs inside the cutoff) and
+// n_tested (ordered pairs not culled by the boxes).
+template <int R, int C, bool F32>
+__device__ __forceinline__ void gather_blocks(const ITile<R> &t, const double *__restrict__ xs,
+                                              const double *__restrict__ ys, const double *__restrict__ nc,
+                                              const double *__restrict__ q, const double *__restrict__ box,
+                                              const Box &ib, int32_t n, int32_t j_begin, int32_t j_end,
+                                              const PairEnv &e, double (&ax)[R], double (&ay)[R],
+                                              uint64_t &n_near, uint64_t &n_tested) {
+    const bool ireal = t.whi <= n;
     for (int32_t jb = j_begin; jb < j_end; jb += kBoxBlock) {
         const int32_t je = jb + kBoxBlock < j_end ? jb + kBoxBlock : j_end;
-        bool dense = false;
+        BlockClass cls = kMixed;
         if (box) {
-            const double *b = box + 4 * (jb / kBoxBlock);
-            const double gx = fmax(0.0, fmax(b[0] - bx1, bx0 - b[1]));
-            const double gy = fmax(0.0, fmax(b[2] - by1, by0 - b[3]));
-            if (gx * gx + gy * gy > rad.rf2) continue;  // no pair within the candidate radius
-            const double hx = fmax(b[1] - bx0, bx1 - b[0]);
-            const double hy = fmax(b[3] - by0, by1 - b[2]);
-            // every pair inside the cutoff, clear of the band (1e-9 covers rounding)
-            dense = ireal && jb + kBoxBlock <= n && (hx * hx + hy * hy) * (1.0 + 1e-9) < cut.lo2;
+            cls = classify(ib, box + 4 * (jb / kBoxBlock), ireal, jb + kBoxBlock <= n, e.rad, e.cut);
+            if (cls == kSkip) continue;
         }
         n_tested += (uint64_t)(je - jb) * (64 * R);
-        for (int32_t j0 = jb; j0 < je; j0 += kChunk) {
-            double cx[kChunk], cy[kChunk], cq[kChunk];
+        for (int32_t j0 = jb; j0 < je; j0 += C) {
+            double cx[C], cy[C], cq[C];
 #pragma unroll
-            for (int u = 0; u < kChunk; ++u) {
+            for (int u = 0; u < C; ++u) {
                 cx[u] = xs[j0 + u];
                 cy[u] = ys[j0 + u];
             }
-            if (!dense) {
-                // prefilter: any lane with a candidate in the chunk?
-                uint64_t hit = 0;
+            if (cls != kDense && !chunk_hit<R, C>(t, cx, cy, nc, j0)) continue;
 #pragma unroll
-                for (int u = 0; u < kChunk; ++u) {
-                    const double cn = nc[j0 + u];
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        const double w = fma(t.xm[r], cx[u], fma(t.ym[r], cy[u], t.nT[r]));
-                        hit |= __ballot(w < cn);
-                    }
-                }
-                if (!hit) continue;
-            }
-#pragma unroll
-            for (int u = 0; u < kChunk; ++u) cq[u] = q[j0 + u];
-            const bool own = j0 < t.whi && j0 + kChunk > t.wlo;
+            for (int u = 0; u < C; ++u) cq[u] = q[j0 + u];
+            const bool own = j0 < t.whi && j0 + C > t.wlo;
             if (own)
-                chunk_exact<R, F32, true, true>(t, cx, cy, cq, j0, n, cut, kappa_f, ec, dtc, rad.dt2, orig,
-                                                    dtmin, n_near, ax, ay);
-            else if (dense)
-                chunk_exact<R, F32, false, false>(t, cx, cy, cq, j0, n, cut, kappa_f, ec, dtc, rad.dt2, orig,
-                                                    dtmin, n_near, ax, ay);
+                chunk_gather<R, C, F32, true, true>(t, cx, cy, cq, j0, e, n_near, ax, ay);
+            else if (cls == kDense)
+                chunk_gather<R, C, F32, false, false>(t, cx, cy, cq, j0, e, n_near, ax, ay);
             else
-                chunk_exact<R, F32, true, false>(t, cx, cy, cq, j0, n, cut, kappa_f, ec, dtc, rad.dt2, orig,
-                                                    dtmin, n_near, ax, ay);
+                chunk_gather<R, C, F32, true, false>(t, cx, cy, cq, j0, e, n_near, ax, ay);
         }
     }
+}
+
+// One wavefront's gather tile: lanes own i = ibase + 64 r (r < R), j runs
+// over [j_begin, j_end).  Pointers are a cloud's arrays (device order), n
+// its particle count; box may be null (no culling).
+template <int R, bool F32>
+__device__ __forceinline__ void wave_tile(const double *__restrict__ xs, const double *__restrict__ ys,
+                                          const double *__restrict__ nc, const double *__restrict__ q,
+                                          const int32_t *__restrict__ orig, const double *__restrict__ box,
+                                          int32_t n, int32_t ibase, int32_t j_begin, int32_t j_end,
+                                          int32_t flags, double kappa, double rc2p, Radii rad,
+                                          const ExpCoef &ec, unsigned int *dtmin, double (&ax)[R],
+                                          double (&ay)[R], uint64_t &n_near, uint64_t &n_tested) {
+    const PairEnv e{make_cutoff(kappa, rc2p), rad, (float)kappa, (flags & kFlagDtCheck) != 0, orig, dtmin, ec};
+    const ITile<R> t = make_itile<R>(xs, ys, nc, q, n, ibase, rad.rf2);
+    Box ib = empty_box();
+    if (box) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) ib = box_union(ib, box + 4 * (t.wlo / kBoxBlock + r));
+    }
+    gather_blocks<R, kChunk, F32>(t, xs, ys, nc, q, box, ib, n, j_begin, j_end, e, ax, ay, n_near, n_tested);
 }
 
 }  // namespace pair

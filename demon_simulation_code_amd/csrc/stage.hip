@@ -33,7 +33,8 @@ __device__ __forceinline__ double da(double a1, double a23, double a4) {
 // Stage force on particle i (caller index oi) in the reference's summation order.
 __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys, double vxs,
                                             double vys, double qi, double mi,
-                                            const double *part, int32_t slices, int64_t npad,
+                                            const PairSums &ps, const unsigned long long *fmask,
+                                            int64_t npad,
                                             int64_t i, double confine, double gamma,
                                             double heat, uint64_t seed, uint64_t step,
                                             int32_t stage, int32_t oi, double &fx, double &fy) {
@@ -50,9 +51,24 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
         fy = fy + dr * vys;
     }
     if (forces & 4) {
-        for (int32_t s = 0; s < slices; ++s) {
-            fx = fx + part[(size_t)(2 * s) * npad + i];
-            fy = fy + part[(size_t)(2 * s + 1) * npad + i];
+        for (int32_t s = 0; s < ps.slices; ++s) {
+            fx = fx + ps.part[(size_t)(2 * s) * npad + i];
+            fy = fy + ps.part[(size_t)(2 * s + 1) * npad + i];
+        }
+        if (ps.react) {
+            // Newton-3 reactions of the evaluated tiles, in increasing
+            // superblock offset d (bit d - 1 of the block's flag mask)
+            const int32_t words = (ps.nsb / 2 + 63) / 64;
+            for (int32_t w = 0; w < words; ++w) {
+                unsigned long long mask = fmask[w];
+                while (mask) {
+                    const int32_t d = 64 * w + __builtin_ctzll(mask) + 1;
+                    mask &= mask - 1;
+                    const double *rr = ps.react + (size_t)(d - 1) * 2 * npad;
+                    fx = fx + rr[i];
+                    fy = fy + rr[npad + i];
+                }
+            }
         }
     }
     if (forces & 16) {
@@ -63,6 +79,23 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
         fx = fx + tv * cos(th);
         fy = fy + tv * sin(th);
     }
+}
+
+// The Newton-3 tile flags of this block's superblock (one 256-thread block
+// = one superblock) as a bitmask in LDS: bit d - 1 of fmask is set when the
+// reactions react[d - 1] were written for it.  Every thread of the block
+// must call it (it ends with a barrier).
+__device__ __forceinline__ void load_flag_mask(const PairSums &ps, unsigned long long *fmask) {
+    if (ps.react && threadIdx.x < 64) {
+        const int32_t half = ps.nsb / 2;
+        const uint32_t *fl = ps.rflag + (size_t)blockIdx.x * half;
+        for (int32_t w = 0; w < (half + 63) / 64; ++w) {
+            const int32_t k = 64 * w + (int32_t)threadIdx.x;
+            const unsigned long long m = __ballot(k < half && fl[k] != 0u);
+            if (threadIdx.x == 0) fmask[w] = m;
+        }
+    }
+    __syncthreads();
 }
 
 // Bounding box of the wave's 64 consecutive particles (one box block), over
@@ -89,9 +122,11 @@ __device__ __forceinline__ void wave_box(bool real, double x, double y, double *
 template <int STAGE>
 __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
     __shared__ double red[4];
+    __shared__ unsigned long long fmask[kMaxHalf / 64];
     Scalars *sc = a.sc;
     if (!sc->active) return;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    load_flag_mask(a.pairs, fmask);
 
     double dt;
     if (STAGE == 1) {
@@ -135,9 +170,17 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
             a.store[2 * a.npad + i] = vxs;
             a.store[3 * a.npad + i] = vys;
             a.store[4 * a.npad + i] = a.cur.nc[i];
+            if ((i & (kBoxBlock - 1)) == 0) {
+                double *bx = a.store + 5 * a.npad + 4 * (i / kBoxBlock);
+                const double *cb = a.cur.box + 4 * (i / kBoxBlock);
+                bx[0] = cb[0];
+                bx[1] = cb[1];
+                bx[2] = cb[2];
+                bx[3] = cb[3];
+            }
         }
         double fx, fy;
-        stage_force(a.forces, xs, ys, vxs, vys, qi, mi, a.part, a.slices, a.npad, i, a.confine,
+        stage_force(a.forces, xs, ys, vxs, vys, qi, mi, a.pairs, fmask, a.npad, i, a.confine,
                     a.gamma, a.heat, a.seed, sc->step, STAGE, a.orig[i], fx, fy);
         const double k = (dt * fx) / mi;
         const double l = dt * vxs;
@@ -215,11 +258,13 @@ __global__ void k_boundary(Scalars *sc, int closing) {
 }
 
 __global__ __launch_bounds__(256) void k_force_out(ForceOutLaunch a) {
+    __shared__ unsigned long long fmask[kMaxHalf / 64];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    load_flag_mask(a.pairs, fmask);
     if (i >= a.n) return;
     double fx, fy;
     stage_force(a.forces, a.st.xs[i], a.st.ys[i], a.st.vxs[i], a.st.vys[i], a.q[i], a.m[i],
-                a.part, a.slices, a.npad, i, a.confine,
+                a.pairs, fmask, a.npad, i, a.confine,
                 a.gamma, a.heat, a.seed, a.step, a.stage, a.orig[i], fx, fy);
     a.fx[i] = fx;
     a.fy[i] = fy;

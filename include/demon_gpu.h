@@ -137,6 +137,18 @@ int demon_get_pair_shape(const demon_ctx *ctx, int32_t *i_per_lane, int32_t *j_s
  * all-pairs kernels and everything else (HIP events on the context stream). */
 int demon_last_timing(const demon_ctx *ctx, double *pair_ms, double *total_ms);
 
+/* All-pairs kernel (no counterpart in the reference's interface, whose
+ * ShieldedCoulombForce always uses Newton's third law,
+ * ShieldedCoulombForce.cpp:51-56).  NEWTON3 evaluates every unordered pair
+ * once and applies it to both particles (the default while its reaction
+ * buffers, about 8 * npad^2 / 256 bytes, stay under 16 GiB); GATHER
+ * evaluates every ordered pair and keeps no reaction buffers.  Both give
+ * the reference's forces to the same tolerance. */
+#define DEMON_PAIRS_GATHER 0
+#define DEMON_PAIRS_NEWTON3 1
+int demon_set_pair_mode(demon_ctx *ctx, int32_t mode);
+int demon_get_pair_mode(const demon_ctx *ctx, int32_t *mode);
+
 /* Pair counts of the all-pairs kernels since the last upload (no counterpart
  * in the reference; instrumentation for the throughput report): ordered
  * pairs that contributed a Yukawa term (inside the cutoff), and ordered

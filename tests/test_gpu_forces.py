@@ -17,39 +17,71 @@ FP64_TOL = 1e-12
 FP32_TOL = 1e-5
 
 
+MODES = [dm.PAIRS_NEWTON3, dm.PAIRS_GATHER]
+
+
 def _ctx(n, q, forces=dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB, precision=dm.FP64,
-         heat=0.0, x=None, y=None, shape=None):
+         heat=0.0, x=None, y=None, shape=None, mode=None):
     c = dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, heat=heat, seed=11,
                    forces=forces, precision=precision)
     if x is None:
         x, y = lattice(n)
     c.upload(x, y, None, None, q, np.full(n, M0))
+    if mode is not None:
+        c.set_pair_mode(mode)
     if shape:
         c.set_pair_shape(*shape)
     return c
 
 
+def test_default_mode_is_newton3():
+    with _ctx(1024, charges(1024)) as c:
+        assert c.pair_mode() == dm.PAIRS_NEWTON3
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("n", [2, 3, 64, 255, 256, 1000, 1024, 4096])
 @pytest.mark.parametrize("stage", [1, 4])
-def test_yukawa_stage_fp64(oracle, n, stage):
+def test_yukawa_stage_fp64(oracle, n, stage, mode):
     x, y = lattice(n, seed=n)
     q = charges(n)
-    with _ctx(n, q, x=x, y=y) as c:
+    with _ctx(n, q, x=x, y=y, mode=mode) as c:
         gx, gy = c.force_stage(dm.SHIELDED_COULOMB, stage, 0, x, y)
     fx, fy = oracle.yukawa(x, y, q, KAPPA)
     scale = pair_abs_sum(x, y, q, KAPPA) + 1e-300
     assert force_rel_err(fx, fy, gx, gy, scale) <= FP64_TOL
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (1, 7), (2, 3), (4, 1), (4, 64)])
-def test_yukawa_pair_shapes(oracle, shape):
+@pytest.mark.parametrize("mode,shape", [(dm.PAIRS_GATHER, (1, 1)), (dm.PAIRS_GATHER, (1, 7)),
+                                        (dm.PAIRS_GATHER, (2, 3)), (dm.PAIRS_GATHER, (4, 1)),
+                                        (dm.PAIRS_GATHER, (4, 64)), (dm.PAIRS_NEWTON3, (1, 1)),
+                                        (dm.PAIRS_NEWTON3, (1, 2)), (dm.PAIRS_NEWTON3, (1, 5))])
+def test_yukawa_pair_shapes(oracle, mode, shape):
     n = 3000
     x, y = lattice(n, seed=5)
     q = charges(n)
-    with _ctx(n, q, x=x, y=y, shape=shape) as c:
+    with _ctx(n, q, x=x, y=y, shape=shape, mode=mode) as c:
         gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 2, 0, x, y)
     fx, fy = oracle.yukawa(x, y, q, KAPPA)
     assert force_rel_err(fx, fy, gx, gy, pair_abs_sum(x, y, q, KAPPA)) <= FP64_TOL
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("precision,tol", [(dm.FP64, FP64_TOL), (dm.FP32_PAIR, FP32_TOL)])
+def test_yukawa_dense_cluster(oracle, mode, precision, tol):
+    """A compact cluster (60 um lattice, 2.1 mm across, inside one cutoff
+    radius): every pair contributes, every block pair takes the straight-line
+    path, and the Newton-3 kernel evaluates every tile."""
+    n = 4096
+    x, y = lattice(n, seed=9, spacing=60e-6, jitter=6e-6)
+    q = charges(n)
+    with _ctx(n, q, x=x, y=y, mode=mode, precision=precision) as c:
+        gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y)
+        near, tested = c.pair_counts()
+    fx, fy = oracle.yukawa(x, y, q, KAPPA)
+    assert near == n * (n - 1)  # every ordered pair contributes
+    assert tested == n * n      # no block pair culled (the diagonal blocks include i == j)
+    assert force_rel_err(fx, fy, gx, gy, pair_abs_sum(x, y, q, KAPPA)) <= tol
 
 
 def test_all_stage_forces_fp64(oracle):
@@ -87,7 +119,8 @@ def test_force_stage_adds_into_arrays(oracle):
     assert force_rel_err(fx, fy, gx, gy, scale) <= FP64_TOL
 
 
-def test_cutoff_boundary_exact(oracle):
+@pytest.mark.parametrize("mode", MODES)
+def test_cutoff_boundary_exact(oracle, mode):
     """Pairs placed 1 ulp either side of u = 10 are classified as the reference does."""
     lam = 1.0 / KAPPA
     r_cut = 10.0 * lam
@@ -102,7 +135,7 @@ def test_cutoff_boundary_exact(oracle):
     x, y = np.array(xs), np.array(ys)
     n = len(x)
     q = np.full(n, Q0)
-    with _ctx(n, q, x=x, y=y) as c:
+    with _ctx(n, q, x=x, y=y, mode=mode) as c:
         gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y)
     fx, fy = oracle.yukawa(x, y, q, KAPPA)
     np.testing.assert_array_equal(fx == 0.0, gx == 0.0)
@@ -110,29 +143,51 @@ def test_cutoff_boundary_exact(oracle):
     assert force_rel_err(fx, fy, gx, gy, scale) <= FP64_TOL
 
 
-def test_newton3_full_size():
+@pytest.mark.parametrize("mode", MODES)
+def test_newton3_full_size(mode):
     """N = 131072: sum of Yukawa forces vanishes (Newton's third law) to rounding."""
     n = 131072
     x, y = lattice(n)
     q = charges(n)
-    with _ctx(n, q, x=x, y=y) as c:
+    with _ctx(n, q, x=x, y=y, mode=mode) as c:
         gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y)
     rows = np.random.default_rng(0).choice(n, 512, replace=False)
     scale = pair_abs_sum(x, y, q, KAPPA, rows).mean() * n
     assert abs(gx.sum()) / scale < 1e-12 and abs(gy.sum()) / scale < 1e-12
 
 
-def test_sampled_rows_full_size(oracle):
-    """N = 131072: 512 sampled particles against the oracle's gather."""
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("spacing", [500e-6, 12e-6])
+def test_sampled_rows_full_size(oracle, mode, spacing):
+    """N = 131072: 512 sampled particles against the oracle's gather, on the
+    benchmark lattice (500 um: most block pairs culled) and on a lattice as
+    compact as the imploded benchmark cloud (12 um: 4.6 mm across, every pair
+    inside the cutoff)."""
     n = 131072
-    x, y = lattice(n)
+    x, y = lattice(n, spacing=spacing, jitter=0.1 * spacing)
     q = charges(n)
     rows = np.random.default_rng(1).choice(n, 512, replace=False)
-    with _ctx(n, q, x=x, y=y) as c:
+    with _ctx(n, q, x=x, y=y, mode=mode) as c:
         gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y)
     fx, fy = oracle.yukawa_rows(x, y, q, KAPPA, rows)
     scale = pair_abs_sum(x, y, q, KAPPA, rows)
     assert force_rel_err(fx, fy, gx[rows], gy[rows], scale) <= FP64_TOL
+
+
+def test_newton3_equals_gather_full_size():
+    """N = 131072 on a 40 um lattice (mixed culled, straight-line and
+    prefiltered tiles): the two kernels agree to rounding on every particle."""
+    n = 131072
+    x, y = lattice(n, spacing=40e-6, jitter=4e-6)
+    q = charges(n)
+    out = []
+    for mode in MODES:
+        with _ctx(n, q, x=x, y=y, mode=mode) as c:
+            out.append(c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y))
+    rows = np.random.default_rng(2).choice(n, 256, replace=False)
+    scale = pair_abs_sum(x, y, q, KAPPA, rows).mean()
+    d = np.hypot(out[0][0] - out[1][0], out[0][1] - out[1][1])
+    assert d.max() / scale <= 1e-12
 
 
 @pytest.mark.parametrize("n", [1024, 4096])

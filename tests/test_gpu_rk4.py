@@ -16,13 +16,16 @@ FORCES = dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB
 POS_TOL = 1e-9
 
 
-def _run_both(oracle, x, y, q, m, steps, dt=1e-3, shape=None, graph=False, forces=FORCES, heat=0.0):
+def _run_both(oracle, x, y, q, m, steps, dt=1e-3, shape=None, graph=False, forces=FORCES, heat=0.0,
+              mode=None):
     n = len(x)
     vx = np.zeros(n)
     vy = np.zeros(n)
     with dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, heat=heat, seed=5,
                     forces=forces) as c:
         c.upload(x, y, vx, vy, q, m)
+        if mode is not None:
+            c.set_pair_mode(mode)
         if shape:
             c.set_pair_shape(*shape)
         c.use_graph(graph)
@@ -34,12 +37,14 @@ def _run_both(oracle, x, y, q, m, steps, dt=1e-3, shape=None, graph=False, force
     return g, t, st, o
 
 
-@pytest.mark.parametrize("n,graph", [(256, False), (1024, False), (1024, True), (1000, False)])
-def test_rk4_100_steps(oracle, n, graph):
+@pytest.mark.parametrize("n,graph,mode", [(256, False, None), (1024, False, None), (1024, True, None),
+                                          (1000, False, None), (1024, False, dm.PAIRS_GATHER),
+                                          (1024, True, dm.PAIRS_GATHER)])
+def test_rk4_100_steps(oracle, n, graph, mode):
     x, y = lattice(n, seed=n)
     q = charges(n)
     m = np.full(n, M0)
-    g, t, st, o = _run_both(oracle, x, y, q, m, 100, graph=graph)
+    g, t, st, o = _run_both(oracle, x, y, q, m, 100, graph=graph, mode=mode)
     assert st.steps == 100
     assert t == o[4]
     assert pos_rel_err(o[0], o[1], g[0], g[1]) <= POS_TOL

@@ -57,7 +57,7 @@ def test_deterministic_16k_100_steps(oracle):
     assert pos_rel_err(o[0], o[1], g[0], g[1]) <= 1e-9
 
 
-def _relaxed(n, seed):
+def _relaxed(n, seed, dt_check=True):
     """The cloud damped to rest at the benchmark's physics: a 200 um lattice
     (near the equilibrium spacing, so the start is gentle) run at gamma = 30/s
     to t = 1.5 s (45 damping times).  A thermal run from the cold 500 um
@@ -66,7 +66,7 @@ def _relaxed(n, seed):
     force, sets the temperature."""
     x, y = lattice(n, seed=seed, spacing=200e-6, jitter=20e-6)
     q, m = np.full(n, Q0), np.full(n, M0)
-    with dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=30.0, forces=FORCES) as c:
+    with dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=30.0, forces=FORCES, dt_check=dt_check) as c:
         c.upload(x, y, None, None, q, m)
         t, steps = c.rk4_move(0.0, 1.5, DT)
         x, y, vx, vy = c.download()
@@ -76,12 +76,12 @@ def _relaxed(n, seed):
     return x, y
 
 
-def _gpu_snapshots(x, y, steps, every, seed):
+def _gpu_snapshots(x, y, steps, every, seed, dt_check=True):
     n = len(x)
     q, m = np.full(n, Q0), np.full(n, M0)
     out = []
     with dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, heat=HEAT, seed=seed,
-                    forces=FORCES | dm.THERMAL) as c:
+                    forces=FORCES | dm.THERMAL, dt_check=dt_check) as c:
         c.upload(x, y, None, None, q, m)
         t = 0.0
         for _ in range(steps // every):
@@ -113,10 +113,14 @@ def test_thermal_statistics_vs_reference():
 
 def test_thermal_16k_temperature():
     """N = 16384 from the relaxed cloud, 5000 steps: the kinetic temperature
-    over the second half is the 300 K that heatVal was set for."""
+    over the second half is the 300 K that heatVal was set for.  The dt check
+    is off here: the relaxed 16k cloud's core has pairs closer than 100 um,
+    where the reference's check would cut dt tenfold, and with it the
+    temperature that its fixed-magnitude thermal force maintains
+    (T ~ heatVal^2 dt); heatVal is set for dt = 1 ms."""
     n, steps, every = 16384, 5000, 250
-    x, y = _relaxed(n, seed=4)
-    _, _, gs = _gpu_snapshots(x, y, steps, every, seed=4)
+    x, y = _relaxed(n, seed=4, dt_check=False)
+    _, _, gs = _gpu_snapshots(x, y, steps, every, seed=4, dt_check=False)
     T = [kinetic_T(s[2], s[3]) for s in gs]
     Tlate = float(np.mean(T[len(T) // 2:]))
     print(f"N={n}, 5000 steps: kinetic T over the second half {Tlate:.1f} K (target {T_TARGET} K)")
