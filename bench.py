@@ -101,6 +101,83 @@ def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mod
                 tested=tested, final=(xs1, ys1, vx1, vy1), shape=shape_used, snaps=snaps)
 
 
+def run_ensemble(steps, warmup):
+    """The batched ensemble: 256 clouds, N log-uniform in [64, 8192] (multiples
+    of 4 for the dt check), lambda, charge and k per cloud, gamma = 2/s, hex
+    lattice (500 um, +-50 um), FP64, dt check on, 500 RK4 steps of 1 ms; one
+    graph-captured launch sequence per step, tile shapes from the autotuner."""
+    import demon_simulation_code_amd as dm
+    from demon_simulation_code_amd.physics import ensemble
+
+    ens = ensemble(256, seed=4, multiple_of=4)
+    sizes = [e["n"] for e in ens]
+    b = dm.Batch(sizes, [e["kappa"] for e in ens], [e["confine"] for e in ens], [e["gamma"] for e in ens])
+
+    def upload():
+        for c, e in enumerate(ens):
+            x, y = dm.hex_lattice(e["n"], 500e-6, 50e-6, 100 + c)
+            b.upload(c, x, y, None, None, np.full(e["n"], e["charge"]), np.full(e["n"], 2.8e-14))
+
+    upload()
+    b.tune(2)
+    b.use_graph(True)
+    if warmup:
+        b.steps(warmup, 1e-3)
+    upload()
+    t0 = time.perf_counter()
+    b.steps(steps, 1e-3)
+    wall = time.perf_counter() - t0
+    pairs = sum(4.0 * n * (n - 1) for n in sizes) * steps
+    out = {"value": pairs / wall, "unit": "pair interactions/s", "clouds": len(sizes), "steps": steps,
+           "n_total": int(sum(sizes)), "wall_s": wall, "device_ms": b.last_ms(),
+           "shapes": [list(b.shape(k)[:2]) for k in range(3)]}
+    b.close()
+    return out
+
+
+def run_adjoint(reps=3):
+    """The differentiable fit: N = 4096 on the 500 um lattice, 200 RK4 steps
+    of 1 ms with the dt check, loss on positions every 10th step, checkpoints
+    every 20; gradients of the 4096 charges, kappa, C and gamma (and of the
+    initial state).  One full gradient = one demon_fit_loss_grad call."""
+    import demon_simulation_code_amd as dm
+    from demon_simulation_code_amd.physics import ADJOINT_4K as wl
+
+    n, steps, obs, ckpt = wl.n, wl.steps, 10, 20
+    x, y = dm.hex_lattice(n, wl.spacing, wl.jitter, wl.seed)
+    rng = np.random.default_rng(11)
+    q_true = wl.charge * (1 + 0.1 * rng.uniform(-1, 1, n))
+    m = np.full(n, wl.mass)
+    with dm.Context(n, shielding=wl.kappa, confine=wl.confine, gamma=wl.gamma) as c:
+        c.upload(x, y, None, None, q_true, m)
+        tx, ty, t = [], [], 0.0
+        for _ in range(steps // obs):
+            t = c.rk4_steps(obs, wl.dt, t)
+            g = c.download()
+            tx.append(g[0])
+            ty.append(g[1])
+    tx, ty = np.array(tx), np.array(ty)
+    q = wl.charge * (1 + 0.1 * np.random.default_rng(12).uniform(-1, 1, n))
+    with dm.Context(n, shielding=wl.kappa * 1.02, confine=wl.confine * 0.97, gamma=wl.gamma * 1.05) as c:
+        c.upload(x, y, None, None, q, m)
+        c.fit_loss_grad(steps, wl.dt, obs, ckpt, tx, ty)  # warm-up (allocates the fit arena)
+        fwd = bwd = 0.0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            c.fit_loss_grad(steps, wl.dt, obs, ckpt, tx, ty)
+            f, b = c.fit_timing()
+            fwd += f
+            bwd += b
+        wall = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            c.fit_loss_grad(steps, wl.dt, obs, ckpt, tx, ty, grad=False)
+        loss_only = (time.perf_counter() - t0) / reps
+    return {"grads_per_s": 1.0 / wall, "unit": "full gradients/s", "n": n, "steps": steps,
+            "params": n + 3, "forward_ms": fwd / reps, "recompute_adjoint_ms": bwd / reps,
+            "gradient_over_forward": (fwd + bwd) / fwd, "loss_only_s": loss_only, "wall_s": wall}
+
+
 CPU_SUBSET = 32768
 
 
@@ -169,6 +246,8 @@ def main():
                          "phase of the trajectory; the headline workload is 0)")
     ap.add_argument("--mode", type=str, default="newton3", choices=["newton3", "gather"],
                     help="all-pairs kernel")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -262,6 +341,15 @@ def main():
             "mean_launch_ms": r32["pair_ms"] / launches,
         }
         log(f"FP32-pair done: {r32['wall']:.3f} s")
+
+    if not args.no_ensemble:
+        log("batched ensemble (256 clouds, 500 steps)")
+        out["ensemble"] = run_ensemble(500, 5)
+        log(f"ensemble done: {out['ensemble']['wall_s']:.3f} s")
+    if not args.no_adjoint:
+        log("trajectory-fit gradient (N=4096, 200 steps)")
+        out["adjoint"] = run_adjoint()
+        log(f"adjoint done: {out['adjoint']['grads_per_s']:.3f} gradients/s")
 
     if not args.no_cpu:
         log("CPU baseline (reference code on host cores)")

@@ -5,11 +5,13 @@
 // Device layout (all FP64 SoA of npad = roundup(n, 1024) entries, in the
 // device order -- a Morton order of the uploaded positions):
 //   x y vx vy q m              base state and per-particle constants
-//   sb[0], sb[1]               ping-pong stage records {xs ys vxs vys nc}
+//   sb[0], sb[1]               ping-pong stage records {xs ys vxs vys nc} and
+//                              their bounding-box tables (npad/16 doubles)
 //   sx                         scratch stage record for demon_force_stage
 //   acc1[4], acc23[4]          RK increments (k l m n) of stage 1, of 2 (+3)
 //   fout[2]                    force output of demon_force_stage
-//   part[slices][2]            all-pairs partial sums per j slice
+//   part[slices][2]            all-pairs partial sums per slice (i side)
+//   react[nsb/2][2], rflag     Newton-3 reactions and tile flags (own buffers)
 // plus orig[npad] (caller index of each device slot) and the Scalars block.
 #include <algorithm>
 #include <cmath>

@@ -90,40 +90,63 @@ def _gpu_snapshots(x, y, steps, every, seed, dt_check=True):
     return q, m, out
 
 
+def _block_mean(values, blocks=5):
+    """Mean of a correlated series and its standard error from block means."""
+    v = np.asarray(values, dtype=np.float64)
+    bm = np.array([b.mean() for b in np.array_split(v, blocks)])
+    return float(v.mean()), float(bm.std(ddof=1) / np.sqrt(blocks))
+
+
 @pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built")
 def test_thermal_statistics_vs_reference():
     """N = 512 from the relaxed cloud, 2000 steps: kinetic temperature and
     g(r) over the second half against the reference's own Runge_Kutta4 with
-    its ThermalForce (single-threaded: faster than OpenMP at this size)."""
-    n, steps, every = 512, 2000, 100
+    its ThermalForce (single-threaded: faster than OpenMP at this size).
+    Temperatures are sampled every 20 steps; each run's standard error comes
+    from five block means, and the two runs must agree within four combined
+    standard errors (and each within 15% of the 300 K heatVal was set for)."""
+    n, steps, every = 512, 2000, 20
     x, y = _relaxed(n, seed=3)
     q, m, gs = _gpu_snapshots(x, y, steps, every, seed=3)
     rec = Reference().rk4_record(x, y, q, m, KAPPA, CONFINE, GAMMA, HEAT, 7 | 16, DT, steps, every)
     late = slice(len(gs) // 2, None)
-    Tg = np.mean([kinetic_T(s[2], s[3]) for s in gs[late]])
-    Tr = np.mean([kinetic_T(r[2], r[3]) for r in rec[late]])
-    hg = np.mean([pair_hist(s[0], s[1]) for s in gs[late]], axis=0)
-    hr = np.mean([pair_hist(r[0], r[1]) for r in rec[late]], axis=0)
+    Tg, eg = _block_mean([kinetic_T(s[2], s[3]) for s in gs[late]])
+    Tr, er = _block_mean([kinetic_T(r[2], r[3]) for r in rec[late]])
+    hg = np.mean([pair_hist(s[0], s[1]) for s in gs[late][::5]], axis=0)
+    hr = np.mean([pair_hist(r[0], r[1]) for r in rec[late][::5]], axis=0)
     l1 = float(np.abs(hg - hr).sum())
-    print(f"N={n}: T_gpu={Tg:.1f} K, T_ref={Tr:.1f} K (target {T_TARGET} K), g(r) L1 distance {l1:.4f}")
-    assert abs(Tg - Tr) / Tr < 0.10
-    assert abs(Tr - T_TARGET) / T_TARGET < 0.15
+    print(f"N={n}: T_gpu={Tg:.1f}+-{eg:.1f} K, T_ref={Tr:.1f}+-{er:.1f} K (target {T_TARGET} K), "
+          f"g(r) L1 distance {l1:.4f}")
+    assert abs(Tg - Tr) <= 4.0 * np.hypot(eg, er)
+    assert abs(Tg - T_TARGET) / T_TARGET < 0.15 and abs(Tr - T_TARGET) / T_TARGET < 0.15
     assert l1 < 0.05
 
 
 def test_thermal_16k_temperature():
-    """N = 16384 from the relaxed cloud, 5000 steps: the kinetic temperature
-    over the second half is the 300 K that heatVal was set for.  The dt check
-    is off here: the relaxed 16k cloud's core has pairs closer than 100 um,
-    where the reference's check would cut dt tenfold, and with it the
-    temperature that its fixed-magnitude thermal force maintains
-    (T ~ heatVal^2 dt); heatVal is set for dt = 1 ms."""
-    n, steps, every = 16384, 5000, 250
+    """N = 16384 from the relaxed cloud: the kinetic temperature over the
+    second half of 10000 steps of 0.1 ms is the 300 K that heatVal is set for.
+    The relaxed 16k crystal's stiffest modes have omega dt ~ 1 at dt = 1 ms,
+    where RK4 driven by a force held constant over each stage no longer
+    delivers the bath temperature (about half of it at 1 ms; the reference
+    integrates the same way); at 0.1 ms the force model itself is checked.
+    The dt check is off: the core has pairs closer than 100 um, where the
+    check would cut dt and with it the temperature (T ~ heatVal^2 dt)."""
+    n, steps, every, dt = 16384, 10000, 500, 1e-4
+    heat = math.sqrt(43.2 * GAMMA * KB * T_TARGET * M0 / dt)
     x, y = _relaxed(n, seed=4, dt_check=False)
-    _, _, gs = _gpu_snapshots(x, y, steps, every, seed=4, dt_check=False)
-    T = [kinetic_T(s[2], s[3]) for s in gs]
-    Tlate = float(np.mean(T[len(T) // 2:]))
-    print(f"N={n}, 5000 steps: kinetic T over the second half {Tlate:.1f} K (target {T_TARGET} K)")
+    q, m = np.full(n, Q0), np.full(n, M0)
+    T = []
+    with dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, heat=heat, seed=4,
+                    forces=FORCES | dm.THERMAL, dt_check=False) as c:
+        c.upload(x, y, None, None, q, m)
+        t = 0.0
+        for _ in range(steps // every):
+            t = c.rk4_steps(every, dt, t)
+            g = c.download()
+            T.append(kinetic_T(g[2], g[3]))
+    Tlate, err = _block_mean(T[len(T) // 2:])
+    print(f"N={n}, {steps} steps of {dt} s: kinetic T over the second half {Tlate:.1f}+-{err:.1f} K "
+          f"(target {T_TARGET} K)")
     assert all(np.isfinite(T))
     assert abs(Tlate - T_TARGET) / T_TARGET < 0.15
 
