@@ -48,6 +48,7 @@ template <int R, bool F32>
 __global__ __launch_bounds__(256) void k_bpairs(const double *__restrict__ xs,
                                                 const double *__restrict__ ys,
                                                 const double *__restrict__ nc,
+                                                const double *__restrict__ box,
                                                 const double *__restrict__ q,
                                                 const int32_t *__restrict__ orig,
                                                 double *__restrict__ part,
@@ -74,7 +75,8 @@ __global__ __launch_bounds__(256) void k_bpairs(const double *__restrict__ xs,
 #pragma unroll
         for (int r = 0; r < R; ++r) ax[r] = ay[r] = 0.0;
         uint64_t n_near = 0, n_tested = 0;  // not reported for ensembles
-        pair::wave_tile<R, F32>(xs + cl.off, ys + cl.off, nc + cl.off, q + cl.off, orig + cl.off, nullptr,
+        pair::wave_tile<R, F32>(xs + cl.off, ys + cl.off, nc + cl.off, q + cl.off, orig + cl.off,
+                                box + cl.off / 16,
                                 cl.n, ibase, j_begin, j_end, base_flags, cl.kappa, cl.rc2p, rad,
                                 ec, &s->dtmin_bits, ax, ay, n_near, n_tested);
         double *px = part + cl.part_off + (size_t)(2 * item.slice) * cl.npad;
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(256) void k_bupdate(BUpdate a) {
     } else {
         dt = sc->dt_step;
     }
-    double r2new = 0.0;
+    double r2new = 0.0, nx = 0.0, ny = 0.0;
     if (i < cl.n) {
         const double xs = a.cur.xs[p], ys = a.cur.ys[p], vxs = a.cur.vxs[p], vys = a.cur.vys[p];
         const double qi = a.q[p], mi = a.m[p];
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(256) void k_bupdate(BUpdate a) {
         const double k = (dt * fx) / mi, l = dt * vxs, mm = (dt * fy) / mi, nn = dt * vys;
         const double x = a.x[p], y = a.y[p], vx = a.vx[p], vy = a.vy[p];
         const int64_t P = a.P;
-        double nx, ny, nvx, nvy;
+        double nvx, nvy;
         if (STAGE == 1 || STAGE == 2) {
             double *A = STAGE == 1 ? a.acc1 : a.acc23;
             A[p] = k;
@@ -212,6 +214,8 @@ __global__ __launch_bounds__(256) void k_bupdate(BUpdate a) {
         a.nxt.nc[p] = -r2 * kNormShrink;
         r2new = r2;
     }
+    // clouds start at multiples of 256, so a 64-block never spans two clouds
+    wave_box(i < cl.n, nx, ny, a.nxt.box + 4 * (p / kBoxBlock));
     if (STAGE == 4) {
         double v = r2new;
         for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
@@ -408,11 +412,11 @@ int enqueue_bpairs(demon_batch *b, const StageBuf &st, int32_t flags, int only_g
         const pair::ExpCoef ec = pair::exp_taylor_coef();
 #define BLAUNCH(RR)                                                                                        \
     if (f32)                                                                                               \
-        hipLaunchKernelGGL((k_bpairs<RR, true>), grid, blk, 0, b->stream, st.xs, st.ys, st.nc, b->q, b->orig, \
-                           b->part, b->d_clouds, it, cnt, ctr, b->d_sc, flags, ec);                       \
+        hipLaunchKernelGGL((k_bpairs<RR, true>), grid, blk, 0, b->stream, st.xs, st.ys, st.nc, st.box, b->q,  \
+                           b->orig, b->part, b->d_clouds, it, cnt, ctr, b->d_sc, flags, ec);             \
     else                                                                                                   \
-        hipLaunchKernelGGL((k_bpairs<RR, false>), grid, blk, 0, b->stream, st.xs, st.ys, st.nc, b->q, b->orig, \
-                           b->part, b->d_clouds, it, cnt, ctr, b->d_sc, flags, ec);
+        hipLaunchKernelGGL((k_bpairs<RR, false>), grid, blk, 0, b->stream, st.xs, st.ys, st.nc, st.box, b->q, \
+                           b->orig, b->part, b->d_clouds, it, cnt, ctr, b->d_sc, flags, ec);
         if (g == 0) { BLAUNCH(1) } else if (g == 1) { BLAUNCH(2) } else { BLAUNCH(4) }
 #undef BLAUNCH
         BHIP(b, hipGetLastError());
@@ -521,14 +525,14 @@ int demon_batch_create(demon_batch **out, int32_t nclouds, const int64_t *n, con
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_HIP, "hipStreamCreate failed"));
     const size_t P = (size_t)b->P;
-    if (hipMalloc((void **)&b->arena, sizeof(double) * P * 24) != hipSuccess ||
+    if (hipMalloc((void **)&b->arena, sizeof(double) * P * 25) != hipSuccess ||
         hipMalloc((void **)&b->orig, sizeof(int32_t) * P) != hipSuccess ||
         hipMalloc((void **)&b->d_block_cloud, sizeof(int32_t) * b->block_cloud.size()) != hipSuccess ||
         hipMalloc((void **)&b->d_clouds, sizeof(BCloud) * (size_t)nclouds) != hipSuccess ||
         hipMalloc((void **)&b->d_sc, sizeof(Scalars) * (size_t)nclouds) != hipSuccess ||
         hipMalloc((void **)&b->d_counters, sizeof(unsigned int) * kShapes) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_ALLOC, "hipMalloc failed"));
-    if (hipMemset(b->arena, 0, sizeof(double) * P * 24) != hipSuccess ||
+    if (hipMemset(b->arena, 0, sizeof(double) * P * 25) != hipSuccess ||
         hipMemset(b->d_sc, 0, sizeof(Scalars) * (size_t)nclouds) != hipSuccess ||
         hipMemset(b->d_counters, 0, sizeof(unsigned int) * kShapes) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_HIP, "hipMemset failed"));
@@ -540,6 +544,10 @@ int demon_batch_create(demon_batch **out, int32_t nclouds, const int64_t *n, con
     }
     b->acc1 = a; a += 4 * P;
     b->acc23 = a; a += 4 * P;
+    for (StageBuf *s : {&b->sb[0], &b->sb[1]}) {  // box tables, P/16 doubles each
+        s->box = a;
+        a += P / 16;
+    }
     if (hipMemcpy(b->d_block_cloud, b->block_cloud.data(), sizeof(int32_t) * b->block_cloud.size(),
                   hipMemcpyHostToDevice) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_HIP, "hipMemcpy failed"));
@@ -621,6 +629,7 @@ int demon_batch_upload(demon_batch *b, int32_t cloud, const double *x, const dou
         BHIP(b, hipMemcpy(rec[a] + cl.off, bs[a] + cl.off, sizeof(double) * np, hipMemcpyDeviceToDevice));
     StageBuf view{b->sb[0].xs + cl.off, b->sb[0].ys + cl.off, b->sb[0].vxs + cl.off, b->sb[0].vys + cl.off,
                   b->sb[0].nc + cl.off};
+    view.box = b->sb[0].box + cl.off / 16;  // 4 doubles per 64-particle block
     BHIP(b, launch_prepare_stage(view, n, np, b->stream));
     Scalars s{};
     s.dtmin_bits = 0x7f800000u;

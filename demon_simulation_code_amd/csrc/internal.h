@@ -36,6 +36,27 @@ constexpr int kSuper = 256;
 // superblock's tile flags as a bitmask of this many bits in LDS.
 constexpr int kMaxHalf = 1024;
 
+// Bounding box of the wave's 64 consecutive particles (one box block), over
+// the lanes with real == true; written by lane 0.  Every lane of the wave
+// must call it.
+__device__ __forceinline__ void wave_box(bool real, double x, double y, double *box) {
+    const double inf = __builtin_inf();
+    double x0 = real ? x : inf, x1 = real ? x : -inf;
+    double y0 = real ? y : inf, y1 = real ? y : -inf;
+    for (int off = 32; off > 0; off >>= 1) {
+        x0 = fmin(x0, __shfl_xor(x0, off));
+        x1 = fmax(x1, __shfl_xor(x1, off));
+        y0 = fmin(y0, __shfl_xor(y0, off));
+        y1 = fmax(y1, __shfl_xor(y1, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        box[0] = x0;
+        box[1] = x1;
+        box[2] = y0;
+        box[3] = y1;
+    }
+}
+
 // Where the all-pairs kernels leave a stage's Yukawa forces (device order):
 // part[s][2][npad] (s < slices), partial sums of the i side; with the
 // Newton-3 kernel also react[d - 1][2][npad] (d = 1 .. nsb/2), reactions

@@ -28,16 +28,26 @@ __device__ __forceinline__ bool dtcheck_covers(int32_t a, int32_t b) {
     return (hi & 3) != ((lo + 1) & 3);
 }
 
+// The lane's running minimum of the reference's FP32 squared distance over
+// the tested pairs it has seen.
 __device__ __forceinline__ void dtcheck_pair(double xi, double yi, double xj, double yj,
-                                             int32_t oi, int32_t oj, unsigned int *dtmin) {
+                                             int32_t oi, int32_t oj, float &smin) {
     if (!dtcheck_covers(oi, oj)) return;
     // (float) conversions and unfused FP32 arithmetic as loadFloatVector and
     // isWithInDistance (Integrator.cpp:162-182).
     const float sx = (float)xi - (float)xj;
     const float sy = (float)yi - (float)yj;
     const float s = sx * sx + sy * sy;
-    // Only distances at or below the first threshold (1e-4f) can reduce dt.
-    if (s <= 1.0001e-8f) atomicMin(dtmin, __float_as_uint(s));
+    smin = fminf(smin, s);
+}
+
+// End of a wave's tile: the wave's minimum into the device-wide minimum, one
+// atomic per wave.  Only distances at or below the first threshold (1e-4f)
+// can reduce dt.  Every lane of the wave must call it.
+__device__ __forceinline__ void commit_dtmin(float smin, unsigned int *dtmin) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) smin = fminf(smin, __shfl_xor(smin, m));
+    if ((threadIdx.x & 63) == 0 && smin <= 1.0001e-8f) atomicMin(dtmin, __float_as_uint(smin));
 }
 
 // Cutoff radius^2 (10/kappa)^2 of a launch and the band around it.
@@ -256,7 +266,6 @@ struct PairEnv {
     float kappa_f;
     bool dtc;
     const int32_t *__restrict__ orig;
-    unsigned int *dtmin;
     ExpCoef ec;
 };
 
@@ -277,11 +286,11 @@ __device__ __forceinline__ bool pair_in(double dx, double dy, double r2, const P
 }
 
 __device__ __forceinline__ void pair_dtcheck(double xi, double yi, double xj, double yj, double r2,
-                                             int32_t i, int32_t j, const PairEnv &e) {
+                                             int32_t i, int32_t j, const PairEnv &e, float &smin) {
     if (e.dtc) {
         const bool close = r2 <= e.rad.dt2;
         if (__builtin_expect(__ballot(close) != 0, 0)) {
-            if (close) dtcheck_pair(xi, yi, xj, yj, e.orig[i], e.orig[j], e.dtmin);
+            if (close) dtcheck_pair(xi, yi, xj, yj, e.orig[i], e.orig[j], smin);
         }
     }
 }
@@ -293,7 +302,7 @@ template <int R, int C, bool F32, bool CHECK, bool SELF>
 __device__ __forceinline__ void chunk_gather(const ITile<R> &t, const double (&cx)[C],
                                              const double (&cy)[C], const double (&cq)[C],
                                              int32_t j0, const PairEnv &e, uint64_t &n_near,
-                                             double (&ax)[R], double (&ay)[R]) {
+                                             float &smin, double (&ax)[R], double (&ay)[R]) {
 #pragma unroll
     for (int u = 0; u < C; ++u) {
         const int32_t j = j0 + u;
@@ -317,7 +326,7 @@ __device__ __forceinline__ void chunk_gather(const ITile<R> &t, const double (&c
                     ay[r] = fma(s, dy, ay[r]);
                 }
             }
-            pair_dtcheck(t.xi[r], t.yi[r], cx[u], cy[u], r2, t.ibase + 64 * r, j, e);
+            pair_dtcheck(t.xi[r], t.yi[r], cx[u], cy[u], r2, t.ibase + 64 * r, j, e, smin);
         }
     }
 }
@@ -333,7 +342,7 @@ __device__ __forceinline__ void gather_blocks(const ITile<R> &t, const double *_
                                               const double *__restrict__ q, const double *__restrict__ box,
                                               const Box &ib, int32_t n, int32_t j_begin, int32_t j_end,
                                               const PairEnv &e, double (&ax)[R], double (&ay)[R],
-                                              uint64_t &n_near, uint64_t &n_tested) {
+                                              uint64_t &n_near, uint64_t &n_tested, float &smin) {
     const bool ireal = t.whi <= n;
     for (int32_t jb = j_begin; jb < j_end; jb += kBoxBlock) {
         const int32_t je = jb + kBoxBlock < j_end ? jb + kBoxBlock : j_end;
@@ -355,11 +364,11 @@ __device__ __forceinline__ void gather_blocks(const ITile<R> &t, const double *_
             for (int u = 0; u < C; ++u) cq[u] = q[j0 + u];
             const bool own = j0 < t.whi && j0 + C > t.wlo;
             if (own)
-                chunk_gather<R, C, F32, true, true>(t, cx, cy, cq, j0, e, n_near, ax, ay);
+                chunk_gather<R, C, F32, true, true>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);
             else if (cls == kDense)
-                chunk_gather<R, C, F32, false, false>(t, cx, cy, cq, j0, e, n_near, ax, ay);
+                chunk_gather<R, C, F32, false, false>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);
             else
-                chunk_gather<R, C, F32, true, false>(t, cx, cy, cq, j0, e, n_near, ax, ay);
+                chunk_gather<R, C, F32, true, false>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);
         }
     }
 }
@@ -375,14 +384,17 @@ __device__ __forceinline__ void wave_tile(const double *__restrict__ xs, const d
                                           int32_t flags, double kappa, double rc2p, Radii rad,
                                           const ExpCoef &ec, unsigned int *dtmin, double (&ax)[R],
                                           double (&ay)[R], uint64_t &n_near, uint64_t &n_tested) {
-    const PairEnv e{make_cutoff(kappa, rc2p), rad, (float)kappa, (flags & kFlagDtCheck) != 0, orig, dtmin, ec};
+    const PairEnv e{make_cutoff(kappa, rc2p), rad, (float)kappa, (flags & kFlagDtCheck) != 0, orig, ec};
     const ITile<R> t = make_itile<R>(xs, ys, nc, q, n, ibase, rad.rf2);
     Box ib = empty_box();
     if (box) {
 #pragma unroll
         for (int r = 0; r < R; ++r) ib = box_union(ib, box + 4 * (t.wlo / kBoxBlock + r));
     }
-    gather_blocks<R, kChunk, F32>(t, xs, ys, nc, q, box, ib, n, j_begin, j_end, e, ax, ay, n_near, n_tested);
+    float smin = __builtin_inff();
+    gather_blocks<R, kChunk, F32>(t, xs, ys, nc, q, box, ib, n, j_begin, j_end, e, ax, ay, n_near, n_tested,
+                                  smin);
+    if (e.dtc) commit_dtmin(smin, dtmin);
 }
 
 }  // namespace pair

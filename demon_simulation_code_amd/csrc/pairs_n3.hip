@@ -64,8 +64,8 @@ __device__ __forceinline__ void store_wave_sums(const double (&f)[2 * kChunkN3],
 template <bool F32, bool CHECK>
 __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[kChunkN3],
                                          const double (&cy)[kChunkN3], const double (&cq)[kChunkN3],
-                                         int32_t j0, const PairEnv &e, uint64_t &n_near, double &ax,
-                                         double &ay, double *scr, double *red, int32_t jj0) {
+                                         int32_t j0, const PairEnv &e, uint64_t &n_near, float &smin,
+                                         double &ax, double &ay, double *scr, double *red, int32_t jj0) {
     double f[2 * kChunkN3];
 #pragma unroll
     for (int u = 0; u < kChunkN3; ++u) {
@@ -88,7 +88,7 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
         ay = ay + fy;
         f[u] = fx;
         f[kChunkN3 + u] = fy;
-        pair_dtcheck(t.xi[0], t.yi[0], cx[u], cy[u], r2, t.ibase, j0 + u, e);
+        pair_dtcheck(t.xi[0], t.yi[0], cx[u], cy[u], r2, t.ibase, j0 + u, e, smin);
     }
     store_wave_sums(f, scr, red, jj0);
 }
@@ -110,8 +110,8 @@ __global__ __launch_bounds__(256) void k_pairs_n3(const double *__restrict__ xs,
     __shared__ double scr[4][2 * kChunkN3 * kScrRow];      // per-wave transpose scratch
     if ((flags & kFlagGated) && !sc->active) return;
     const Radii rad = launch_radii(rc2p, flags, sc->rmax2_bits);
-    const PairEnv e{make_cutoff(kappa, rc2p), rad, (float)kappa, (flags & kFlagDtCheck) != 0, orig,
-                    &sc->dtmin_bits, ec};
+    const PairEnv e{make_cutoff(kappa, rc2p), rad, (float)kappa, (flags & kFlagDtCheck) != 0, orig, ec};
+    float smin = __builtin_inff();
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int32_t nsb = npad / kSuper;
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_pairs_n3(const double *__restrict__ xs,
         if (d == 0) {
             // the superblock with itself, every ordered pair, j == i skipped
             gather_blocks<1, kChunkN3, F32>(t, xs, ys, nc, q, box, ib, n, S * kSuper, S * kSuper + kSuper,
-                                            e, ax, ay, n_near, n_tested);
+                                            e, ax, ay, n_near, n_tested, smin);
             continue;
         }
         const int32_t T = S + d < nsb ? S + d : S + d - nsb;
@@ -165,10 +165,10 @@ __global__ __launch_bounds__(256) void k_pairs_n3(const double *__restrict__ xs,
 #pragma unroll
                 for (int u = 0; u < kChunkN3; ++u) cq[u] = q[j0 + u];
                 if (cls == kDense)
-                    chunk_n3<F32, false>(t, cx, cy, cq, j0, e, n_near, ax[0], ay[0], scr[wave], r,
+                    chunk_n3<F32, false>(t, cx, cy, cq, j0, e, n_near, smin, ax[0], ay[0], scr[wave], r,
                                          j0 - T * kSuper);
                 else
-                    chunk_n3<F32, true>(t, cx, cy, cq, j0, e, n_near, ax[0], ay[0], scr[wave], r,
+                    chunk_n3<F32, true>(t, cx, cy, cq, j0, e, n_near, smin, ax[0], ay[0], scr[wave], r,
                                         j0 - T * kSuper);
             }
         }
@@ -182,6 +182,7 @@ __global__ __launch_bounds__(256) void k_pairs_n3(const double *__restrict__ xs,
         rr[npad + T * kSuper + jj] = -ry;
         if (threadIdx.x == 0) *fl = 1u;
     }
+    if (e.dtc) commit_dtmin(smin, &sc->dtmin_bits);
     double *px = part + (size_t)(2 * blockIdx.y) * npad;
     px[ibase] = ax[0];
     px[npad + ibase] = ay[0];

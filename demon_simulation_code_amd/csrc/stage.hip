@@ -98,27 +98,6 @@ __device__ __forceinline__ void load_flag_mask(const PairSums &ps, unsigned long
     __syncthreads();
 }
 
-// Bounding box of the wave's 64 consecutive particles (one box block), over
-// the lanes with real == true; written by lane 0.  Every lane of the wave
-// must call it.
-__device__ __forceinline__ void wave_box(bool real, double x, double y, double *box) {
-    const double inf = __builtin_inf();
-    double x0 = real ? x : inf, x1 = real ? x : -inf;
-    double y0 = real ? y : inf, y1 = real ? y : -inf;
-    for (int off = 32; off > 0; off >>= 1) {
-        x0 = fmin(x0, __shfl_xor(x0, off));
-        x1 = fmax(x1, __shfl_xor(x1, off));
-        y0 = fmin(y0, __shfl_xor(y0, off));
-        y1 = fmax(y1, __shfl_xor(y1, off));
-    }
-    if ((threadIdx.x & 63) == 0) {
-        box[0] = x0;
-        box[1] = x1;
-        box[2] = y0;
-        box[3] = y1;
-    }
-}
-
 template <int STAGE>
 __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
     __shared__ double red[4];
