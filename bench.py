@@ -39,10 +39,36 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK = 78.6e12       # MI355X FP64 vector, FLOP/s
 FLOP_FAR, FLOP_NEAR = 7, 20
+# HBM traffic of the FP64 all-pairs kernel per launch, averaged over the
+# timed launches of the default run: FETCH_SIZE and WRITE_SIZE from two
+# rocprofv3 --pmc passes, scaled by what they read for a plain copy of known
+# size (the summary file gives the commands, the counts and the scale).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pairs_n3_hex131072_fp64_traffic.md")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(workload, steps, warmup, start, mode):
+    """Bytes per launch from the committed counter summary, when it was taken
+    on the same workload, step count, start step and pair kernel as this run;
+    None otherwise."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    fields = {}
+    for line in text.splitlines():
+        if line.startswith("traffic:"):
+            for kv in line[len("traffic:"):].split():
+                k, _, v = kv.partition("=")
+                fields[k] = v
+    want = {"workload": workload, "steps": str(steps), "warmup": str(warmup), "start_step": str(start),
+            "pair_kernel": mode}
+    if any(fields.get(k) != v for k, v in want.items()) or "bytes_per_launch" not in fields:
+        return None
+    return float(fields["bytes_per_launch"])
 
 
 def setup(wl):
@@ -283,7 +309,8 @@ def main():
         "peak": FP64_PEAK / 1e12,
         "unit": "TFLOP/s",
         "frac": achieved / FP64_PEAK,
-        "traffic": None,
+        "traffic": measured_traffic(wl.name + "_rk4", K, W, args.start_step, args.mode),
+        "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
         "kernel": kernel + " (FP64 all-pairs)",
         "mean_launch_ms": pair_s * 1e3 / launches,
         "flop_per_launch": flop_total / launches,
