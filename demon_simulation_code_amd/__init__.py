@@ -11,10 +11,13 @@ from ._lib import (  # noqa: F401
     FP32_PAIR,
     FP64,
     LIB_PATH,
+    BATCH_NEWTON3,
     PAIRS_GATHER,
     PAIRS_NEWTON3,
+    MAGNETIC,
     SHIELDED_COULOMB,
     THERMAL,
+    TIME_VARYING_DRAG,
     Context,
     DemonError,
     Params,
@@ -26,7 +29,7 @@ from .physics import ELECTRON_CHARGE, EPSILON0, COULOMB, confine_from_k  # noqa:
 
 __all__ = [
     "Batch", "Context", "DemonError", "Params", "Stats", "hex_lattice", "lib", "LIB_PATH",
-    "CONFINEMENT", "DRAG", "SHIELDED_COULOMB", "THERMAL", "FP64", "FP32_PAIR",
-    "PAIRS_GATHER", "PAIRS_NEWTON3",
+    "CONFINEMENT", "DRAG", "SHIELDED_COULOMB", "THERMAL", "TIME_VARYING_DRAG", "MAGNETIC", "FP64", "FP32_PAIR",
+    "BATCH_NEWTON3", "PAIRS_GATHER", "PAIRS_NEWTON3",
     "ELECTRON_CHARGE", "EPSILON0", "COULOMB", "confine_from_k",
 ]

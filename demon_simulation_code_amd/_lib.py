@@ -27,11 +27,14 @@ CONFINEMENT = 1
 DRAG = 2
 SHIELDED_COULOMB = 4
 THERMAL = 16
+TIME_VARYING_DRAG = 256
+MAGNETIC = 1024
 
 FP64 = 0
 FP32_PAIR = 1
 PAIRS_GATHER = 0
 PAIRS_NEWTON3 = 1
+BATCH_NEWTON3 = 0  # Batch.set_shape i_per_lane: the Newton-3 form
 
 
 class DemonError(RuntimeError):
@@ -53,6 +56,8 @@ class Params(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("dt_check", ctypes.c_int32),
         ("device", ctypes.c_int32),
+        ("magnetic_field", ctypes.c_double),
+        ("drag_scale", ctypes.c_double),
     ]
 
 
@@ -83,8 +88,12 @@ SIGNATURES = {
     "demon_upload": (ctypes.c_int, [_CTX, _D, _D, _D, _D, _D, _D]),
     "demon_download": (ctypes.c_int, [_CTX, _D, _D, _D, _D]),
     "demon_force_stage": (ctypes.c_int, [_CTX, _I32, _I32, ctypes.c_uint64, _D, _D, _D, _D, _D, _D]),
+    "demon_force_stage_at": (ctypes.c_int, [_CTX, _I32, _I32, ctypes.c_uint64, ctypes.c_double, _D, _D, _D, _D,
+                                            _D, _D]),
     "demon_rk4_move": (ctypes.c_int, [_CTX, _D, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_I64)]),
     "demon_rk4_steps": (ctypes.c_int, [_CTX, _I64, ctypes.c_double, _D]),
+    "demon_rk2_move": (ctypes.c_int, [_CTX, _D, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_I64)]),
+    "demon_rk2_steps": (ctypes.c_int, [_CTX, _I64, ctypes.c_double, _D]),
     "demon_use_graph": (ctypes.c_int, [_CTX, _I32]),
     "demon_get_stats": (ctypes.c_int, [_CTX, ctypes.POINTER(Stats)]),
     "demon_synchronize": (ctypes.c_int, [_CTX]),
@@ -158,10 +167,11 @@ class Context:
 
     def __init__(self, n: int, *, shielding: float, confine: float = 0.0, gamma: float = 0.0,
                  heat: float = 0.0, seed: int = 0, forces: int = CONFINEMENT | DRAG | SHIELDED_COULOMB,
-                 precision: int = FP64, dt_check: bool = True, device: int = 0):
+                 precision: int = FP64, dt_check: bool = True, device: int = 0,
+                 magnetic_field: float = 0.0, drag_scale: float = 0.0):
         self.n = int(n)
         self.params = Params(shielding, confine, gamma, heat, seed, forces, precision,
-                             1 if dt_check else 0, device)
+                             1 if dt_check else 0, device, magnetic_field, drag_scale)
         h = _CTX()
         _check(lib.demon_create(ctypes.byref(h), self.n, ctypes.byref(self.params)))
         self._h = h
@@ -201,11 +211,17 @@ class Context:
         self._c(lib.demon_download(self._h, *[_ptr(a) for a in out]))
         return tuple(out)
 
-    def force_stage(self, forces, stage, step, xs, ys, vxs=None, vys=None, fx=None, fy=None):
+    def force_stage(self, forces, stage, step, xs, ys, vxs=None, vys=None, fx=None, fy=None, t=None):
+        """Adds the stage force into fx/fy; t is the stage time (required for
+        TIME_VARYING_DRAG)."""
         fx = np.zeros(self.n) if fx is None else fx
         fy = np.zeros(self.n) if fy is None else fy
-        self._c(lib.demon_force_stage(self._h, forces, stage, step, _ptr(xs), _ptr(ys), _ptr(vxs),
-                                      _ptr(vys), _ptr(fx), _ptr(fy)))
+        if t is None:
+            self._c(lib.demon_force_stage(self._h, forces, stage, step, _ptr(xs), _ptr(ys), _ptr(vxs),
+                                          _ptr(vys), _ptr(fx), _ptr(fy)))
+        else:
+            self._c(lib.demon_force_stage_at(self._h, forces, stage, step, t, _ptr(xs), _ptr(ys), _ptr(vxs),
+                                             _ptr(vys), _ptr(fx), _ptr(fy)))
         return fx, fy
 
     def rk4_steps(self, steps: int, dt: float, t: float = 0.0) -> float:
@@ -217,6 +233,18 @@ class Context:
         tt = ctypes.c_double(t)
         st = _I64(0)
         self._c(lib.demon_rk4_move(self._h, ctypes.byref(tt), end_time, dt, ctypes.byref(st)))
+        return tt.value, st.value
+
+    def rk2_steps(self, steps: int, dt: float, t: float = 0.0) -> float:
+        """Runge_Kutta2 (midpoint) steps; returns the time."""
+        tt = ctypes.c_double(t)
+        self._c(lib.demon_rk2_steps(self._h, int(steps), dt, ctypes.byref(tt)))
+        return tt.value
+
+    def rk2_move(self, t: float, end_time: float, dt: float):
+        tt = ctypes.c_double(t)
+        st = _I64(0)
+        self._c(lib.demon_rk2_move(self._h, ctypes.byref(tt), end_time, dt, ctypes.byref(st)))
         return tt.value, st.value
 
     def use_graph(self, on: bool = True):
@@ -354,6 +382,8 @@ class Batch:
         return v.value
 
     def set_shape(self, bucket, i_per_lane, jslice):
+        """i_per_lane 1, 2, 4: gather form, j slices of jslice particles;
+        BATCH_NEWTON3: Newton-3 form, jslice superblock offsets per item."""
         self._c(lib.demon_batch_set_shape(self._h, bucket, i_per_lane, jslice))
 
     def shape(self, bucket):

@@ -38,6 +38,7 @@ struct demon_ctx {
     int32_t *d_orig = nullptr;
     Scalars *d_sc = nullptr;
     double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *q = nullptr, *m = nullptr;
+    float *qf = nullptr;  // the charges rounded to FP32 (FP32 pair arithmetic)
     StageBuf sb[2]{}, sx{};
     double *acc1 = nullptr, *acc23 = nullptr, *fout = nullptr;
     std::vector<int64_t> perm;   // perm[k] = caller index stored in device slot k
@@ -51,6 +52,7 @@ struct demon_ctx {
     double *d_react = nullptr;   // Newton-3 reactions [nsb/2][2][npad]
     uint32_t *d_rflag = nullptr; // Newton-3 tile flags [nsb][nsb/2]
     bool use_graph = false;
+    int32_t order = 4;           // integrator of the captured step: 4 (RK4) or 2 (RK2)
     hipGraphExec_t graph = nullptr;
     bool timing = true;
     std::vector<hipEvent_t> ev;  // pairs of events around each all-pairs launch
@@ -92,12 +94,14 @@ int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 int check_params(demon_ctx *c, const demon_params *p) {
     if (!p) return fail(c, DEMON_ERR_ARG, "params is NULL");
-    const int32_t known = DEMON_FORCE_CONFINEMENT | DEMON_FORCE_DRAG |
-                          DEMON_FORCE_SHIELDED_COULOMB | DEMON_FORCE_THERMAL;
+    const int32_t known = DEMON_FORCE_CONFINEMENT | DEMON_FORCE_DRAG | DEMON_FORCE_SHIELDED_COULOMB |
+                          DEMON_FORCE_THERMAL | DEMON_FORCE_TIME_VARYING_DRAG | DEMON_FORCE_MAGNETIC;
     if (p->forces & ~known)
         return fail(c, DEMON_ERR_UNSUPPORTED,
-                    "force flags outside Confinement|Drag|ShieldedCoulomb|Thermal are not "
-                    "implemented on the device");
+                    "force flags outside Confinement|Drag|ShieldedCoulomb|Thermal|TimeVaryingDrag|"
+                    "Magnetic are not implemented on the device");
+    if (!std::isfinite(p->magnetic_field) || !std::isfinite(p->drag_scale))
+        return fail(c, DEMON_ERR_ARG, "magnetic field and drag scale must be finite");
     if ((p->forces & DEMON_FORCE_SHIELDED_COULOMB) && !(p->shielding > 0.0 && std::isfinite(p->shielding)))
         return fail(c, DEMON_ERR_ARG, "shielding constant must be positive and finite");
     if (p->precision != DEMON_PRECISION_FP64 && p->precision != DEMON_PRECISION_FP32_PAIR)
@@ -220,7 +224,7 @@ int put(demon_ctx *c, double *dst, const double *src, double fill, size_t slot) 
 
 int enqueue_pairs(demon_ctx *c, const StageBuf &st, int gated, int dtcheck) {
     PairLaunch a{};
-    a.xs = st.xs; a.ys = st.ys; a.nc = st.nc; a.q = c->q; a.orig = c->d_orig; a.box = st.box;
+    a.xs = st.xs; a.ys = st.ys; a.nc = st.nc; a.q = c->q; a.qf = c->qf; a.orig = c->d_orig; a.box = st.box;
     a.part = c->d_part; a.sc = c->d_sc; a.n = c->n; a.npad = c->npad;
     a.jslice = c->jslice; a.slices = c->slices; a.i_per_lane = c->i_per_lane;
     a.dtcheck = dtcheck; a.gated = gated;
@@ -257,8 +261,9 @@ int enqueue_pairs(demon_ctx *c, const StageBuf &st, int gated, int dtcheck) {
 
 UpdateLaunch make_update(demon_ctx *c, int stage, int dtcheck, double *store) {
     UpdateLaunch u{};
-    // stage records: 1 reads sb0, 2 reads sb1, 3 reads sb0, 4 reads sb1
-    const int cur = (stage - 1) & 1;
+    // stage records: 1 reads sb0, 2 reads sb1, 3 reads sb0, 4 reads sb1;
+    // RK2's second stage reads sb1
+    const int cur = stage == kRk2Final ? 1 : (stage - 1) & 1;
     u.cur = c->sb[cur];
     u.nxt = c->sb[cur ^ 1];
     u.x = c->x; u.y = c->y; u.vx = c->vx; u.vy = c->vy; u.q = c->q; u.m = c->m;
@@ -266,14 +271,30 @@ UpdateLaunch make_update(demon_ctx *c, int stage, int dtcheck, double *store) {
     u.acc1 = c->acc1; u.acc23 = c->acc23; u.sc = c->d_sc;
     u.n = c->n; u.npad = c->npad; u.forces = c->p.forces; u.dtcheck = dtcheck;
     u.confine = c->p.confine; u.gamma = c->p.drag_gamma; u.heat = c->p.heat_val; u.seed = c->p.seed;
+    u.bfield = c->p.magnetic_field; u.drag_scale = c->p.drag_scale;
     u.store = store;
     return u;
 }
 
-// One RK4 step.  store (optional): 4 consecutive stage slots of 5*npad doubles.
+// One step of the context's integrator (c->order): RK4 stages 1..4, or RK2's
+// stage 1 and kRk2Final.  store (optional, RK4 only): 4 consecutive stage
+// slots of 5*npad doubles.
 int enqueue_step_ex(demon_ctx *c, int dtcheck, double *store) {
     HIP_TRY(c, launch_boundary(c->d_sc, 0, c->stream));
     const bool yk = c->p.forces & DEMON_FORCE_SHIELDED_COULOMB;
+    if (c->order == 2) {
+        for (int s : {1, (int)kRk2Final}) {
+            if (yk) {
+                const int rc = enqueue_pairs(c, c->sb[s == 1 ? 0 : 1], 1, s == 1 && dtcheck);
+                if (rc) return rc;
+            } else if (s == 1 && dtcheck) {
+                return fail(c, DEMON_ERR_UNSUPPORTED,
+                            "the dt check runs inside the all-pairs pass; enable ShieldedCoulomb");
+            }
+            HIP_TRY(c, launch_update(s, make_update(c, s, dtcheck, nullptr), c->stream));
+        }
+        return DEMON_OK;
+    }
     for (int s = 1; s <= 4; ++s) {
         const StageBuf &st = c->sb[(s - 1) & 1];
         if (yk) {
@@ -352,7 +373,7 @@ int run_steps(demon_ctx *c, int64_t steps, Scalars *out) {
         HIP_TRY(c, hipEventElapsedTime(&pm, c->ev[k], c->ev[k + 1]));
         c->last_pair_ms += pm;
     }
-    c->force_evals += 4 * steps;
+    c->force_evals += c->order * steps;
     c->rng_step = out->step;
     if (out->err == 2)
         return fail(c, DEMON_ERR_STATE,
@@ -411,6 +432,62 @@ double host_sum(demon_ctx *c, const double *dev, std::vector<double> &tmp, int *
     return acc;
 }
 
+// Selects the integrator whose step is queued (and captured).
+void set_order(demon_ctx *c, int32_t order) {
+    if (c->order != order) {
+        drop_graph(c);
+        c->order = order;
+    }
+}
+
+int rk_steps(demon_ctx *c, int32_t order, int64_t steps, double init_dt, double *current_time) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
+    if (!current_time || steps < 0 || !(init_dt > 0.0)) return fail(c, DEMON_ERR_ARG, "bad steps/dt/time");
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    set_order(c, order);
+    c->last_pair_ms = c->last_total_ms = 0.0;
+    hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, *current_time,
+                       std::numeric_limits<double>::infinity(), init_dt,
+                       (unsigned long long)steps);
+    HIP_TRY(c, hipGetLastError());
+    Scalars s{};
+    const int rc = run_steps(c, steps, &s);
+    if (rc) return rc;
+    *current_time = s.t;
+    return DEMON_OK;
+}
+
+int rk_move(demon_ctx *c, int32_t order, double *current_time, double end_time, double init_dt,
+            int64_t *steps_done) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
+    if (!current_time || !(init_dt > 0.0)) return fail(c, DEMON_ERR_ARG, "bad dt/time");
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    set_order(c, order);
+    c->last_pair_ms = c->last_total_ms = 0.0;
+    Scalars s{};
+    HIP_TRY(c, hipMemcpy(&s, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    const unsigned long long before = s.steps_done;
+    hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, *current_time,
+                       end_time, init_dt, ~0ull);
+    HIP_TRY(c, hipGetLastError());
+    double t = *current_time;
+    for (;;) {
+        // steps the while loop needs if dt stays init_dt (same accumulation)
+        int64_t est = 0;
+        for (double tt = t; tt < end_time && est < 4096; tt += init_dt) ++est;
+        if (est == 0) break;
+        const int rc = run_steps(c, est, &s);
+        if (rc) return rc;
+        if (!(s.t > t)) return fail(c, DEMON_ERR_STATE, "time does not advance (dt underflow)");
+        t = s.t;
+    }
+    *current_time = t;
+    if (steps_done) *steps_done = (int64_t)(s.steps_done - before);
+    return DEMON_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -465,10 +542,13 @@ int demon_create(demon_ctx **out, int64_t n, const demon_params *p) {
     c->acc23 = b; b += 4 * P;
     c->fout = b; b += 2 * P;
     // one more array holds the three stages' box tables (P/16 doubles each)
+    // and the FP32 charges (P/2 doubles)
     for (StageBuf *s : {&c->sb[0], &c->sb[1], &c->sx}) {
         s->box = b;
         b += P / 16;
     }
+    c->qf = reinterpret_cast<float *>(b);
+    b += P / 2;
     for (int k = 0; k < 2; ++k) {
         e = hipEventCreate(&c->ev_call[k]);
         if (e != hipSuccess) return bail(hip_fail(c, e, "hipEventCreate"));
@@ -538,6 +618,7 @@ int demon_upload(demon_ctx *c, const double *x, const double *y, const double *v
         return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if ((rc = put(c, c->q, charge, 0.0, 0)) || (rc = put(c, c->m, mass, 1.0, 1))) return rc;
+    HIP_TRY(c, launch_round_fp32(c->q, c->qf, c->npad, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     // stage-1 record = base state
     const size_t bytes = sizeof(double) * c->npad;
@@ -575,13 +656,16 @@ int demon_download(demon_ctx *c, double *x, double *y, double *vx, double *vy) {
     return DEMON_OK;
 }
 
-int demon_force_stage(demon_ctx *c, int32_t forces, int32_t stage, uint64_t step,
-                      const double *xs, const double *ys, const double *vxs,
-                      const double *vys, double *fx, double *fy) {
+int demon_force_stage_at(demon_ctx *c, int32_t forces, int32_t stage, uint64_t step, double t,
+                         const double *xs, const double *ys, const double *vxs,
+                         const double *vys, double *fx, double *fy) {
     if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
     if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "demon_upload must provide charge and mass first");
     if (!xs || !ys || !fx || !fy) return fail(c, DEMON_ERR_ARG, "NULL array");
-    if ((forces & DEMON_FORCE_DRAG) && (!vxs || !vys)) return fail(c, DEMON_ERR_ARG, "drag needs velocities");
+    const int32_t needs_v = DEMON_FORCE_DRAG | DEMON_FORCE_MAGNETIC | DEMON_FORCE_TIME_VARYING_DRAG;
+    if ((forces & needs_v) && (!vxs || !vys))
+        return fail(c, DEMON_ERR_ARG, "drag and magnetic forces need velocities");
+    if (!std::isfinite(t)) return fail(c, DEMON_ERR_ARG, "stage time must be finite");
     if (forces & ~c->p.forces) return fail(c, DEMON_ERR_ARG, "force not enabled in the context");
     if (stage < 1 || stage > 4) return fail(c, DEMON_ERR_ARG, "stage must be 1..4");
     HIP_TRY(c, hipSetDevice(c->p.device));
@@ -597,6 +681,7 @@ int demon_force_stage(demon_ctx *c, int32_t forces, int32_t stage, uint64_t step
     f.st = c->sx; f.q = c->q; f.m = c->m; f.orig = c->d_orig; f.pairs = pair_sums(c);
     f.fx = c->fout; f.fy = c->fout + c->npad; f.n = c->n; f.npad = c->npad;
     f.forces = forces; f.confine = c->p.confine; f.gamma = c->p.drag_gamma; f.heat = c->p.heat_val;
+    f.bfield = c->p.magnetic_field; f.drag_scale = c->p.drag_scale; f.t = t;
     f.seed = c->p.seed; f.step = step; f.stage = stage;
     HIP_TRY(c, launch_force_out(f, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->h_tmp.data(), c->fout, sizeof(double) * 2 * c->npad,
@@ -611,50 +696,30 @@ int demon_force_stage(demon_ctx *c, int32_t forces, int32_t stage, uint64_t step
     return DEMON_OK;
 }
 
+int demon_force_stage(demon_ctx *c, int32_t forces, int32_t stage, uint64_t step,
+                      const double *xs, const double *ys, const double *vxs,
+                      const double *vys, double *fx, double *fy) {
+    if (forces & DEMON_FORCE_TIME_VARYING_DRAG)
+        return fail(c, DEMON_ERR_ARG, "TimeVaryingDrag needs the stage time: use demon_force_stage_at");
+    return demon_force_stage_at(c, forces, stage, step, 0.0, xs, ys, vxs, vys, fx, fy);
+}
+
 int demon_rk4_steps(demon_ctx *c, int64_t steps, double init_dt, double *current_time) {
-    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
-    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
-    if (!current_time || steps < 0 || !(init_dt > 0.0)) return fail(c, DEMON_ERR_ARG, "bad steps/dt/time");
-    HIP_TRY(c, hipSetDevice(c->p.device));
-    c->last_pair_ms = c->last_total_ms = 0.0;
-    hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, *current_time,
-                       std::numeric_limits<double>::infinity(), init_dt,
-                       (unsigned long long)steps);
-    HIP_TRY(c, hipGetLastError());
-    Scalars s{};
-    const int rc = run_steps(c, steps, &s);
-    if (rc) return rc;
-    *current_time = s.t;
-    return DEMON_OK;
+    return rk_steps(c, 4, steps, init_dt, current_time);
 }
 
 int demon_rk4_move(demon_ctx *c, double *current_time, double end_time, double init_dt,
                    int64_t *steps_done) {
-    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
-    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
-    if (!current_time || !(init_dt > 0.0)) return fail(c, DEMON_ERR_ARG, "bad dt/time");
-    HIP_TRY(c, hipSetDevice(c->p.device));
-    c->last_pair_ms = c->last_total_ms = 0.0;
-    Scalars s{};
-    HIP_TRY(c, hipMemcpy(&s, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost));
-    const unsigned long long before = s.steps_done;
-    hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, *current_time,
-                       end_time, init_dt, ~0ull);
-    HIP_TRY(c, hipGetLastError());
-    double t = *current_time;
-    for (;;) {
-        // steps the while loop needs if dt stays init_dt (same accumulation)
-        int64_t est = 0;
-        for (double tt = t; tt < end_time && est < 4096; tt += init_dt) ++est;
-        if (est == 0) break;
-        const int rc = run_steps(c, est, &s);
-        if (rc) return rc;
-        if (!(s.t > t)) return fail(c, DEMON_ERR_STATE, "time does not advance (dt underflow)");
-        t = s.t;
-    }
-    *current_time = t;
-    if (steps_done) *steps_done = (int64_t)(s.steps_done - before);
-    return DEMON_OK;
+    return rk_move(c, 4, current_time, end_time, init_dt, steps_done);
+}
+
+int demon_rk2_steps(demon_ctx *c, int64_t steps, double init_dt, double *current_time) {
+    return rk_steps(c, 2, steps, init_dt, current_time);
+}
+
+int demon_rk2_move(demon_ctx *c, double *current_time, double end_time, double init_dt,
+                   int64_t *steps_done) {
+    return rk_move(c, 2, current_time, end_time, init_dt, steps_done);
 }
 
 int demon_use_graph(demon_ctx *c, int32_t enable) {
@@ -753,6 +818,7 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     const int32_t allowed = DEMON_FORCE_CONFINEMENT | DEMON_FORCE_DRAG | DEMON_FORCE_SHIELDED_COULOMB;
     if ((c->p.forces & ~allowed) || !(c->p.forces & DEMON_FORCE_SHIELDED_COULOMB))
         return fail(c, DEMON_ERR_UNSUPPORTED, "the adjoint covers Confinement, Drag and ShieldedCoulomb only");
+    set_order(c, 4);  // the fit differentiates the RK4 map
     if (c->p.precision != DEMON_PRECISION_FP64)
         return fail(c, DEMON_ERR_UNSUPPORTED, "the adjoint is FP64 only");
     const int64_t n_obs = steps / obs_every;

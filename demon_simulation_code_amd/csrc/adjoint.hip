@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_adj_pairs(const double *__restrict__ xs
                                                    double *__restrict__ part, int32_t n, int32_t npad,
                                                    int32_t jslice, double kappa, double rc2p, ExpCoef ec) {
     const Radii rad{rc2p, -1.0};
-    const PairEnv e{make_cutoff(kappa, rc2p), rad, (float)kappa, false, nullptr, ec};
+    const PairEnv e{make_cutoff(kappa, rc2p), rad, fp32_kl2(kappa), false, nullptr, ec};
     const int32_t ibase = (int32_t)blockIdx.x * 256 + (int32_t)threadIdx.x;
     const ITile<1> t = make_itile<1>(xs, ys, nc, q, n, ibase, rad.rf2);
     const double qi = q[ibase], uxi = ux[ibase], uyi = uy[ibase];
@@ -231,7 +231,7 @@ hipError_t launch_adj_pairs(const StageBuf &st, const double *q, const double *u
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_adj_pairs, dim3((unsigned)(npad / 256), (unsigned)slices), dim3(256), 0, s, st.xs,
                        st.ys, st.nc, st.box, q, ux, uy, part, (int32_t)n, (int32_t)npad, jslice, kappa, rc2p,
-                       exp_taylor_coef());
+                       exp_coef());
     return hipGetLastError();
 }
 

@@ -6,13 +6,17 @@
 // clouds live in one arena; each cloud's slots are padded to a multiple of 256
 // so that a 256-thread block of the O(N) kernels never straddles two clouds.
 //
-// All-pairs work is split into items (cloud, 64R i-slots, j slice).  One
-// persistent launch per tile shape R pulls items from a device counter, one
-// wavefront per item, largest items first; every wave leaves the loop when the
-// counter passes the item count.  The per-stage update runs over all clouds
-// in one launch.  A step (boundary, 4 x (pairs, update)) is captured once as a
-// hipGraph and replayed.  The tile shape per N bucket (R and the j-slice
-// length) comes from the autotuner, demon_batch_tune.
+// All-pairs work is split into items.  A cloud in the gather form has items
+// (cloud, 64R i-slots, j slice): one persistent launch per tile shape R pulls
+// them from a device counter, one wavefront per item.  A cloud in the
+// Newton-3 form (every unordered pair once, pairs_n3.h) has items (cloud,
+// superblock row, slice of superblock offsets): one persistent launch pulls
+// them, one workgroup per item.  Items are taken largest first; every wave
+// leaves the loop when the counter passes the item count.  The per-stage
+// update runs over all clouds in one launch and adds each Newton-3 cloud's
+// reactions as the single-cloud stage kernel does.  A step (boundary,
+// 4 x (pairs, update)) is captured once as a hipGraph and replayed.  The form
+// and tile shape per N bucket come from the autotuner, demon_batch_tune.
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -22,6 +26,7 @@
 
 #include "../../include/demon_gpu.h"
 #include "pair_core.h"
+#include "pairs_n3.h"
 #include "philox.h"
 
 namespace demon {
@@ -29,17 +34,23 @@ namespace {
 
 struct BCloud {
     int32_t off, n, npad;        // first slot, particles, padded size (multiple of 256)
-    int32_t R, jslice, slices;   // tile shape
+    int32_t R;                   // i per lane (gather form), 0: Newton-3 form
+    int32_t jslice;              // gather: j per slice; Newton-3: superblock offsets per item
+    int32_t slices;              // slices per i block (gather) or per row (Newton-3)
     int64_t part_off;            // partial sums: part + part_off, layout [slices][2][npad]
+    int64_t react_off;           // Newton-3: reactions, layout [nsb/2][2][npad]
+    int64_t rflag_off;           // Newton-3: tile flags, layout [nsb][nsb/2]
     double kappa, rc2p, confine, gamma;
     uint64_t seed;
 };
 
 struct BItem {
-    int32_t cloud, ibase, slice, pad_;
+    int32_t cloud, ibase, slice, pad_;  // ibase: first i slot (gather) or superblock row (Newton-3)
 };
 
-constexpr int kShapes = 3;  // R = 1, 2, 4
+constexpr int kShapes = 3;       // gather forms R = 1, 2, 4
+constexpr int kGroups = 4;       // item groups: the three gather shapes, then Newton-3
+constexpr int kGroupN3 = 3;
 constexpr int kCloudPad = 256;
 
 __device__ __forceinline__ int32_t first_lane(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -89,12 +100,48 @@ __global__ __launch_bounds__(256) void k_bpairs(const double *__restrict__ xs,
     }
 }
 
+// Newton-3 items: one workgroup per item (row S of a cloud against a slice
+// of superblock offsets).
+template <bool F32, bool DTC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bpairs_n3(
+    const double *__restrict__ xs, const double *__restrict__ ys, const double *__restrict__ nc,
+    const double *__restrict__ box, const double *__restrict__ q, const float *__restrict__ qf,
+    const int32_t *__restrict__ orig, double *__restrict__ part, double *__restrict__ react,
+    uint32_t *__restrict__ rflag, const BCloud *__restrict__ clouds, const BItem *__restrict__ items,
+    int32_t n_items, unsigned int *__restrict__ counter, Scalars *__restrict__ sc, int32_t base_flags,
+    pair::ExpCoef ec) {
+    __shared__ n3::RedLds red;
+    __shared__ n3::ScrLds scr;
+    __shared__ int32_t slot;
+    for (;;) {
+        // the previous item is done with the LDS sums and with slot
+        __syncthreads();
+        if (threadIdx.x == 0) slot = (int32_t)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int32_t it = first_lane(slot);
+        if (it >= n_items) break;
+        const BItem item = items[it];
+        const BCloud cl = clouds[item.cloud];
+        Scalars *s = sc + item.cloud;
+        if ((base_flags & pair::kFlagGated) && !s->active) continue;
+        const int32_t half = cl.npad / kSuper / 2;
+        const int32_t d0 = item.slice * cl.jslice;
+        const int32_t d1 = d0 + cl.jslice < half + 1 ? d0 + cl.jslice : half + 1;
+        const n3::N3Row row{xs + cl.off, ys + cl.off, nc + cl.off, q + cl.off, qf + cl.off, orig + cl.off,
+                            box + cl.off / 16, part + cl.part_off, react + cl.react_off,
+                            rflag + cl.rflag_off, s, cl.n, cl.npad, base_flags, cl.kappa, cl.rc2p};
+        n3::n3_row<F32, DTC>(row, ec, item.ibase, d0, d1, item.slice, red, scr);
+    }
+}
+
 struct BUpdate {
     StageBuf cur, nxt;
     double *x, *y, *vx, *vy;
     const double *q, *m;
     const int32_t *orig, *block_cloud;
     const double *part;
+    const double *react;     // Newton-3 clouds' reactions (BCloud::react_off)
+    const uint32_t *rflag;   // and tile flags (BCloud::rflag_off)
     double *acc1, *acc23;    // [4][P]
     const BCloud *clouds;
     Scalars *sc;
@@ -111,13 +158,27 @@ __device__ __forceinline__ double bda(double a1, double a23, double a4) {
 template <int STAGE>
 __global__ __launch_bounds__(256) void k_bupdate(BUpdate a) {
     __shared__ double red[4];
-    if (blockIdx.x == 0 && threadIdx.x < kShapes) a.counters[threadIdx.x] = 0u;
+    __shared__ unsigned long long fmask[kMaxHalf / 64];
+    if (blockIdx.x == 0 && threadIdx.x < kGroups) a.counters[threadIdx.x] = 0u;
     const int32_t c = a.block_cloud[blockIdx.x];
     const BCloud cl = a.clouds[c];
     Scalars *sc = a.sc + c;
     if (!sc->active) return;  // uniform per block
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int32_t i = (int32_t)(p - cl.off);
+    // a block is one superblock of one cloud; for a Newton-3 cloud, the flags
+    // of the tiles whose reactions land on it, as a bitmask over d - 1
+    const bool n3c = cl.R == 0 && (a.forces & 4);
+    const int32_t half = cl.npad / kSuper / 2;
+    if (n3c && threadIdx.x < 64) {
+        const uint32_t *fl = a.rflag + cl.rflag_off + (size_t)(i / kSuper) * half;
+        for (int32_t w = 0; w < (half + 63) / 64; ++w) {
+            const int32_t k = 64 * w + (int32_t)threadIdx.x;
+            const unsigned long long m = __ballot(k < half && fl[k] != 0u);
+            if (threadIdx.x == 0) fmask[w] = m;
+        }
+    }
+    __syncthreads();
     double dt;
     if (STAGE == 1) {
         dt = sc->init_dt;
@@ -164,6 +225,19 @@ __global__ __launch_bounds__(256) void k_bupdate(BUpdate a) {
             for (int32_t s = 0; s < cl.slices; ++s) {
                 fx = fx + pp[(size_t)(2 * s) * cl.npad + i];
                 fy = fy + pp[(size_t)(2 * s + 1) * cl.npad + i];
+            }
+            if (n3c) {
+                // reactions in increasing superblock offset d (bit d - 1)
+                const double *rr = a.react + cl.react_off;
+                for (int32_t w = 0; w < (half + 63) / 64; ++w) {
+                    unsigned long long mask = fmask[w];
+                    while (mask) {
+                        const int32_t d = 64 * w + __builtin_ctzll(mask) + 1;
+                        mask &= mask - 1;
+                        fx = fx + rr[(size_t)(d - 1) * 2 * cl.npad + i];
+                        fy = fy + rr[(size_t)(d - 1) * 2 * cl.npad + cl.npad + i];
+                    }
+                }
             }
         }
         if (a.forces & 16) {
@@ -231,7 +305,7 @@ __global__ __launch_bounds__(256) void k_bupdate(BUpdate a) {
 __global__ void k_bboundary(Scalars *sc, int32_t nclouds, int closing, unsigned int *counters) {
     const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c == 0)
-        for (int k = 0; k < kShapes; ++k) counters[k] = 0u;
+        for (int k = 0; k < kGroups; ++k) counters[k] = 0u;
     if (c >= nclouds) return;
     Scalars *s = sc + c;
     if (s->active) {
@@ -291,17 +365,24 @@ struct demon_batch {
     double *arena = nullptr;     // 30 arrays of P
     double *part = nullptr;
     size_t part_doubles = 0;
+    double *react = nullptr;     // Newton-3 clouds' reactions
+    size_t react_doubles = 0;
+    uint32_t *rflag = nullptr;   // and tile flags
+    size_t rflag_words = 0;
+    float *qf = nullptr;         // charges in FP32
     int32_t *orig = nullptr, *d_block_cloud = nullptr;
     BCloud *d_clouds = nullptr;
     Scalars *d_sc = nullptr;
     BItem *d_items = nullptr;
     size_t items_cap = 0;
     unsigned int *d_counters = nullptr;
-    int32_t group_off[kShapes]{}, group_count[kShapes]{};
+    int32_t group_off[kGroups]{}, group_count[kGroups]{};
     double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *q = nullptr, *m = nullptr;
     StageBuf sb[2]{};
     double *acc1 = nullptr, *acc23 = nullptr;
-    // tile shape per N bucket (bucket = floor(log2 npad), 0..31)
+    // form and tile shape per N bucket (bucket = floor(log2 npad), 0..31):
+    // R = 1, 2, 4 gather form with j slices of js; R = 0 Newton-3 form with
+    // js superblock offsets per item
     int32_t bucket_R[32], bucket_js[32];
     float bucket_us[32];
     bool use_graph = true;
@@ -334,32 +415,58 @@ int bucket_of(int64_t npad) {
     return k;
 }
 
-int shape_index(int R) { return R == 1 ? 0 : R == 2 ? 1 : 2; }
+int shape_index(int R) { return R == 0 ? kGroupN3 : R == 1 ? 0 : R == 2 ? 1 : 2; }
 
-// Recomputes shapes, partial offsets and the item lists from the bucket table.
+// Items of one cloud, with their cost (pair evaluations) for the ordering.
+void cloud_items(const BCloud &cl, int32_t c, std::vector<std::pair<int64_t, BItem>> &out) {
+    if (cl.R == 0) {
+        const int32_t nsb = cl.npad / kSuper, half = nsb / 2;
+        for (int32_t S = 0; S < nsb; ++S)
+            for (int32_t sl = 0; sl < cl.slices; ++sl) {
+                const int32_t d0 = sl * cl.jslice, d1 = std::min(d0 + cl.jslice, half + 1);
+                out.push_back({(int64_t)(d1 - d0) * kSuper * kSuper, BItem{c, S, sl, 0}});
+            }
+        return;
+    }
+    for (int32_t ib = 0; ib < cl.npad; ib += 64 * cl.R)
+        for (int32_t sl = 0; sl < cl.slices; ++sl) {
+            const int32_t jl = std::min(cl.jslice, cl.npad - sl * cl.jslice);
+            out.push_back({(int64_t)jl * 64 * cl.R, BItem{c, ib, sl, 0}});
+        }
+}
+
+// Recomputes forms, shapes, buffer offsets and the item lists from the
+// bucket table.
 int rebuild_items(demon_batch *b) {
-    int64_t part_off = 0;
+    int64_t part_off = 0, react_off = 0, rflag_off = 0;
     for (int32_t c = 0; c < b->nclouds; ++c) {
         BCloud &cl = b->hc[(size_t)c];
         const int bk = bucket_of(cl.npad);
         cl.R = b->bucket_R[bk];
-        cl.jslice = std::min<int32_t>(b->bucket_js[bk], cl.npad);
-        cl.slices = (cl.npad + cl.jslice - 1) / cl.jslice;
+        const int32_t nsb = cl.npad / kSuper, half = nsb / 2;
+        if (cl.R == 0 && half > kMaxHalf) cl.R = 1;  // the update's flag mask holds kMaxHalf tiles
+        if (cl.R == 0) {
+            cl.jslice = std::max<int32_t>(1, std::min<int32_t>(b->bucket_js[bk], half + 1));
+            cl.slices = (half + 1 + cl.jslice - 1) / cl.jslice;
+            cl.react_off = react_off;
+            cl.rflag_off = rflag_off;
+            react_off += (int64_t)half * 2 * cl.npad;
+            rflag_off += (int64_t)nsb * half;
+        } else {
+            cl.jslice = std::min<int32_t>(b->bucket_R[bk] == 0 ? 512 : b->bucket_js[bk], cl.npad);
+            cl.slices = (cl.npad + cl.jslice - 1) / cl.jslice;
+            cl.react_off = cl.rflag_off = 0;
+        }
         cl.part_off = part_off;
         part_off += (int64_t)cl.slices * 2 * cl.npad;
     }
-    std::vector<std::pair<int64_t, BItem>> all[kShapes];
+    std::vector<std::pair<int64_t, BItem>> all[kGroups];
     for (int32_t c = 0; c < b->nclouds; ++c) {
         const BCloud &cl = b->hc[(size_t)c];
-        const int g = shape_index(cl.R);
-        for (int32_t ib = 0; ib < cl.npad; ib += 64 * cl.R)
-            for (int32_t s = 0; s < cl.slices; ++s) {
-                const int32_t jl = std::min(cl.jslice, cl.npad - s * cl.jslice);
-                all[g].push_back({(int64_t)jl * 64 * cl.R, BItem{c, ib, s, 0}});
-            }
+        cloud_items(cl, c, all[shape_index(cl.R)]);
     }
     std::vector<BItem> items;
-    for (int g = 0; g < kShapes; ++g) {
+    for (int g = 0; g < kGroups; ++g) {
         std::stable_sort(all[g].begin(), all[g].end(),
                          [](const std::pair<int64_t, BItem> &a, const std::pair<int64_t, BItem> &b2) {
                              return a.first > b2.first;
@@ -383,6 +490,23 @@ int rebuild_items(demon_batch *b) {
             return bfail(b, DEMON_ERR_ALLOC, "hipMalloc partial sums failed");
         b->part_doubles = (size_t)part_off;
     }
+    // at least one element each, so that the kernels always get valid pointers
+    if ((size_t)std::max<int64_t>(react_off, 1) > b->react_doubles) {
+        if (b->react) (void)hipFree(b->react);
+        b->react = nullptr;
+        const size_t cnt = (size_t)std::max<int64_t>(react_off, 1);
+        if (hipMalloc((void **)&b->react, sizeof(double) * cnt) != hipSuccess)
+            return bfail(b, DEMON_ERR_ALLOC, "hipMalloc reaction sums failed");
+        b->react_doubles = cnt;
+    }
+    if ((size_t)std::max<int64_t>(rflag_off, 1) > b->rflag_words) {
+        if (b->rflag) (void)hipFree(b->rflag);
+        b->rflag = nullptr;
+        const size_t cnt = (size_t)std::max<int64_t>(rflag_off, 1);
+        if (hipMalloc((void **)&b->rflag, sizeof(uint32_t) * cnt) != hipSuccess)
+            return bfail(b, DEMON_ERR_ALLOC, "hipMalloc tile flags failed");
+        b->rflag_words = cnt;
+    }
     BHIP(b, hipMemcpy(b->d_items, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice));
     BHIP(b, hipMemcpy(b->d_clouds, b->hc.data(), sizeof(BCloud) * b->hc.size(), hipMemcpyHostToDevice));
     if (b->graph) {
@@ -392,24 +516,35 @@ int rebuild_items(demon_batch *b) {
     return DEMON_OK;
 }
 
-int grid_for(int32_t count) {
+int grid_for(int32_t count, int32_t items_per_block) {
     // persistent grid: enough waves to fill 256 CUs several times over, never
-    // more blocks than needed for one item per wave
-    const int32_t blocks = std::min<int32_t>(2048, (count + 3) / 4);
+    // more blocks than needed for one item per wave (gather) or per
+    // workgroup (Newton-3)
+    const int32_t blocks = std::min<int32_t>(2048, (count + items_per_block - 1) / items_per_block);
     return std::max(blocks, 1);
 }
 
 int enqueue_bpairs(demon_batch *b, const StageBuf &st, int32_t flags, int only_group = -1,
                    const BItem *items = nullptr, int32_t count = -1) {
     const bool f32 = b->precision == DEMON_PRECISION_FP32_PAIR;
-    for (int g = 0; g < kShapes; ++g) {
+    for (int g = 0; g < kGroups; ++g) {
         if (only_group >= 0 && g != only_group) continue;
         const int32_t cnt = count >= 0 ? count : b->group_count[g];
         if (cnt == 0) continue;
         const BItem *it = items ? items : b->d_items + b->group_off[g];
         unsigned int *ctr = b->d_counters + g;
-        const dim3 grid((unsigned)grid_for(cnt)), blk(256);
-        const pair::ExpCoef ec = pair::exp_taylor_coef();
+        const pair::ExpCoef ec = pair::exp_coef();
+        if (g == kGroupN3) {
+            const dim3 grid((unsigned)grid_for(cnt, 1)), blk(256);
+            const bool dtc = (flags & pair::kFlagDtCheck) != 0;
+            auto kern = f32 ? (dtc ? k_bpairs_n3<true, true> : k_bpairs_n3<true, false>)
+                            : (dtc ? k_bpairs_n3<false, true> : k_bpairs_n3<false, false>);
+            hipLaunchKernelGGL(kern, grid, blk, 0, b->stream, st.xs, st.ys, st.nc, st.box, b->q, b->qf, b->orig,
+                               b->part, b->react, b->rflag, b->d_clouds, it, cnt, ctr, b->d_sc, flags, ec);
+            BHIP(b, hipGetLastError());
+            continue;
+        }
+        const dim3 grid((unsigned)grid_for(cnt, 4)), blk(256);
 #define BLAUNCH(RR)                                                                                        \
     if (f32)                                                                                               \
         hipLaunchKernelGGL((k_bpairs<RR, true>), grid, blk, 0, b->stream, st.xs, st.ys, st.nc, st.box, b->q,  \
@@ -431,6 +566,7 @@ BUpdate bupdate(demon_batch *b, int stage) {
     u.nxt = b->sb[cur ^ 1];
     u.x = b->x; u.y = b->y; u.vx = b->vx; u.vy = b->vy; u.q = b->q; u.m = b->m;
     u.orig = b->orig; u.block_cloud = b->d_block_cloud; u.part = b->part;
+    u.react = b->react; u.rflag = b->rflag;
     u.acc1 = b->acc1; u.acc23 = b->acc23; u.clouds = b->d_clouds; u.sc = b->d_sc;
     u.counters = b->d_counters; u.P = b->P; u.forces = b->forces; u.dtcheck = b->dt_check;
     u.heat = b->heat;
@@ -530,11 +666,11 @@ int demon_batch_create(demon_batch **out, int32_t nclouds, const int64_t *n, con
         hipMalloc((void **)&b->d_block_cloud, sizeof(int32_t) * b->block_cloud.size()) != hipSuccess ||
         hipMalloc((void **)&b->d_clouds, sizeof(BCloud) * (size_t)nclouds) != hipSuccess ||
         hipMalloc((void **)&b->d_sc, sizeof(Scalars) * (size_t)nclouds) != hipSuccess ||
-        hipMalloc((void **)&b->d_counters, sizeof(unsigned int) * kShapes) != hipSuccess)
+        hipMalloc((void **)&b->d_counters, sizeof(unsigned int) * kGroups) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_ALLOC, "hipMalloc failed"));
     if (hipMemset(b->arena, 0, sizeof(double) * P * 25) != hipSuccess ||
         hipMemset(b->d_sc, 0, sizeof(Scalars) * (size_t)nclouds) != hipSuccess ||
-        hipMemset(b->d_counters, 0, sizeof(unsigned int) * kShapes) != hipSuccess)
+        hipMemset(b->d_counters, 0, sizeof(unsigned int) * kGroups) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_HIP, "hipMemset failed"));
     double *a = b->arena;
     auto take = [&]() { double *r = a; a += P; return r; };
@@ -548,6 +684,8 @@ int demon_batch_create(demon_batch **out, int32_t nclouds, const int64_t *n, con
         s->box = a;
         a += P / 16;
     }
+    b->qf = reinterpret_cast<float *>(a);  // P floats
+    a += P / 2;
     if (hipMemcpy(b->d_block_cloud, b->block_cloud.data(), sizeof(int32_t) * b->block_cloud.size(),
                   hipMemcpyHostToDevice) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_HIP, "hipMemcpy failed"));
@@ -569,7 +707,8 @@ void demon_batch_destroy(demon_batch *b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (hipEvent_t e : b->ev)
         if (e) (void)hipEventDestroy(e);
-    for (void *ptr : {(void *)b->arena, (void *)b->part, (void *)b->orig, (void *)b->d_block_cloud,
+    for (void *ptr : {(void *)b->arena, (void *)b->part, (void *)b->react, (void *)b->rflag, (void *)b->orig,
+                      (void *)b->d_block_cloud,
                       (void *)b->d_clouds, (void *)b->d_sc, (void *)b->d_items, (void *)b->d_counters})
         if (ptr) (void)hipFree(ptr);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -622,6 +761,7 @@ int demon_batch_upload(demon_batch *b, int32_t cloud, const double *x, const dou
     for (int a = 0; a < 6; ++a)
         BHIP(b, hipMemcpy(dst[a] + cl.off, h.data() + (size_t)a * np, sizeof(double) * np, hipMemcpyHostToDevice));
     BHIP(b, hipMemcpy(b->orig + cl.off, o.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+    BHIP(b, launch_round_fp32(b->q + cl.off, b->qf + cl.off, np, b->stream));
     // stage-1 record of this cloud
     const double *bs[4] = {b->x, b->y, b->vx, b->vy};
     double *rec[4] = {b->sb[0].xs, b->sb[0].ys, b->sb[0].vxs, b->sb[0].vys};
@@ -740,8 +880,11 @@ int demon_batch_last_ms(const demon_batch *b, double *ms) {
 
 int demon_batch_set_shape(demon_batch *b, int32_t bucket, int32_t i_per_lane, int32_t jslice) {
     if (!b || bucket < 0 || bucket > 31) return bfail(b, DEMON_ERR_ARG, "bad bucket");
-    if ((i_per_lane != 1 && i_per_lane != 2 && i_per_lane != 4) || jslice < 64 || jslice % 64)
-        return bfail(b, DEMON_ERR_ARG, "i_per_lane in {1,2,4}, jslice a multiple of 64");
+    if (i_per_lane == DEMON_BATCH_NEWTON3) {
+        if (jslice < 1) return bfail(b, DEMON_ERR_ARG, "Newton-3: jslice (superblock offsets per item) >= 1");
+    } else if ((i_per_lane != 1 && i_per_lane != 2 && i_per_lane != 4) || jslice < 64 || jslice % 64) {
+        return bfail(b, DEMON_ERR_ARG, "i_per_lane in {0 (Newton-3), 1, 2, 4}, gather jslice a multiple of 64");
+    }
     b->bucket_R[bucket] = i_per_lane;
     b->bucket_js[bucket] = jslice;
     return rebuild_items(b);
@@ -757,8 +900,11 @@ int demon_batch_get_shape(const demon_batch *b, int32_t bucket, int32_t *i_per_l
 }
 
 // Autotuner: for every N bucket present, times the all-pairs pass of that
-// bucket's clouds (ungated, on the uploaded state) for each candidate shape
-// (R in {1,2,4}, j slice 64..8192) and keeps the fastest.
+// bucket's clouds (ungated, on the uploaded state) for each candidate: the
+// gather form (R in {1,2,4}, j slice 64..8192) and the Newton-3 form (1, 2,
+// 4, ... superblock offsets per item), and keeps the fastest.  The Newton-3
+// form also adds its reactions in the update kernel (one read per evaluated
+// tile row), which the timing leaves out.
 int demon_batch_tune(demon_batch *b, int32_t reps) {
     if (!b || reps < 1) return bfail(b, DEMON_ERR_ARG, "bad tune arguments");
     if (!b->uploaded_all) return bfail(b, DEMON_ERR_ARG, "upload every cloud first");
@@ -768,55 +914,63 @@ int demon_batch_tune(demon_batch *b, int32_t reps) {
         const int bk = bucket_of(cl.npad);
         if (std::find(buckets.begin(), buckets.end(), bk) == buckets.end()) buckets.push_back(bk);
     }
-    const int R_cand[3] = {1, 2, 4};
     for (int bk : buckets) {
-        float best = std::numeric_limits<float>::infinity();
-        int bestR = b->bucket_R[bk], bestJ = b->bucket_js[bk];
-        for (int R : R_cand) {
+        std::vector<std::pair<int, int>> cand;  // (R, js)
+        for (int R : {1, 2, 4})
             for (int js = 64; js <= 8192; js *= 2) {
                 if (js > (int64_t(1) << (bk + 1)) * 2) break;
-                b->bucket_R[bk] = R;
-                b->bucket_js[bk] = js;
-                int rc = rebuild_items(b);
-                if (rc) return rc;
-                // items of this bucket only: they form a contiguous run? no -- build them
-                std::vector<BItem> items;
-                for (int32_t c = 0; c < b->nclouds; ++c) {
-                    const BCloud &cl = b->hc[(size_t)c];
-                    if (bucket_of(cl.npad) != bk) continue;
-                    for (int32_t ib = 0; ib < cl.npad; ib += 64 * cl.R)
-                        for (int32_t s = 0; s < cl.slices; ++s) items.push_back(BItem{c, ib, s, 0});
-                }
-                BItem *d_it = nullptr;
-                BHIP(b, hipMalloc((void **)&d_it, sizeof(BItem) * items.size()));
-                BHIP(b, hipMemcpy(d_it, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice));
-                float tot = 0.0f;
-                int ok = DEMON_OK;
-                for (int r = 0; r <= reps && !ok; ++r) {
-                    hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * kShapes, b->stream);
-                    hipEventRecord(b->ev[0], b->stream);
-                    ok = enqueue_bpairs(b, b->sb[0], 0, shape_index(R), d_it, (int32_t)items.size());
-                    hipEventRecord(b->ev[1], b->stream);
-                    if (hipStreamSynchronize(b->stream) != hipSuccess) ok = bfail(b, DEMON_ERR_HIP, "tune sync");
-                    float ms = 0.0f;
-                    hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
-                    if (r > 0) tot += ms;  // first run is a warm-up
-                }
-                (void)hipFree(d_it);
-                if (ok) return ok;
-                const float us = tot / reps * 1000.0f;
-                if (us < best) {
-                    best = us;
-                    bestR = R;
-                    bestJ = js;
-                }
+                cand.push_back({R, js});
+            }
+        const int32_t half_max = (int32_t)((int64_t(1) << (bk + 1)) / kSuper / 2);
+        for (int span = 1; span <= half_max + 1 && half_max <= kMaxHalf; span *= 2) {
+            cand.push_back({0, span});
+            if (span > half_max) break;
+        }
+        float best = std::numeric_limits<float>::infinity();
+        int bestR = b->bucket_R[bk], bestJ = b->bucket_js[bk];
+        for (const auto &cj : cand) {
+            const int R = cj.first, js = cj.second;
+            b->bucket_R[bk] = R;
+            b->bucket_js[bk] = js;
+            int rc = rebuild_items(b);
+            if (rc) return rc;
+            // the items of this bucket's clouds only
+            std::vector<std::pair<int64_t, BItem>> all;
+            for (int32_t c = 0; c < b->nclouds; ++c) {
+                const BCloud &cl = b->hc[(size_t)c];
+                if (bucket_of(cl.npad) == bk) cloud_items(cl, c, all);
+            }
+            std::vector<BItem> items;
+            for (auto &e : all) items.push_back(e.second);
+            BItem *d_it = nullptr;
+            BHIP(b, hipMalloc((void **)&d_it, sizeof(BItem) * items.size()));
+            BHIP(b, hipMemcpy(d_it, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice));
+            float tot = 0.0f;
+            int ok = DEMON_OK;
+            for (int r = 0; r <= reps && !ok; ++r) {
+                hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * kGroups, b->stream);
+                hipEventRecord(b->ev[0], b->stream);
+                ok = enqueue_bpairs(b, b->sb[0], 0, shape_index(R), d_it, (int32_t)items.size());
+                hipEventRecord(b->ev[1], b->stream);
+                if (hipStreamSynchronize(b->stream) != hipSuccess) ok = bfail(b, DEMON_ERR_HIP, "tune sync");
+                float ms = 0.0f;
+                hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+                if (r > 0) tot += ms;  // first run is a warm-up
+            }
+            (void)hipFree(d_it);
+            if (ok) return ok;
+            const float us = tot / reps * 1000.0f;
+            if (us < best) {
+                best = us;
+                bestR = R;
+                bestJ = js;
             }
         }
         b->bucket_R[bk] = bestR;
         b->bucket_js[bk] = bestJ;
         b->bucket_us[bk] = best;
     }
-    BHIP(b, hipMemset(b->d_counters, 0, sizeof(unsigned int) * kShapes));
+    BHIP(b, hipMemset(b->d_counters, 0, sizeof(unsigned int) * kGroups));
     return rebuild_items(b);
 }
 

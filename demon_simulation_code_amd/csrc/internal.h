@@ -91,6 +91,7 @@ struct Scalars {
 
 struct PairLaunch {
     const double *xs, *ys, *nc, *q;
+    const float *qf;         // q rounded to FP32 (Newton-3 kernel, FP32 pair arithmetic)
     const double *box;       // per-block bounding boxes of xs/ys, or null
     double *react;           // Newton-3 kernel: reactions [nsb/2][2][npad]
     uint32_t *rflag;         // Newton-3 kernel: tile flags [nsb][nsb/2]
@@ -123,11 +124,15 @@ struct UpdateLaunch {
     int32_t forces;
     int32_t dtcheck;
     double confine, gamma, heat;
+    double bfield, drag_scale;      // MagneticForce B, TimeVaryingDragForce scale
     uint64_t seed;
     double *store;                  // optional copy of the current stage record: {xs ys vxs vys nc}[npad]
                                     // then the box table [npad/64][4] (store_doubles(npad) in all)
 };
 
+// stage 1..4: Runge_Kutta4's stages; kRk2Final: Runge_Kutta2's second stage
+// and combine (its first stage is stage 1).
+constexpr int kRk2Final = 5;
 hipError_t launch_update(int stage, const UpdateLaunch &a, hipStream_t s);
 
 // Doubles in one stored stage record (UpdateLaunch::store) and its view.
@@ -150,6 +155,8 @@ struct ForceOutLaunch {
     int64_t n, npad;
     int32_t forces;
     double confine, gamma, heat;
+    double bfield, drag_scale;
+    double t;                       // stage time (TimeVaryingDragForce)
     uint64_t seed, step;
     int32_t stage;
 };
@@ -159,6 +166,7 @@ hipError_t launch_force_out(const ForceOutLaunch &a, hipStream_t s);
 hipError_t launch_prepare_stage(StageBuf st, int64_t n, int64_t npad, hipStream_t s);
 // rmax2 = max |r|^2 over the stage positions (into sc->rmax2_bits, after reset).
 hipError_t launch_rmax(StageBuf st, int64_t n, Scalars *sc, hipStream_t s);
+hipError_t launch_round_fp32(const double *src, float *dst, int64_t npad, hipStream_t s);
 
 // Discrete adjoint (adjoint.hip).  st: a stored stage record (with boxes).
 hipError_t launch_adj_pairs(const StageBuf &st, const double *q, const double *ux, const double *uy,

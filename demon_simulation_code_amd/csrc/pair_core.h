@@ -67,82 +67,112 @@ __device__ __forceinline__ bool exact_contributes(double dx, double dy, double k
     return sqrt(r2) * kappa < 10.0;
 }
 
-// 1/sqrt(r2) to about 1 ulp: the hardware estimate refined by two Newton
-// steps y <- y + y (1 - r2 y^2) / 2 (each doubles the correct bits).
-__device__ __forceinline__ double rsqrt_fp64(double r2) {
-    double y = __builtin_amdgcn_rsq(r2);
-    double e = fma(-r2 * y, y, 1.0);
-    y = fma(0.5 * y, e, y);
-    e = fma(-r2 * y, y, 1.0);
-    return fma(0.5 * y, e, y);
+// 1/sqrt(r2) and sqrt(r2), each to about 1 ulp: the hardware estimate y0
+// (relative error far below 2^-20) refined by one Halley step.  With
+// e = 1 - r2 y0^2, 1/sqrt(r2) = y0 (1 - e)^-1/2 = y0 (1 + e/2 + 3e^2/8 + ...),
+// so y = y0 (1 + c) with c = e (1/2 + 3e/8) leaves a relative error of about
+// 5e^3/16, and sqrt(r2) = r2 y = t (1 + c) with t = r2 y0 comes from the
+// same correction.
+struct InvSqrt {
+    double y, r;  // 1/sqrt(r2), sqrt(r2)
+};
+
+__device__ __forceinline__ InvSqrt inv_sqrt_fp64(double r2) {
+    const double y0 = __builtin_amdgcn_rsq(r2);
+    const double t = r2 * y0;
+    const double e = fma(-t, y0, 1.0);
+    const double c = e * fma(e, 0.375, 0.5);
+    return InvSqrt{fma(y0, c, y0), fma(t, c, t)};
 }
 
-// Taylor coefficients 1/k! of e^f for k = 12 .. 3, passed to the kernels by
-// value.  As kernel arguments they arrive in SGPRs, so every Horner step is
-// one three-operand v_fma_f64 with an SGPR addend; as literals they would
-// sit in VGPRs, where the compiler's two-operand v_fmac_f64 (which
+__device__ __forceinline__ double rsqrt_fp64(double r2) { return inv_sqrt_fp64(r2).y; }
+
+// Coefficients q_0 .. q_9 (highest power first) of the polynomial q with
+// e^f ~ 1 + f + f^2 q(f) on [-ln2/2, ln2/2]: the Chebyshev interpolant of
+// degree 9 of (e^f - 1 - f) / f^2 (near-minimax; 1.6e-17 relative from e^f with the
+// coefficients rounded to double), produced by gen_exp_coef.py.  Passed to the
+// kernels by value: as kernel arguments they arrive in SGPRs, so every Horner
+// step is one three-operand v_fma_f64 with an SGPR addend; as literals they
+// would sit in VGPRs, where the compiler's two-operand v_fmac_f64 (which
 // overwrites its addend) costs a register copy per coefficient per pair.
 struct ExpCoef {
     double c[10];
 };
 
-inline ExpCoef exp_taylor_coef() {
-    ExpCoef e{};
-    double f = 1.0;  // k!
-    for (int k = 1; k <= 12; ++k) {
-        f *= k;
-        if (k >= 3) e.c[12 - k] = 1.0 / f;
-    }
-    return e;
+inline ExpCoef exp_coef() {
+    return ExpCoef{{2.5100375832561234e-08, 2.7620075879983367e-07, 2.7557268480310024e-06,
+                    2.4801521322368692e-05, 0.00019841269863040545, 0.0013888888917196719,
+                    0.008333333333330065, 0.041666666666624164, 0.16666666666666669,
+                    0.5000000000000001}};
 }
 
-// e^-u for 0 <= u <= 16 (callers pass u < 10), to about 1 ulp.
+// e^-u for 0 <= u <= 16 (callers pass u < 10), to about 1.5 ulp.
 // Cody-Waite reduction e^-u = 2^n e^f with n = rint(-u / ln 2) and
 // f = -u - n ln 2 in [-ln2/2, ln2/2], ln 2 split as in fdlibm's exp
-// (ln2_hi has 11 trailing zero bits, so n * ln2_hi is exact); e^f from its
-// Taylor polynomial of degree 12 (remainder |f|^13/13! < 2.3e-16 relative,
-// coefficients in ec);
-// the scaling by 2^n is exact (no overflow or underflow in this range).
+// (ln2_hi has 11 trailing zero bits, so n * ln2_hi is exact).  n is rounded
+// by adding 1.5 * 2^52 inside the fma that forms -u / ln 2: the sum's low
+// word is then n itself (two's complement), so neither a rounding nor a
+// conversion instruction is needed.  e^f = 1 + f (1 + f q(f)) by Horner; the
+// scaling by 2^n is exact (no overflow or underflow in this range).
 __device__ __forceinline__ double exp_neg(double u, const ExpCoef &ec) {
-    constexpr double kInvLn2 = 1.4426950408889634074;
+    constexpr double kLog2e = 1.4426950408889634074;
+    constexpr double kRound = 0x1.8p52;
     constexpr double kLn2Hi = 6.93147180369123816490e-01;
     constexpr double kLn2Lo = 1.90821492927058770002e-10;
-    const double n = __builtin_rint(u * -kInvLn2);
+    const double m = fma(u, -kLog2e, kRound);
+    const double n = m - kRound;
+    const int ni = __double2loint(m);
     double f = fma(n, -kLn2Hi, -u);
     f = fma(n, -kLn2Lo, f);
     double p = ec.c[0];
 #pragma unroll
     for (int k = 1; k < 10; ++k) p = fma(p, f, ec.c[k]);
-    p = fma(p, f, 0.5);
     p = fma(p, f, 1.0);
     p = fma(p, f, 1.0);
-    return __builtin_ldexp(p, (int)n);
+    return __builtin_ldexp(p, ni);
 }
 
 // Pair scale for a contributing pair (i <- j):
 //   F_i += s (r_i - r_j),  s = C q_i q_j (1 + u) e^-u / r^3,  u = kappa r,
 // the reference's cross-pair formula (ShieldedCoulombForce.cpp:183-184);
-// cqi = C q_i.  r and 1/r^3 come from one refined reciprocal square root.
+// cqi = C q_i.  r and 1/r^3 come from one refined reciprocal square root;
+// (1 + u) / r^3 is one fma.
 __device__ __forceinline__ double pair_scale_fp64(double r2, double cqi, double qj, double kappa,
                                                   const ExpCoef &ec) {
-    const double y = rsqrt_fp64(r2);
-    const double u = (r2 * y) * kappa;
-    return (cqi * qj) * ((1.0 + u) * exp_neg(u, ec)) * ((y * y) * y);
+    const InvSqrt s = inv_sqrt_fp64(r2);
+    const double u = s.r * kappa;
+    const double y3 = (s.y * s.y) * s.y;
+    return ((cqi * qj) * fma(u, y3, y3)) * exp_neg(u, ec);
 }
 
 // FP32 pair force: distance, exp and force in FP32 (hardware reciprocal
-// square root and 2^x); the FP64 difference is rounded once and the
-// components are returned in FP64 for FP64 accumulation.
-__device__ __forceinline__ void pair_force_fp32(double dx, double dy, float cqi, float qj, float kappa,
-                                                double &fx, double &fy) {
+// square root and 2^x) from the FP64 difference rounded once.  kl2 =
+// -kappa log2(e), so a = kl2 r gives e^-u = 2^a and 1 + u = 1 - a ln 2.
+__device__ __forceinline__ void pair_force_fp32f(double dx, double dy, float cqi, float qj, float kl2,
+                                                 float &fx, float &fy) {
     const float gx = (float)dx, gy = (float)dy;
     const float r2 = fmaf(gx, gx, gy * gy);
     const float y = __builtin_amdgcn_rsqf(r2);
-    const float u = (r2 * y) * kappa;
-    const float e = __builtin_amdgcn_exp2f(u * -1.44269504f);
-    const float s = (cqi * qj) * ((1.0f + u) * e) * ((y * y) * y);
-    fx = (double)(s * gx);
-    fy = (double)(s * gy);
+    const float a = (r2 * y) * kl2;
+    const float e = __builtin_amdgcn_exp2f(a);
+    const float b = fmaf(a, -0.693147182f, 1.0f);
+    const float s = ((cqi * qj) * (b * e)) * ((y * y) * y);
+    fx = s * gx;
+    fy = s * gy;
+}
+
+// -kappa log2(e) in FP32, the FP32 path's exp scale.
+__host__ __device__ __forceinline__ float fp32_kl2(double kappa) {
+    return (float)(-kappa * 1.4426950408889634074);
+}
+
+// The same, with the components returned in FP64 for FP64 accumulation.
+__device__ __forceinline__ void pair_force_fp32(double dx, double dy, float cqi, float qj, float kl2,
+                                                double &fx, double &fy) {
+    float gx, gy;
+    pair_force_fp32f(dx, dy, cqi, qj, kl2, gx, gy);
+    fx = (double)gx;
+    fy = (double)gy;
 }
 
 // Radii^2 of a launch: the candidate radius (the cutoff, widened in the
@@ -180,7 +210,8 @@ __device__ __forceinline__ ITile<R> make_itile(const double *__restrict__ xs, co
                                                int32_t n, int32_t ibase, double rf2) {
     ITile<R> t;
     t.ibase = ibase;
-    t.wlo = ibase - (int32_t)(threadIdx.x & 63);
+    // wave-uniform; readfirstlane tells the compiler so (SGPR, scalar branches)
+    t.wlo = __builtin_amdgcn_readfirstlane(ibase - (int32_t)(threadIdx.x & 63));
     t.whi = t.wlo + 64 * R;
     const double nan = __builtin_nan("");
 #pragma unroll
@@ -263,7 +294,7 @@ __device__ __forceinline__ bool chunk_hit(const ITile<R> &t, const double (&cx)[
 struct PairEnv {
     Cutoff cut;
     Radii rad;
-    float kappa_f;
+    float kl2_f;  // -kappa log2(e), FP32 pair arithmetic
     bool dtc;
     const int32_t *__restrict__ orig;
     ExpCoef ec;
@@ -317,7 +348,7 @@ __device__ __forceinline__ void chunk_gather(const ITile<R> &t, const double (&c
             if (in) {
                 if (F32) {
                     double fx, fy;
-                    pair_force_fp32(dx, dy, t.cqf[r], (float)cq[u], e.kappa_f, fx, fy);
+                    pair_force_fp32(dx, dy, t.cqf[r], (float)cq[u], e.kl2_f, fx, fy);
                     ax[r] = ax[r] + fx;
                     ay[r] = ay[r] + fy;
                 } else {
@@ -384,7 +415,7 @@ __device__ __forceinline__ void wave_tile(const double *__restrict__ xs, const d
                                           int32_t flags, double kappa, double rc2p, Radii rad,
                                           const ExpCoef &ec, unsigned int *dtmin, double (&ax)[R],
                                           double (&ay)[R], uint64_t &n_near, uint64_t &n_tested) {
-    const PairEnv e{make_cutoff(kappa, rc2p), rad, (float)kappa, (flags & kFlagDtCheck) != 0, orig, ec};
+    const PairEnv e{make_cutoff(kappa, rc2p), rad, fp32_kl2(kappa), (flags & kFlagDtCheck) != 0, orig, ec};
     const ITile<R> t = make_itile<R>(xs, ys, nc, q, n, ibase, rad.rf2);
     Box ib = empty_box();
     if (box) {

@@ -81,7 +81,7 @@ hipError_t launch_shape(const PairLaunch &a, hipStream_t s) {
     if (a.gated) flags |= kFlagGated;
     hipLaunchKernelGGL((k_pairs<R, F32>), grid, dim3(256), 0, s, a.xs, a.ys, a.nc, a.q, a.orig,
                        a.box, a.part, a.sc, a.n, a.npad, a.jslice, flags, a.kappa, a.rc2p,
-                       exp_taylor_coef());
+                       exp_coef());
     return hipGetLastError();
 }
 

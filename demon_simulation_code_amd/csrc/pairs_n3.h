@@ -1,0 +1,295 @@
+// pairs_n3.h -- tile code of the Newton-3 all-pairs pass, shared by the
+// single-cloud kernel (pairs_n3.hip) and the batched ensemble (batch.hip).
+//
+// Every unordered pair is evaluated once and applied to both particles, as
+// the reference does (ShieldedCoulombForce.cpp:51-56 add the force to one
+// particle and subtract it from the other).  Same physics, contribution test
+// and dt check as the gather form (pair_core.h); half the pair evaluations.
+//
+// Work decomposition.  Particles (device order) form superblocks of 256
+// (four 64-particle box blocks).  With nsb superblocks, row S pairs with
+// superblocks T = S + d (mod nsb) for d = 1 .. floor(nsb/2) (for even nsb,
+// d = nsb/2 only for S < nsb/2, so that each unordered superblock pair is
+// taken once), plus itself (d = 0, gather form).  Every row does the same
+// amount of work.
+// A workgroup of four waves owns row S and a contiguous range of d (a
+// slice); wave w holds the 64 i-particles of block 4S + w in registers and
+// accumulates their forces across the slice (written once, to part[slice]).
+// For each pair tile (S, T) the reaction on every j of T is summed over the
+// wave's 64 lanes, then over the four waves in LDS in a fixed order, and
+// written to react[d - 1][.][j]; rflag[T nsb/2 + d - 1] says whether the
+// tile was evaluated (1) or culled / not owned (0).  Nothing is atomic, so
+// the sums are deterministic: the stage kernel adds part[0..slices) and then
+// react[d - 1] for d = 1 .. nsb/2 in that order.
+#pragma once
+
+#include <type_traits>
+
+#include "pair_core.h"
+
+namespace demon {
+namespace n3 {
+
+using namespace pair;
+
+// j particles per chunk: four keeps the chunk's j records (x, y, q) in 24
+// SGPRs next to the exp coefficients without spilling.
+constexpr int kChunkN3 = 4;
+
+// v from the lane given by the DPP control CTRL (a permutation within each
+// group of 8 lanes here, so every source lane exists), as two 32-bit DPP
+// moves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+constexpr int kDppSwap1 = 0xB1;        // quad_perm [1, 0, 3, 2]: lane ^ 1
+constexpr int kDppSwap2 = 0x4E;        // quad_perm [2, 3, 0, 1]: lane ^ 2
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror: lane 7 - lane within 8
+
+// Wave sums of the 8 per-lane values v[4 c + u] (component c of the pair
+// force on j0 + u), transposed through LDS: every lane writes its 8 values
+// into scr[v][lane], then lane (v = l / 8, p = l % 8) adds scr[v][8p .. 8p+7]
+// and three DPP butterflies over the 8 lanes of value v (swap 1, swap 2,
+// then the mirror, which pairs the two quad sums) finish the sum, which
+// lane p = 0 stores into red[2 (jj0 + u) + c] (FP64).  Rows are padded to
+// 66 doubles or 68 floats, which keeps both the writes and the 16-byte reads
+// at the minimum bank load.  The order of the additions is fixed, so the
+// result is deterministic.
+template <typename V>
+struct Scr;
+template <>
+struct Scr<double> {
+    static constexpr int kRow = 66;
+};
+template <>
+struct Scr<float> {
+    static constexpr int kRow = 68;
+};
+
+__device__ __forceinline__ double dpp_swap1(double v) { return dpp_f64<kDppSwap1>(v); }
+__device__ __forceinline__ double dpp_swap2(double v) { return dpp_f64<kDppSwap2>(v); }
+__device__ __forceinline__ double dpp_mirror(double v) { return dpp_f64<kDppHalfMirror>(v); }
+__device__ __forceinline__ float dpp_swap1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppSwap1, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float dpp_swap2(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppSwap2, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float dpp_mirror(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppHalfMirror, 0xF, 0xF, true));
+}
+
+template <typename V>
+__device__ __forceinline__ void store_wave_sums(const V (&f)[2 * kChunkN3], V *scr, double *red, int32_t jj0) {
+    constexpr int kRow = Scr<V>::kRow;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int v = 0; v < 2 * kChunkN3; ++v) scr[v * kRow + lane] = f[v];
+    __builtin_amdgcn_wave_barrier();
+    const int v = lane >> 3, p = lane & 7;
+    const V *row = scr + v * kRow + 8 * p;
+    V t = row[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) t += row[k];
+    __builtin_amdgcn_wave_barrier();
+    t += dpp_swap1(t);
+    t += dpp_swap2(t);
+    t += dpp_mirror(t);
+    if (p == 0) red[2 * (jj0 + (v & 3)) + (v >> 2)] = (double)t;
+}
+
+// LDS scratch per wave for the transposes: 8 rows of the FP64 row length
+// (which also holds the 8 FP32 rows).
+constexpr int kScrDoubles = 2 * kChunkN3 * Scr<double>::kRow;
+static_assert(2 * kChunkN3 * Scr<float>::kRow <= 2 * kScrDoubles, "FP32 rows fit the scratch");
+
+// Newton-3 form over one chunk of 4 j: adds F_ij into ax/ay and stores the
+// wave's sum of F_ij for each j into red[2 jj], red[2 jj + 1] (the
+// reaction, negated later).  CHECK: the chunk may hold pairs outside the
+// cutoff (counted into n_near here; a dense block's count is added by the
+// caller).  DTC: the stage-1 launch, with the dt check.  FP32 pair
+// arithmetic keeps the chunk's values in FP32 (the transposed sums of 64
+// terms, and the 4-term i-side sum added to ax/ay in FP64).
+template <bool F32>
+using PairVal = typename std::conditional<F32, float, double>::type;
+
+template <bool F32, bool CHECK, bool DTC>
+__device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[kChunkN3],
+                                         const double (&cy)[kChunkN3], const PairVal<F32> (&cq)[kChunkN3],
+                                         int32_t j0, const PairEnv &e, uint64_t &n_near, float &smin,
+                                         double &ax, double &ay, double *scr, double *red, int32_t jj0) {
+    using V = PairVal<F32>;
+    V f[2 * kChunkN3];
+#pragma unroll
+    for (int u = 0; u < kChunkN3; ++u) {
+        const double dx = t.xi[0] - cx[u];
+        const double dy = t.yi[0] - cy[u];
+        const double r2 = fma(dx, dx, dy * dy);
+        const bool in = pair_in<CHECK>(dx, dy, r2, e);
+        // a dense block has 64 real i and every pair inside the cutoff
+        if (CHECK) n_near += 2 * (uint64_t)__builtin_popcountll(__ballot(in));
+        V fx = 0, fy = 0;
+        if (in) {
+            if constexpr (F32) {
+                pair_force_fp32f(dx, dy, t.cqf[0], cq[u], e.kl2_f, fx, fy);
+            } else {
+                const double s = pair_scale_fp64(r2, t.cqi[0], cq[u], e.cut.kappa, e.ec);
+                fx = s * dx;
+                fy = s * dy;
+            }
+        }
+        if constexpr (!F32) {
+            ax = ax + fx;
+            ay = ay + fy;
+        }
+        f[u] = fx;
+        f[kChunkN3 + u] = fy;
+        if (DTC) pair_dtcheck(t.xi[0], t.yi[0], cx[u], cy[u], r2, t.ibase, j0 + u, e, smin);
+    }
+    if constexpr (F32) {
+        float sx = f[0], sy = f[kChunkN3];
+#pragma unroll
+        for (int u = 1; u < kChunkN3; ++u) {
+            sx += f[u];
+            sy += f[kChunkN3 + u];
+        }
+        ax = ax + (double)sx;
+        ay = ay + (double)sy;
+    }
+    store_wave_sums<V>(f, reinterpret_cast<V *>(scr), red, jj0);
+}
+
+// One cloud's arrays and launch constants for the row function.
+struct N3Row {
+    const double *xs, *ys, *nc, *q;
+    const float *qf;  // q in FP32 (FP32 pair arithmetic)
+    const int32_t *orig;
+    const double *box;
+    double *part;     // [slices][2][npad]
+    double *react;    // [nsb/2][2][npad]
+    uint32_t *rflag;  // [nsb][nsb/2]
+    Scalars *sc;
+    int32_t n, npad, flags;
+    double kappa, rc2p;
+};
+
+using RedLds = double[4][2 * kSuper];  // [wave][2 jj + component]
+using ScrLds = double[4][kScrDoubles]; // per-wave transpose scratch
+
+// One workgroup's share: row S against superblocks S + d for d in [d0, d1),
+// its i-side sums written to part[slice].  Called by all 256 threads.
+template <bool F32, bool DTC>
+__device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_t S, int32_t d0, int32_t d1,
+                                       int32_t slice, RedLds &red, ScrLds &scr) {
+    const double *__restrict__ xs = a.xs;
+    const double *__restrict__ ys = a.ys;
+    const double *__restrict__ nc = a.nc;
+    const double *__restrict__ q = a.q;
+    const float *__restrict__ qf = a.qf;
+    const double *__restrict__ box = a.box;
+    const int32_t n = a.n, npad = a.npad;
+    const Radii rad = launch_radii(a.rc2p, a.flags, a.sc->rmax2_bits);
+    const PairEnv e{make_cutoff(a.kappa, a.rc2p), rad, fp32_kl2(a.kappa), DTC, a.orig, ec};
+    float smin = __builtin_inff();
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int32_t nsb = npad / kSuper;
+    const int32_t half = nsb / 2;
+    const int32_t ibase = S * kSuper + wave * 64 + lane;
+    const ITile<1> t = make_itile<1>(xs, ys, nc, q, n, ibase, rad.rf2);
+    const Box ib = box_union(empty_box(), box + 4 * (t.wlo / kBoxBlock));
+    Box sbS = empty_box();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sbS = box_union(sbS, box + 4 * (S * 4 + k));
+    const bool ireal = t.whi <= n;
+    double ax[1] = {0.0}, ay[1] = {0.0};
+    uint64_t n_near = 0, n_tested = 0;
+    for (int32_t d = d0; d < d1; ++d) {
+        if (d == 0) {
+            // the superblock with itself, every ordered pair, j == i skipped
+            gather_blocks<1, kChunkN3, F32>(t, xs, ys, nc, q, box, ib, n, S * kSuper, S * kSuper + kSuper,
+                                            e, ax, ay, n_near, n_tested, smin);
+            continue;
+        }
+        const int32_t T = S + d < nsb ? S + d : S + d - nsb;
+        uint32_t *fl = a.rflag + (size_t)T * half + (d - 1);
+        Box sbT = empty_box();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sbT = box_union(sbT, box + 4 * (T * 4 + k));
+        // for even nsb the pair (S, S + nsb/2) is reached from both rows and
+        // belongs to the lower one; for odd nsb every offset is distinct
+        if ((d == half && !(nsb & 1) && S >= half) || box_gap2(sbS, sbT) > rad.rf2) {
+            if (threadIdx.x == 0) *fl = 0u;
+            continue;
+        }
+        // the previous tile's sums have been read by all threads
+        __syncthreads();
+        double *r = red[wave];
+        for (int k = lane; k < 2 * kSuper; k += 64) r[k] = 0.0;
+        for (int jb4 = 0; jb4 < 4; ++jb4) {
+            const int32_t jb = T * kSuper + jb4 * kBoxBlock;
+            const BlockClass cls = classify(ib, box + 4 * (jb / kBoxBlock), ireal, jb + kBoxBlock <= n, rad, e.cut);
+            if (cls == kSkip) continue;
+            n_tested += 2 * 64 * kBoxBlock;
+            if (cls == kDense) {
+                n_near += 2 * 64 * kBoxBlock;
+                for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
+                    double cx[kChunkN3], cy[kChunkN3];
+                    PairVal<F32> cq[kChunkN3];
+#pragma unroll
+                    for (int u = 0; u < kChunkN3; ++u) {
+                        cx[u] = xs[j0 + u];
+                        cy[u] = ys[j0 + u];
+                        if constexpr (F32) cq[u] = qf[j0 + u];
+                        else cq[u] = q[j0 + u];
+                    }
+                    chunk_n3<F32, false, DTC>(t, cx, cy, cq, j0, e, n_near, smin, ax[0], ay[0], scr[wave], r,
+                                              j0 - T * kSuper);
+                }
+                continue;
+            }
+            for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
+                double cx[kChunkN3], cy[kChunkN3];
+                PairVal<F32> cq[kChunkN3];
+#pragma unroll
+                for (int u = 0; u < kChunkN3; ++u) {
+                    cx[u] = xs[j0 + u];
+                    cy[u] = ys[j0 + u];
+                }
+                if (!chunk_hit<1, kChunkN3>(t, cx, cy, nc, j0)) continue;
+#pragma unroll
+                for (int u = 0; u < kChunkN3; ++u) {
+                    if constexpr (F32) cq[u] = qf[j0 + u];
+                    else cq[u] = q[j0 + u];
+                }
+                chunk_n3<F32, true, DTC>(t, cx, cy, cq, j0, e, n_near, smin, ax[0], ay[0], scr[wave], r,
+                                         j0 - T * kSuper);
+            }
+        }
+        __syncthreads();
+        // the four waves' sums in a fixed order; the reaction is their negation
+        const int jj = threadIdx.x;
+        const double rx = ((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj];
+        const double ry = ((red[0][2 * jj + 1] + red[1][2 * jj + 1]) + red[2][2 * jj + 1]) + red[3][2 * jj + 1];
+        double *rr = a.react + (size_t)(d - 1) * 2 * npad;
+        rr[T * kSuper + jj] = -rx;
+        rr[npad + T * kSuper + jj] = -ry;
+        if (threadIdx.x == 0) *fl = 1u;
+    }
+    if (DTC) commit_dtmin(smin, &a.sc->dtmin_bits);
+    double *px = a.part + (size_t)(2 * slice) * npad;
+    px[ibase] = ax[0];
+    px[npad + ibase] = ay[0];
+    if (lane == 0) {
+        atomicAdd(&a.sc->pairs_near, (unsigned long long)n_near);
+        atomicAdd(&a.sc->pairs_tested, (unsigned long long)n_tested);
+    }
+}
+
+}  // namespace n3
+}  // namespace demon

@@ -2,11 +2,15 @@
 // into one pass per stage.
 //
 // For stage s the update kernel, per particle:
-//   * sums the stage force in the driver's order (driver.cpp:372-392):
+//   * sums the stage force in the driver's order (driver.cpp:372-396):
 //       Confinement  F  = (q C) r_s            ConfinementForce.cpp:48-54
-//       Drag         F += (m (-gamma)) v_s     DragForce.cpp:143-149
+//       Drag         F += (m (-gamma)) v_s     DragForce.cpp:49-55
+//       Magnetic     F += (q B) (vy_s, -vx_s)  MagneticForce.cpp:44-49
 //       Yukawa       F += sum of the all-pairs partial sums (fixed order)
 //       Thermal      F += heat U (cos t, sin t) ThermalForce.cpp:118-122
+//       TimeVaryingDrag F += (m (-(scale t_s + gamma))) v_s at the stage time
+//                    t_s = t, t + dt/2, t + dt/2, t + dt
+//                    (TimeVaryingDragForce.cpp:12-39, Runge_Kutta4.cpp:31-88)
 //   * forms the RK increments k = (dt F)/m, l = dt v_s (Runge_Kutta4.cpp:32-104),
 //   * builds the next stage state (CacheOperator.cpp:28-73: x + l/2, x + l)
 //     or, after stage 4, the combine x += (a1 + ((a2 + a3) 2 + a4)) / 6
@@ -14,6 +18,9 @@
 //   * writes the next stage's pair records (x, y, v, nc).
 // Stage 1 also turns the device-wide FP32 minimum distance into the step's dt
 // (Integrator::tryToReduceTimeStep, Integrator.cpp:144-155).
+// Runge_Kutta2::moveParticles (Runge_Kutta2.cpp:23-78) shares stage 1 (its
+// operation2 is the same x + l/2); its second stage (kRk2Final) evaluates the
+// force at t + dt/2 on that record and ends the step with x += l2, v += k2.
 // The force arrays of the reference are never materialised between stages,
 // so nothing needs zeroing (Runge_Kutta4.cpp:44-46).
 #include "internal.h"
@@ -36,7 +43,8 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
                                             const PairSums &ps, const unsigned long long *fmask,
                                             int64_t npad,
                                             int64_t i, double confine, double gamma,
-                                            double heat, uint64_t seed, uint64_t step,
+                                            double heat, double bfield, double drag_scale,
+                                            double t_stage, uint64_t seed, uint64_t step,
                                             int32_t stage, int32_t oi, double &fx, double &fy) {
     fx = 0.0;
     fy = 0.0;
@@ -49,6 +57,11 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
         const double dr = mi * (-gamma);
         fx = fx + dr * vxs;
         fy = fy + dr * vys;
+    }
+    if (forces & 1024) {
+        const double qb = qi * bfield;
+        fx = fx + qb * vys;
+        fy = fy - qb * vxs;
     }
     if (forces & 4) {
         for (int32_t s = 0; s < ps.slices; ++s) {
@@ -79,6 +92,11 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
         fx = fx + tv * cos(th);
         fy = fy + tv * sin(th);
     }
+    if (forces & 256) {
+        const double dr = mi * (-(drag_scale * t_stage + gamma));
+        fx = fx + dr * vxs;
+        fy = fy + dr * vys;
+    }
 }
 
 // The Newton-3 tile flags of this block's superblock (one 256-thread block
@@ -107,6 +125,9 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     load_flag_mask(a.pairs, fmask);
 
+    // the stage whose force this launch evaluates (RK2's second stage is a
+    // stage-2 evaluation: record x + l1/2, time t + dt/2, thermal key 2)
+    constexpr int FS = STAGE == kRk2Final ? 2 : STAGE;
     double dt;
     if (STAGE == 1) {
         // Integrator::modifyTimeStep: while any tested pair has
@@ -159,8 +180,10 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
             }
         }
         double fx, fy;
+        // Runge_Kutta4.cpp:31,50,69,88 (Runge_Kutta2.cpp:31,50): the stage times
+        const double ts = FS == 1 ? sc->t : FS == 4 ? sc->t + dt : sc->t + dt / 2.0;
         stage_force(a.forces, xs, ys, vxs, vys, qi, mi, a.pairs, fmask, a.npad, i, a.confine,
-                    a.gamma, a.heat, a.seed, sc->step, STAGE, a.orig[i], fx, fy);
+                    a.gamma, a.heat, a.bfield, a.drag_scale, ts, a.seed, sc->step, FS, a.orig[i], fx, fy);
         const double k = (dt * fx) / mi;
         const double l = dt * vxs;
         const double mm = (dt * fy) / mi;
@@ -179,6 +202,16 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
             ny = 0.5 * nn + y;
             nvx = 0.5 * k + vx;
             nvy = 0.5 * mm + vy;
+        } else if (STAGE == kRk2Final) {
+            // Runge_Kutta2.cpp:68-74
+            nvx = vx + k;
+            nx = x + l;
+            nvy = vy + mm;
+            ny = y + nn;
+            a.x[i] = nx;
+            a.y[i] = ny;
+            a.vx[i] = nvx;
+            a.vy[i] = nvy;
         } else if (STAGE == 3) {
             A23[i] = A23[i] + k;
             A23[P + i] = A23[P + i] + l;
@@ -207,7 +240,7 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
         r2new = r2;
     }
     if (a.nxt.box) wave_box(i < a.n, nx, ny, a.nxt.box + 4 * (i / kBoxBlock));
-    if (STAGE == 4) {
+    if (STAGE == 4 || STAGE == kRk2Final) {
         // block max of |r|^2 for the next stage-1 candidate radius
         double v = r2new;
         for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
@@ -244,7 +277,7 @@ __global__ __launch_bounds__(256) void k_force_out(ForceOutLaunch a) {
     double fx, fy;
     stage_force(a.forces, a.st.xs[i], a.st.ys[i], a.st.vxs[i], a.st.vys[i], a.q[i], a.m[i],
                 a.pairs, fmask, a.npad, i, a.confine,
-                a.gamma, a.heat, a.seed, a.step, a.stage, a.orig[i], fx, fy);
+                a.gamma, a.heat, a.bfield, a.drag_scale, a.t, a.seed, a.step, a.stage, a.orig[i], fx, fy);
     a.fx[i] = fx;
     a.fy[i] = fy;
 }
@@ -292,6 +325,7 @@ hipError_t launch_update(int stage, const UpdateLaunch &a, hipStream_t s) {
         case 2: hipLaunchKernelGGL(k_update<2>, grid, block, 0, s, a); break;
         case 3: hipLaunchKernelGGL(k_update<3>, grid, block, 0, s, a); break;
         case 4: hipLaunchKernelGGL(k_update<4>, grid, block, 0, s, a); break;
+        case kRk2Final: hipLaunchKernelGGL(k_update<kRk2Final>, grid, block, 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -311,6 +345,17 @@ hipError_t launch_force_out(const ForceOutLaunch &a, hipStream_t s) {
 hipError_t launch_prepare_stage(StageBuf st, int64_t n, int64_t npad, hipStream_t s) {
     if (npad % 256) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_prepare, dim3(blocks_for(npad)), dim3(256), 0, s, st, n, npad);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_round_fp32(const double *__restrict__ src, float *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    dst[i] = (float)src[i];
+}
+
+hipError_t launch_round_fp32(const double *src, float *dst, int64_t npad, hipStream_t s) {
+    if (npad % 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_round_fp32, dim3(blocks_for(npad)), dim3(256), 0, s, src, dst);
     return hipGetLastError();
 }
 

@@ -111,7 +111,7 @@ static demon_params base_params(int device) {
     return p;
 }
 
-void Force::evaluate(int stage, uint64_t step) {
+void Force::evaluate(int stage, uint64_t step, double t) {
     demon_params p = base_params(cloud->device());
     p.forces = (int32_t)flag();
     contribute(p);
@@ -119,32 +119,52 @@ void Force::evaluate(int stage, uint64_t step) {
     // charge and mass come from the cloud's host arrays
     check(demon_upload(ctx, cloud->x.data(), cloud->y.data(), cloud->Vx.data(), cloud->Vy.data(),
                        cloud->charge.data(), cloud->mass.data()), ctx);
-    check(demon_force_stage(ctx, (int32_t)flag(), stage, step, cloud->stageX(stage),
-                            cloud->stageY(stage), cloud->stageVx(stage), cloud->stageVy(stage),
-                            cloud->forceX.data(), cloud->forceY.data()),
+    check(demon_force_stage_at(ctx, (int32_t)flag(), stage, step, t, cloud->stageX(stage),
+                               cloud->stageY(stage), cloud->stageVx(stage), cloud->stageVy(stage),
+                               cloud->forceX.data(), cloud->forceY.data()),
           ctx);
 }
 
-void Runge_Kutta4::moveParticles(double endTime) {
+namespace {
+
+// The device parameter block from the integrator's forces, then the
+// moveParticles loop of the given order (4: Runge_Kutta4, 2: Runge_Kutta2).
+int64_t move_on_device(Cloud *cloud, const ForceArray &forces, double init_dt, double &currentTime,
+                       double endTime, int order) {
     demon_params p = base_params(cloud->device());
-    const force_flags device_forces =
-        ConfinementForceFlag | DragForceFlag | ShieldedCoulombForceFlag | ThermalForceFlag;
+    const force_flags device_forces = ConfinementForceFlag | DragForceFlag | ShieldedCoulombForceFlag |
+                                      ThermalForceFlag | MagneticForceFlag | TimeVaryingDragForceFlag;
     for (const Force *f : forces) {
         if (!(f->flag() & device_forces) || (p.forces & f->flag()))
             throw Error(DEMON_ERR_UNSUPPORTED,
-                        "Runge_Kutta4: every force must be a distinct device force "
-                        "(Confinement, Drag, ShieldedCoulomb, Thermal)");
+                        "moveParticles: every force must be a distinct device force "
+                        "(Confinement, Drag, Magnetic, ShieldedCoulomb, Thermal, TimeVaryingDrag)");
         p.forces |= (int32_t)f->flag();
         f->contribute(p);
     }
+    if ((p.forces & DragForceFlag) && (p.forces & TimeVaryingDragForceFlag))
+        throw Error(DEMON_ERR_UNSUPPORTED,
+                    "moveParticles: DragForce and TimeVaryingDragForce together (the reference's driver "
+                    "uses one or the other)");
     if (!(p.forces & ShieldedCoulombForceFlag))
-        throw Error(DEMON_ERR_UNSUPPORTED, "Runge_Kutta4: the device integrator needs ShieldedCoulombForce");
+        throw Error(DEMON_ERR_UNSUPPORTED, "moveParticles: the device integrator needs ShieldedCoulombForce");
     demon_ctx *ctx = cloud->context(p);
     cloud->syncToDevice();
     int64_t steps = 0;
-    check(demon_rk4_move(ctx, &currentTime, endTime, init_dt, &steps), ctx);
-    lastSteps_ = steps;
+    if (order == 4) check(demon_rk4_move(ctx, &currentTime, endTime, init_dt, &steps), ctx);
+    else check(demon_rk2_move(ctx, &currentTime, endTime, init_dt, &steps), ctx);
     cloud->syncFromDevice();
+    return steps;
+}
+
+}  // namespace
+
+void Runge_Kutta4::moveParticles(double endTime) {
+    lastSteps_ = move_on_device(cloud, forces, init_dt, currentTime, endTime, 4);
+}
+
+void Runge_Kutta2::moveParticles(double endTime) {
+    lastSteps_ = move_on_device(cloud, forces, init_dt, currentTime, endTime, 2);
 }
 
 // ---------------------------------------------------------------------------

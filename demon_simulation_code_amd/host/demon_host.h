@@ -129,9 +129,10 @@ public:
     virtual void contribute(demon_params &p) const = 0;
 
 protected:
-    // Evaluates this force on the cloud's stage-S state on the device and adds
-    // it into cloud->forceX / forceY (the contract of Force::forceS).
-    void evaluate(int stage, uint64_t step);
+    // Evaluates this force on the cloud's stage-S state at the stage time t on
+    // the device and adds it into cloud->forceX / forceY (the contract of
+    // Force::forceS).
+    void evaluate(int stage, uint64_t step, double t);
 };
 
 typedef std::vector<Force *> ForceArray;
@@ -139,10 +140,10 @@ typedef std::vector<Force *> ForceArray;
 class ShieldedCoulombForce : public Force {
 public:
     ShieldedCoulombForce(Cloud *C, double shieldingConstant) : Force(C), shielding(shieldingConstant) {}
-    void force1(double) override { evaluate(1, 0); }
-    void force2(double) override { evaluate(2, 0); }
-    void force3(double) override { evaluate(3, 0); }
-    void force4(double) override { evaluate(4, 0); }
+    void force1(double t) override { evaluate(1, 0, t); }
+    void force2(double t) override { evaluate(2, 0, t); }
+    void force3(double t) override { evaluate(3, 0, t); }
+    void force4(double t) override { evaluate(4, 0, t); }
     force_flags flag() const override { return ShieldedCoulombForceFlag; }
     void contribute(demon_params &p) const override { p.shielding = shielding; }
     double shielding;  // inverse shielding length [1/m]
@@ -151,10 +152,10 @@ public:
 class ConfinementForce : public Force {
 public:
     ConfinementForce(Cloud *C, double confineConst) : Force(C), confine(confineConst) {}
-    void force1(double) override { evaluate(1, 0); }
-    void force2(double) override { evaluate(2, 0); }
-    void force3(double) override { evaluate(3, 0); }
-    void force4(double) override { evaluate(4, 0); }
+    void force1(double t) override { evaluate(1, 0, t); }
+    void force2(double t) override { evaluate(2, 0, t); }
+    void force3(double t) override { evaluate(3, 0, t); }
+    void force4(double t) override { evaluate(4, 0, t); }
     force_flags flag() const override { return ConfinementForceFlag; }
     void contribute(demon_params &p) const override { p.confine = confine; }
     double confine;  // [V/m^2]
@@ -163,10 +164,10 @@ public:
 class DragForce : public Force {
 public:
     DragForce(Cloud *C, double gamma) : Force(C), dragConst(-gamma) {}
-    void force1(double) override { evaluate(1, 0); }
-    void force2(double) override { evaluate(2, 0); }
-    void force3(double) override { evaluate(3, 0); }
-    void force4(double) override { evaluate(4, 0); }
+    void force1(double t) override { evaluate(1, 0, t); }
+    void force2(double t) override { evaluate(2, 0, t); }
+    void force3(double t) override { evaluate(3, 0, t); }
+    void force4(double t) override { evaluate(4, 0, t); }
     force_flags flag() const override { return DragForceFlag; }
     void contribute(demon_params &p) const override { p.drag_gamma = -dragConst; }
     double dragConst;  // -gamma [1/s] (DragForce.h:17)
@@ -175,10 +176,10 @@ public:
 class ThermalForce : public Force {
 public:
     ThermalForce(Cloud *C, double redFactor, uint64_t seed = 0) : Force(C), heatVal(redFactor), seed(seed) {}
-    void force1(double) override { evaluate(1, step_++); }
-    void force2(double) override { evaluate(2, step_); }
-    void force3(double) override { evaluate(3, step_); }
-    void force4(double) override { evaluate(4, step_); }
+    void force1(double t) override { evaluate(1, step_++, t); }
+    void force2(double t) override { evaluate(2, step_, t); }
+    void force3(double t) override { evaluate(3, step_, t); }
+    void force4(double t) override { evaluate(4, step_, t); }
     force_flags flag() const override { return ThermalForceFlag; }
     void contribute(demon_params &p) const override {
         p.heat_val = heatVal;
@@ -189,6 +190,39 @@ public:
 
 private:
     uint64_t step_ = 0;
+};
+
+// MagneticForce (MagneticForce.h:14-33): F += q B (vy, -vx), B along z.
+class MagneticForce : public Force {
+public:
+    MagneticForce(Cloud *C, double magneticField) : Force(C), BField(magneticField) {}
+    void force1(double t) override { evaluate(1, 0, t); }
+    void force2(double t) override { evaluate(2, 0, t); }
+    void force3(double t) override { evaluate(3, 0, t); }
+    void force4(double t) override { evaluate(4, 0, t); }
+    force_flags flag() const override { return MagneticForceFlag; }
+    void contribute(demon_params &p) const override { p.magnetic_field = BField; }
+    double BField;  // [T]
+};
+
+// TimeVaryingDragForce (TimeVaryingDragForce.h:14-33): the drag constant at
+// the stage time t is -(scale t + offset).  The reference's driver gives it
+// offset = dragGamma and then leaves DragForce out (driver.cpp:322-323,394).
+class TimeVaryingDragForce : public Force {
+public:
+    TimeVaryingDragForce(Cloud *C, double scale, double offset)
+        : Force(C), scaleConst(scale), offsetConst(offset) {}
+    void force1(double t) override { evaluate(1, 0, t); }
+    void force2(double t) override { evaluate(2, 0, t); }
+    void force3(double t) override { evaluate(3, 0, t); }
+    void force4(double t) override { evaluate(4, 0, t); }
+    force_flags flag() const override { return TimeVaryingDragForceFlag; }
+    void contribute(demon_params &p) const override {
+        p.drag_scale = scaleConst;
+        p.drag_gamma = offsetConst;
+    }
+    double scaleConst;   // [Hz/s]
+    double offsetConst;  // [Hz]
 };
 
 class Integrator {
@@ -213,6 +247,19 @@ public:
     // of Integrator::modifyTimeStep, entirely on the device.
     void moveParticles(double endTime) override;
     // Steps taken by the last moveParticles call.
+    int64_t lastSteps() const { return lastSteps_; }
+
+private:
+    int64_t lastSteps_ = 0;
+};
+
+class Runge_Kutta2 : public Integrator {
+public:
+    Runge_Kutta2(Cloud *C, const ForceArray &FA, double timeStep, double startTime)
+        : Integrator(C, FA, timeStep, startTime) {}
+    // Runge_Kutta2::moveParticles (Runge_Kutta2.cpp:23-78): the midpoint rule
+    // with the same dt check, entirely on the device.
+    void moveParticles(double endTime) override;
     int64_t lastSteps() const { return lastSteps_; }
 
 private:

@@ -2,14 +2,17 @@
 // (driver.cpp:315-475) over the device integrator.
 //
 //   demon [-P Params.cfg] [-n rows [cols]] [-t dt] [-e end] [-o dataTimeStep]
-//         [-s shielding] [-C confine] [-g gamma] [-T heat] [-q mean sigma]
+//         [-s shielding] [-C confine] [-g gamma] [-T heat] [-B field]
+//         [-D scale offset] [-I] [-q mean sigma]
 //         [-r mean sigma] [-i spacing] [-p x y] [-k vx vy] [-d density]
 //         [-O snapshots.csv] [--hex N] [--seed S] [--device D] [-h]
 //
 // Options are applied left to right, so options after -P override the file
-// (driver.cpp:834-1030).  Always-on forces: Drag, Confinement, ShieldedCoulomb
-// (driver.cpp:322-326); -T adds the thermal force.  The reference's other
-// force plugins and -I/-M/-c/-f are not part of this build and are rejected.
+// (driver.cpp:834-1030).  Always-on forces: Confinement, ShieldedCoulomb and
+// Drag, unless TimeVaryingDrag (-D, forceFlags D) replaces it (driver.cpp:
+// 322-326); -T adds the thermal force, -B the magnetic force; -I selects
+// Runge_Kutta2 (driver.cpp:437-438).  The reference's other force plugins and
+// -M/-c/-f are not part of this build and are rejected.
 // Snapshots (t, index, x, y, vx, vy) go to a CSV file every dataTimeStep.
 #include <chrono>
 #include <cstdio>
@@ -30,7 +33,7 @@ void help() {
                  "  -n R [C]       R*C particles on a grid (rounded up to a multiple of 4)\n"
                  "  --hex N        N particles on a jittered hexagonal lattice instead\n"
                  "  -t dt -e end -o dataTimeStep\n"
-                 "  -s kappa -C confine -g gamma -T heat\n"
+                 "  -s kappa -C confine -g gamma -T heat -B field -D scale offset -I (RK2)\n"
                  "  -q qMean qSigma -r rMean rSigma -i spacing -p x y -k vx vy -d density\n"
                  "  -O file.csv    write snapshots; --seed S; --device D; -h help\n";
 }
@@ -43,6 +46,7 @@ bool is_number(const char *s) {
 
 struct Cli {
     Params P;
+    bool rk4 = true;  // driver.cpp:64
     cloud_index rowX = 4, rowY = 0;
     int64_t hex = 0;
     uint64_t seed = 0;
@@ -109,6 +113,17 @@ Cli parse(int argc, char **argv) {
                 throw Error(DEMON_ERR_ARG, "option -T already set or conflicts with -L/-v");
             P.usedForces |= ThermalForceFlag;
             P.thermRed = take(argc, argv, i, P.thermRed, "heat factor");
+        } else if (a == "-B") {
+            if (P.usedForces & MagneticForceFlag) throw Error(DEMON_ERR_ARG, "option -B already set");
+            P.usedForces |= MagneticForceFlag;
+            P.magneticFieldStrength = take(argc, argv, i, P.magneticFieldStrength, "magnetic field");
+        } else if (a == "-D") {
+            if (P.usedForces & TimeVaryingDragForceFlag) throw Error(DEMON_ERR_ARG, "option -D already set");
+            P.usedForces |= TimeVaryingDragForceFlag;
+            P.dragScale = take(argc, argv, i, P.dragScale, "scale factor");
+            P.dragGamma = take(argc, argv, i, P.dragGamma, "offset");
+        } else if (a == "-I") {
+            c.rk4 = false;
         } else if (a == "-q") {
             P.qMean = take(argc, argv, i, P.qMean, "mean number of charges");
             P.qSigma = take(argc, argv, i, P.qSigma, "number of charges sigma");
@@ -130,10 +145,12 @@ Cli parse(int argc, char **argv) {
             throw Error(DEMON_ERR_UNSUPPORTED, "Unknown or unsupported option " + a);
         }
     }
-    const force_flags device_forces = ConfinementForceFlag | DragForceFlag | ShieldedCoulombForceFlag | ThermalForceFlag;
+    const force_flags device_forces = ConfinementForceFlag | DragForceFlag | ShieldedCoulombForceFlag |
+                                      ThermalForceFlag | MagneticForceFlag | TimeVaryingDragForceFlag;
     if (c.P.usedForces & ~device_forces)
         throw Error(DEMON_ERR_UNSUPPORTED, "forceFlags select forces that are not part of this build "
-                                           "(only Confinement, Drag, ShieldedCoulomb and Thermal run on the device)");
+                                           "(Confinement, Drag, Magnetic, ShieldedCoulomb, Thermal and "
+                                           "TimeVaryingDrag run on the device)");
     return c;
 }
 
@@ -153,16 +170,21 @@ int main(int argc, char **argv) {
         else
             cloud = Cloud::initializeGrid(P.numParticles, c.rowX, c.rowY, P.rMean, P.rSigma, P.qMean,
                                           P.qSigma, c.seed, c.device);
-        // driver.cpp:372-392: construction order is summation order
+        // driver.cpp:372-394: construction order is summation order; Drag
+        // only without TimeVaryingDrag (driver.cpp:322-323)
         ForceArray forces;
         ConfinementForce conf(cloud.get(), P.confinementConst);
         DragForce drag(cloud.get(), P.dragGamma);
+        MagneticForce mag(cloud.get(), P.magneticFieldStrength);
         ShieldedCoulombForce yuk(cloud.get(), P.shieldingConstant);
         ThermalForce therm(cloud.get(), P.thermRed, c.seed);
+        TimeVaryingDragForce tvdrag(cloud.get(), P.dragScale, P.dragGamma);
         forces.push_back(&conf);
-        forces.push_back(&drag);
+        if (!(P.usedForces & TimeVaryingDragForceFlag)) forces.push_back(&drag);
+        if (P.usedForces & MagneticForceFlag) forces.push_back(&mag);
         forces.push_back(&yuk);
         if (P.usedForces & ThermalForceFlag) forces.push_back(&therm);
+        if (P.usedForces & TimeVaryingDragForceFlag) forces.push_back(&tvdrag);
 
         FILE *out = c.out.empty() ? nullptr : std::fopen(c.out.c_str(), "w");
         if (!c.out.empty() && !out) throw Error(DEMON_ERR_ARG, "cannot open " + c.out);
@@ -174,19 +196,21 @@ int main(int argc, char **argv) {
             std::fflush(out);
         };
         snapshot(P.startTime);
-        Runge_Kutta4 I(cloud.get(), forces, P.simTimeStep, P.startTime);
+        Runge_Kutta4 I4(cloud.get(), forces, P.simTimeStep, P.startTime);
+        Runge_Kutta2 I2(cloud.get(), forces, P.simTimeStep, P.startTime);
+        Integrator &I = c.rk4 ? static_cast<Integrator &>(I4) : static_cast<Integrator &>(I2);
         double startTime = P.startTime;
         int64_t steps = 0;
         while (startTime < P.endTime) {  // driver.cpp:443-450
             std::cout << "\rCurrent Time: " << I.currentTime << "s (" << I.currentTime / P.endTime * 100.0
                       << "% Complete)" << std::flush;
             I.moveParticles(startTime += P.dataTimeStep);
-            steps += I.lastSteps();
+            steps += c.rk4 ? I4.lastSteps() : I2.lastSteps();
             snapshot(I.currentTime);
         }
         if (out) std::fclose(out);
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-        std::cout << "\nN = " << cloud->n << ", " << steps << " RK4 steps, t = " << I.currentTime
+        std::cout << "\nN = " << cloud->n << ", " << steps << (c.rk4 ? " RK4" : " RK2") << " steps, t = " << I.currentTime
                   << " s, elapsed " << secs << " s" << std::endl;
     } catch (const Error &e) {
         std::cerr << "Error: " << e.what() << std::endl;

@@ -7,15 +7,21 @@
 //                            (ShieldedCoulombForce.h:15-39): force1..force4
 //                            read the same stage getters' data (x / xCache)
 //                            and add into Cloud::forceX/forceY.
-//   GpuConfinementForce, GpuDragForce, GpuThermalForce
-//                            the same for ConfinementForce, DragForce and
-//                            ThermalForce, carrying their parameters.
+//   GpuConfinementForce, GpuDragForce, GpuThermalForce, GpuMagneticForce,
+//   GpuTimeVaryingDragForce
+//                            the same for ConfinementForce, DragForce,
+//                            ThermalForce, MagneticForce and
+//                            TimeVaryingDragForce, carrying their parameters
+//                            (the stage time reaches the device for the
+//                            time-varying drag).
 //   GpuRungeKutta4           drop-in for Runge_Kutta4 (Runge_Kutta4.h:14-31):
 //                            moveParticles(endTime) uploads the Cloud, runs
 //                            the whole while (t < end) loop -- dt check,
 //                            four force stages, RK4 combine -- on the device
 //                            and downloads x, y, Vx, Vy.  Every force in its
 //                            ForceArray must be one of the Gpu* forces.
+//   GpuRungeKutta2           the same for Runge_Kutta2 (Runge_Kutta2.h), the
+//                            driver's -I.
 // Errors of the C ABI are reported on stderr and end the program with
 // exit(1), the reference's convention for FITS errors (driver.cpp:202-213).
 #ifndef DEMON_GPU_FORCES_H
@@ -54,7 +60,7 @@ public:
     virtual void contribute(demon_params &p) const = 0;
 
 protected:
-    void stage(int s) {
+    void stage(int s, double t) {
         demon_params p{};
         p.shielding = 1.0;
         p.forces = flag_;
@@ -63,9 +69,9 @@ protected:
         else require(demon_set_params(ctx_, &p), ctx_, "demon_set_params");
         require(demon_upload(ctx_, cloud->x, cloud->y, cloud->Vx, cloud->Vy, cloud->charge, cloud->mass),
                 ctx_, "demon_upload");
-        require(demon_force_stage(ctx_, flag_, s, step_, stage_x(cloud, s), stage_y(cloud, s),
-                                  stage_vx(cloud, s), stage_vy(cloud, s), cloud->forceX, cloud->forceY),
-                ctx_, "demon_force_stage");
+        require(demon_force_stage_at(ctx_, flag_, s, step_, t, stage_x(cloud, s), stage_y(cloud, s),
+                                     stage_vx(cloud, s), stage_vy(cloud, s), cloud->forceX, cloud->forceY),
+                ctx_, "demon_force_stage_at");
         if (s == 4) ++step_;
     }
 
@@ -76,10 +82,10 @@ private:
 };
 
 #define DEMON_GPU_FORCE_STAGES                        \
-    void force1(const double) { stage(1); }           \
-    void force2(const double) { stage(2); }           \
-    void force3(const double) { stage(3); }           \
-    void force4(const double) { stage(4); }
+    void force1(const double t) { stage(1, t); }      \
+    void force2(const double t) { stage(2, t); }      \
+    void force3(const double t) { stage(3, t); }      \
+    void force4(const double t) { stage(4, t); }
 
 // FITS keywords as the CPU classes write them (e.g. ShieldedCoulombForce.cpp:239-277).
 inline void write_key(fitsfile *file, int *error, long bit, const char *key, double value, const char *comment) {
@@ -158,36 +164,90 @@ public:
     uint64_t seed;
 };
 
+class GpuMagneticForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuMagneticForce(Cloud *C, double magneticField) : GpuForceBase(C, DEMON_FORCE_MAGNETIC), BField(magneticField) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const { p.magnetic_field = BField; }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, MagneticForceFlag, "magneticField", BField, "[T] (MagneticForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) { demon_gpu_binding::read_key(f, e, "magneticField", &BField); }
+    double BField;
+};
+
+// TimeVaryingDragForce(C, scale, offset) (TimeVaryingDragForce.h:16-17): the
+// drag constant at stage time t is -(scale t + offset).
+class GpuTimeVaryingDragForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuTimeVaryingDragForce(Cloud *C, double scale, double offset)
+        : GpuForceBase(C, DEMON_FORCE_TIME_VARYING_DRAG), scaleConst(scale), offsetConst(offset) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const {
+        p.drag_scale = scaleConst;
+        p.drag_gamma = offsetConst;
+    }
+    void writeForce(fitsfile *const f, int *const e) const {
+        // as the CPU class: DragForce's keyword and bit, then its own (TimeVaryingDragForce.cpp:41-64)
+        demon_gpu_binding::write_key(f, e, DragForceFlag, "dragConst", -offsetConst, "[Hz] (DragForce)");
+        demon_gpu_binding::write_key(f, e, TimeVaryingDragForceFlag, "TVDragScaleConst", scaleConst,
+                                     "[Hz/s] (TimeVaryingDragForce)");
+        demon_gpu_binding::write_key(f, e, TimeVaryingDragForceFlag, "TVDragOffsetConst", offsetConst,
+                                     "[Hz] (TimeVaryingDragForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) {
+        demon_gpu_binding::read_key(f, e, "TVDragScaleConst", &scaleConst);
+        demon_gpu_binding::read_key(f, e, "TVDragOffsetConst", &offsetConst);
+    }
+    double scaleConst, offsetConst;
+};
+
 #undef DEMON_GPU_FORCE_STAGES
 
-class GpuRungeKutta4 : public Integrator {
+namespace demon_gpu_binding {
+
+// The device parameter block of a ForceArray of Gpu* forces.
+inline demon_params gather_params(const ForceArray &forces, int device) {
+    demon_params p{};
+    p.shielding = 1.0;
+    p.dt_check = 1;  // moveParticles always calls modifyTimeStep (Runge_Kutta4.cpp:25, Runge_Kutta2.cpp:25)
+    p.device = device;
+    for (Force *const F : forces) {
+        GpuForceBase *g = dynamic_cast<GpuForceBase *>(F);
+        if (!g) {
+            std::fprintf(stderr, "Error: the Gpu integrators need Gpu* forces only\n");
+            std::exit(1);
+        }
+        p.forces |= g->flag();
+        g->contribute(p);
+    }
+    return p;
+}
+
+}  // namespace demon_gpu_binding
+
+// order 4: GpuRungeKutta4, 2: GpuRungeKutta2.
+template <int ORDER>
+class GpuRungeKutta : public Integrator {
 public:
-    GpuRungeKutta4(Cloud *const C, const ForceArray &FA, const double timeStep, const double startTime,
-                   int device = 0)
+    GpuRungeKutta(Cloud *const C, const ForceArray &FA, const double timeStep, const double startTime,
+                  int device = 0)
         : Integrator(C, FA, timeStep, startTime), device_(device) {}
-    ~GpuRungeKutta4() { demon_destroy(ctx_); }
+    ~GpuRungeKutta() { demon_destroy(ctx_); }
 
     void moveParticles(const double endTime) {
-        demon_params p{};
-        p.shielding = 1.0;
-        p.dt_check = 1;  // Runge_Kutta4.cpp:25 always calls modifyTimeStep
-        p.device = device_;
-        for (Force *const F : forces) {
-            demon_gpu_binding::GpuForceBase *g = dynamic_cast<demon_gpu_binding::GpuForceBase *>(F);
-            if (!g) {
-                std::fprintf(stderr, "Error: GpuRungeKutta4 needs Gpu* forces only\n");
-                std::exit(1);
-            }
-            p.forces |= g->flag();
-            g->contribute(p);
-        }
+        const demon_params p = demon_gpu_binding::gather_params(forces, device_);
         if (!ctx_) demon_gpu_binding::require(demon_create(&ctx_, (int64_t)cloud->n, &p), nullptr, "demon_create");
         else demon_gpu_binding::require(demon_set_params(ctx_, &p), ctx_, "demon_set_params");
         demon_gpu_binding::require(
             demon_upload(ctx_, cloud->x, cloud->y, cloud->Vx, cloud->Vy, cloud->charge, cloud->mass), ctx_,
             "demon_upload");
-        demon_gpu_binding::require(demon_rk4_move(ctx_, &currentTime, endTime, init_dt, nullptr), ctx_,
-                                   "demon_rk4_move");
+        if (ORDER == 4)
+            demon_gpu_binding::require(demon_rk4_move(ctx_, &currentTime, endTime, init_dt, nullptr), ctx_,
+                                       "demon_rk4_move");
+        else
+            demon_gpu_binding::require(demon_rk2_move(ctx_, &currentTime, endTime, init_dt, nullptr), ctx_,
+                                       "demon_rk2_move");
         demon_gpu_binding::require(demon_download(ctx_, cloud->x, cloud->y, cloud->Vx, cloud->Vy), ctx_,
                                    "demon_download");
     }
@@ -196,5 +256,8 @@ private:
     int device_;
     demon_ctx *ctx_ = nullptr;
 };
+
+typedef GpuRungeKutta<4> GpuRungeKutta4;
+typedef GpuRungeKutta<2> GpuRungeKutta2;
 
 #endif  // DEMON_GPU_FORCES_H

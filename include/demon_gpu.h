@@ -12,15 +12,20 @@
  * Reference interfaces replaced (AU-PSL/demon_simulation_code):
  *   Cloud (Cloud.h:16-87)                  -> demon_create / demon_upload /
  *                                             demon_download
- *   Force::force1..force4 (Force.h:15-69)  -> demon_force_stage
+ *   Force::force1..force4 (Force.h:15-69)  -> demon_force_stage_at /
+ *                                             demon_force_stage
  *     ShieldedCoulombForce (ShieldedCoulombForce.cpp:37-237)
  *     ConfinementForce     (ConfinementForce.cpp:13-54)
  *     DragForce            (DragForce.cpp:13-55)
  *     ThermalForce         (ThermalForce.cpp:42-122)
+ *     MagneticForce        (MagneticForce.cpp:13-49)
+ *     TimeVaryingDragForce (TimeVaryingDragForce.cpp:12-39)
  *   Force::readForce / constructor args    -> demon_set_params
  *   Runge_Kutta4::moveParticles (Runge_Kutta4.cpp:23-137), including
  *     Integrator::modifyTimeStep (Integrator.cpp:68-182)
  *                                          -> demon_rk4_move / demon_rk4_steps
+ *   Runge_Kutta2::moveParticles (Runge_Kutta2.cpp:23-78)
+ *                                          -> demon_rk2_move / demon_rk2_steps
  */
 #ifndef DEMON_GPU_H
 #define DEMON_GPU_H
@@ -31,7 +36,7 @@
 extern "C" {
 #endif
 
-#define DEMON_ABI_VERSION 1
+#define DEMON_ABI_VERSION 2
 
 /* Return codes. */
 #define DEMON_OK 0
@@ -46,6 +51,8 @@ extern "C" {
 #define DEMON_FORCE_DRAG 2
 #define DEMON_FORCE_SHIELDED_COULOMB 4
 #define DEMON_FORCE_THERMAL 16
+#define DEMON_FORCE_TIME_VARYING_DRAG 256 /* the reference's driver then leaves out DRAG */
+#define DEMON_FORCE_MAGNETIC 1024
 
 /* Pair-force arithmetic. */
 #define DEMON_PRECISION_FP64 0      /* FP64 throughout */
@@ -61,6 +68,9 @@ typedef struct demon_params {
     int32_t precision; /* DEMON_PRECISION_* */
     int32_t dt_check;  /* 1: Integrator::modifyTimeStep each step (reference behaviour) */
     int32_t device;    /* HIP device ordinal */
+    double magnetic_field; /* B [T]; MagneticForce(cloud, B): F += q*B*(vy, -vx) */
+    double drag_scale;     /* [Hz/s]; TimeVaryingDragForce(cloud, scale, drag_gamma):
+                              F += m*(-(scale*t + drag_gamma))*v at the stage time t */
 } demon_params;
 
 /* Counters of the work done since demon_upload (or the last reset). */
@@ -102,10 +112,19 @@ int demon_upload(demon_ctx *ctx, const double *x, const double *y, const double 
 int demon_download(demon_ctx *ctx, double *x, double *y, double *vx, double *vy);
 
 /* Force::force<stage>(t) for the forces in `forces` (a subset of the
- * context's flags) evaluated on the given stage state, ADDED into fx/fy
- * (host arrays, caller's order), as Force::forceS adds into Cloud::forceX/Y.
- * stage (1..4) only keys the thermal RNG (with `step`).  vxs/vys may be NULL
- * when DRAG is not requested.  Uses the context's charge and mass. */
+ * context's flags) evaluated on the given stage state at time t, ADDED into
+ * fx/fy (host arrays, caller's order), as Force::forceS adds into
+ * Cloud::forceX/Y, in the driver's order (Confinement, Drag, Magnetic,
+ * ShieldedCoulomb, Thermal, TimeVaryingDrag; driver.cpp:373-394).  stage
+ * (1..4) keys the thermal RNG (with `step`); t is the stage time the
+ * integrator passes (Runge_Kutta4.cpp:31,50,69,88), read by TimeVaryingDrag
+ * only.  vxs/vys may be NULL when no velocity-dependent force is requested.
+ * Uses the context's charge and mass. */
+int demon_force_stage_at(demon_ctx *ctx, int32_t forces, int32_t stage, uint64_t step, double t,
+                         const double *xs, const double *ys, const double *vxs,
+                         const double *vys, double *fx, double *fy);
+
+/* The same without the time; rejects TIME_VARYING_DRAG. */
 int demon_force_stage(demon_ctx *ctx, int32_t forces, int32_t stage, uint64_t step,
                       const double *xs, const double *ys, const double *vxs,
                       const double *vys, double *fx, double *fy);
@@ -119,8 +138,15 @@ int demon_rk4_move(demon_ctx *ctx, double *current_time, double end_time, double
 /* Exactly `steps` iterations of the same loop body (no end-time test). */
 int demon_rk4_steps(demon_ctx *ctx, int64_t steps, double init_dt, double *current_time);
 
-/* Captures one RK4 step into a HIP graph and replays it for later
- * demon_rk4_* calls (1 = on, 0 = plain launches). */
+/* Runge_Kutta2::moveParticles(end_time) (Runge_Kutta2.cpp:23-78, the
+ * driver's -I): the same loop and dt check with the midpoint rule -- force at
+ * (t, x), then at (t + dt/2, x + l1/2), then x += l2, v += k2. */
+int demon_rk2_move(demon_ctx *ctx, double *current_time, double end_time, double init_dt,
+                   int64_t *steps_done);
+int demon_rk2_steps(demon_ctx *ctx, int64_t steps, double init_dt, double *current_time);
+
+/* Captures one step into a HIP graph and replays it for later demon_rk4_* /
+ * demon_rk2_* calls (1 = on, 0 = plain launches). */
 int demon_use_graph(demon_ctx *ctx, int32_t enable);
 
 int demon_get_stats(const demon_ctx *ctx, demon_stats *s);
@@ -200,12 +226,18 @@ int demon_batch_steps(demon_batch *b, int64_t steps, double init_dt);
 int demon_batch_cloud_stats(const demon_batch *b, int32_t cloud, demon_stats *s);
 int demon_batch_use_graph(demon_batch *b, int32_t enable);
 int demon_batch_last_ms(const demon_batch *b, double *ms);
-/* Tile shape of the clouds whose padded size falls in [2^bucket, 2^(bucket+1)). */
+/* Form and tile shape of the clouds whose padded size falls in
+ * [2^bucket, 2^(bucket+1)): i_per_lane 1, 2 or 4 is the gather form (each
+ * ordered pair evaluated for its i; j slices of jslice particles, a multiple
+ * of 64); i_per_lane DEMON_BATCH_NEWTON3 is the Newton-3 form (each unordered
+ * pair once, as ShieldedCoulombForce.cpp:51-56; work items of one 256-particle
+ * row against jslice superblock offsets). */
+#define DEMON_BATCH_NEWTON3 0
 int demon_batch_set_shape(demon_batch *b, int32_t bucket, int32_t i_per_lane, int32_t jslice);
 int demon_batch_get_shape(const demon_batch *b, int32_t bucket, int32_t *i_per_lane,
                           int32_t *jslice, double *tuned_us);
-/* Autotuner: times every candidate shape per N bucket present and keeps the
- * fastest (reps timed runs each, after one warm-up). */
+/* Autotuner: times every candidate form and shape per N bucket present and
+ * keeps the fastest (reps timed runs each, after one warm-up). */
 int demon_batch_tune(demon_batch *b, int32_t reps);
 
 /* Host-only helper (no device needed): the synthetic hexagonal-lattice input

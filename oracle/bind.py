@@ -32,6 +32,8 @@ class OracleParams(ctypes.Structure):
         ("forces", ctypes.c_int32),
         ("dt_check", ctypes.c_int32),
         ("gather", ctypes.c_int32),
+        ("magnetic_field", ctypes.c_double),
+        ("drag_scale", ctypes.c_double),
     ]
 
 
@@ -84,11 +86,13 @@ class Oracle:
         L.oracle_dtcheck_covers.restype = ctypes.c_int
         L.oracle_rk4_steps.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.POINTER(OracleParams),
                                        ctypes.c_double, _I64, _D, ctypes.POINTER(ctypes.c_uint64)]
+        L.oracle_rk2_steps.argtypes = L.oracle_rk4_steps.argtypes
+        L.oracle_rk2_steps.restype = ctypes.c_int
         L.oracle_rk4_move.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.POINTER(OracleParams),
                                       ctypes.c_double, ctypes.c_double, _D,
                                       ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_I64)]
         L.oracle_stage_force.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.POINTER(OracleParams),
-                                         ctypes.c_uint64, ctypes.c_int, _D, _D]
+                                         ctypes.c_uint64, ctypes.c_int, ctypes.c_double, _D, _D]
         L.oracle_fit_loss_grad.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_int, _I64, _I64, _D, _D, _D, _D, _D, _D, _D,
@@ -132,11 +136,11 @@ class Oracle:
     def near_pairs(self, x, y, kappa) -> int:
         return int(self.L.oracle_near_pairs(len(x), p(_f64(x)), p(_f64(y)), kappa))
 
-    def stage_force(self, xs, ys, vxs, vys, q, m, params: OracleParams, step=0, stage=1):
+    def stage_force(self, xs, ys, vxs, vys, q, m, params: OracleParams, step=0, stage=1, t=0.0):
         n = len(xs)
         fx, fy = np.zeros(n), np.zeros(n)
         rc = self.L.oracle_stage_force(n, p(_f64(xs)), p(_f64(ys)), p(_f64(vxs)), p(_f64(vys)),
-                                       p(_f64(q)), p(_f64(m)), ctypes.byref(params), step, stage,
+                                       p(_f64(q)), p(_f64(m)), ctypes.byref(params), step, stage, t,
                                        p(fx), p(fy))
         assert rc == 0
         return fx, fy
@@ -193,6 +197,16 @@ class Oracle:
         assert rc == 0, "oracle_rk4_steps rejected the input"
         return (*s, float(tt[0]), sc.value)
 
+    def rk2_steps(self, x, y, vx, vy, q, m, params: OracleParams, dt, steps, t=0.0, step0=0):
+        """Runge_Kutta2 steps; returns (x, y, vx, vy, t, step) (copies)."""
+        s = [_f64(a).copy() for a in (x, y, vx, vy)]
+        tt = np.array([t])
+        sc = ctypes.c_uint64(step0)
+        rc = self.L.oracle_rk2_steps(len(s[0]), *[p(a) for a in s], p(_f64(q)), p(_f64(m)),
+                                     ctypes.byref(params), dt, steps, p(tt), ctypes.byref(sc))
+        assert rc == 0, "oracle_rk2_steps rejected the input"
+        return (*s, float(tt[0]), sc.value)
+
     def rk4_move(self, x, y, vx, vy, q, m, params: OracleParams, dt, end_time, t=0.0):
         s = [_f64(a).copy() for a in (x, y, vx, vy)]
         tt = np.array([t])
@@ -236,6 +250,10 @@ class Reference:
         L.ref_rk4.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double,
                               ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                               ctypes.c_int, _I64, ctypes.c_double, _D, ctypes.c_int]
+        L.ref_rk4_ext.argtypes = [_I64, _D, _D, _D, _D, _D, _D, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_int, ctypes.c_double, _I64, _D, ctypes.c_int]
+        L.ref_rk2_ext.argtypes = L.ref_rk4_ext.argtypes
 
     @staticmethod
     def available(openmp: bool = False) -> bool:
@@ -258,6 +276,18 @@ class Reference:
         tt = np.array([t])
         rc = self.L.ref_rk4(len(s[0]), *[p(a) for a in s], p(_f64(q)), p(_f64(m)), kappa, confine,
                             gamma, heat, forces, dt, 0, steps, 0.0, p(tt), threads or self.threads)
+        assert rc == 0
+        return (*s, float(tt[0]))
+
+    def rk4_steps_ext(self, x, y, vx, vy, q, m, kappa, confine, gamma, heat, bfield, drag_scale, forces,
+                      dt, steps, t=0.0, threads=0, order=4):
+        """rk4_steps with the Magnetic and TimeVaryingDrag forces available;
+        order=2 runs Runge_Kutta2 instead."""
+        s = [_f64(a).copy() for a in (x, y, vx, vy)]
+        tt = np.array([t])
+        fn = self.L.ref_rk4_ext if order == 4 else self.L.ref_rk2_ext
+        rc = fn(len(s[0]), *[p(a) for a in s], p(_f64(q)), p(_f64(m)), kappa, confine,
+                gamma, heat, bfield, drag_scale, forces, dt, steps, p(tt), threads or self.threads)
         assert rc == 0
         return (*s, float(tt[0]))
 

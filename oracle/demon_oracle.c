@@ -382,16 +382,39 @@ float oracle_dtcheck_min_s(int64_t n, const double *x, const double *y)
 /* Runge_Kutta4::moveParticles                                              */
 /* ------------------------------------------------------------------------ */
 
+void oracle_magnetic_add(int64_t n, const double *vxs, const double *vys,
+                         const double *q, double bfield, double *fx, double *fy)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const double qb = q[i] * bfield;           /* MagneticForce.cpp:46 */
+        fx[i] = fx[i] + qb * vys[i];               /* :47 */
+        fy[i] = fy[i] - qb * vxs[i];               /* :48 */
+    }
+}
+
+void oracle_tvdrag_add(int64_t n, const double *vxs, const double *vys, const double *m,
+                       double scale, double offset, double t, double *fx, double *fy)
+{
+    const double drag_const = -(scale * t + offset);  /* TimeVaryingDragForce.cpp:37-39 */
+    for (int64_t i = 0; i < n; ++i) {
+        const double d = m[i] * drag_const;            /* DragForce.cpp:50 */
+        fx[i] = fx[i] + d * vxs[i];                    /* :52 */
+        fy[i] = fy[i] + d * vys[i];                    /* :53 */
+    }
+}
+
 int oracle_stage_force(int64_t n, const double *xs, const double *ys,
                        const double *vxs, const double *vys, const double *q,
                        const double *m, const oracle_params *p, uint64_t step,
-                       int stage, double *fx, double *fy)
+                       int stage, double t_stage, double *fx, double *fy)
 {
-    /* driver.cpp:372-392 construction order = summation order */
+    /* driver.cpp:372-396 construction order = summation order */
     if (p->forces & ORACLE_CONFINEMENT)
         oracle_confinement_add(n, xs, ys, q, p->confine, fx, fy);
     if (p->forces & ORACLE_DRAG)
         oracle_drag_add(n, vxs, vys, m, p->gamma, fx, fy);
+    if (p->forces & ORACLE_MAGNETIC)
+        oracle_magnetic_add(n, vxs, vys, q, p->magnetic_field, fx, fy);
     if (p->forces & ORACLE_SHIELDED_COULOMB) {
         if ((n & 1) || p->gather)
             oracle_yukawa_gather_add(n, xs, ys, q, p->shielding, fx, fy);
@@ -400,6 +423,8 @@ int oracle_stage_force(int64_t n, const double *xs, const double *ys,
     }
     if (p->forces & ORACLE_THERMAL)
         oracle_thermal_add(n, p->heat_val, p->seed, step, stage, fx, fy);
+    if (p->forces & ORACLE_TIME_VARYING_DRAG)
+        oracle_tvdrag_add(n, vxs, vys, m, p->drag_scale, p->gamma, t_stage, fx, fy);
     return 0;
 }
 
@@ -467,7 +492,9 @@ static int rk4_one_step(int64_t n, double *x, double *y, double *vx, double *vy,
             }
             xs = w->xc; ys = w->yc; vxs = w->vxc; vys = w->vyc;
         }
-        if (oracle_stage_force(n, xs, ys, vxs, vys, q, m, p, step, s + 1, w->fx, w->fy))
+        /* stage times t, t + dt/2, t + dt/2, t + dt (Runge_Kutta4.cpp:31,50,69,88) */
+        const double ts = s == 0 ? *t : s == 3 ? *t + dt : *t + dt / 2.0;
+        if (oracle_stage_force(n, xs, ys, vxs, vys, q, m, p, step, s + 1, ts, w->fx, w->fy))
             return -1;
         /* stage loop, Runge_Kutta4.cpp:32-47 */
         for (int64_t i = 0; i < n; ++i) {
@@ -505,6 +532,68 @@ int oracle_rk4_steps(int64_t n, double *x, double *y, double *vx, double *vy,
     int rc = 0;
     for (int64_t s = 0; s < steps && !rc; ++s)
         rc = rk4_one_step(n, x, y, vx, vy, q, m, p, init_dt, t, step_counter, &w);
+    free(w.block);
+    return rc;
+}
+
+/* Runge_Kutta2::moveParticles loop body (Runge_Kutta2.cpp:24-77). */
+static int rk2_one_step(int64_t n, double *x, double *y, double *vx, double *vy,
+                        const double *q, const double *m, const oracle_params *p,
+                        double init_dt, double *t, uint64_t *step_counter, rk_work *w)
+{
+    const double dt = p->dt_check ? oracle_modify_time_step(n, x, y, 1.0e-4f, init_dt) : init_dt; /* :25 */
+    const uint64_t step = *step_counter;
+    /* stage 1 on (x, Vx) at t (:29-46) */
+    if (oracle_stage_force(n, x, y, vx, vy, q, m, p, step, 1, *t, w->fx, w->fy))
+        return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        w->k[0][i] = (dt * w->fx[i]) / m[i];
+        w->l[0][i] = dt * vx[i];
+        w->mm[0][i] = (dt * w->fy[i]) / m[i];
+        w->nn[0][i] = dt * vy[i];
+        w->fx[i] = 0.0;
+        w->fy[i] = 0.0;
+    }
+    /* operation2 (CacheOperator.cpp:28-44), stage 2 at t + dt/2 (:49-65) */
+    for (int64_t i = 0; i < n; ++i) {
+        w->xc[i] = 0.5 * w->l[0][i] + x[i];
+        w->yc[i] = 0.5 * w->nn[0][i] + y[i];
+        w->vxc[i] = 0.5 * w->k[0][i] + vx[i];
+        w->vyc[i] = 0.5 * w->mm[0][i] + vy[i];
+    }
+    if (oracle_stage_force(n, w->xc, w->yc, w->vxc, w->vyc, q, m, p, step, 2, *t + dt / 2.0, w->fx, w->fy))
+        return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        const double k2 = (dt * w->fx[i]) / m[i];
+        const double l2 = dt * w->vxc[i];
+        const double m2 = (dt * w->fy[i]) / m[i];
+        const double n2 = dt * w->vyc[i];
+        w->fx[i] = 0.0;
+        w->fy[i] = 0.0;
+        vx[i] += k2;                             /* :68-74 */
+        x[i] += l2;
+        vy[i] += m2;
+        y[i] += n2;
+    }
+    *t += dt;                                    /* :76 */
+    *step_counter = step + 1;
+    return 0;
+}
+
+int oracle_rk2_steps(int64_t n, double *x, double *y, double *vx, double *vy,
+                     const double *q, const double *m, const oracle_params *p,
+                     double init_dt, int64_t steps, double *t, uint64_t *step_counter)
+{
+    if (n < 1)
+        return -1;
+    if (p->dt_check && (n & 3))
+        return -1;
+    rk_work w;
+    if (rk_alloc(&w, n))
+        return -1;
+    int rc = 0;
+    for (int64_t s = 0; s < steps && !rc; ++s)
+        rc = rk2_one_step(n, x, y, vx, vy, q, m, p, init_dt, t, step_counter, &w);
     free(w.block);
     return rc;
 }

@@ -19,10 +19,15 @@
  *   oracle_yukawa_gather_add long-double brute force, equality with
  *                            oracle_yukawa_add to rounding
  *   oracle_confinement_add,
- *   oracle_drag_add          linear closed forms; bitwise vs reference
+ *   oracle_drag_add,
+ *   oracle_magnetic_add,
+ *   oracle_tvdrag_add        linear closed forms (cyclotron rotation for the
+ *                            magnetic force); bitwise vs reference trajectories
  *   oracle_rk4_steps         bitwise vs compiled reference trajectories;
  *                            RK4 amplification polynomial on the linear
  *                            (confinement + drag) system
+ *   oracle_rk2_steps         bitwise vs compiled reference trajectories;
+ *                            the midpoint rule's polynomial 1 + hA + (hA)^2/2
  *   oracle_modify_time_step  constructed cases incl. the coverage hole;
  *                            bitwise vs compiled reference
  *   oracle_dtcheck_min_s     brute force over the coverage predicate
@@ -46,7 +51,9 @@ enum {
     ORACLE_CONFINEMENT = 1,
     ORACLE_DRAG = 2,
     ORACLE_SHIELDED_COULOMB = 4,
-    ORACLE_THERMAL = 16
+    ORACLE_THERMAL = 16,
+    ORACLE_TIME_VARYING_DRAG = 256,
+    ORACLE_MAGNETIC = 1024
 };
 
 typedef struct oracle_params {
@@ -59,6 +66,9 @@ typedef struct oracle_params {
     int32_t dt_check;   /* 1: run Integrator::modifyTimeStep each step */
     int32_t gather;     /* 1: Yukawa in gather form (j ascending per i), OpenMP over i;
                            deterministic, differs from the reference order by rounding */
+    double magnetic_field; /* B [T]                             (MagneticForce.h:29) */
+    double drag_scale;  /* TimeVaryingDragForce scaleConst [Hz/s]; its offset is gamma
+                           (driver.cpp:394, TimeVaryingDragForce.h:16-17) */
 } oracle_params;
 
 /* ShieldedCoulombForce::forceS (ShieldedCoulombForce.cpp:37-237), serial
@@ -94,6 +104,16 @@ void oracle_confinement_add(int64_t n, const double *xs, const double *ys,
 void oracle_drag_add(int64_t n, const double *vxs, const double *vys,
                      const double *m, double gamma, double *fx, double *fy);
 
+/* MagneticForce::force (MagneticForce.cpp:44-49): qB = q*B;
+ * Fx += qB*vy, Fy -= qB*vx. */
+void oracle_magnetic_add(int64_t n, const double *vxs, const double *vys,
+                         const double *q, double bfield, double *fx, double *fy);
+
+/* TimeVaryingDragForce::forceS(t) (TimeVaryingDragForce.cpp:12-31, 37-39):
+ * dragConst = -(scale*t + offset), then DragForce::force: F += (m*dragConst)*v. */
+void oracle_tvdrag_add(int64_t n, const double *vxs, const double *vys, const double *m,
+                       double scale, double offset, double t, double *fx, double *fy);
+
 /* Philox4x32-10 (Salmon et al., SC'11).  ctr[4], key[2] -> out[4]. */
 void oracle_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
@@ -119,9 +139,18 @@ int oracle_dtcheck_covers(int64_t i, int64_t j);
 /* Runge_Kutta4::moveParticles (Runge_Kutta4.cpp:23-137) driven for exactly
  * `steps` steps (the while-loop body, with the dt check when enabled).
  * State arrays are updated in place; *t is advanced by each step's dt.
- * Forces are summed in the driver's order: Confinement, Drag,
- * ShieldedCoulomb, Thermal (driver.cpp:372-392).  Returns 0 or -1 (bad n). */
+ * Forces are summed in the driver's order: Confinement, Drag, Magnetic,
+ * ShieldedCoulomb, Thermal, TimeVaryingDrag (driver.cpp:372-396); stage s
+ * runs at t, t + dt/2, t + dt/2, t + dt (Runge_Kutta4.cpp:31,50,69,88).
+ * Returns 0 or -1 (bad n). */
 int oracle_rk4_steps(int64_t n, double *x, double *y, double *vx, double *vy,
+                     const double *q, const double *m, const oracle_params *p,
+                     double init_dt, int64_t steps, double *t, uint64_t *step_counter);
+
+/* Runge_Kutta2::moveParticles (Runge_Kutta2.cpp:23-78) driven for exactly
+ * `steps` steps: the midpoint rule with the same dt check, stage times t and
+ * t + dt/2, and the same force order as oracle_rk4_steps. */
+int oracle_rk2_steps(int64_t n, double *x, double *y, double *vx, double *vy,
                      const double *q, const double *m, const oracle_params *p,
                      double init_dt, int64_t steps, double *t, uint64_t *step_counter);
 
@@ -131,11 +160,12 @@ int oracle_rk4_move(int64_t n, double *x, double *y, double *vx, double *vy,
                     double init_dt, double end_time, double *t,
                     uint64_t *step_counter, int64_t *steps_done);
 
-/* The stage force of the RK4 loop on a given stage state (all enabled forces). */
+/* The stage force of the RK4 loop on a given stage state (all enabled
+ * forces) at the stage time t_stage. */
 int oracle_stage_force(int64_t n, const double *xs, const double *ys,
                        const double *vxs, const double *vys, const double *q,
                        const double *m, const oracle_params *p, uint64_t step,
-                       int stage, double *fx, double *fy);
+                       int stage, double t_stage, double *fx, double *fy);
 
 /* Trajectory-fit loss and its exact gradient (discrete adjoint of the RK4
  * map of oracle_rk4_steps with Confinement + Drag + ShieldedCoulomb,

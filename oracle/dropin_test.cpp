@@ -7,6 +7,11 @@
 //   alacarte  reference Runge_Kutta4 + Confinement + Drag, GpuShieldedCoulombForce
 //   gpuforces reference Runge_Kutta4 + GpuConfinement + GpuDrag + GpuShieldedCoulomb
 //   gpurk4    GpuRungeKutta4 with the three Gpu* forces
+//   bd_gpuforces, bd_gpurk4: the same two against the reference with the
+//             driver's forceFlags B,D set (Confinement, Magnetic,
+//             ShieldedCoulomb, TimeVaryingDrag; driver.cpp:373-394)
+//   rk2_gpurk2  GpuRungeKutta2 with the three Gpu* forces against the
+//             reference's Runge_Kutta2 (the driver's -I)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -16,8 +21,11 @@
 #include "ConfinementForce.h"
 #include "DragForce.h"
 #include "GpuForces.h"
+#include "MagneticForce.h"
+#include "Runge_Kutta2.h"
 #include "Runge_Kutta4.h"
 #include "ShieldedCoulombForce.h"
+#include "TimeVaryingDragForce.h"
 
 double Cloud::interParticleSpacing = 0.003;
 double Cloud::dustParticleMassDensity = 2200;
@@ -30,6 +38,7 @@ namespace {
 
 const double kQ = 6000 * -1.602e-19, kM = 2.8e-14, kKappa = 2000.0, kGamma = 2.0, kDt = 1e-3;
 const double kConf = 5e-11 / (6000 * 1.602e-19);
+const double kB = 0.8, kScale = -3.0;
 
 struct State {
     std::vector<double> x, y, vx, vy;
@@ -52,7 +61,31 @@ Cloud *make(int n, const std::vector<double> &x, const std::vector<double> &y) {
 State run(int variant, int n, int steps, const std::vector<double> &x0, const std::vector<double> &y0) {
     Cloud *c = make(n, x0, y0);
     ForceArray fa;
-    if (variant <= 1) {
+    if (variant >= 7) {
+        // Runge_Kutta2: reference forces (7) or Gpu* forces (8)
+        if (variant == 7) {
+            fa.push_back(new ConfinementForce(c, kConf));
+            fa.push_back(new DragForce(c, kGamma));
+            fa.push_back(new ShieldedCoulombForce(c, kKappa));
+        } else {
+            fa.push_back(new GpuConfinementForce(c, kConf));
+            fa.push_back(new GpuDragForce(c, kGamma));
+            fa.push_back(new GpuShieldedCoulombForce(c, kKappa));
+        }
+    } else if (variant >= 4) {
+        // forceFlags B,D: reference classes (4), Gpu* forces (5, 6)
+        if (variant == 4) {
+            fa.push_back(new ConfinementForce(c, kConf));
+            fa.push_back(new MagneticForce(c, kB));
+            fa.push_back(new ShieldedCoulombForce(c, kKappa));
+            fa.push_back(new TimeVaryingDragForce(c, kScale, kGamma));
+        } else {
+            fa.push_back(new GpuConfinementForce(c, kConf));
+            fa.push_back(new GpuMagneticForce(c, kB));
+            fa.push_back(new GpuShieldedCoulombForce(c, kKappa));
+            fa.push_back(new GpuTimeVaryingDragForce(c, kScale, kGamma));
+        }
+    } else if (variant <= 1) {
         fa.push_back(new ConfinementForce(c, kConf));
         fa.push_back(new DragForce(c, kGamma));
         if (variant == 0) fa.push_back(new ShieldedCoulombForce(c, kKappa));
@@ -62,8 +95,11 @@ State run(int variant, int n, int steps, const std::vector<double> &x0, const st
         fa.push_back(new GpuDragForce(c, kGamma));
         fa.push_back(new GpuShieldedCoulombForce(c, kKappa));
     }
-    Integrator *I = variant == 3 ? static_cast<Integrator *>(new GpuRungeKutta4(c, fa, kDt, 0.0))
-                                 : static_cast<Integrator *>(new Runge_Kutta4(c, fa, kDt, 0.0));
+    Integrator *I;
+    if (variant == 3 || variant == 6) I = new GpuRungeKutta4(c, fa, kDt, 0.0);
+    else if (variant == 7) I = new Runge_Kutta2(c, fa, kDt, 0.0);
+    else if (variant == 8) I = new GpuRungeKutta2(c, fa, kDt, 0.0);
+    else I = new Runge_Kutta4(c, fa, kDt, 0.0);
     for (int s = 0; s < steps; ++s) I->moveParticles(std::nextafter(I->currentTime, INFINITY));
     State st{std::vector<double>(c->x, c->x + n), std::vector<double>(c->y, c->y + n),
              std::vector<double>(c->Vx, c->Vx + n), std::vector<double>(c->Vy, c->Vy + n), I->currentTime};
@@ -80,21 +116,25 @@ int main(int argc, char **argv) {
     const int steps = argc > 2 ? std::atoi(argv[2]) : 20;
     std::vector<double> x(n), y(n);
     if (demon_hex_lattice(n, 500e-6, 50e-6, 21, x.data(), y.data()) != DEMON_OK) return 2;
-    const State ref = run(0, n, steps, x, y);
-    double rmax = 0, vmax = 0;
-    for (int i = 0; i < n; ++i) {
-        rmax = std::fmax(rmax, std::hypot(ref.x[i], ref.y[i]));
-        vmax = std::fmax(vmax, std::hypot(ref.vx[i], ref.vy[i]));
-    }
-    const char *names[4] = {"reference", "alacarte", "gpuforces", "gpurk4"};
-    for (int v = 1; v <= 3; ++v) {
-        const State s = run(v, n, steps, x, y);
-        double dr = 0, dv = 0;
+    const char *names[9] = {"reference", "alacarte", "gpuforces", "gpurk4", "bd_reference", "bd_gpuforces",
+                            "bd_gpurk4", "rk2_reference", "rk2_gpurk2"};
+    for (int base : {0, 4, 7}) {
+        const State ref = run(base, n, steps, x, y);
+        double rmax = 0, vmax = 0;
         for (int i = 0; i < n; ++i) {
-            dr = std::fmax(dr, std::hypot(s.x[i] - ref.x[i], s.y[i] - ref.y[i]));
-            dv = std::fmax(dv, std::hypot(s.vx[i] - ref.vx[i], s.vy[i] - ref.vy[i]));
+            rmax = std::fmax(rmax, std::hypot(ref.x[i], ref.y[i]));
+            vmax = std::fmax(vmax, std::hypot(ref.vx[i], ref.vy[i]));
         }
-        std::printf("%s %.3e %.3e %.17g %.17g\n", names[v], dr / rmax, dv / vmax, s.t, ref.t);
+        const int last = base == 0 ? 3 : base == 4 ? 6 : 8;
+        for (int v = base + 1; v <= last; ++v) {
+            const State s = run(v, n, steps, x, y);
+            double dr = 0, dv = 0;
+            for (int i = 0; i < n; ++i) {
+                dr = std::fmax(dr, std::hypot(s.x[i] - ref.x[i], s.y[i] - ref.y[i]));
+                dv = std::fmax(dv, std::hypot(s.vx[i] - ref.vx[i], s.vy[i] - ref.vy[i]));
+            }
+            std::printf("%s %.3e %.3e %.17g %.17g\n", names[v], dr / rmax, dv / vmax, s.t, ref.t);
+        }
     }
     return 0;
 }

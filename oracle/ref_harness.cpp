@@ -7,9 +7,12 @@
 #include "Cloud.h"
 #include "ConfinementForce.h"
 #include "DragForce.h"
+#include "MagneticForce.h"
+#include "Runge_Kutta2.h"
 #include "Runge_Kutta4.h"
 #include "ShieldedCoulombForce.h"
 #include "ThermalForce.h"
+#include "TimeVaryingDragForce.h"
 
 #include <cmath>
 #include <cstdint>
@@ -145,6 +148,70 @@ int ref_rk4(int64_t n, double *x, double *y, double *vx, double *vy, const doubl
         } else {
             I.moveParticles(end_time);
         }
+        *t = I.currentTime;
+    }
+    for (Force *f : fa) delete f;
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = c->x[i];
+        y[i] = c->y[i];
+        vx[i] = c->Vx[i];
+        vy[i] = c->Vy[i];
+    }
+    delete c;
+    return 0;
+}
+
+// ref_rk4 (mode 0) with two more of the driver's forces in its order
+// (driver.cpp:373-394): Confinement, Drag, Magnetic(bfield), ShieldedCoulomb,
+// Thermal, TimeVaryingDrag(drag_scale, gamma).  Flags as in Force.h:75-91.
+int ref_rk4_ext(int64_t n, double *x, double *y, double *vx, double *vy, const double *q,
+                const double *m, double kappa, double confine, double gamma, double heat,
+                double bfield, double drag_scale, int forces, double dt, int64_t steps, double *t,
+                int nthreads) {
+    if (n < 4 || (n & 3)) return -1;
+    Cloud *c = make_cloud(n, x, y, vx, vy, q, m);
+    set_threads(nthreads);
+    ForceArray fa;
+    if (forces & ConfinementForceFlag) fa.push_back(new ConfinementForce(c, confine));
+    if (forces & DragForceFlag) fa.push_back(new DragForce(c, gamma));
+    if (forces & MagneticForceFlag) fa.push_back(new MagneticForce(c, bfield));
+    if (forces & ShieldedCoulombForceFlag) fa.push_back(new ShieldedCoulombForce(c, kappa));
+    if (forces & ThermalForceFlag) fa.push_back(new ThermalForce(c, heat));
+    if (forces & TimeVaryingDragForceFlag) fa.push_back(new TimeVaryingDragForce(c, drag_scale, gamma));
+    {
+        Runge_Kutta4 I(c, fa, dt, *t);
+        for (int64_t s = 0; s < steps; ++s) I.moveParticles(std::nextafter(I.currentTime, INFINITY));
+        *t = I.currentTime;
+    }
+    for (Force *f : fa) delete f;
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = c->x[i];
+        y[i] = c->y[i];
+        vx[i] = c->Vx[i];
+        vy[i] = c->Vy[i];
+    }
+    delete c;
+    return 0;
+}
+
+// ref_rk4_ext with Runge_Kutta2 (the driver's -I, driver.cpp:437-438).
+int ref_rk2_ext(int64_t n, double *x, double *y, double *vx, double *vy, const double *q,
+                const double *m, double kappa, double confine, double gamma, double heat,
+                double bfield, double drag_scale, int forces, double dt, int64_t steps, double *t,
+                int nthreads) {
+    if (n < 4 || (n & 3)) return -1;
+    Cloud *c = make_cloud(n, x, y, vx, vy, q, m);
+    set_threads(nthreads);
+    ForceArray fa;
+    if (forces & ConfinementForceFlag) fa.push_back(new ConfinementForce(c, confine));
+    if (forces & DragForceFlag) fa.push_back(new DragForce(c, gamma));
+    if (forces & MagneticForceFlag) fa.push_back(new MagneticForce(c, bfield));
+    if (forces & ShieldedCoulombForceFlag) fa.push_back(new ShieldedCoulombForce(c, kappa));
+    if (forces & ThermalForceFlag) fa.push_back(new ThermalForce(c, heat));
+    if (forces & TimeVaryingDragForceFlag) fa.push_back(new TimeVaryingDragForce(c, drag_scale, gamma));
+    {
+        Runge_Kutta2 I(c, fa, dt, *t);
+        for (int64_t s = 0; s < steps; ++s) I.moveParticles(std::nextafter(I.currentTime, INFINITY));
         *t = I.currentTime;
     }
     for (Force *f : fa) delete f;

@@ -29,15 +29,16 @@ def test_every_declared_symbol_is_exported_and_bound():
 
 
 def test_versions():
-    assert dm.lib.demon_abi_version() == 1
+    assert dm.lib.demon_abi_version() == 2
     assert b"gfx950" in dm.lib.demon_version()
 
 
 def test_struct_layouts_match_header():
-    # demon_params: 4 doubles, 1 uint64, 4 int32 ; demon_stats: 3 int64, 2 doubles
-    assert ctypes.sizeof(_lib.Params) == 4 * 8 + 8 + 4 * 4
+    # demon_params: 4 doubles, 1 uint64, 4 int32, 2 doubles ; demon_stats: 3 int64, 2 doubles
+    assert ctypes.sizeof(_lib.Params) == 4 * 8 + 8 + 4 * 4 + 2 * 8
     assert ctypes.sizeof(_lib.Stats) == 5 * 8
     assert _lib.Params.forces.offset == 40 and _lib.Params.device.offset == 52
+    assert _lib.Params.magnetic_field.offset == 56 and _lib.Params.drag_scale.offset == 64
 
 
 def test_create_rejects_bad_arguments_before_touching_the_device():
@@ -45,7 +46,7 @@ def test_create_rejects_bad_arguments_before_touching_the_device():
     h = ctypes.c_void_p()
     assert dm.lib.demon_create(ctypes.byref(h), 1, ctypes.byref(p)) == _lib.ERR_ARG
     assert b"n must be" in dm.lib.demon_last_error(None)
-    bad = _lib.Params(2000.0, 1.0, 2.0, 0.0, 0, 7 | 1024, 0, 1, 0)  # MagneticForce flag
+    bad = _lib.Params(2000.0, 1.0, 2.0, 0.0, 0, 7 | 2048, 0, 1, 0)  # ConfinementForceVoid flag
     assert dm.lib.demon_create(ctypes.byref(h), 16, ctypes.byref(bad)) == _lib.ERR_UNSUPPORTED
     neg = _lib.Params(-1.0, 1.0, 2.0, 0.0, 0, 7, 0, 1, 0)
     assert dm.lib.demon_create(ctypes.byref(h), 16, ctypes.byref(neg)) == _lib.ERR_ARG

@@ -200,3 +200,38 @@ def test_yukawa_fp32_pair(oracle, n):
     err = force_rel_err(fx, fy, gx, gy, pair_abs_sum(x, y, q, KAPPA))
     assert err <= FP32_TOL
     assert err > 0.0  # it really is the FP32 path
+
+
+def test_magnetic_and_time_varying_drag_stage_forces(oracle):
+    """MagneticForce and TimeVaryingDragForce at a stage time, in the driver's
+    order: bit-exact with the oracle without the pair sums, and within the FP64
+    bar with them and the thermal force (Force::forceS(t) through
+    demon_force_stage_at)."""
+    n = 1024
+    x, y = lattice(n, seed=5)
+    rng = np.random.default_rng(6)
+    vx, vy = rng.normal(0, 1e-2, n), rng.normal(0, 1e-2, n)
+    q = charges(n)
+    m = np.full(n, M0) * (1 + 0.05 * rng.uniform(-1, 1, n))
+    bfield, scale_d = 0.9, -2.5
+    elem = dm.CONFINEMENT | dm.MAGNETIC | dm.TIME_VARYING_DRAG
+    full = elem | dm.SHIELDED_COULOMB | dm.THERMAL | dm.DRAG
+    heat = 2.0e-15
+    c = dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, heat=heat, seed=7, forces=full,
+                   magnetic_field=bfield, drag_scale=scale_d)
+    with c:
+        c.upload(x, y, vx, vy, q, m)
+        for stage, t in ((1, 0.25), (2, 0.2505), (4, 0.251)):
+            gx, gy = c.force_stage(elem, stage, 3, x, y, vx, vy, t=t)
+            op = OracleParams(KAPPA, CONFINE, GAMMA, heat, 7, elem, 1, 0, bfield, scale_d)
+            fx, fy = oracle.stage_force(x, y, vx, vy, q, m, op, step=3, stage=stage, t=t)
+            assert np.array_equal(fx, gx) and np.array_equal(fy, gy), stage
+            gx, gy = c.force_stage(full, stage, 3, x, y, vx, vy, t=t)
+            op = OracleParams(KAPPA, CONFINE, GAMMA, heat, 7, full, 1, 0, bfield, scale_d)
+            fx, fy = oracle.stage_force(x, y, vx, vy, q, m, op, step=3, stage=stage, t=t)
+            scale = (pair_abs_sum(x, y, q, KAPPA) + np.abs(q * CONFINE) * np.hypot(x, y)
+                     + m * (GAMMA + abs(scale_d * t + GAMMA)) * np.hypot(vx, vy)
+                     + np.abs(q * bfield) * np.hypot(vx, vy) + heat)
+            assert force_rel_err(fx, fy, gx, gy, scale) <= FP64_TOL, stage
+        with pytest.raises(dm.DemonError):
+            c.force_stage(dm.TIME_VARYING_DRAG, 1, 0, x, y, vx, vy)  # needs the stage time

@@ -111,3 +111,60 @@ def test_rk4_state_is_finite_at_full_size():
         g = c.download()
     assert t == pytest.approx(3e-3)
     assert all(np.isfinite(a).all() for a in g)
+
+
+@pytest.mark.parametrize("mode", [None, dm.PAIRS_GATHER])
+def test_rk4_magnetic_time_varying_drag(oracle, mode):
+    """forceFlags B,D as the reference's driver builds them: Confinement,
+    Magnetic, ShieldedCoulomb, TimeVaryingDrag (no Drag), 100 steps from
+    t = 0.5 s with the dt check."""
+    n = 1024
+    x, y = lattice(n, seed=12)
+    q = charges(n)
+    m = np.full(n, M0)
+    rng = np.random.default_rng(13)
+    vx, vy = rng.normal(0, 5e-3, n), rng.normal(0, 5e-3, n)
+    forces = dm.CONFINEMENT | dm.MAGNETIC | dm.SHIELDED_COULOMB | dm.TIME_VARYING_DRAG
+    bfield, scale_d = 1.3, -1.5
+    with dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, forces=forces,
+                    magnetic_field=bfield, drag_scale=scale_d) as c:
+        c.upload(x, y, vx, vy, q, m)
+        if mode is not None:
+            c.set_pair_mode(mode)
+        t = c.rk4_steps(100, 1e-3, 0.5)
+        g = c.download()
+    op = OracleParams(KAPPA, CONFINE, GAMMA, 0.0, 0, forces, 1, 0, bfield, scale_d)
+    o = oracle.rk4_steps(x, y, vx, vy, q, m, op, 1e-3, 100, t=0.5)
+    assert t == o[4]
+    assert pos_rel_err(o[0], o[1], g[0], g[1]) <= POS_TOL
+    vscale = np.max(np.hypot(o[2], o[3]))
+    assert np.max(np.hypot(o[2] - g[2], o[3] - g[3])) / vscale <= 1e-8
+
+
+@pytest.mark.parametrize("graph,mode", [(False, None), (True, None), (False, dm.PAIRS_GATHER)])
+def test_rk2_100_steps(oracle, graph, mode):
+    """Runge_Kutta2 (the driver's -I): the midpoint rule with the dt check,
+    against the oracle's restatement; then RK4 again on the same context (the
+    captured step follows the integrator)."""
+    n = 1024
+    x, y = lattice(n, seed=14)
+    q = charges(n)
+    m = np.full(n, M0)
+    z = np.zeros(n)
+    with dm.Context(n, shielding=KAPPA, confine=CONFINE, gamma=GAMMA, forces=FORCES) as c:
+        c.upload(x, y, z, z, q, m)
+        if mode is not None:
+            c.set_pair_mode(mode)
+        c.use_graph(graph)
+        t = c.rk2_steps(100, 1e-3, 0.0)
+        g = c.download()
+        assert c.stats().steps == 100
+        t4 = c.rk4_steps(5, 1e-3, t)
+        g4 = c.download()
+    op = OracleParams(KAPPA, CONFINE, GAMMA, 0.0, 0, FORCES, 1)
+    o = oracle.rk2_steps(x, y, z, z, q, m, op, 1e-3, 100)
+    assert t == o[4]
+    assert pos_rel_err(o[0], o[1], g[0], g[1]) <= POS_TOL
+    o4 = oracle.rk4_steps(o[0], o[1], o[2], o[3], q, m, op, 1e-3, 5, t=o[4])
+    assert t4 == o4[4]
+    assert pos_rel_err(o4[0], o4[1], g4[0], g4[1]) <= POS_TOL

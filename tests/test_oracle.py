@@ -277,3 +277,122 @@ def test_thermal_moments(oracle):
     assert np.mean(fx**2) == pytest.approx(1 / 6, abs=3e-3)
     gx, _ = oracle.thermal(n, heat, 123, 7, 3)
     assert abs(np.corrcoef(fx, gx)[0, 1]) < 0.01                  # stages independent
+
+
+# --- Magnetic and TimeVaryingDrag (driver order, stage times) ---------------
+
+MAGNETIC, TV_DRAG = 1024, 256
+
+
+@pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_magnetic_tvdrag_bitwise_vs_live_reference(oracle):
+    """Confinement, Drag/TimeVaryingDrag, Magnetic and Yukawa in the driver's
+    order, with the dt check, against the reference's own classes."""
+    ref = Reference()
+    rng = np.random.default_rng(11)
+    for trial, forces in enumerate((1 | 2 | 4 | MAGNETIC, 1 | 4 | TV_DRAG | MAGNETIC, 1 | 4 | TV_DRAG)):
+        n = 4 * int(rng.integers(3, 30))
+        x = rng.uniform(-2e-3, 2e-3, n)
+        y = rng.uniform(-2e-3, 2e-3, n)
+        vx = rng.normal(size=n) * 1e-4
+        vy = rng.normal(size=n) * 1e-4
+        q = Q0 * rng.uniform(0.5, 1.5, n)
+        m = M0 * rng.uniform(0.8, 1.2, n)
+        kappa, bfield, scale = float(rng.uniform(500, 8000)), float(rng.uniform(-2, 2)), float(rng.uniform(-5, 5))
+        op = OracleParams(kappa, CONFINE, 2.0, 0.0, 0, forces, 1, 0, bfield, scale)
+        o = oracle.rk4_steps(x, y, vx, vy, q, m, op, 1e-4, 6, t=0.25)
+        r = ref.rk4_steps_ext(x, y, vx, vy, q, m, kappa, CONFINE, 2.0, 0.0, bfield, scale, forces, 1e-4, 6,
+                              t=0.25)
+        for a, b in zip(o[:4], r[:4]):
+            assert bits_equal(a, b), trial
+        assert o[4] == r[4]
+
+
+def test_magnetic_rk4_step_is_amplification_polynomial(oracle):
+    """Magnetic force alone is linear: dv/dt = w (vy, -vx) with w = qB/m, and
+    one RK4 step is T4(hA) on (x, y, vx, vy)."""
+    n = 3
+    rng = np.random.default_rng(12)
+    x, y, vx, vy = rng.normal(size=(4, n)) * 1e-3
+    q, m = np.full(n, Q0), np.full(n, M0)
+    bfield, h = 0.7, 1e-3
+    op = OracleParams(KAPPA, CONFINE, 2.0, 0.0, 0, MAGNETIC, 0, 0, bfield, 0.0)
+    o = oracle.rk4_steps(x, y, vx, vy, q, m, op, h, 1)
+    w = Q0 * bfield / M0
+    A = np.zeros((4, 4))
+    A[0, 2] = A[1, 3] = 1.0
+    A[2, 3], A[3, 2] = w, -w
+    hA = h * A
+    T = np.eye(4)
+    term = np.eye(4)
+    for k in range(1, 5):
+        term = term @ hA / k
+        T = T + term
+    for i in range(n):
+        e = T @ np.array([x[i], y[i], vx[i], vy[i]])
+        got = np.array([o[0][i], o[1][i], o[2][i], o[3][i]])
+        np.testing.assert_allclose(got, e, rtol=1e-13, atol=1e-13 * np.abs(e).max())
+
+
+def test_tvdrag_uses_the_stage_times(oracle):
+    """TimeVaryingDrag alone: dv/dt = -(s t + g) v.  One RK4 step from t0
+    evaluates the rate at t0, t0 + h/2, t0 + h/2, t0 + h."""
+    n = 2
+    vx = np.array([1e-3, -2e-3])
+    vy = np.array([3e-4, 5e-4])
+    z = np.zeros(n)
+    q, m = np.full(n, Q0), np.full(n, M0)
+    s, g, h, t0 = 40.0, 2.0, 1e-2, 0.3
+    op = OracleParams(KAPPA, CONFINE, g, 0.0, 0, TV_DRAG, 0, 0, 0.0, s)
+    o = oracle.rk4_steps(z, z, vx, vy, q, m, op, h, 1, t=t0)
+
+    def rate(t):
+        return -(s * t + g)
+
+    for v0, got in ((vx, o[2]), (vy, o[3])):
+        k1 = h * rate(t0) * v0
+        k2 = h * rate(t0 + h / 2) * (v0 + k1 / 2)
+        k3 = h * rate(t0 + h / 2) * (v0 + k2 / 2)
+        k4 = h * rate(t0 + h) * (v0 + k3)
+        np.testing.assert_allclose(got, v0 + (k1 + 2 * k2 + 2 * k3 + k4) / 6, rtol=1e-14)
+    assert o[4] == t0 + h
+
+
+# --- Runge_Kutta2 (the driver's -I) -----------------------------------------
+
+@pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_rk2_bitwise_vs_live_reference(oracle):
+    ref = Reference()
+    rng = np.random.default_rng(21)
+    for forces in (7, 1 | 4 | TV_DRAG | MAGNETIC):
+        n = 4 * int(rng.integers(3, 30))
+        x = rng.uniform(-2e-3, 2e-3, n)
+        y = rng.uniform(-2e-3, 2e-3, n)
+        vx, vy = rng.normal(size=(2, n)) * 1e-4
+        q = Q0 * rng.uniform(0.5, 1.5, n)
+        m = M0 * rng.uniform(0.8, 1.2, n)
+        kappa = float(rng.uniform(500, 8000))
+        op = OracleParams(kappa, CONFINE, 2.0, 0.0, 0, forces, 1, 0, 0.6, -2.0)
+        o = oracle.rk2_steps(x, y, vx, vy, q, m, op, 1e-4, 7, t=0.1)
+        r = ref.rk4_steps_ext(x, y, vx, vy, q, m, kappa, CONFINE, 2.0, 0.0, 0.6, -2.0, forces, 1e-4, 7, t=0.1,
+                              order=2)
+        for a, b in zip(o[:4], r[:4]):
+            assert bits_equal(a, b), forces
+        assert o[4] == r[4]
+
+
+def test_rk2_step_is_midpoint_polynomial(oracle):
+    """Confinement + drag only: one Runge_Kutta2 step is 1 + hA + (hA)^2/2."""
+    n = 4
+    rng = np.random.default_rng(22)
+    x, y, vx, vy = rng.normal(size=(4, n)) * 1e-3
+    q, m = np.full(n, Q0), np.full(n, M0)
+    gamma, h = 2.0, 1e-3
+    op = OracleParams(KAPPA, CONFINE, gamma, 0.0, 0, 1 | 2, 0)
+    o = oracle.rk2_steps(x, y, vx, vy, q, m, op, h, 1)
+    hA = h * np.array([[0.0, 1.0], [Q0 * CONFINE / M0, -gamma]])
+    T = np.eye(2) + hA + hA @ hA / 2
+    for i in range(n):
+        ex, ey = T @ np.array([x[i], vx[i]]), T @ np.array([y[i], vy[i]])
+        assert o[0][i] == pytest.approx(ex[0], rel=1e-13) and o[2][i] == pytest.approx(ex[1], rel=1e-12)
+        assert o[1][i] == pytest.approx(ey[0], rel=1e-13) and o[3][i] == pytest.approx(ey[1], rel=1e-12)
