@@ -67,7 +67,7 @@ def build_host(verbose: bool = False) -> list[Path]:
     lib = PKG / HOST_LIB
     steps = [
         [cxx(), *CXX_FLAGS, *inc, "-shared", "-o", str(lib), str(HOST / "demon_host.cpp"),
-         f"-L{PKG}", "-ldemon_gpu", rpath],
+         str(HOST / "demon_fits.cpp"), f"-L{PKG}", "-ldemon_gpu", rpath],
         [cxx(), *CXX_FLAGS, *inc, "-o", str(PKG / "demon"), str(HOST / "driver.cpp"),
          f"-L{PKG}", "-ldemon_host", "-ldemon_gpu", rpath],
     ]

@@ -10,8 +10,12 @@
 //   ConfinementForce           ConfinementForce.h:13-30
 //   DragForce                  DragForce.h:13-30
 //   ThermalForce               ThermalForce.h:13-30
-//   Integrator, Runge_Kutta4   Integrator.h:17-38, Runge_Kutta4.h:14-31
+//   MagneticForce              MagneticForce.h:14-33
+//   TimeVaryingDragForce       TimeVaryingDragForce.h:14-33
+//   Integrator, Runge_Kutta4,  Integrator.h:17-38, Runge_Kutta4.h:14-31,
+//   Runge_Kutta2               Runge_Kutta2.h
 //   checkParams                driver.cpp:626-824 (Params.cfg)
+//   writeForce / readForce     the FITS keywords of each force (demon_fits.h)
 // The forces evaluate on the device; Runge_Kutta4::moveParticles runs the
 // whole while (t < end) loop on the device and copies the state back.
 // There is no CPU code path: a Force that is not one of the device forces
@@ -25,6 +29,7 @@
 #include <vector>
 
 #include "../../include/demon_gpu.h"
+#include "demon_fits.h"
 
 namespace demon_amd {
 
@@ -127,6 +132,11 @@ public:
     // Runge_Kutta4 builds one parameter block from all forces.
     virtual force_flags flag() const = 0;
     virtual void contribute(demon_params &p) const = 0;
+    // Force::writeForce / readForce: the FORCES bits and the parameter
+    // keywords of the output file's primary header (e.g.
+    // ShieldedCoulombForce.cpp:239-277).
+    virtual void writeForce(long &flags, std::vector<FitsKey> &keys) const = 0;
+    virtual void readForce(const FitsCloud &file) = 0;
 
 protected:
     // Evaluates this force on the cloud's stage-S state at the stage time t on
@@ -146,6 +156,11 @@ public:
     void force4(double t) override { evaluate(4, 0, t); }
     force_flags flag() const override { return ShieldedCoulombForceFlag; }
     void contribute(demon_params &p) const override { p.shielding = shielding; }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= ShieldedCoulombForceFlag;
+        keys.push_back({"shieldingConstant", shielding, "[m^-1] (ShieldedCoulombForce)"});
+    }
+    void readForce(const FitsCloud &f) override { shielding = f.key("shieldingConstant", shielding); }
     double shielding;  // inverse shielding length [1/m]
 };
 
@@ -158,6 +173,11 @@ public:
     void force4(double t) override { evaluate(4, 0, t); }
     force_flags flag() const override { return ConfinementForceFlag; }
     void contribute(demon_params &p) const override { p.confine = confine; }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= ConfinementForceFlag;
+        keys.push_back({"confineConst", confine, "[V/m^2] (ConfinementForce)"});
+    }
+    void readForce(const FitsCloud &f) override { confine = f.key("confineConst", confine); }
     double confine;  // [V/m^2]
 };
 
@@ -170,6 +190,11 @@ public:
     void force4(double t) override { evaluate(4, 0, t); }
     force_flags flag() const override { return DragForceFlag; }
     void contribute(demon_params &p) const override { p.drag_gamma = -dragConst; }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= DragForceFlag;
+        keys.push_back({"dragConst", dragConst, "[Hz] (DragForce)"});
+    }
+    void readForce(const FitsCloud &f) override { dragConst = f.key("dragConst", dragConst); }
     double dragConst;  // -gamma [1/s] (DragForce.h:17)
 };
 
@@ -185,6 +210,11 @@ public:
         p.heat_val = heatVal;
         p.seed = seed;
     }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= ThermalForceFlag;
+        keys.push_back({"heatingValue", heatVal, "[N] (ThermalForce)"});
+    }
+    void readForce(const FitsCloud &f) override { heatVal = f.key("heatingValue", heatVal); }
     double heatVal;  // [N]
     uint64_t seed;
 
@@ -202,6 +232,11 @@ public:
     void force4(double t) override { evaluate(4, 0, t); }
     force_flags flag() const override { return MagneticForceFlag; }
     void contribute(demon_params &p) const override { p.magnetic_field = BField; }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= MagneticForceFlag;
+        keys.push_back({"magneticField", BField, "[T] (MagneticForce)"});
+    }
+    void readForce(const FitsCloud &f) override { BField = f.key("magneticField", BField); }
     double BField;  // [T]
 };
 
@@ -220,6 +255,19 @@ public:
     void contribute(demon_params &p) const override {
         p.drag_scale = scaleConst;
         p.drag_gamma = offsetConst;
+    }
+    // as the reference's: DragForce's keyword (its dragConst is +offset
+    // between stages, TimeVaryingDragForce.h:16-17) and bit, then its own
+    // (TimeVaryingDragForce.cpp:41-64)
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= DragForceFlag | TimeVaryingDragForceFlag;
+        keys.push_back({"dragConst", offsetConst, "[Hz] (DragForce)"});
+        keys.push_back({"TVDragScaleConst", scaleConst, "[Hz/s] (TimeVaryingDragForce)"});
+        keys.push_back({"TVDragOffsetConst", offsetConst, "[Hz] (TimeVaryingDragForce)"});
+    }
+    void readForce(const FitsCloud &f) override {
+        scaleConst = f.key("TVDragScaleConst", scaleConst);
+        offsetConst = f.key("TVDragOffsetConst", offsetConst);
     }
     double scaleConst;   // [Hz/s]
     double offsetConst;  // [Hz]

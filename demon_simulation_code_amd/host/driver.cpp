@@ -5,15 +5,20 @@
 //         [-s shielding] [-C confine] [-g gamma] [-T heat] [-B field]
 //         [-D scale offset] [-I] [-q mean sigma]
 //         [-r mean sigma] [-i spacing] [-p x y] [-k vx vy] [-d density]
-//         [-O snapshots.csv] [--hex N] [--seed S] [--device D] [-h]
+//         [-O data.fits | snapshots.csv] [-c file.fits] [-f file.fits]
+//         [--hex N] [--seed S] [--device D] [-h]
 //
 // Options are applied left to right, so options after -P override the file
 // (driver.cpp:834-1030).  Always-on forces: Confinement, ShieldedCoulomb and
 // Drag, unless TimeVaryingDrag (-D, forceFlags D) replaces it (driver.cpp:
 // 322-326); -T adds the thermal force, -B the magnetic force; -I selects
 // Runge_Kutta2 (driver.cpp:437-438).  The reference's other force plugins and
-// -M/-c/-f are not part of this build and are rejected.
-// Snapshots (t, index, x, y, vx, vy) go to a CSV file every dataTimeStep.
+// -M are not part of this build and are rejected.
+// Output (driver.cpp:356-420, 443-450): the reference's data.fits (or the -O
+// name) with a TIME_STEP row every dataTimeStep (demon_fits.h); -c continues
+// a file (its forces, parameters, last row and time) and appends to it; -f
+// starts from a file's last row and writes a new file.  A name ending in .csv
+// writes rows (t, index, x, y, vx, vy) as text instead.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -35,7 +40,9 @@ void help() {
                  "  -t dt -e end -o dataTimeStep\n"
                  "  -s kappa -C confine -g gamma -T heat -B field -D scale offset -I (RK2)\n"
                  "  -q qMean qSigma -r rMean rSigma -i spacing -p x y -k vx vy -d density\n"
-                 "  -O file.csv    write snapshots; --seed S; --device D; -h help\n";
+                 "  -O data.fits   output file (default data.fits; *.csv writes text rows)\n"
+                 "  -c file.fits   continue a run; -f file.fits  start from a run's final state\n"
+                 "  --seed S; --device D; -h help\n";
 }
 
 bool is_number(const char *s) {
@@ -51,7 +58,8 @@ struct Cli {
     int64_t hex = 0;
     uint64_t seed = 0;
     int device = 0;
-    std::string out;
+    std::string out = "data.fits";  // driver.cpp:361
+    std::string cont, finals;       // -c, -f
 };
 
 double take(int argc, char **argv, int &i, double fallback, const char *what) {
@@ -141,6 +149,12 @@ Cli parse(int argc, char **argv) {
         else if (a == "-O") {
             if (i + 1 >= argc) throw Error(DEMON_ERR_ARG, "-O needs a file");
             c.out = argv[++i];
+        } else if (a == "-c") {
+            if (i + 1 >= argc) throw Error(DEMON_ERR_ARG, "-c needs a file");
+            c.cont = argv[++i];
+        } else if (a == "-f") {
+            if (i + 1 >= argc) throw Error(DEMON_ERR_ARG, "-f needs a file");
+            c.finals = argv[++i];
         } else {
             throw Error(DEMON_ERR_UNSUPPORTED, "Unknown or unsupported option " + a);
         }
@@ -161,8 +175,29 @@ int main(int argc, char **argv) {
     try {
         Cli c = parse(argc, argv);
         Params &P = c.P;
+        if (!c.cont.empty() && !c.finals.empty()) throw Error(DEMON_ERR_ARG, "-c and -f exclude each other");
         std::unique_ptr<Cloud> cloud;
-        if (c.hex > 0)
+        FitsCloud file;
+        const std::string &src = !c.cont.empty() ? c.cont : c.finals;
+        if (!src.empty()) {
+            // Cloud::initializeFromFile (driver.cpp:332-356): the last TIME_STEP row
+            file = readFits(src, -1);
+            cloud.reset(new Cloud(file.n, c.device));
+            cloud->x = file.x;
+            cloud->y = file.y;
+            cloud->Vx = file.vx;
+            cloud->Vy = file.vy;
+            cloud->charge = file.charge;
+            cloud->mass = file.mass;
+            if (!c.cont.empty()) {
+                P.usedForces = file.forces;  // driver.cpp:340
+                P.startTime = file.time;     // driver.cpp:343
+                const force_flags known = ConfinementForceFlag | DragForceFlag | ShieldedCoulombForceFlag |
+                                          ThermalForceFlag | MagneticForceFlag | TimeVaryingDragForceFlag;
+                if (P.usedForces & ~known)
+                    throw Error(DEMON_ERR_UNSUPPORTED, src + " was written with forces that are not part of this build");
+            }
+        } else if (c.hex > 0)
             cloud = Cloud::initializeHexLattice(c.hex, Cloud::interParticleSpacing, 0.1 * Cloud::interParticleSpacing,
                                                 c.seed, P.qMean * Cloud::electronCharge,
                                                 (4.0 / 3.0) * kPi * Cloud::dustParticleMassDensity * P.rMean * P.rMean * P.rMean,
@@ -185,17 +220,37 @@ int main(int argc, char **argv) {
         forces.push_back(&yuk);
         if (P.usedForces & ThermalForceFlag) forces.push_back(&therm);
         if (P.usedForces & TimeVaryingDragForceFlag) forces.push_back(&tvdrag);
+        // -c: the forces' parameters from the file (driver.cpp:405-408)
+        if (!c.cont.empty())
+            for (Force *f : forces) f->readForce(file);
 
-        FILE *out = c.out.empty() ? nullptr : std::fopen(c.out.c_str(), "w");
-        if (!c.out.empty() && !out) throw Error(DEMON_ERR_ARG, "cannot open " + c.out);
+        const bool csv = c.cont.empty() && c.out.size() > 4 && c.out.compare(c.out.size() - 4, 4, ".csv") == 0;
+        FILE *out = nullptr;
+        std::unique_ptr<FitsWriter> fits;
+        if (csv) {
+            out = std::fopen(c.out.c_str(), "w");
+            if (!out) throw Error(DEMON_ERR_ARG, "cannot open " + c.out);
+        } else if (!c.cont.empty()) {
+            fits.reset(new FitsWriter(c.cont));  // driver.cpp:420: append to TIME_STEP
+        } else {
+            // writeForce in force order, then Cloud::writeCloudSetup (driver.cpp:410-418)
+            long flags = 0;
+            std::vector<FitsKey> keys;
+            for (const Force *f : forces) f->writeForce(flags, keys);
+            fits.reset(new FitsWriter(c.out, flags, keys, cloud->n, cloud->mass.data(), cloud->charge.data(),
+                                      cloud->x.data(), cloud->y.data(), cloud->Vx.data(), cloud->Vy.data()));
+        }
         auto snapshot = [&](double t) {
-            if (!out) return;
+            if (fits) {
+                fits->writeTimeStep(t, cloud->x.data(), cloud->y.data(), cloud->Vx.data(), cloud->Vy.data());
+                return;
+            }
             for (cloud_index i = 0; i < cloud->n; ++i)
                 std::fprintf(out, "%.17g,%lld,%.17g,%.17g,%.17g,%.17g\n", t, (long long)i, cloud->x[i],
                              cloud->y[i], cloud->Vx[i], cloud->Vy[i]);
             std::fflush(out);
         };
-        snapshot(P.startTime);
+        if (csv) snapshot(P.startTime);
         Runge_Kutta4 I4(cloud.get(), forces, P.simTimeStep, P.startTime);
         Runge_Kutta2 I2(cloud.get(), forces, P.simTimeStep, P.startTime);
         Integrator &I = c.rk4 ? static_cast<Integrator &>(I4) : static_cast<Integrator &>(I2);
@@ -206,7 +261,7 @@ int main(int argc, char **argv) {
                       << "% Complete)" << std::flush;
             I.moveParticles(startTime += P.dataTimeStep);
             steps += c.rk4 ? I4.lastSteps() : I2.lastSteps();
-            snapshot(I.currentTime);
+            snapshot(I.currentTime);  // Cloud::writeTimeStep (driver.cpp:447)
         }
         if (out) std::fclose(out);
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();

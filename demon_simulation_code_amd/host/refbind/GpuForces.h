@@ -188,8 +188,10 @@ public:
         p.drag_gamma = offsetConst;
     }
     void writeForce(fitsfile *const f, int *const e) const {
-        // as the CPU class: DragForce's keyword and bit, then its own (TimeVaryingDragForce.cpp:41-64)
-        demon_gpu_binding::write_key(f, e, DragForceFlag, "dragConst", -offsetConst, "[Hz] (DragForce)");
+        // as the CPU class: DragForce's keyword and bit (its dragConst is +offset between stages:
+        // TimeVaryingDragForce.h:16-17 builds DragForce(C, -offset)), then its own
+        // (TimeVaryingDragForce.cpp:41-64)
+        demon_gpu_binding::write_key(f, e, DragForceFlag, "dragConst", offsetConst, "[Hz] (DragForce)");
         demon_gpu_binding::write_key(f, e, TimeVaryingDragForceFlag, "TVDragScaleConst", scaleConst,
                                      "[Hz/s] (TimeVaryingDragForce)");
         demon_gpu_binding::write_key(f, e, TimeVaryingDragForceFlag, "TVDragOffsetConst", offsetConst,

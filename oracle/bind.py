@@ -254,6 +254,9 @@ class Reference:
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_int, ctypes.c_double, _I64, _D, ctypes.c_int]
         L.ref_rk2_ext.argtypes = L.ref_rk4_ext.argtypes
+        L.ref_fits_write.argtypes = [ctypes.c_char_p, _I64, _D, _D, _D, _D, _D, _D, ctypes.c_int, _D, _I64]
+        L.ref_fits_read.argtypes = [ctypes.c_char_p, _I64, _D, _D, _D, _D, _D, _D,
+                                    ctypes.POINTER(ctypes.c_long), _D, _D]
 
     @staticmethod
     def available(openmp: bool = False) -> bool:
@@ -290,6 +293,23 @@ class Reference:
                 gamma, heat, bfield, drag_scale, forces, dt, steps, p(tt), threads or self.threads)
         assert rc == 0
         return (*s, float(tt[0]))
+
+    def fits_write(self, path, x, y, vx, vy, q, m, forces, par, rows):
+        """The reference's own writer (driver + Cloud::writeCloudSetup/writeTimeStep);
+        par = (confine, gamma, B, kappa, heat, TV scale); row k has x + 1e-6 k at t = 0.01 k."""
+        rc = self.L.ref_fits_write(str(path).encode(), len(x), *[p(_f64(a)) for a in (x, y, vx, vy, q, m)],
+                                   forces, p(_f64(np.asarray(par, dtype=np.float64))), rows)
+        assert rc == 0, rc
+
+    def fits_read(self, path, n):
+        """The reference's reader: (x, y, vx, vy, q, m, forces, par, t) of the last row."""
+        out = [np.zeros(n) for _ in range(6)]
+        par = np.zeros(6)
+        t = np.zeros(1)
+        fl = ctypes.c_long(0)
+        rc = self.L.ref_fits_read(str(path).encode(), n, *[p(a) for a in out], ctypes.byref(fl), p(par), p(t))
+        assert rc == 0, rc
+        return (*out, fl.value, par, float(t[0]))
 
     def rk4_record(self, x, y, q, m, kappa, confine, gamma, heat, forces, dt, steps, every, threads=0):
         """Snapshots [steps//every, 4, n] (x, y, vx, vy) of one integrator run."""

@@ -16,6 +16,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <string>
 #include <cstring>
 #include <vector>
 #ifdef _OPENMP
@@ -223,6 +224,90 @@ int ref_rk2_ext(int64_t n, double *x, double *y, double *vx, double *vy, const d
     }
     delete c;
     return 0;
+}
+
+// A data.fits as the reference's driver writes it (driver.cpp:364-420,
+// Cloud.cpp:198-271): the primary image, each force's writeForce in the
+// driver's order (flags as Force.h:75-91; TimeVaryingDrag instead of Drag),
+// writeCloudSetup, then rows - 1 writeTimeStep calls (row k: x + 1e-6 k,
+// t = 0.01 k).  Parameters: confine, gamma, B, kappa, heat, TV scale.
+int ref_fits_write(const char *path, int64_t n, const double *x, const double *y, const double *vx,
+                   const double *vy, const double *q, const double *m, int forces, const double *par,
+                   int64_t rows) {
+    Cloud *c = make_cloud(n, x, y, vx, vy, q, m);
+    ForceArray fa;
+    if (forces & ConfinementForceFlag) fa.push_back(new ConfinementForce(c, par[0]));
+    if ((forces & DragForceFlag) && !(forces & TimeVaryingDragForceFlag)) fa.push_back(new DragForce(c, par[1]));
+    if (forces & MagneticForceFlag) fa.push_back(new MagneticForce(c, par[2]));
+    if (forces & ShieldedCoulombForceFlag) fa.push_back(new ShieldedCoulombForce(c, par[3]));
+    if (forces & ThermalForceFlag) fa.push_back(new ThermalForce(c, par[4]));
+    if (forces & TimeVaryingDragForceFlag) fa.push_back(new TimeVaryingDragForce(c, par[5], par[1]));
+    int error = 0;
+    fitsfile *file = nullptr;
+    const std::string name = std::string("!") + path;  // '!' replaces an existing file
+    fits_create_file(&file, name.c_str(), &error);
+    fits_create_img(file, 16, 0, NULL, &error);
+    for (Force *f : fa) f->writeForce(file, &error);
+    c->writeCloudSetup(file, error);
+    for (int64_t k = 1; k < rows && !error; ++k) {
+        for (int64_t i = 0; i < n; ++i) c->x[i] = x[i] + 1e-6 * (double)k;
+        c->writeTimeStep(file, error, 0.01 * (double)k);
+    }
+    fits_close_file(file, &error);
+    for (Force *f : fa) delete f;
+    delete c;
+    return error;
+}
+
+// The driver's -c reading (driver.cpp:332-343, 404-408): FORCES, the last
+// TIME_STEP row (Cloud::initializeFromFile) and each force's readForce.
+// out: x, y, vx, vy, q, m (n each); par as ref_fits_write; *t the row's time.
+int ref_fits_read(const char *path, int64_t n, double *x, double *y, double *vx, double *vy, double *q,
+                  double *m, long *forces, double *par, double *t) {
+    int error = 0;
+    fitsfile *file = nullptr;
+    fits_open_file(&file, path, READONLY, &error);
+    fits_read_key_lng(file, const_cast<char *>("FORCES"), forces, NULL, &error);
+    if (error) return error;
+    Cloud *c = Cloud::initializeFromFile(file, error, t);
+    if (error || (int64_t)c->n != n) {
+        delete c;
+        return error ? error : -1;
+    }
+    fits_movabs_hdu(file, 1, NULL, &error);
+    ConfinementForce conf(c, 0.0);
+    DragForce drag(c, 0.0);
+    MagneticForce mag(c, 0.0);
+    ShieldedCoulombForce yuk(c, 1.0);
+    ThermalForce therm(c, 0.0);
+    TimeVaryingDragForce tvd(c, 0.0, 0.0);
+    if (*forces & ConfinementForceFlag) conf.readForce(file, &error);
+    if ((*forces & DragForceFlag) && !(*forces & TimeVaryingDragForceFlag)) drag.readForce(file, &error);
+    if (*forces & MagneticForceFlag) mag.readForce(file, &error);
+    if (*forces & ShieldedCoulombForceFlag) yuk.readForce(file, &error);
+    if (*forces & ThermalForceFlag) therm.readForce(file, &error);
+    if (*forces & TimeVaryingDragForceFlag) tvd.readForce(file, &error);
+    // the members those readForce calls filled are private: report the same
+    // keywords, read the same way (fits_read_key_dbl by name)
+    const char *names[6] = {"confineConst", "dragConst", "magneticField", "shieldingConstant", "heatingValue",
+                            "TVDragScaleConst"};
+    const long bits[6] = {ConfinementForceFlag, DragForceFlag, MagneticForceFlag, ShieldedCoulombForceFlag,
+                          ThermalForceFlag, TimeVaryingDragForceFlag};
+    for (int k = 0; k < 6; ++k) {
+        par[k] = 0.0;
+        if (*forces & bits[k]) fits_read_key_dbl(file, const_cast<char *>(names[k]), &par[k], NULL, &error);
+    }
+    fits_close_file(file, &error);
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = c->x[i];
+        y[i] = c->y[i];
+        vx[i] = c->Vx[i];
+        vy[i] = c->Vy[i];
+        q[i] = c->charge[i];
+        m[i] = c->mass[i];
+    }
+    delete c;
+    return error;
 }
 
 // One Runge_Kutta4 instance (so one RNG stream) run for `steps` steps,

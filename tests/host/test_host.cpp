@@ -7,10 +7,20 @@
 //                                   device; prints x y q fx1 fy1 fx3 fy3 per particle
 //   test_host rk4 N STEPS           Runge_Kutta4 with Confinement, Drag and
 //                                   ShieldedCoulomb; prints t, steps, then x y vx vy
+//   test_host fits_write PATH N ROWS FLAGS
+//                                   writes a DEMON output file (no device): the
+//                                   forces of FLAGS' keys, the hex cloud, ROWS
+//                                   TIME_STEP rows (row k: x + k 1e-6, t = k 0.01)
+//   test_host fits_append PATH ROWS appends ROWS rows to an existing file
+//                                   (row k of them: x + 1e-3 k, t = 100 + k)
+//   test_host fits_read PATH ROW    prints n rows time forces, the keys, then
+//                                   mass charge x y vx vy of row ROW (-1: last)
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "demon_host.h"
 
@@ -38,6 +48,61 @@ int main(int argc, char **argv) {
             if (mode == "parse") checkParams(argv[2], P);
             else checkParamsText(argv[2], P);
             dump(P);
+            return 0;
+        }
+        if (mode == "fits_read") {
+            const FitsCloud f = readFits(argv[2], std::atol(argv[3]));
+            std::printf("%lld %lld %.17g %ld\n", (long long)f.n, (long long)f.rows, f.time, f.forces);
+            std::printf("%zu\n", f.keys.size());
+            for (const FitsKey &k : f.keys) std::printf("%s %.17g\n", k.name.c_str(), k.value);
+            for (int64_t i = 0; i < f.n; ++i)
+                std::printf("%.17g %.17g %.17g %.17g %.17g %.17g\n", f.mass[(size_t)i], f.charge[(size_t)i],
+                            f.x[(size_t)i], f.y[(size_t)i], f.vx[(size_t)i], f.vy[(size_t)i]);
+            return 0;
+        }
+        if (mode == "fits_append") {
+            FitsWriter w(argv[2]);
+            const FitsCloud f = readFits(argv[2], -1);
+            const long rows = std::atol(argv[3]);
+            std::vector<double> x = f.x;
+            for (long k = 1; k <= rows; ++k) {
+                for (auto &v : x) v += 1e-3;
+                w.writeTimeStep(100.0 + (double)k, x.data(), f.y.data(), f.vx.data(), f.vy.data());
+            }
+            return 0;
+        }
+        if (mode == "fits_write") {
+            const long n = std::atol(argv[3]);
+            const long rows = std::atol(argv[4]);
+            const long flags = std::atol(argv[5]);
+            auto cloud = Cloud::initializeHexLattice(n, 500e-6, 50e-6, 1, 6000 * Cloud::electronCharge, 2.8e-14);
+            ConfinementForce conf(cloud.get(), 52018.31);
+            DragForce drag(cloud.get(), 2.0);
+            MagneticForce mag(cloud.get(), 0.75);
+            ShieldedCoulombForce yuk(cloud.get(), 2000.0);
+            ThermalForce therm(cloud.get(), 3e-15);
+            TimeVaryingDragForce tvd(cloud.get(), -1.5, 4.0);
+            ForceArray fa;
+            if (flags & ConfinementForceFlag) fa.push_back(&conf);
+            if ((flags & DragForceFlag) && !(flags & TimeVaryingDragForceFlag)) fa.push_back(&drag);
+            if (flags & MagneticForceFlag) fa.push_back(&mag);
+            if (flags & ShieldedCoulombForceFlag) fa.push_back(&yuk);
+            if (flags & ThermalForceFlag) fa.push_back(&therm);
+            if (flags & TimeVaryingDragForceFlag) fa.push_back(&tvd);
+            long fl = 0;
+            std::vector<FitsKey> keys;
+            for (const Force *f : fa) f->writeForce(fl, keys);
+            for (long i = 0; i < n; ++i) {
+                cloud->Vx[i] = 1e-4 * std::sin(0.3 * (double)i);
+                cloud->Vy[i] = -2e-4 * std::cos(0.7 * (double)i);
+            }
+            FitsWriter w(argv[2], fl, keys, n, cloud->mass.data(), cloud->charge.data(), cloud->x.data(),
+                         cloud->y.data(), cloud->Vx.data(), cloud->Vy.data());
+            for (long k = 1; k < rows; ++k) {
+                std::vector<double> x = cloud->x;
+                for (auto &v : x) v += 1e-6 * (double)k;
+                w.writeTimeStep(0.01 * (double)k, x.data(), cloud->y.data(), cloud->Vx.data(), cloud->Vy.data());
+            }
             return 0;
         }
         const long n = std::atol(argv[2]);
