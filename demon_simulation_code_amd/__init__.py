@@ -18,6 +18,16 @@ from ._lib import (  # noqa: F401
     SHIELDED_COULOMB,
     THERMAL,
     TIME_VARYING_DRAG,
+    GRAVITATIONAL,
+    THERMAL_LOCALIZED,
+    DRIVING,
+    ROTATIONAL,
+    TIME_VARYING_THERMAL,
+    CONFINEMENT_VOID,
+    ELECTRIC,
+    RECT_CONFINEMENT,
+    VERT_ELECTRIC,
+    PLUGIN_PARAMS,
     Context,
     DemonError,
     Params,
@@ -30,6 +40,8 @@ from .physics import ELECTRON_CHARGE, EPSILON0, COULOMB, confine_from_k  # noqa:
 __all__ = [
     "Batch", "Context", "DemonError", "Params", "Stats", "hex_lattice", "lib", "LIB_PATH",
     "CONFINEMENT", "DRAG", "SHIELDED_COULOMB", "THERMAL", "TIME_VARYING_DRAG", "MAGNETIC", "FP64", "FP32_PAIR",
-    "BATCH_NEWTON3", "PAIRS_GATHER", "PAIRS_NEWTON3",
+    "BATCH_NEWTON3", "PAIRS_GATHER", "PAIRS_NEWTON3", "GRAVITATIONAL", "THERMAL_LOCALIZED", "DRIVING",
+    "ROTATIONAL", "TIME_VARYING_THERMAL", "CONFINEMENT_VOID", "ELECTRIC", "RECT_CONFINEMENT", "VERT_ELECTRIC",
+    "PLUGIN_PARAMS",
     "ELECTRON_CHARGE", "EPSILON0", "COULOMB", "confine_from_k",
 ]

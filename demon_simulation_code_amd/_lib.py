@@ -26,9 +26,24 @@ ERR_UNSUPPORTED = -5
 CONFINEMENT = 1
 DRAG = 2
 SHIELDED_COULOMB = 4
+GRAVITATIONAL = 8
 THERMAL = 16
+THERMAL_LOCALIZED = 32
+DRIVING = 64
+ROTATIONAL = 128
 TIME_VARYING_DRAG = 256
+TIME_VARYING_THERMAL = 512
 MAGNETIC = 1024
+CONFINEMENT_VOID = 2048
+ELECTRIC = 4096
+RECT_CONFINEMENT = 8192
+VERT_ELECTRIC = 16384
+
+# constants of the further force plugins (demon_params fields after drag_scale)
+PLUGIN_PARAMS = ("confine_x", "confine_y", "void_decay", "rot_inner", "rot_outer", "rot_const",
+                 "drive_const", "drive_amplitude", "drive_shift", "electric_field", "plasma_radius",
+                 "vert_electric_field", "vert_decay", "gravity", "heat_in", "heat_out", "heat_radius",
+                 "heat_scale", "heat_offset")
 
 FP64 = 0
 FP32_PAIR = 1
@@ -58,7 +73,7 @@ class Params(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("magnetic_field", ctypes.c_double),
         ("drag_scale", ctypes.c_double),
-    ]
+    ] + [(name, ctypes.c_double) for name in PLUGIN_PARAMS]
 
 
 class Stats(ctypes.Structure):
@@ -168,10 +183,16 @@ class Context:
     def __init__(self, n: int, *, shielding: float, confine: float = 0.0, gamma: float = 0.0,
                  heat: float = 0.0, seed: int = 0, forces: int = CONFINEMENT | DRAG | SHIELDED_COULOMB,
                  precision: int = FP64, dt_check: bool = True, device: int = 0,
-                 magnetic_field: float = 0.0, drag_scale: float = 0.0):
+                 magnetic_field: float = 0.0, drag_scale: float = 0.0, **plugin):
+        """plugin: constants of the further force plugins by demon_params
+        field name (confine_x, void_decay, rot_const, heat_in, ...)."""
         self.n = int(n)
         self.params = Params(shielding, confine, gamma, heat, seed, forces, precision,
                              1 if dt_check else 0, device, magnetic_field, drag_scale)
+        for k, v in plugin.items():
+            if k not in PLUGIN_PARAMS:
+                raise TypeError(f"unknown force constant {k!r}")
+            setattr(self.params, k, v)
         h = _CTX()
         _check(lib.demon_create(ctypes.byref(h), self.n, ctypes.byref(self.params)))
         self._h = h
@@ -213,7 +234,7 @@ class Context:
 
     def force_stage(self, forces, stage, step, xs, ys, vxs=None, vys=None, fx=None, fy=None, t=None):
         """Adds the stage force into fx/fy; t is the stage time (required for
-        TIME_VARYING_DRAG)."""
+        TIME_VARYING_DRAG, TIME_VARYING_THERMAL and DRIVING)."""
         fx = np.zeros(self.n) if fx is None else fx
         fy = np.zeros(self.n) if fy is None else fy
         if t is None:

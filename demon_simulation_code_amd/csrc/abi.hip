@@ -92,14 +92,37 @@ int hip_fail(demon_ctx *c, hipError_t e, const char *what) {
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+// Every ForceFlag of the reference (Force.h:75-91).
+constexpr int32_t kAllForces = 0x7FFF;
+
+// The force constants of a parameter block as the kernels read them.
+ForceParams force_params(const demon_params &p) {
+    ForceParams f{};
+    f.confine = p.confine; f.gamma = p.drag_gamma; f.heat = p.heat_val;
+    f.bfield = p.magnetic_field; f.drag_scale = p.drag_scale;
+    f.confine_x = p.confine_x; f.confine_y = p.confine_y; f.void_decay = p.void_decay;
+    f.rot_inner = p.rot_inner; f.rot_outer = p.rot_outer; f.rot_const = p.rot_const;
+    f.drive_const = p.drive_const; f.drive_amplitude = p.drive_amplitude; f.drive_shift = p.drive_shift;
+    f.electric_field = p.electric_field; f.plasma_radius = p.plasma_radius;
+    f.vert_electric_field = p.vert_electric_field; f.vert_decay = p.vert_decay;
+    f.gravity = p.gravity;
+    f.heat_in = p.heat_in; f.heat_out = p.heat_out; f.heat_radius = p.heat_radius;
+    f.heat_scale = p.heat_scale; f.heat_offset = p.heat_offset;
+    f.seed = p.seed;
+    return f;
+}
+
 int check_params(demon_ctx *c, const demon_params *p) {
     if (!p) return fail(c, DEMON_ERR_ARG, "params is NULL");
-    const int32_t known = DEMON_FORCE_CONFINEMENT | DEMON_FORCE_DRAG | DEMON_FORCE_SHIELDED_COULOMB |
-                          DEMON_FORCE_THERMAL | DEMON_FORCE_TIME_VARYING_DRAG | DEMON_FORCE_MAGNETIC;
-    if (p->forces & ~known)
-        return fail(c, DEMON_ERR_UNSUPPORTED,
-                    "force flags outside Confinement|Drag|ShieldedCoulomb|Thermal|TimeVaryingDrag|"
-                    "Magnetic are not implemented on the device");
+    if (p->forces & ~kAllForces)
+        return fail(c, DEMON_ERR_UNSUPPORTED, "unknown force flags (the reference's ForceFlag bits are 1..16384)");
+    const double consts[] = {p->magnetic_field, p->drag_scale, p->confine_x, p->confine_y, p->void_decay,
+                             p->rot_inner, p->rot_outer, p->rot_const, p->drive_const, p->drive_amplitude,
+                             p->drive_shift, p->electric_field, p->plasma_radius, p->vert_electric_field,
+                             p->vert_decay, p->gravity, p->heat_in, p->heat_out, p->heat_radius,
+                             p->heat_scale, p->heat_offset};
+    for (double v : consts)
+        if (std::isnan(v)) return fail(c, DEMON_ERR_ARG, "force constants must not be NaN");
     if (!std::isfinite(p->magnetic_field) || !std::isfinite(p->drag_scale))
         return fail(c, DEMON_ERR_ARG, "magnetic field and drag scale must be finite");
     if ((p->forces & DEMON_FORCE_SHIELDED_COULOMB) && !(p->shielding > 0.0 && std::isfinite(p->shielding)))
@@ -270,8 +293,7 @@ UpdateLaunch make_update(demon_ctx *c, int stage, int dtcheck, double *store) {
     u.orig = c->d_orig; u.pairs = pair_sums(c);
     u.acc1 = c->acc1; u.acc23 = c->acc23; u.sc = c->d_sc;
     u.n = c->n; u.npad = c->npad; u.forces = c->p.forces; u.dtcheck = dtcheck;
-    u.confine = c->p.confine; u.gamma = c->p.drag_gamma; u.heat = c->p.heat_val; u.seed = c->p.seed;
-    u.bfield = c->p.magnetic_field; u.drag_scale = c->p.drag_scale;
+    u.fp = force_params(c->p);
     u.store = store;
     return u;
 }
@@ -680,9 +702,8 @@ int demon_force_stage_at(demon_ctx *c, int32_t forces, int32_t stage, uint64_t s
     ForceOutLaunch f{};
     f.st = c->sx; f.q = c->q; f.m = c->m; f.orig = c->d_orig; f.pairs = pair_sums(c);
     f.fx = c->fout; f.fy = c->fout + c->npad; f.n = c->n; f.npad = c->npad;
-    f.forces = forces; f.confine = c->p.confine; f.gamma = c->p.drag_gamma; f.heat = c->p.heat_val;
-    f.bfield = c->p.magnetic_field; f.drag_scale = c->p.drag_scale; f.t = t;
-    f.seed = c->p.seed; f.step = step; f.stage = stage;
+    f.forces = forces; f.fp = force_params(c->p); f.t = t;
+    f.step = step; f.stage = stage;
     HIP_TRY(c, launch_force_out(f, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->h_tmp.data(), c->fout, sizeof(double) * 2 * c->npad,
                               hipMemcpyDeviceToHost, c->stream));
@@ -699,8 +720,9 @@ int demon_force_stage_at(demon_ctx *c, int32_t forces, int32_t stage, uint64_t s
 int demon_force_stage(demon_ctx *c, int32_t forces, int32_t stage, uint64_t step,
                       const double *xs, const double *ys, const double *vxs,
                       const double *vys, double *fx, double *fy) {
-    if (forces & DEMON_FORCE_TIME_VARYING_DRAG)
-        return fail(c, DEMON_ERR_ARG, "TimeVaryingDrag needs the stage time: use demon_force_stage_at");
+    if (forces & (DEMON_FORCE_TIME_VARYING_DRAG | DEMON_FORCE_TIME_VARYING_THERMAL | DEMON_FORCE_DRIVING))
+        return fail(c, DEMON_ERR_ARG,
+                    "TimeVaryingDrag, TimeVaryingThermal and Driving need the stage time: use demon_force_stage_at");
     return demon_force_stage_at(c, forces, stage, step, 0.0, xs, ys, vxs, vys, fx, fy);
 }
 

@@ -112,6 +112,23 @@ struct PairLaunch {
 hipError_t launch_pairs(const PairLaunch &a, hipStream_t s);
 hipError_t launch_pairs_n3(const PairLaunch &a, hipStream_t s);
 
+// Constants of the per-particle force plugins (everything but the pairs),
+// one field per constructor argument of the reference's classes.
+struct ForceParams {
+    double confine, gamma, heat;           // Confinement C, Drag gamma, Thermal heatVal
+    double bfield, drag_scale;             // Magnetic B; TimeVaryingDrag scale (offset gamma)
+    double confine_x, confine_y;           // RectConfinement
+    double void_decay;                     // ConfinementVoid (with confine)
+    double rot_inner, rot_outer, rot_const;// Rotational
+    double drive_const, drive_amplitude, drive_shift;  // Driving (drive_const = dampConst)
+    double electric_field, plasma_radius;  // Electric
+    double vert_electric_field, vert_decay;// VertElectric
+    double gravity;                        // Gravitational
+    double heat_in, heat_out, heat_radius; // ThermalLocalized
+    double heat_scale, heat_offset;        // TimeVaryingThermal
+    uint64_t seed;                         // thermal RNG key
+};
+
 struct UpdateLaunch {
     StageBuf cur, nxt;
     double *x, *y, *vx, *vy;        // base state
@@ -123,9 +140,7 @@ struct UpdateLaunch {
     int64_t n, npad;
     int32_t forces;
     int32_t dtcheck;
-    double confine, gamma, heat;
-    double bfield, drag_scale;      // MagneticForce B, TimeVaryingDragForce scale
-    uint64_t seed;
+    ForceParams fp;
     double *store;                  // optional copy of the current stage record: {xs ys vxs vys nc}[npad]
                                     // then the box table [npad/64][4] (store_doubles(npad) in all)
 };
@@ -154,10 +169,9 @@ struct ForceOutLaunch {
     double *fx, *fy;
     int64_t n, npad;
     int32_t forces;
-    double confine, gamma, heat;
-    double bfield, drag_scale;
-    double t;                       // stage time (TimeVaryingDragForce)
-    uint64_t seed, step;
+    ForceParams fp;
+    double t;                       // stage time (the time-dependent forces)
+    uint64_t step;
     int32_t stage;
 };
 hipError_t launch_force_out(const ForceOutLaunch &a, hipStream_t s);

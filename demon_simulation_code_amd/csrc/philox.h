@@ -47,7 +47,12 @@ DEMON_HD double u01(uint32_t lo, uint32_t hi) {
 }
 
 // Counter domains (top byte of the fourth counter word).
-enum : uint32_t { kDomainThermal = 1u, kDomainLattice = 2u };
+enum : uint32_t {
+    kDomainThermal = 1u,
+    kDomainLattice = 2u,
+    kDomainThermalLocalized = 3u,
+    kDomainTimeVaryingThermal = 4u
+};
 
 DEMON_HD U32x4 draw(uint64_t seed, uint64_t particle, uint64_t step, uint32_t domain,
                     uint32_t stage) {

@@ -2,15 +2,29 @@
 // into one pass per stage.
 //
 // For stage s the update kernel, per particle:
-//   * sums the stage force in the driver's order (driver.cpp:372-396):
+//   * sums the stage force in the driver's construction order
+//     (driver.cpp:372-402), each plugin present when its flag is set:
 //       Confinement  F  = (q C) r_s            ConfinementForce.cpp:48-54
+//       ConfinementVoid  the confinement term, then
+//                    F += ((q (-d)) (e^{-d r}/r)) r_s  ConfinementForceVoid.cpp:13-55
 //       Drag         F += (m (-gamma)) v_s     DragForce.cpp:49-55
+//       Driving      Fx += (sin(k x - w t_s) e^0) A      DrivingForce.cpp:14-61
 //       Magnetic     F += (q B) (vy_s, -vx_s)  MagneticForce.cpp:44-49
+//       RectConfinement F += ((q cX) x, (q cY) y)      RectConfinementForce.cpp:50-59
+//       Rotational   F += c (-y, x)/r for rmin < r < rmax  RotationalForce.cpp:50-66
 //       Yukawa       F += sum of the all-pairs partial sums (fixed order)
 //       Thermal      F += heat U (cos t, sin t) ThermalForce.cpp:118-122
-//       TimeVaryingDrag F += (m (-(scale t_s + gamma))) v_s at the stage time
-//                    t_s = t, t + dt/2, t + dt/2, t + dt
-//                    (TimeVaryingDragForce.cpp:12-39, Runge_Kutta4.cpp:31-88)
+//       ThermalLocalized the same with heat = r < R ? in : out
+//                    (ThermalForceLocalized.cpp:124-133)
+//       TimeVaryingDrag F += (m (-(scale t_s + gamma))) v_s
+//       TimeVaryingThermal the thermal force with heat = scale t_s + offset
+//                    (TimeVaryingThermalForce.cpp:14-42)
+//       Electric     F += ((q E) e^{r/(-R)}) r_s   ElectricForce.cpp:48-62
+//       Gravitational Fy += m g                GravitationalForce.cpp:46-50
+//       VertElectric Fy += (q E) e^{y/d}       VertElectricForce.cpp:47-53
+//     with the stage time t_s = t, t + dt/2, t + dt/2, t + dt
+//     (Runge_Kutta4.cpp:31-88); each thermal plugin draws from its own
+//     Philox counter domain
 //   * forms the RK increments k = (dt F)/m, l = dt v_s (Runge_Kutta4.cpp:32-104),
 //   * builds the next stage state (CacheOperator.cpp:28-73: x + l/2, x + l)
 //     or, after stage 4, the combine x += (a1 + ((a2 + a3) 2 + a4)) / 6
@@ -31,6 +45,22 @@ namespace demon {
 namespace {
 
 constexpr double kTwoPi = 6.283185307179586476925286766559;
+// DrivingForce::waveNum and angFreq (DrivingForce.cpp:14-15), the same doubles
+constexpr double kPiD = 3.14159265358979323846;
+constexpr double kWaveNum = 2.0 * kPiD / 0.002;
+constexpr double kAngFreq = 2.0 * kPiD * 10.0;
+
+// One thermal kick (ThermalForce.cpp:118-122): U in [0,1), theta in [0,2pi)
+// from the Philox block of (particle, step, domain, stage).
+__device__ __forceinline__ void thermal_kick(double heat, uint64_t seed, int32_t oi, uint64_t step,
+                                             uint32_t domain, int32_t stage, double &fx, double &fy) {
+    const U32x4 o = draw(seed, (uint64_t)oi, step, domain, (uint32_t)stage);
+    const double uu = u01(o.v[0], o.v[1]);
+    const double th = u01(o.v[2], o.v[3]) * kTwoPi;
+    const double tv = uu * heat;
+    fx = fx + tv * cos(th);
+    fy = fy + tv * sin(th);
+}
 
 __device__ __forceinline__ double da(double a1, double a23, double a4) {
     // Runge_Kutta4::da with a2 + a3 already summed (same rounding: a2 + a3 first).
@@ -41,27 +71,52 @@ __device__ __forceinline__ double da(double a1, double a23, double a4) {
 __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys, double vxs,
                                             double vys, double qi, double mi,
                                             const PairSums &ps, const unsigned long long *fmask,
-                                            int64_t npad,
-                                            int64_t i, double confine, double gamma,
-                                            double heat, double bfield, double drag_scale,
-                                            double t_stage, uint64_t seed, uint64_t step,
-                                            int32_t stage, int32_t oi, double &fx, double &fy) {
+                                            int64_t npad, int64_t i, const ForceParams &fp,
+                                            double t_stage, uint64_t step, int32_t stage,
+                                            int32_t oi, double &fx, double &fy) {
     fx = 0.0;
     fy = 0.0;
     if (forces & 1) {
-        const double cv = qi * confine;
+        const double cv = qi * fp.confine;
         fx = fx + cv * xs;
         fy = fy + cv * ys;
     }
+    if (forces & 2048) {
+        const double cv = qi * fp.confine;
+        fx = fx + cv * xs;
+        fy = fy + cv * ys;
+        const double dv = qi * (-fp.void_decay);
+        const double r = sqrt(xs * xs + ys * ys);
+        const double er = exp(r * (-fp.void_decay)) / r;
+        fx = fx + (dv * er) * xs;
+        fy = fy + (dv * er) * ys;
+    }
     if (forces & 2) {
-        const double dr = mi * (-gamma);
+        const double dr = mi * (-fp.gamma);
         fx = fx + dr * vxs;
         fy = fy + dr * vys;
     }
+    if (forces & 64) {
+        const double dist = xs - fp.drive_shift;
+        const double arg = xs * kWaveNum - t_stage * kAngFreq;
+        const double earg = (dist - dist) / fp.drive_const;  // -driveConst = dampConst
+        fx = fx + (sin(arg) * exp(earg)) * fp.drive_amplitude;
+    }
     if (forces & 1024) {
-        const double qb = qi * bfield;
+        const double qb = qi * fp.bfield;
         fx = fx + qb * vys;
         fy = fy - qb * vxs;
+    }
+    if (forces & 8192) {
+        fx = fx + (qi * fp.confine_x) * xs;
+        fy = fy + (qi * fp.confine_y) * ys;
+    }
+    if (forces & 128) {
+        const double r = sqrt(xs * xs + ys * ys);
+        if (r > fp.rot_inner && r < fp.rot_outer) {
+            fx = fx - (fp.rot_const * ys) / r;
+            fy = fy + (fp.rot_const * xs) / r;
+        }
     }
     if (forces & 4) {
         for (int32_t s = 0; s < ps.slices; ++s) {
@@ -84,19 +139,28 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
             }
         }
     }
-    if (forces & 16) {
-        const U32x4 o = draw(seed, (uint64_t)oi, step, kDomainThermal, (uint32_t)stage);
-        const double uu = u01(o.v[0], o.v[1]);
-        const double th = u01(o.v[2], o.v[3]) * kTwoPi;
-        const double tv = uu * heat;
-        fx = fx + tv * cos(th);
-        fy = fy + tv * sin(th);
+    if (forces & 16) thermal_kick(fp.heat, fp.seed, oi, step, kDomainThermal, stage, fx, fy);
+    if (forces & 32) {
+        const double r = sqrt(xs * xs + ys * ys);
+        thermal_kick(r < fp.heat_radius ? fp.heat_in : fp.heat_out, fp.seed, oi, step,
+                     kDomainThermalLocalized, stage, fx, fy);
     }
     if (forces & 256) {
-        const double dr = mi * (-(drag_scale * t_stage + gamma));
+        const double dr = mi * (-(fp.drag_scale * t_stage + fp.gamma));
         fx = fx + dr * vxs;
         fy = fy + dr * vys;
     }
+    if (forces & 512)
+        thermal_kick(fp.heat_scale * t_stage + fp.heat_offset, fp.seed, oi, step,
+                     kDomainTimeVaryingThermal, stage, fx, fy);
+    if (forces & 4096) {
+        const double r = sqrt(xs * xs + ys * ys);
+        const double ev = (qi * fp.electric_field) * exp(r / (fp.plasma_radius * (-1)));
+        fx = fx + ev * xs;
+        fy = fy + ev * ys;
+    }
+    if (forces & 8) fy = fy + mi * fp.gravity;
+    if (forces & 16384) fy = fy + (qi * fp.vert_electric_field) * exp(ys / fp.vert_decay);
 }
 
 // The Newton-3 tile flags of this block's superblock (one 256-thread block
@@ -182,8 +246,8 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
         double fx, fy;
         // Runge_Kutta4.cpp:31,50,69,88 (Runge_Kutta2.cpp:31,50): the stage times
         const double ts = FS == 1 ? sc->t : FS == 4 ? sc->t + dt : sc->t + dt / 2.0;
-        stage_force(a.forces, xs, ys, vxs, vys, qi, mi, a.pairs, fmask, a.npad, i, a.confine,
-                    a.gamma, a.heat, a.bfield, a.drag_scale, ts, a.seed, sc->step, FS, a.orig[i], fx, fy);
+        stage_force(a.forces, xs, ys, vxs, vys, qi, mi, a.pairs, fmask, a.npad, i, a.fp, ts,
+                    sc->step, FS, a.orig[i], fx, fy);
         const double k = (dt * fx) / mi;
         const double l = dt * vxs;
         const double mm = (dt * fy) / mi;
@@ -276,8 +340,7 @@ __global__ __launch_bounds__(256) void k_force_out(ForceOutLaunch a) {
     if (i >= a.n) return;
     double fx, fy;
     stage_force(a.forces, a.st.xs[i], a.st.ys[i], a.st.vxs[i], a.st.vys[i], a.q[i], a.m[i],
-                a.pairs, fmask, a.npad, i, a.confine,
-                a.gamma, a.heat, a.bfield, a.drag_scale, a.t, a.seed, a.step, a.stage, a.orig[i], fx, fy);
+                a.pairs, fmask, a.npad, i, a.fp, a.t, a.step, a.stage, a.orig[i], fx, fy);
     a.fx[i] = fx;
     a.fy[i] = fy;
 }

@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <random>
 #include <sstream>
@@ -83,10 +84,7 @@ demon_ctx *Cloud::context(const demon_params &p) {
         check(demon_create(&ctx_, n, &p), nullptr);
         ctxParams_ = p;
         syncToDevice();
-    } else if (p.forces != ctxParams_.forces || p.shielding != ctxParams_.shielding ||
-               p.confine != ctxParams_.confine || p.drag_gamma != ctxParams_.drag_gamma ||
-               p.heat_val != ctxParams_.heat_val || p.seed != ctxParams_.seed ||
-               p.precision != ctxParams_.precision || p.dt_check != ctxParams_.dt_check) {
+    } else if (std::memcmp(&p, &ctxParams_, sizeof p) != 0) {
         check(demon_set_params(ctx_, &p), ctx_);
         ctxParams_ = p;
     }
@@ -132,13 +130,9 @@ namespace {
 int64_t move_on_device(Cloud *cloud, const ForceArray &forces, double init_dt, double &currentTime,
                        double endTime, int order) {
     demon_params p = base_params(cloud->device());
-    const force_flags device_forces = ConfinementForceFlag | DragForceFlag | ShieldedCoulombForceFlag |
-                                      ThermalForceFlag | MagneticForceFlag | TimeVaryingDragForceFlag;
     for (const Force *f : forces) {
-        if (!(f->flag() & device_forces) || (p.forces & f->flag()))
-            throw Error(DEMON_ERR_UNSUPPORTED,
-                        "moveParticles: every force must be a distinct device force "
-                        "(Confinement, Drag, Magnetic, ShieldedCoulomb, Thermal, TimeVaryingDrag)");
+        if (p.forces & f->flag())
+            throw Error(DEMON_ERR_UNSUPPORTED, "moveParticles: two forces of the same kind in one ForceArray");
         p.forces |= (int32_t)f->flag();
         f->contribute(p);
     }

@@ -12,6 +12,11 @@
 //   ThermalForce               ThermalForce.h:13-30
 //   MagneticForce              MagneticForce.h:14-33
 //   TimeVaryingDragForce       TimeVaryingDragForce.h:14-33
+//   RectConfinementForce, ConfinementForceVoid, RotationalForce,
+//   DrivingForce, ElectricForce, VertElectricForce, GravitationalForce,
+//   ThermalForceLocalized, TimeVaryingThermalForce
+//                              their headers' constructors (e.g.
+//                              RotationalForce.h:20-21, DrivingForce.h:17-18)
 //   Integrator, Runge_Kutta4,  Integrator.h:17-38, Runge_Kutta4.h:14-31,
 //   Runge_Kutta2               Runge_Kutta2.h
 //   checkParams                driver.cpp:626-824 (Params.cfg)
@@ -272,6 +277,243 @@ public:
     double scaleConst;   // [Hz/s]
     double offsetConst;  // [Hz]
 };
+
+#define DEMON_HOST_STAGES(STEP_FIRST, STEP)                         \
+    void force1(double t) override { evaluate(1, STEP_FIRST, t); }  \
+    void force2(double t) override { evaluate(2, STEP, t); }        \
+    void force3(double t) override { evaluate(3, STEP, t); }        \
+    void force4(double t) override { evaluate(4, STEP, t); }
+
+// RectConfinementForce (RectConfinementForce.h): F += (q cX x, q cY y).
+class RectConfinementForce : public Force {
+public:
+    RectConfinementForce(Cloud *C, double confineConstX, double confineConstY)
+        : Force(C), confineX(confineConstX), confineY(confineConstY) {}
+    DEMON_HOST_STAGES(0, 0)
+    force_flags flag() const override { return RectConfinementForceFlag; }
+    void contribute(demon_params &p) const override {
+        p.confine_x = confineX;
+        p.confine_y = confineY;
+    }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= RectConfinementForceFlag;
+        keys.push_back({"confineConstX", confineX, "[V/m^2] (RectConfinementForce)"});
+        keys.push_back({"confineConstY", confineY, "[V/m^2] (RectConfinementForce)"});
+    }
+    void readForce(const FitsCloud &f) override {
+        confineX = f.key("confineConstX", confineX);
+        confineY = f.key("confineConstY", confineY);
+    }
+    double confineX, confineY;  // [V/m^2]
+};
+
+// ConfinementForceVoid (ConfinementForceVoid.h:14-20): the confinement force
+// plus -q d e^{-d r} r_hat, one plugin; it writes ConfinementForce's keyword
+// and bit too (ConfinementForceVoid.cpp:57-80).
+class ConfinementForceVoid : public ConfinementForce {
+public:
+    ConfinementForceVoid(Cloud *C, double confineConst, double voidDecay)
+        : ConfinementForce(C, confineConst), decay(voidDecay) {}
+    force_flags flag() const override { return ConfinementForceVoidFlag; }
+    void contribute(demon_params &p) const override {
+        p.confine = confine;
+        p.void_decay = decay;
+    }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        ConfinementForce::writeForce(flags, keys);
+        flags |= ConfinementForceVoidFlag;
+        keys.push_back({"decay", decay, "[m^-1] (ConfinementForceVoid)"});
+    }
+    void readForce(const FitsCloud &f) override {
+        ConfinementForce::readForce(f);
+        decay = f.key("decay", decay);
+    }
+    double decay;  // [1/m]
+};
+
+// RotationalForce (RotationalForce.h:20-21): c (-y, x)/r for rmin < r < rmax.
+class RotationalForce : public Force {
+public:
+    RotationalForce(Cloud *C, double rmin, double rmax, double rotConst)
+        : Force(C), innerRad(rmin), outerRad(rmax), rotationalConst(rotConst) {}
+    DEMON_HOST_STAGES(0, 0)
+    force_flags flag() const override { return RotationalForceFlag; }
+    void contribute(demon_params &p) const override {
+        p.rot_inner = innerRad;
+        p.rot_outer = outerRad;
+        p.rot_const = rotationalConst;
+    }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= RotationalForceFlag;
+        keys.push_back({"rotationalConst", rotationalConst, "[N] (RotationalForce)"});
+        keys.push_back({"innerRadius", innerRad, "[m] (RotationalForce)"});
+        keys.push_back({"outerRadius", outerRad, "[m] (RotationalForce)"});
+    }
+    void readForce(const FitsCloud &f) override {
+        rotationalConst = f.key("rotationalConst", rotationalConst);
+        innerRad = f.key("innerRadius", innerRad);
+        outerRad = f.key("outerRadius", outerRad);
+    }
+    double innerRad, outerRad;  // [m]
+    double rotationalConst;     // [N]
+};
+
+// DrivingForce (DrivingForce.h:17-18): the member driveConst is -dampConst,
+// and the FITS keyword holds the member.
+class DrivingForce : public Force {
+public:
+    DrivingForce(Cloud *C, double dampConst, double amp, double drivingShift)
+        : Force(C), amplitude(amp), driveConst(-dampConst), shift(drivingShift) {}
+    DEMON_HOST_STAGES(0, 0)
+    force_flags flag() const override { return DrivingForceFlag; }
+    void contribute(demon_params &p) const override {
+        p.drive_const = -driveConst;
+        p.drive_amplitude = amplitude;
+        p.drive_shift = shift;
+    }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= DrivingForceFlag;
+        keys.push_back({"drivingAmplitude", amplitude, "[N] (DrivingForce)"});
+        keys.push_back({"drivingConst", driveConst, "[m^2] (DrivingForce)"});
+        keys.push_back({"drivingShift", shift, "[m] (DrivingForce)"});
+    }
+    void readForce(const FitsCloud &f) override {
+        amplitude = f.key("drivingAmplitude", amplitude);
+        driveConst = f.key("drivingConst", driveConst);
+        shift = f.key("drivingShift", shift);
+    }
+    double amplitude;   // [N]
+    double driveConst;  // [m^2]
+    double shift;       // [m]
+};
+
+// ElectricForce (ElectricForce.h:16-17): F += q E e^{-r/R} r.
+class ElectricForce : public Force {
+public:
+    ElectricForce(Cloud *C, double electricField, double plasmaRad)
+        : Force(C), electric(electricField), radius(plasmaRad) {}
+    DEMON_HOST_STAGES(0, 0)
+    force_flags flag() const override { return ElectricForceFlag; }
+    void contribute(demon_params &p) const override {
+        p.electric_field = electric;
+        p.plasma_radius = radius;
+    }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= ElectricForceFlag;
+        keys.push_back({"ElectricFieldStrength", electric, "[V/m^2] (ElectricForce)"});
+        keys.push_back({"plasmaRadius", radius, "[m] (ElectricForce)"});
+    }
+    void readForce(const FitsCloud &f) override {
+        electric = f.key("electricFieldStrength", electric);
+        radius = f.key("plasmaRadius", radius);
+    }
+    double electric;  // [V/m^2]
+    double radius;    // [m]
+};
+
+// VertElectricForce (VertElectricForce.h:16-17): Fy += q E e^{y/d}.
+class VertElectricForce : public Force {
+public:
+    VertElectricForce(Cloud *C, double vertElectricField, double vertDecay)
+        : Force(C), vertElectric(vertElectricField), vertDec(vertDecay) {}
+    DEMON_HOST_STAGES(0, 0)
+    force_flags flag() const override { return VertElectricForceFlag; }
+    void contribute(demon_params &p) const override {
+        p.vert_electric_field = vertElectric;
+        p.vert_decay = vertDec;
+    }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= VertElectricForceFlag;
+        keys.push_back({"VertElectricFieldStrength", vertElectric, "[V/m^2] (VertElectricForce)"});
+        keys.push_back({"verticalDecay", vertDec, "[m] (VertElectricForce)"});
+    }
+    void readForce(const FitsCloud &f) override {
+        vertElectric = f.key("vertElectricFieldStrength", vertElectric);
+        vertDec = f.key("verticalDecay", vertDec);
+    }
+    double vertElectric;  // [V/m^2]
+    double vertDec;       // [m]
+};
+
+// GravitationalForce: Fy += m g.
+class GravitationalForce : public Force {
+public:
+    GravitationalForce(Cloud *C, double g) : Force(C), gravitational(g) {}
+    DEMON_HOST_STAGES(0, 0)
+    force_flags flag() const override { return GravitationalForceFlag; }
+    void contribute(demon_params &p) const override { p.gravity = gravitational; }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= GravitationalForceFlag;
+        keys.push_back({"GravitationalFieldStrength", gravitational, "[m/s^2] (GravitationalForce)"});
+    }
+    void readForce(const FitsCloud &f) override {
+        gravitational = f.key("gravitationalFieldStrength", gravitational);
+    }
+    double gravitational;  // [m/s^2]
+};
+
+// ThermalForceLocalized (ThermalForceLocalized.h:16-31): the thermal force
+// with heatVal1 strictly inside heatingRadius and heatVal2 elsewhere.
+class ThermalForceLocalized : public Force {
+public:
+    ThermalForceLocalized(Cloud *C, double thermRed1, double thermRed2, double specifiedRadius, uint64_t seed = 0)
+        : Force(C), heatingRadius(specifiedRadius), heatVal1(thermRed1), heatVal2(thermRed2), seed(seed) {}
+    DEMON_HOST_STAGES(step_++, step_)
+    force_flags flag() const override { return ThermalForceLocalizedFlag; }
+    void contribute(demon_params &p) const override {
+        p.heat_in = heatVal1;
+        p.heat_out = heatVal2;
+        p.heat_radius = heatingRadius;
+        p.seed = seed;
+    }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        flags |= ThermalForceLocalizedFlag;
+        keys.push_back({"heatingValue1", heatVal1, "[N] (ThermalForceLocalized)"});
+        keys.push_back({"heatingValue2", heatVal2, "[N] (ThermalForceLocalized)"});
+        keys.push_back({"heatingRadius", heatingRadius, "[m] (ThermalForceLocalized)"});
+    }
+    void readForce(const FitsCloud &f) override {
+        heatVal1 = f.key("heatingValue1", heatVal1);
+        heatVal2 = f.key("heatingValue2", heatVal2);
+        heatingRadius = f.key("heatingRadius", heatingRadius);
+    }
+    double heatingRadius;       // [m]
+    double heatVal1, heatVal2;  // [N]
+    uint64_t seed;
+
+private:
+    uint64_t step_ = 0;
+};
+
+// TimeVaryingThermalForce (TimeVaryingThermalForce.h:14-30): the thermal
+// force with heatVal = scale t + offset at the stage time; it writes
+// ThermalForce's keyword and bit too (TimeVaryingThermalForce.cpp:44-69).
+class TimeVaryingThermalForce : public ThermalForce {
+public:
+    TimeVaryingThermalForce(Cloud *C, double scale, double offset, uint64_t seed = 0)
+        : ThermalForce(C, offset, seed), heatValScale(scale), heatValOffset(offset) {}
+    force_flags flag() const override { return TimeVaryingThermalForceFlag; }
+    void contribute(demon_params &p) const override {
+        p.heat_scale = heatValScale;
+        p.heat_offset = heatValOffset;
+        p.seed = seed;
+    }
+    void writeForce(long &flags, std::vector<FitsKey> &keys) const override {
+        ThermalForce::writeForce(flags, keys);
+        flags |= TimeVaryingThermalForceFlag;
+        keys.push_back({"heatingValueScale", heatValScale, "[N/s] (TimeVaryingThermalForce)"});
+        keys.push_back({"heatingValueOffset", heatValOffset, "[N] (TimeVaryingThermalForce)"});
+    }
+    void readForce(const FitsCloud &f) override {
+        ThermalForce::readForce(f);
+        heatValScale = f.key("heatingValueScale", heatValScale);
+        heatValOffset = f.key("heatingValueOffset", heatValOffset);
+    }
+    double heatValScale;   // [N/s]
+    double heatValOffset;  // [N]
+};
+
+#undef DEMON_HOST_STAGES
 
 class Integrator {
 public:

@@ -3,17 +3,20 @@
 //
 //   demon [-P Params.cfg] [-n rows [cols]] [-t dt] [-e end] [-o dataTimeStep]
 //         [-s shielding] [-C confine] [-g gamma] [-T heat] [-B field]
-//         [-D scale offset] [-I] [-q mean sigma]
-//         [-r mean sigma] [-i spacing] [-p x y] [-k vx vy] [-d density]
-//         [-O data.fits | snapshots.csv] [-c file.fits] [-f file.fits]
-//         [--hex N] [--seed S] [--device D] [-h]
+//         [-D scale offset] [-R X Y] [-V decay] [-S c rmin rmax]
+//         [-w amplitude shift driveConst] [-E field radius] [-F field decay]
+//         [-G g] [-L radius heat_in heat_out] [-v scale offset] [-I]
+//         [-q mean sigma] [-r mean sigma] [-i spacing] [-p x y] [-k vx vy]
+//         [-d density] [-O data.fits | snapshots.csv] [-c file.fits]
+//         [-f file.fits] [--hex N] [--seed S] [--device D] [-h]
 //
 // Options are applied left to right, so options after -P override the file
-// (driver.cpp:834-1030).  Always-on forces: Confinement, ShieldedCoulomb and
-// Drag, unless TimeVaryingDrag (-D, forceFlags D) replaces it (driver.cpp:
-// 322-326); -T adds the thermal force, -B the magnetic force; -I selects
-// Runge_Kutta2 (driver.cpp:437-438).  The reference's other force plugins and
-// -M are not part of this build and are rejected.
+// (driver.cpp:834-1030).  Always-on forces: ShieldedCoulomb; Confinement
+// unless RectConfinement (-R) or ConfinementVoid (-V) replaces it; Drag
+// unless TimeVaryingDrag (-D) replaces it (driver.cpp:322-326).  -T, -L and
+// -v (the thermal forces) exclude each other (driver.cpp:908-973).  -I
+// selects Runge_Kutta2 (driver.cpp:437-438).  -M (the Mach cone) is not part
+// of this build and is rejected.
 // Output (driver.cpp:356-420, 443-450): the reference's data.fits (or the -O
 // name) with a TIME_STEP row every dataTimeStep (demon_fits.h); -c continues
 // a file (its forces, parameters, last row and time) and appends to it; -f
@@ -23,6 +26,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <iostream>
 #include <string>
 
@@ -39,6 +43,8 @@ void help() {
                  "  --hex N        N particles on a jittered hexagonal lattice instead\n"
                  "  -t dt -e end -o dataTimeStep\n"
                  "  -s kappa -C confine -g gamma -T heat -B field -D scale offset -I (RK2)\n"
+                 "  -R X Y (rect) -V decay (void) -S c rmin rmax (shear) -w amp shift driveConst\n"
+                 "  -E field radius -F field decay (vertical) -G g -L radius in out -v scale offset\n"
                  "  -q qMean qSigma -r rMean rSigma -i spacing -p x y -k vx vy -d density\n"
                  "  -O data.fits   output file (default data.fits; *.csv writes text rows)\n"
                  "  -c file.fits   continue a run; -f file.fits  start from a run's final state\n"
@@ -61,6 +67,15 @@ struct Cli {
     std::string out = "data.fits";  // driver.cpp:361
     std::string cont, finals;       // -c, -f
 };
+
+// checkForce (driver.cpp:483-511): a force option once, and not with the
+// options it excludes.
+void use_force(Params &P, char letter, force_flags flag, force_flags excludes) {
+    if (P.usedForces & flag) throw Error(DEMON_ERR_ARG, std::string("option -") + letter + " already set");
+    if (P.usedForces & excludes)
+        throw Error(DEMON_ERR_ARG, std::string("option -") + letter + " conflicts with another thermal option");
+    P.usedForces |= flag;
+}
 
 double take(int argc, char **argv, int &i, double fallback, const char *what) {
     if (i + 1 < argc && is_number(argv[i + 1])) return std::atof(argv[++i]);
@@ -117,19 +132,52 @@ Cli parse(int argc, char **argv) {
         else if (a == "-C") P.confinementConst = take(argc, argv, i, P.confinementConst, "confinementConst");
         else if (a == "-g") P.dragGamma = take(argc, argv, i, P.dragGamma, "dragGamma");
         else if (a == "-T") {
-            if (P.usedForces & (ThermalForceFlag | ThermalForceLocalizedFlag | TimeVaryingThermalForceFlag))
-                throw Error(DEMON_ERR_ARG, "option -T already set or conflicts with -L/-v");
-            P.usedForces |= ThermalForceFlag;
+            use_force(P, 'T', ThermalForceFlag, ThermalForceLocalizedFlag | TimeVaryingThermalForceFlag);
             P.thermRed = take(argc, argv, i, P.thermRed, "heat factor");
+        } else if (a == "-L") {
+            use_force(P, 'L', ThermalForceLocalizedFlag, ThermalForceFlag | TimeVaryingThermalForceFlag);
+            P.heatRadius = take(argc, argv, i, P.heatRadius, "radius");
+            P.thermRed = take(argc, argv, i, P.thermRed, "heat factor1");
+            P.thermRed1 = take(argc, argv, i, P.thermRed1, "heat factor2");
+        } else if (a == "-v") {
+            use_force(P, 'v', TimeVaryingThermalForceFlag, ThermalForceFlag | ThermalForceLocalizedFlag);
+            P.thermScale = take(argc, argv, i, P.thermScale, "heat value scale");
+            P.thermOffset = take(argc, argv, i, P.thermOffset, "heat value offset");
         } else if (a == "-B") {
-            if (P.usedForces & MagneticForceFlag) throw Error(DEMON_ERR_ARG, "option -B already set");
-            P.usedForces |= MagneticForceFlag;
+            use_force(P, 'B', MagneticForceFlag, 0);
             P.magneticFieldStrength = take(argc, argv, i, P.magneticFieldStrength, "magnetic field");
         } else if (a == "-D") {
-            if (P.usedForces & TimeVaryingDragForceFlag) throw Error(DEMON_ERR_ARG, "option -D already set");
-            P.usedForces |= TimeVaryingDragForceFlag;
+            use_force(P, 'D', TimeVaryingDragForceFlag, 0);
             P.dragScale = take(argc, argv, i, P.dragScale, "scale factor");
             P.dragGamma = take(argc, argv, i, P.dragGamma, "offset");
+        } else if (a == "-R") {
+            use_force(P, 'R', RectConfinementForceFlag, 0);
+            P.confinementConstX = take(argc, argv, i, P.confinementConstX, "confine constantX");
+            P.confinementConstY = take(argc, argv, i, P.confinementConstY, "confine constantY");
+        } else if (a == "-V") {
+            use_force(P, 'V', ConfinementForceVoidFlag, 0);
+            P.voidDecay = take(argc, argv, i, P.voidDecay, "void decay");
+        } else if (a == "-S") {
+            use_force(P, 'S', RotationalForceFlag, 0);
+            P.rotConst = take(argc, argv, i, P.rotConst, "force constant");
+            P.rmin = take(argc, argv, i, P.rmin, "rmin");
+            P.rmax = take(argc, argv, i, P.rmax, "rmax");
+        } else if (a == "-w") {
+            use_force(P, 'w', DrivingForceFlag, 0);
+            P.waveAmplitude = take(argc, argv, i, P.waveAmplitude, "amplitude");
+            P.waveShift = take(argc, argv, i, P.waveShift, "wave shift");
+            P.driveConst = take(argc, argv, i, P.driveConst, "driving constant");
+        } else if (a == "-E") {
+            use_force(P, 'E', ElectricForceFlag, 0);
+            P.electricFieldStrength = take(argc, argv, i, P.electricFieldStrength, "electric field const");
+            P.plasmaRadius = take(argc, argv, i, P.plasmaRadius, "plasma radius");
+        } else if (a == "-F") {
+            use_force(P, 'F', VertElectricForceFlag, 0);
+            P.vertElectricFieldStrength = take(argc, argv, i, P.vertElectricFieldStrength, "vert electric field const");
+            P.verticalDecay = take(argc, argv, i, P.verticalDecay, "vertical decay const");
+        } else if (a == "-G") {
+            use_force(P, 'G', GravitationalForceFlag, 0);
+            P.gravitationalFieldStrength = take(argc, argv, i, P.gravitationalFieldStrength, "gravitational field const");
         } else if (a == "-I") {
             c.rk4 = false;
         } else if (a == "-q") {
@@ -159,12 +207,6 @@ Cli parse(int argc, char **argv) {
             throw Error(DEMON_ERR_UNSUPPORTED, "Unknown or unsupported option " + a);
         }
     }
-    const force_flags device_forces = ConfinementForceFlag | DragForceFlag | ShieldedCoulombForceFlag |
-                                      ThermalForceFlag | MagneticForceFlag | TimeVaryingDragForceFlag;
-    if (c.P.usedForces & ~device_forces)
-        throw Error(DEMON_ERR_UNSUPPORTED, "forceFlags select forces that are not part of this build "
-                                           "(Confinement, Drag, Magnetic, ShieldedCoulomb, Thermal and "
-                                           "TimeVaryingDrag run on the device)");
     return c;
 }
 
@@ -192,10 +234,13 @@ int main(int argc, char **argv) {
             if (!c.cont.empty()) {
                 P.usedForces = file.forces;  // driver.cpp:340
                 P.startTime = file.time;     // driver.cpp:343
-                const force_flags known = ConfinementForceFlag | DragForceFlag | ShieldedCoulombForceFlag |
-                                          ThermalForceFlag | MagneticForceFlag | TimeVaryingDragForceFlag;
-                if (P.usedForces & ~known)
-                    throw Error(DEMON_ERR_UNSUPPORTED, src + " was written with forces that are not part of this build");
+                if (P.usedForces & ~(force_flags)0x7FFF)
+                    throw Error(DEMON_ERR_UNSUPPORTED, src + " has FORCES bits that are no ForceFlag");
+                // A plugin that writes its base class's bit too (Void: Confinement,
+                // TimeVaryingDrag: Drag, TimeVaryingThermal: Thermal) stands for one
+                // force: continue with the forces of the run that wrote the file.
+                if (P.usedForces & ConfinementForceVoidFlag) P.usedForces &= ~(force_flags)ConfinementForceFlag;
+                if (P.usedForces & TimeVaryingThermalForceFlag) P.usedForces &= ~(force_flags)ThermalForceFlag;
             }
         } else if (c.hex > 0)
             cloud = Cloud::initializeHexLattice(c.hex, Cloud::interParticleSpacing, 0.1 * Cloud::interParticleSpacing,
@@ -205,21 +250,32 @@ int main(int argc, char **argv) {
         else
             cloud = Cloud::initializeGrid(P.numParticles, c.rowX, c.rowY, P.rMean, P.rSigma, P.qMean,
                                           P.qSigma, c.seed, c.device);
-        // driver.cpp:372-394: construction order is summation order; Drag
-        // only without TimeVaryingDrag (driver.cpp:322-323)
+        // driver.cpp:319-326: the always-on forces unless a substitute is used
+        if (!(P.usedForces & TimeVaryingDragForceFlag)) P.usedForces |= DragForceFlag;
+        else P.usedForces &= ~(force_flags)DragForceFlag;
+        if (!(P.usedForces & (RectConfinementForceFlag | ConfinementForceVoidFlag))) P.usedForces |= ConfinementForceFlag;
+        P.usedForces |= ShieldedCoulombForceFlag;
+        // driver.cpp:372-402: construction order is summation order
+        Cloud *cl = cloud.get();
+        ConfinementForce conf(cl, P.confinementConst);
+        ConfinementForceVoid voidf(cl, P.confinementConst, P.voidDecay);
+        DragForce drag(cl, P.dragGamma);
+        DrivingForce drive(cl, P.driveConst, P.waveAmplitude, P.waveShift);
+        MagneticForce mag(cl, P.magneticFieldStrength);
+        RectConfinementForce rect(cl, P.confinementConstX, P.confinementConstY);
+        RotationalForce rot(cl, P.rmin, P.rmax, P.rotConst);
+        ShieldedCoulombForce yuk(cl, P.shieldingConstant);
+        ThermalForce therm(cl, P.thermRed, c.seed);
+        ThermalForceLocalized local(cl, P.thermRed, P.thermRed1, P.heatRadius, c.seed);
+        TimeVaryingDragForce tvdrag(cl, P.dragScale, P.dragGamma);
+        TimeVaryingThermalForce tvtherm(cl, P.thermScale, P.thermOffset, c.seed);
+        ElectricForce elec(cl, P.electricFieldStrength, P.plasmaRadius);
+        GravitationalForce grav(cl, P.gravitationalFieldStrength);
+        VertElectricForce vert(cl, P.vertElectricFieldStrength, P.verticalDecay);
         ForceArray forces;
-        ConfinementForce conf(cloud.get(), P.confinementConst);
-        DragForce drag(cloud.get(), P.dragGamma);
-        MagneticForce mag(cloud.get(), P.magneticFieldStrength);
-        ShieldedCoulombForce yuk(cloud.get(), P.shieldingConstant);
-        ThermalForce therm(cloud.get(), P.thermRed, c.seed);
-        TimeVaryingDragForce tvdrag(cloud.get(), P.dragScale, P.dragGamma);
-        forces.push_back(&conf);
-        if (!(P.usedForces & TimeVaryingDragForceFlag)) forces.push_back(&drag);
-        if (P.usedForces & MagneticForceFlag) forces.push_back(&mag);
-        forces.push_back(&yuk);
-        if (P.usedForces & ThermalForceFlag) forces.push_back(&therm);
-        if (P.usedForces & TimeVaryingDragForceFlag) forces.push_back(&tvdrag);
+        for (Force *f : std::initializer_list<Force *>{&conf, &voidf, &drag, &drive, &mag, &rect, &rot, &yuk, &therm,
+                                                      &local, &tvdrag, &tvtherm, &elec, &grav, &vert})
+            if (P.usedForces & f->flag()) forces.push_back(f);
         // -c: the forces' parameters from the file (driver.cpp:405-408)
         if (!c.cont.empty())
             for (Force *f : forces) f->readForce(file);

@@ -14,6 +14,13 @@
 //                            TimeVaryingDragForce, carrying their parameters
 //                            (the stage time reaches the device for the
 //                            time-varying drag).
+//   GpuRectConfinementForce, GpuConfinementForceVoid, GpuRotationalForce,
+//   GpuDrivingForce, GpuElectricForce, GpuVertElectricForce,
+//   GpuGravitationalForce, GpuThermalForceLocalized,
+//   GpuTimeVaryingThermalForce
+//                            the same for the reference's further force
+//                            plugins, with their constructor arguments and
+//                            FITS keywords.
 //   GpuRungeKutta4           drop-in for Runge_Kutta4 (Runge_Kutta4.h:14-31):
 //                            moveParticles(endTime) uploads the Cloud, runs
 //                            the whole while (t < end) loop -- dt check,
@@ -202,6 +209,213 @@ public:
         demon_gpu_binding::read_key(f, e, "TVDragOffsetConst", &offsetConst);
     }
     double scaleConst, offsetConst;
+};
+
+class GpuRectConfinementForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuRectConfinementForce(Cloud *C, double confineConstX, double confineConstY)
+        : GpuForceBase(C, DEMON_FORCE_RECT_CONFINEMENT), confineX(confineConstX), confineY(confineConstY) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const {
+        p.confine_x = confineX;
+        p.confine_y = confineY;
+    }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, RectConfinementForceFlag, "confineConstX", confineX,
+                                     "[V/m^2] (RectConfinementForce)");
+        demon_gpu_binding::write_key(f, e, RectConfinementForceFlag, "confineConstY", confineY,
+                                     "[V/m^2] (RectConfinementForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) {
+        demon_gpu_binding::read_key(f, e, "confineConstX", &confineX);
+        demon_gpu_binding::read_key(f, e, "confineConstY", &confineY);
+    }
+    double confineX, confineY;
+};
+
+// ConfinementForceVoid(C, confineConst, decay): the confinement term plus the
+// radial void term, as one plugin (ConfinementForceVoid.cpp:13-55).
+class GpuConfinementForceVoid : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuConfinementForceVoid(Cloud *C, double confineConst, double voidDecay)
+        : GpuForceBase(C, DEMON_FORCE_CONFINEMENT_VOID), confine(confineConst), decay(voidDecay) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const {
+        p.confine = confine;
+        p.void_decay = decay;
+    }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, ConfinementForceFlag, "confineConst", confine, "[V/m^2] (ConfinementForce)");
+        demon_gpu_binding::write_key(f, e, ConfinementForceVoidFlag, "decay", decay, "[m^-1] (ConfinementForceVoid)");
+    }
+    void readForce(fitsfile *const f, int *const e) {
+        demon_gpu_binding::read_key(f, e, "confineConst", &confine);
+        demon_gpu_binding::read_key(f, e, "decay", &decay);
+    }
+    double confine, decay;
+};
+
+class GpuRotationalForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuRotationalForce(Cloud *C, double rmin, double rmax, double rotConst)
+        : GpuForceBase(C, DEMON_FORCE_ROTATIONAL), innerRad(rmin), outerRad(rmax), rotationalConst(rotConst) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const {
+        p.rot_inner = innerRad;
+        p.rot_outer = outerRad;
+        p.rot_const = rotationalConst;
+    }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, RotationalForceFlag, "rotationalConst", rotationalConst, "[N] (RotationalForce)");
+        demon_gpu_binding::write_key(f, e, RotationalForceFlag, "innerRadius", innerRad, "[m] (RotationalForce)");
+        demon_gpu_binding::write_key(f, e, RotationalForceFlag, "outerRadius", outerRad, "[m] (RotationalForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) {
+        demon_gpu_binding::read_key(f, e, "rotationalConst", &rotationalConst);
+        demon_gpu_binding::read_key(f, e, "innerRadius", &innerRad);
+        demon_gpu_binding::read_key(f, e, "outerRadius", &outerRad);
+    }
+    double innerRad, outerRad, rotationalConst;
+};
+
+// DrivingForce(C, dampConst, amplitude, shift): the member driveConst is
+// -dampConst (DrivingForce.h:17-18) and is what the FITS keyword holds.
+class GpuDrivingForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuDrivingForce(Cloud *C, double dampConst, double amp, double drivingShift)
+        : GpuForceBase(C, DEMON_FORCE_DRIVING), amplitude(amp), driveConst(-dampConst), shift(drivingShift) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const {
+        p.drive_const = -driveConst;
+        p.drive_amplitude = amplitude;
+        p.drive_shift = shift;
+    }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, DrivingForceFlag, "drivingAmplitude", amplitude, "[N] (DrivingForce)");
+        demon_gpu_binding::write_key(f, e, DrivingForceFlag, "drivingConst", driveConst, "[m^2] (DrivingForce)");
+        demon_gpu_binding::write_key(f, e, DrivingForceFlag, "drivingShift", shift, "[m] (DrivingForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) {
+        demon_gpu_binding::read_key(f, e, "drivingAmplitude", &amplitude);
+        demon_gpu_binding::read_key(f, e, "drivingConst", &driveConst);
+        demon_gpu_binding::read_key(f, e, "drivingShift", &shift);
+    }
+    double amplitude, driveConst, shift;
+};
+
+class GpuElectricForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuElectricForce(Cloud *C, double electricField, double plasmaRad)
+        : GpuForceBase(C, DEMON_FORCE_ELECTRIC), electric(electricField), radius(plasmaRad) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const {
+        p.electric_field = electric;
+        p.plasma_radius = radius;
+    }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, ElectricForceFlag, "ElectricFieldStrength", electric, "[V/m^2] (ElectricForce)");
+        demon_gpu_binding::write_key(f, e, ElectricForceFlag, "plasmaRadius", radius, "[m] (ElectricForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) {
+        demon_gpu_binding::read_key(f, e, "electricFieldStrength", &electric);
+        demon_gpu_binding::read_key(f, e, "plasmaRadius", &radius);
+    }
+    double electric, radius;
+};
+
+class GpuVertElectricForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuVertElectricForce(Cloud *C, double vertElectricField, double vertDecay)
+        : GpuForceBase(C, DEMON_FORCE_VERT_ELECTRIC), vertElectric(vertElectricField), vertDec(vertDecay) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const {
+        p.vert_electric_field = vertElectric;
+        p.vert_decay = vertDec;
+    }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, VertElectricForceFlag, "VertElectricFieldStrength", vertElectric,
+                                     "[V/m^2] (VertElectricForce)");
+        demon_gpu_binding::write_key(f, e, VertElectricForceFlag, "verticalDecay", vertDec, "[m] (VertElectricForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) {
+        demon_gpu_binding::read_key(f, e, "vertElectricFieldStrength", &vertElectric);
+        demon_gpu_binding::read_key(f, e, "verticalDecay", &vertDec);
+    }
+    double vertElectric, vertDec;
+};
+
+class GpuGravitationalForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuGravitationalForce(Cloud *C, double g) : GpuForceBase(C, DEMON_FORCE_GRAVITATIONAL), gravitational(g) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const { p.gravity = gravitational; }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, GravitationalForceFlag, "GravitationalFieldStrength", gravitational,
+                                     "[m/s^2] (GravitationalForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) {
+        demon_gpu_binding::read_key(f, e, "gravitationalFieldStrength", &gravitational);
+    }
+    double gravitational;
+};
+
+// ThermalForceLocalized(C, heatVal1, heatVal2, radius): heatVal1 strictly
+// inside the radius, heatVal2 elsewhere (ThermalForceLocalized.cpp:124-133).
+class GpuThermalForceLocalized : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuThermalForceLocalized(Cloud *C, double thermRed1, double thermRed2, double specifiedRadius, uint64_t seed = 0)
+        : GpuForceBase(C, DEMON_FORCE_THERMAL_LOCALIZED), heatingRadius(specifiedRadius), heatVal1(thermRed1),
+          heatVal2(thermRed2), seed(seed) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const {
+        p.heat_in = heatVal1;
+        p.heat_out = heatVal2;
+        p.heat_radius = heatingRadius;
+        p.seed = seed;
+    }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, ThermalForceLocalizedFlag, "heatingValue1", heatVal1, "[N] (ThermalForceLocalized)");
+        demon_gpu_binding::write_key(f, e, ThermalForceLocalizedFlag, "heatingValue2", heatVal2, "[N] (ThermalForceLocalized)");
+        demon_gpu_binding::write_key(f, e, ThermalForceLocalizedFlag, "heatingRadius", heatingRadius,
+                                     "[m] (ThermalForceLocalized)");
+    }
+    void readForce(fitsfile *const f, int *const e) {
+        demon_gpu_binding::read_key(f, e, "heatingValue1", &heatVal1);
+        demon_gpu_binding::read_key(f, e, "heatingValue2", &heatVal2);
+        demon_gpu_binding::read_key(f, e, "heatingRadius", &heatingRadius);
+    }
+    double heatingRadius, heatVal1, heatVal2;
+    uint64_t seed;
+};
+
+// TimeVaryingThermalForce(C, scale, offset): the thermal force with heatVal =
+// scale t + offset at the stage time (TimeVaryingThermalForce.cpp:14-42); it
+// writes ThermalForce's keyword and bit as well, as the CPU class does.
+class GpuTimeVaryingThermalForce : public demon_gpu_binding::GpuForceBase {
+public:
+    GpuTimeVaryingThermalForce(Cloud *C, double scale, double offset, uint64_t seed = 0)
+        : GpuForceBase(C, DEMON_FORCE_TIME_VARYING_THERMAL), heatVal(offset), heatValScale(scale),
+          heatValOffset(offset), seed(seed) {}
+    DEMON_GPU_FORCE_STAGES
+    void contribute(demon_params &p) const {
+        p.heat_scale = heatValScale;
+        p.heat_offset = heatValOffset;
+        p.seed = seed;
+    }
+    void writeForce(fitsfile *const f, int *const e) const {
+        demon_gpu_binding::write_key(f, e, ThermalForceFlag, "heatingValue", heatVal, "[N] (ThermalForce)");
+        demon_gpu_binding::write_key(f, e, TimeVaryingThermalForceFlag, "heatingValueScale", heatValScale,
+                                     "[N/s] (TimeVaryingThermalForce)");
+        demon_gpu_binding::write_key(f, e, TimeVaryingThermalForceFlag, "heatingValueOffset", heatValOffset,
+                                     "[N] (TimeVaryingThermalForce)");
+    }
+    void readForce(fitsfile *const f, int *const e) {
+        demon_gpu_binding::read_key(f, e, "heatingValue", &heatVal);
+        demon_gpu_binding::read_key(f, e, "heatingValueScale", &heatValScale);
+        demon_gpu_binding::read_key(f, e, "heatingValueOffset", &heatValOffset);
+    }
+    double heatVal, heatValScale, heatValOffset;
+    uint64_t seed;
 };
 
 #undef DEMON_GPU_FORCE_STAGES

@@ -1,8 +1,9 @@
 /*
  * demon_gpu.h -- C ABI of the MI355X (gfx950) implementation of DEMON's hot
  * path: the O(N^2) shielded-Coulomb (Yukawa) force with its 10-lambda cutoff,
- * confinement, drag and thermal forces, and Runge-Kutta-4 stepping with the
- * reference's adaptive time-step check.
+ * the reference's per-particle force plugins (confinement, drag, thermal and
+ * the rest), and Runge-Kutta-4 / -2 stepping with the reference's adaptive
+ * time-step check.
  *
  * Plain C types only: host arrays are double* of length n in the caller's
  * particle order; nothing here exposes HIP or torch types.  Every call returns
@@ -20,6 +21,15 @@
  *     ThermalForce         (ThermalForce.cpp:42-122)
  *     MagneticForce        (MagneticForce.cpp:13-49)
  *     TimeVaryingDragForce (TimeVaryingDragForce.cpp:12-39)
+ *     RectConfinementForce  (RectConfinementForce.cpp:13-59)
+ *     ConfinementForceVoid  (ConfinementForceVoid.cpp:13-55)
+ *     RotationalForce       (RotationalForce.cpp:13-66)
+ *     DrivingForce          (DrivingForce.cpp:14-61)
+ *     ElectricForce         (ElectricForce.cpp:13-62)
+ *     VertElectricForce     (VertElectricForce.cpp:13-53)
+ *     GravitationalForce    (GravitationalForce.cpp:13-50)
+ *     ThermalForceLocalized (ThermalForceLocalized.cpp:39-133)
+ *     TimeVaryingThermalForce (TimeVaryingThermalForce.cpp:14-42)
  *   Force::readForce / constructor args    -> demon_set_params
  *   Runge_Kutta4::moveParticles (Runge_Kutta4.cpp:23-137), including
  *     Integrator::modifyTimeStep (Integrator.cpp:68-182)
@@ -36,7 +46,7 @@
 extern "C" {
 #endif
 
-#define DEMON_ABI_VERSION 2
+#define DEMON_ABI_VERSION 3
 
 /* Return codes. */
 #define DEMON_OK 0
@@ -51,8 +61,17 @@ extern "C" {
 #define DEMON_FORCE_DRAG 2
 #define DEMON_FORCE_SHIELDED_COULOMB 4
 #define DEMON_FORCE_THERMAL 16
+#define DEMON_FORCE_GRAVITATIONAL 8
+#define DEMON_FORCE_THERMAL_LOCALIZED 32
+#define DEMON_FORCE_DRIVING 64
+#define DEMON_FORCE_ROTATIONAL 128
 #define DEMON_FORCE_TIME_VARYING_DRAG 256 /* the reference's driver then leaves out DRAG */
+#define DEMON_FORCE_TIME_VARYING_THERMAL 512
 #define DEMON_FORCE_MAGNETIC 1024
+#define DEMON_FORCE_CONFINEMENT_VOID 2048 /* the driver then leaves out CONFINEMENT */
+#define DEMON_FORCE_ELECTRIC 4096
+#define DEMON_FORCE_RECT_CONFINEMENT 8192 /* the driver then leaves out CONFINEMENT */
+#define DEMON_FORCE_VERT_ELECTRIC 16384
 
 /* Pair-force arithmetic. */
 #define DEMON_PRECISION_FP64 0      /* FP64 throughout */
@@ -71,6 +90,26 @@ typedef struct demon_params {
     double magnetic_field; /* B [T]; MagneticForce(cloud, B): F += q*B*(vy, -vx) */
     double drag_scale;     /* [Hz/s]; TimeVaryingDragForce(cloud, scale, drag_gamma):
                               F += m*(-(scale*t + drag_gamma))*v at the stage time t */
+    /* the further force plugins, with the driver's constructor arguments
+       (driver.cpp:373-402); each is used only when its flag is set */
+    double confine_x, confine_y;   /* RectConfinementForce(cloud, X, Y) [V/m^2] */
+    double void_decay;             /* ConfinementForceVoid(cloud, confine, decay) [1/m] */
+    double rot_inner, rot_outer;   /* RotationalForce(cloud, rmin, rmax, rot_const) [m] */
+    double rot_const;              /* [N] */
+    double drive_const;            /* DrivingForce(cloud, driveConst, amplitude, shift) [m^2] */
+    double drive_amplitude;        /* [N] */
+    double drive_shift;            /* [m] */
+    double electric_field;         /* ElectricForce(cloud, E, plasma_radius) [V/m^2] */
+    double plasma_radius;          /* [m] */
+    double vert_electric_field;    /* VertElectricForce(cloud, E, vert_decay) [V/m^2] */
+    double vert_decay;             /* [m] */
+    double gravity;                /* GravitationalForce(cloud, g) [m/s^2] */
+    double heat_in, heat_out;      /* ThermalForceLocalized(cloud, heat_in, heat_out, heat_radius):
+                                      the thermal magnitude inside / outside heat_radius [N] */
+    double heat_radius;            /* [m] */
+    double heat_scale, heat_offset;/* TimeVaryingThermalForce(cloud, heat_scale, heat_offset):
+                                      the thermal magnitude heat_scale*t + heat_offset
+                                      at the stage time t [N/s, N] */
 } demon_params;
 
 /* Counters of the work done since demon_upload (or the last reset). */
@@ -114,17 +153,23 @@ int demon_download(demon_ctx *ctx, double *x, double *y, double *vx, double *vy)
 /* Force::force<stage>(t) for the forces in `forces` (a subset of the
  * context's flags) evaluated on the given stage state at time t, ADDED into
  * fx/fy (host arrays, caller's order), as Force::forceS adds into
- * Cloud::forceX/Y, in the driver's order (Confinement, Drag, Magnetic,
- * ShieldedCoulomb, Thermal, TimeVaryingDrag; driver.cpp:373-394).  stage
- * (1..4) keys the thermal RNG (with `step`); t is the stage time the
- * integrator passes (Runge_Kutta4.cpp:31,50,69,88), read by TimeVaryingDrag
- * only.  vxs/vys may be NULL when no velocity-dependent force is requested.
+ * Cloud::forceX/Y, in the driver's construction order (Confinement,
+ * ConfinementVoid, Drag, Driving, Magnetic, RectConfinement, Rotational,
+ * ShieldedCoulomb, Thermal, ThermalLocalized, TimeVaryingDrag,
+ * TimeVaryingThermal, Electric, Gravitational, VertElectric;
+ * driver.cpp:372-402); each flag is one plugin object, so CONFINEMENT |
+ * CONFINEMENT_VOID adds the confinement term twice, as two such objects do.
+ * stage (1..4) keys the thermal RNGs (with `step`); t is the stage time the
+ * integrator passes (Runge_Kutta4.cpp:31,50,69,88), read by TimeVaryingDrag,
+ * TimeVaryingThermal and Driving.  vxs/vys may be NULL when no
+ * velocity-dependent force is requested.
  * Uses the context's charge and mass. */
 int demon_force_stage_at(demon_ctx *ctx, int32_t forces, int32_t stage, uint64_t step, double t,
                          const double *xs, const double *ys, const double *vxs,
                          const double *vys, double *fx, double *fy);
 
-/* The same without the time; rejects TIME_VARYING_DRAG. */
+/* The same without the time; rejects TIME_VARYING_DRAG, TIME_VARYING_THERMAL
+ * and DRIVING. */
 int demon_force_stage(demon_ctx *ctx, int32_t forces, int32_t stage, uint64_t step,
                       const double *xs, const double *ys, const double *vxs,
                       const double *vys, double *fx, double *fy);

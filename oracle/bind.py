@@ -34,7 +34,23 @@ class OracleParams(ctypes.Structure):
         ("gather", ctypes.c_int32),
         ("magnetic_field", ctypes.c_double),
         ("drag_scale", ctypes.c_double),
-    ]
+    ] + [(name, ctypes.c_double) for name in (
+        "confine_x", "confine_y", "void_decay", "rot_inner", "rot_outer", "rot_const",
+        "drive_const", "drive_amplitude", "drive_shift", "electric_field", "plasma_radius",
+        "vert_electric_field", "vert_decay", "gravity", "heat_in", "heat_out", "heat_radius",
+        "heat_scale", "heat_offset")]
+
+
+# Order of the parameter vector of ref_rk_all / ref_force_all (ref_harness.cpp RefPar).
+REF_PAR = ("shielding", "confine", "gamma", "heat_val", "magnetic_field", "drag_scale", "confine_x",
+           "confine_y", "void_decay", "rot_inner", "rot_outer", "rot_const", "drive_const",
+           "drive_amplitude", "drive_shift", "electric_field", "plasma_radius", "vert_electric_field",
+           "vert_decay", "gravity", "heat_in", "heat_out", "heat_radius", "heat_scale", "heat_offset")
+
+
+def ref_par(op: "OracleParams") -> np.ndarray:
+    """The OracleParams constants as the reference harness's parameter vector."""
+    return np.array([getattr(op, k) for k in REF_PAR], dtype=np.float64)
 
 
 def build(ref: bool = True) -> None:
@@ -255,6 +271,13 @@ class Reference:
                                   ctypes.c_int, ctypes.c_double, _I64, _D, ctypes.c_int]
         L.ref_rk2_ext.argtypes = L.ref_rk4_ext.argtypes
         L.ref_fits_write.argtypes = [ctypes.c_char_p, _I64, _D, _D, _D, _D, _D, _D, ctypes.c_int, _D, _I64]
+        L.ref_fits_forces_roundtrip.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        L.ref_num_par.restype = ctypes.c_int
+        assert L.ref_num_par() == len(REF_PAR)
+        L.ref_rk_all.argtypes = [ctypes.c_int, _I64, _D, _D, _D, _D, _D, _D, _D, ctypes.c_int,
+                                 ctypes.c_double, _I64, _D, ctypes.c_int]
+        L.ref_force_all.argtypes = [_I64, _D, _D, _D, _D, _D, _D, _D, ctypes.c_int, ctypes.c_double,
+                                    _I64, _D, _D]
         L.ref_fits_read.argtypes = [ctypes.c_char_p, _I64, _D, _D, _D, _D, _D, _D,
                                     ctypes.POINTER(ctypes.c_long), _D, _D]
 
@@ -294,12 +317,38 @@ class Reference:
         assert rc == 0
         return (*s, float(tt[0]))
 
+    def rk_all(self, x, y, vx, vy, q, m, op, dt, steps, t=0.0, order=4, threads=0):
+        """`steps` steps of Runge_Kutta4 / Runge_Kutta2 with every force of op.forces,
+        constructed as the driver does from the OracleParams constants."""
+        s = [_f64(a).copy() for a in (x, y, vx, vy)]
+        tt = np.array([t])
+        rc = self.L.ref_rk_all(order, len(s[0]), *[p(a) for a in s], p(_f64(q)), p(_f64(m)),
+                               p(ref_par(op)), op.forces, dt, steps, p(tt), threads or self.threads)
+        assert rc == 0
+        return (*s, float(tt[0]))
+
+    def force_all(self, x, y, vx, vy, q, m, op, t=0.0, reps=1):
+        """The forces of op.forces on one state at time t, `reps` evaluations
+        (force1..force4 in turn): fx, fy of shape [reps, n]."""
+        n = len(x)
+        fx, fy = np.zeros((reps, n)), np.zeros((reps, n))
+        rc = self.L.ref_force_all(n, *[p(_f64(a)) for a in (x, y, vx, vy, q, m)], p(ref_par(op)),
+                                  op.forces, t, reps, p(fx), p(fy))
+        assert rc == 0
+        return fx, fy
+
     def fits_write(self, path, x, y, vx, vy, q, m, forces, par, rows):
         """The reference's own writer (driver + Cloud::writeCloudSetup/writeTimeStep);
         par = (confine, gamma, B, kappa, heat, TV scale); row k has x + 1e-6 k at t = 0.01 k."""
         rc = self.L.ref_fits_write(str(path).encode(), len(x), *[p(_f64(a)) for a in (x, y, vx, vy, q, m)],
                                    forces, p(_f64(np.asarray(par, dtype=np.float64))), rows)
         assert rc == 0, rc
+
+    def fits_forces_roundtrip(self, src, dst):
+        """The reference's readForce of every force in src's FORCES, then its
+        writeForce of the same objects into a new file dst (primary header only)."""
+        rc = self.L.ref_fits_forces_roundtrip(str(src).encode(), str(dst).encode())
+        assert rc == 0, f"cfitsio status {rc}"
 
     def fits_read(self, path, n):
         """The reference's reader: (x, y, vx, vy, q, m, forces, par, t) of the last row."""

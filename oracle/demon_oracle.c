@@ -251,23 +251,50 @@ static inline double u53(uint32_t lo, uint32_t hi)
     return (double)v * 0x1.0p-53;
 }
 
+/* One thermal kick of particle i: U and theta from the Philox block keyed
+ * (seed; i, step, domain << 24 | stage << 16 | step >> 32).  Domains: 1
+ * ThermalForce, 3 ThermalForceLocalized, 4 TimeVaryingThermalForce. */
+static void thermal_kick(int64_t i, double heat, uint64_t seed, uint64_t step, uint32_t domain,
+                         int stage, double *fx, double *fy)
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), (uint32_t)step,
+                             (domain << 24) | ((uint32_t)stage << 16) |
+                                 ((uint32_t)(step >> 32) & 0xFFFFu)};
+    uint32_t o[4];
+    oracle_philox4x32(ctr, key, o);
+    const double uu = u53(o[0], o[1]);                  /* RandCache::r  in [0,1) */
+    const double th = u53(o[2], o[3]) * (2.0 * M_PI);   /* RandCache::r1 in [0,2pi) */
+    const double tv = uu * heat;                        /* ThermalForce.cpp:119 */
+    *fx += tv * cos(th);                                /* :120 */
+    *fy += tv * sin(th);                                /* :121 */
+}
+
 void oracle_thermal_add(int64_t n, double heat, uint64_t seed, uint64_t step,
                         int stage, double *fx, double *fy)
 {
-    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int64_t i = 0; i < n; ++i)
+        thermal_kick(i, heat, seed, step, 1u, stage, &fx[i], &fy[i]);
+}
+
+void oracle_thermal_localized_add(int64_t n, const double *xs, const double *ys,
+                                  double heat_in, double heat_out, double radius,
+                                  uint64_t seed, uint64_t step, int stage, double *fx,
+                                  double *fy)
+{
     for (int64_t i = 0; i < n; ++i) {
-        /* counter: particle, step, (domain 1 = thermal, stage) */
-        const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), (uint32_t)step,
-                                 (1u << 24) | ((uint32_t)stage << 16) |
-                                     ((uint32_t)(step >> 32) & 0xFFFFu)};
-        uint32_t o[4];
-        oracle_philox4x32(ctr, key, o);
-        const double uu = u53(o[0], o[1]);                  /* RandCache::r  in [0,1) */
-        const double th = u53(o[2], o[3]) * (2.0 * M_PI);   /* RandCache::r1 in [0,2pi) */
-        const double tv = uu * heat;                        /* ThermalForce.cpp:119 */
-        fx[i] += tv * cos(th);                              /* :120 */
-        fy[i] += tv * sin(th);                              /* :121 */
+        const double r = sqrt(xs[i] * xs[i] + ys[i] * ys[i]);     /* ThermalForceLocalized.cpp:126 */
+        const double heat = r < radius ? heat_in : heat_out;      /* :127-128 */
+        thermal_kick(i, heat, seed, step, 3u, stage, &fx[i], &fy[i]);
     }
+}
+
+void oracle_tvthermal_add(int64_t n, double scale, double offset, double t, uint64_t seed,
+                          uint64_t step, int stage, double *fx, double *fy)
+{
+    const double heat = scale * t + offset;     /* TimeVaryingThermalForce.cpp:40-42 */
+    for (int64_t i = 0; i < n; ++i)
+        thermal_kick(i, heat, seed, step, 4u, stage, &fx[i], &fy[i]);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -403,28 +430,124 @@ void oracle_tvdrag_add(int64_t n, const double *vxs, const double *vys, const do
     }
 }
 
+void oracle_rect_add(int64_t n, const double *xs, const double *ys, const double *q,
+                     double cx, double cy, double *fx, double *fy)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        fx[i] = fx[i] + (q[i] * cx) * xs[i];      /* RectConfinementForce.cpp:55-56 */
+        fy[i] = fy[i] + (q[i] * cy) * ys[i];      /* :57-58 */
+    }
+}
+
+void oracle_void_add(int64_t n, const double *xs, const double *ys, const double *q,
+                     double confine, double decay, double *fx, double *fy)
+{
+    oracle_confinement_add(n, xs, ys, q, confine, fx, fy);   /* ConfinementForceVoid.cpp:14 */
+    for (int64_t i = 0; i < n; ++i) {
+        const double dv = q[i] * (-decay);                    /* :49 */
+        const double r = sqrt(xs[i] * xs[i] + ys[i] * ys[i]); /* :50 (length_pd) */
+        const double er = exp(r * (-decay)) / r;              /* :51 */
+        fx[i] = fx[i] + (dv * er) * xs[i];                    /* :53 */
+        fy[i] = fy[i] + (dv * er) * ys[i];                    /* :54 */
+    }
+}
+
+void oracle_rotational_add(int64_t n, const double *xs, const double *ys, double rmin,
+                           double rmax, double c, double *fx, double *fy)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const double r = sqrt(xs[i] * xs[i] + ys[i] * ys[i]); /* RotationalForce.cpp:53 */
+        if (!(r > rmin && r < rmax))                          /* :56-60 */
+            continue;
+        fx[i] = fx[i] - (c * ys[i]) / r;                      /* :64 */
+        fy[i] = fy[i] + (c * xs[i]) / r;                      /* :65 */
+    }
+}
+
+void oracle_driving_add(int64_t n, const double *xs, double t, double damp_const,
+                        double amplitude, double shift, double *fx)
+{
+    const double wave_num = 2.0 * M_PI / 0.002;   /* DrivingForce.cpp:14 */
+    const double ang_freq = 2.0 * M_PI * 10.0;    /* :15 */
+    const double drive_const = -damp_const;       /* DrivingForce.h:18 */
+    for (int64_t i = 0; i < n; ++i) {
+        const double dist = xs[i] - shift;                          /* DrivingForce.cpp:55 */
+        const double arg = xs[i] * wave_num - t * ang_freq;         /* :56 */
+        const double earg = (dist - dist) / (-drive_const);         /* :57 */
+        fx[i] = fx[i] + (sin(arg) * exp(earg)) * amplitude;         /* :59-60 */
+    }
+}
+
+void oracle_electric_add(int64_t n, const double *xs, const double *ys, const double *q,
+                         double e_field, double radius, double *fx, double *fy)
+{
+    const double rad = radius * (-1);             /* ElectricForce.cpp:55 */
+    for (int64_t i = 0; i < n; ++i) {
+        const double cv = q[i] * e_field;                         /* :51 */
+        const double rr = sqrt(xs[i] * xs[i] + ys[i] * ys[i]);    /* :52-54 */
+        const double ev = cv * exp(rr / rad);                     /* :56 */
+        fx[i] = fx[i] + ev * xs[i];                               /* :59 */
+        fy[i] = fy[i] + ev * ys[i];                               /* :60 */
+    }
+}
+
+void oracle_vert_electric_add(int64_t n, const double *ys, const double *q, double e_field,
+                              double decay, double *fy)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const double cv = q[i] * e_field;                 /* VertElectricForce.cpp:49 */
+        fy[i] = fy[i] + cv * exp(ys[i] / decay);          /* :50-52 */
+    }
+}
+
+void oracle_gravity_add(int64_t n, const double *m, double g, double *fy)
+{
+    for (int64_t i = 0; i < n; ++i)
+        fy[i] = fy[i] + m[i] * g;                         /* GravitationalForce.cpp:47-49 */
+}
+
 int oracle_stage_force(int64_t n, const double *xs, const double *ys,
                        const double *vxs, const double *vys, const double *q,
                        const double *m, const oracle_params *p, uint64_t step,
                        int stage, double t_stage, double *fx, double *fy)
 {
-    /* driver.cpp:372-396 construction order = summation order */
-    if (p->forces & ORACLE_CONFINEMENT)
+    /* driver.cpp:372-402 construction order = summation order */
+    const int f = p->forces;
+    if (f & ORACLE_CONFINEMENT)
         oracle_confinement_add(n, xs, ys, q, p->confine, fx, fy);
-    if (p->forces & ORACLE_DRAG)
+    if (f & ORACLE_CONFINEMENT_VOID)
+        oracle_void_add(n, xs, ys, q, p->confine, p->void_decay, fx, fy);
+    if (f & ORACLE_DRAG)
         oracle_drag_add(n, vxs, vys, m, p->gamma, fx, fy);
-    if (p->forces & ORACLE_MAGNETIC)
+    if (f & ORACLE_DRIVING)
+        oracle_driving_add(n, xs, t_stage, p->drive_const, p->drive_amplitude, p->drive_shift, fx);
+    if (f & ORACLE_MAGNETIC)
         oracle_magnetic_add(n, vxs, vys, q, p->magnetic_field, fx, fy);
+    if (f & ORACLE_RECT_CONFINEMENT)
+        oracle_rect_add(n, xs, ys, q, p->confine_x, p->confine_y, fx, fy);
+    if (f & ORACLE_ROTATIONAL)
+        oracle_rotational_add(n, xs, ys, p->rot_inner, p->rot_outer, p->rot_const, fx, fy);
     if (p->forces & ORACLE_SHIELDED_COULOMB) {
         if ((n & 1) || p->gather)
             oracle_yukawa_gather_add(n, xs, ys, q, p->shielding, fx, fy);
         else if (oracle_yukawa_add(n, xs, ys, q, p->shielding, fx, fy))
             return -1;
     }
-    if (p->forces & ORACLE_THERMAL)
+    if (f & ORACLE_THERMAL)
         oracle_thermal_add(n, p->heat_val, p->seed, step, stage, fx, fy);
-    if (p->forces & ORACLE_TIME_VARYING_DRAG)
+    if (f & ORACLE_THERMAL_LOCALIZED)
+        oracle_thermal_localized_add(n, xs, ys, p->heat_in, p->heat_out, p->heat_radius, p->seed,
+                                     step, stage, fx, fy);
+    if (f & ORACLE_TIME_VARYING_DRAG)
         oracle_tvdrag_add(n, vxs, vys, m, p->drag_scale, p->gamma, t_stage, fx, fy);
+    if (f & ORACLE_TIME_VARYING_THERMAL)
+        oracle_tvthermal_add(n, p->heat_scale, p->heat_offset, t_stage, p->seed, step, stage, fx, fy);
+    if (f & ORACLE_ELECTRIC)
+        oracle_electric_add(n, xs, ys, q, p->electric_field, p->plasma_radius, fx, fy);
+    if (f & ORACLE_GRAVITATIONAL)
+        oracle_gravity_add(n, m, p->gravity, fy);
+    if (f & ORACLE_VERT_ELECTRIC)
+        oracle_vert_electric_add(n, ys, q, p->vert_electric_field, p->vert_decay, fy);
     return 0;
 }
 

@@ -33,6 +33,16 @@
  *   oracle_dtcheck_min_s     brute force over the coverage predicate
  *   oracle_philox4x32        published Random123 known-answer vectors
  *   oracle_thermal_add       moments of the published uniform/angle model
+ *   oracle_rect_add, oracle_void_add, oracle_rotational_add,
+ *   oracle_driving_add, oracle_electric_add, oracle_vert_electric_add,
+ *   oracle_gravity_add       bitwise vs the compiled reference's force1 and
+ *                            its RK4 / RK2 trajectories; closed forms (linear
+ *                            rect field, radial void field, the tangential
+ *                            rotation of the shear layer, gravity = m g)
+ *   oracle_thermal_localized_add, oracle_tvthermal_add
+ *                            the reference's own draws: per-particle magnitude
+ *                            bounds inside / outside the radius, and |F|^2
+ *                            moments at the stage time
  */
 #ifndef DEMON_ORACLE_H
 #define DEMON_ORACLE_H
@@ -51,9 +61,18 @@ enum {
     ORACLE_CONFINEMENT = 1,
     ORACLE_DRAG = 2,
     ORACLE_SHIELDED_COULOMB = 4,
+    ORACLE_GRAVITATIONAL = 8,
     ORACLE_THERMAL = 16,
+    ORACLE_THERMAL_LOCALIZED = 32,
+    ORACLE_DRIVING = 64,
+    ORACLE_ROTATIONAL = 128,
     ORACLE_TIME_VARYING_DRAG = 256,
-    ORACLE_MAGNETIC = 1024
+    ORACLE_TIME_VARYING_THERMAL = 512,
+    ORACLE_MAGNETIC = 1024,
+    ORACLE_CONFINEMENT_VOID = 2048,
+    ORACLE_ELECTRIC = 4096,
+    ORACLE_RECT_CONFINEMENT = 8192,
+    ORACLE_VERT_ELECTRIC = 16384
 };
 
 typedef struct oracle_params {
@@ -69,6 +88,18 @@ typedef struct oracle_params {
     double magnetic_field; /* B [T]                             (MagneticForce.h:29) */
     double drag_scale;  /* TimeVaryingDragForce scaleConst [Hz/s]; its offset is gamma
                            (driver.cpp:394, TimeVaryingDragForce.h:16-17) */
+    /* constructor arguments of the further plugins (driver.cpp:373-402) */
+    double confine_x, confine_y;      /* RectConfinementForce.h confineX, confineY */
+    double void_decay;                /* ConfinementForceVoid decay [1/m] */
+    double rot_inner, rot_outer, rot_const; /* RotationalForce.h:20-21 */
+    double drive_const;               /* DrivingForce(C, dampConst, ...): the member
+                                         driveConst is -dampConst (DrivingForce.h:17-18) */
+    double drive_amplitude, drive_shift;
+    double electric_field, plasma_radius;         /* ElectricForce.h:16-17 */
+    double vert_electric_field, vert_decay;       /* VertElectricForce.h:16-17 */
+    double gravity;                               /* GravitationalForce */
+    double heat_in, heat_out, heat_radius;        /* ThermalForceLocalized.h:16-17 */
+    double heat_scale, heat_offset;               /* TimeVaryingThermalForce.h:16-17 */
 } oracle_params;
 
 /* ShieldedCoulombForce::forceS (ShieldedCoulombForce.cpp:37-237), serial
@@ -123,6 +154,55 @@ void oracle_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 void oracle_thermal_add(int64_t n, double heat, uint64_t seed, uint64_t step,
                         int stage, double *fx, double *fy);
 
+/* RectConfinementForce::force (RectConfinementForce.cpp:50-59):
+ * Fx += (q*cX)*x, Fy += (q*cY)*y. */
+void oracle_rect_add(int64_t n, const double *xs, const double *ys, const double *q,
+                     double cx, double cy, double *fx, double *fy);
+
+/* ConfinementForceVoid::forceS (ConfinementForceVoid.cpp:13-55): the
+ * ConfinementForce term, then with r = sqrt(x^2+y^2):
+ * F += ((q*(-decay)) * (exp(r*(-decay))/r)) * (x, y). */
+void oracle_void_add(int64_t n, const double *xs, const double *ys, const double *q,
+                     double confine, double decay, double *fx, double *fy);
+
+/* RotationalForce::force (RotationalForce.cpp:50-66): for rmin < r < rmax,
+ * Fx -= (c*y)/r, Fy += (c*x)/r. */
+void oracle_rotational_add(int64_t n, const double *xs, const double *ys, double rmin,
+                           double rmax, double c, double *fx, double *fy);
+
+/* DrivingForce::force (DrivingForce.cpp:14-15, 52-61) at stage time t:
+ * Fx += (sin(x*k - t*w) * exp(((x-shift)-(x-shift))/dampConst)) * amplitude,
+ * k = 2 pi/0.002, w = 2 pi 10. */
+void oracle_driving_add(int64_t n, const double *xs, double t, double damp_const,
+                        double amplitude, double shift, double *fx);
+
+/* ElectricForce::force (ElectricForce.cpp:48-62):
+ * F += ((q*E) * exp(sqrt(x^2+y^2)/(-radius))) * (x, y). */
+void oracle_electric_add(int64_t n, const double *xs, const double *ys, const double *q,
+                         double e_field, double radius, double *fx, double *fy);
+
+/* VertElectricForce::force (VertElectricForce.cpp:47-53):
+ * Fy += (q*E) * exp(y/decay). */
+void oracle_vert_electric_add(int64_t n, const double *ys, const double *q, double e_field,
+                              double decay, double *fy);
+
+/* GravitationalForce::force (GravitationalForce.cpp:46-50): Fy += m*g. */
+void oracle_gravity_add(int64_t n, const double *m, double g, double *fy);
+
+/* ThermalForceLocalized::force (ThermalForceLocalized.cpp:124-133) with
+ * Philox draws of counter domain 3: heat = r < radius ? heat_in : heat_out
+ * on the stage position, F += (heat*U)*(cos, sin). */
+void oracle_thermal_localized_add(int64_t n, const double *xs, const double *ys,
+                                  double heat_in, double heat_out, double radius,
+                                  uint64_t seed, uint64_t step, int stage, double *fx,
+                                  double *fy);
+
+/* TimeVaryingThermalForce::forceS(t) (TimeVaryingThermalForce.cpp:14-42):
+ * heat = scale*t + offset, then the thermal force with Philox draws of
+ * counter domain 4. */
+void oracle_tvthermal_add(int64_t n, double scale, double offset, double t, uint64_t seed,
+                          uint64_t step, int stage, double *fx, double *fy);
+
 /* Integrator::modifyTimeStep (Integrator.cpp:68-134, 144-182), serial, with
  * the reference's exact pair coverage.  n must be a multiple of 4. */
 double oracle_modify_time_step(int64_t n, const double *x, const double *y,
@@ -139,8 +219,10 @@ int oracle_dtcheck_covers(int64_t i, int64_t j);
 /* Runge_Kutta4::moveParticles (Runge_Kutta4.cpp:23-137) driven for exactly
  * `steps` steps (the while-loop body, with the dt check when enabled).
  * State arrays are updated in place; *t is advanced by each step's dt.
- * Forces are summed in the driver's order: Confinement, Drag, Magnetic,
- * ShieldedCoulomb, Thermal, TimeVaryingDrag (driver.cpp:372-396); stage s
+ * Forces are summed in the driver's construction order (driver.cpp:372-402):
+ * Confinement, ConfinementVoid, Drag, Driving, Magnetic, RectConfinement,
+ * Rotational, ShieldedCoulomb, Thermal, ThermalLocalized, TimeVaryingDrag,
+ * TimeVaryingThermal, Electric, Gravitational, VertElectric; stage s
  * runs at t, t + dt/2, t + dt/2, t + dt (Runge_Kutta4.cpp:31,50,69,88).
  * Returns 0 or -1 (bad n). */
 int oracle_rk4_steps(int64_t n, double *x, double *y, double *vx, double *vy,

@@ -6,13 +6,22 @@
 // The reference sources are not copied: oracle/Makefile compiles them in place.
 #include "Cloud.h"
 #include "ConfinementForce.h"
+#include "ConfinementForceVoid.h"
 #include "DragForce.h"
+#include "DrivingForce.h"
+#include "ElectricForce.h"
+#include "GravitationalForce.h"
 #include "MagneticForce.h"
+#include "RectConfinementForce.h"
+#include "RotationalForce.h"
 #include "Runge_Kutta2.h"
 #include "Runge_Kutta4.h"
 #include "ShieldedCoulombForce.h"
 #include "ThermalForce.h"
+#include "ThermalForceLocalized.h"
 #include "TimeVaryingDragForce.h"
+#include "TimeVaryingThermalForce.h"
+#include "VertElectricForce.h"
 
 #include <cmath>
 #include <cstdint>
@@ -67,9 +76,112 @@ struct RK4Probe : public Runge_Kutta4 {
     double check(float dist, double dt) const { return modifyTimeStep(dist, dt); }
 };
 
+// Parameter vector of ref_rk_all / ref_force_all.
+enum RefPar {
+    kKappa, kConfine, kGamma, kHeat, kBField, kDragScale, kConfineX, kConfineY, kVoidDecay,
+    kRotInner, kRotOuter, kRotConst, kDriveConst, kDriveAmplitude, kDriveShift, kElectric,
+    kPlasmaRadius, kVertElectric, kVertDecay, kGravity, kHeatIn, kHeatOut, kHeatRadius,
+    kHeatScale, kHeatOffset, kNumPar
+};
+
+// Every force of `forces` with the driver's constructors, in its
+// construction order (driver.cpp:372-402).
+void make_forces(Cloud *c, int forces, const double *p, ForceArray &fa) {
+    if (forces & ConfinementForceFlag) fa.push_back(new ConfinementForce(c, p[kConfine]));
+    if (forces & ConfinementForceVoidFlag) fa.push_back(new ConfinementForceVoid(c, p[kConfine], p[kVoidDecay]));
+    if (forces & DragForceFlag) fa.push_back(new DragForce(c, p[kGamma]));
+    if (forces & DrivingForceFlag)
+        fa.push_back(new DrivingForce(c, p[kDriveConst], p[kDriveAmplitude], p[kDriveShift]));
+    if (forces & MagneticForceFlag) fa.push_back(new MagneticForce(c, p[kBField]));
+    if (forces & RectConfinementForceFlag) fa.push_back(new RectConfinementForce(c, p[kConfineX], p[kConfineY]));
+    if (forces & RotationalForceFlag) fa.push_back(new RotationalForce(c, p[kRotInner], p[kRotOuter], p[kRotConst]));
+    if (forces & ShieldedCoulombForceFlag) fa.push_back(new ShieldedCoulombForce(c, p[kKappa]));
+    if (forces & ThermalForceFlag) fa.push_back(new ThermalForce(c, p[kHeat]));
+    if (forces & ThermalForceLocalizedFlag)
+        fa.push_back(new ThermalForceLocalized(c, p[kHeatIn], p[kHeatOut], p[kHeatRadius]));
+    if (forces & TimeVaryingDragForceFlag) fa.push_back(new TimeVaryingDragForce(c, p[kDragScale], p[kGamma]));
+    if (forces & TimeVaryingThermalForceFlag)
+        fa.push_back(new TimeVaryingThermalForce(c, p[kHeatScale], p[kHeatOffset]));
+    if (forces & ElectricForceFlag) fa.push_back(new ElectricForce(c, p[kElectric], p[kPlasmaRadius]));
+    if (forces & GravitationalForceFlag) fa.push_back(new GravitationalForce(c, p[kGravity]));
+    if (forces & VertElectricForceFlag) fa.push_back(new VertElectricForce(c, p[kVertElectric], p[kVertDecay]));
+}
+
 }  // namespace
 
 extern "C" {
+
+int ref_num_par(void) { return kNumPar; }
+
+// `steps` steps of Runge_Kutta4 (order 4) or Runge_Kutta2 (order 2) with
+// every force of `forces` (par: RefPar order).  State updated in place.
+int ref_rk_all(int order, int64_t n, double *x, double *y, double *vx, double *vy, const double *q,
+               const double *m, const double *par, int forces, double dt, int64_t steps, double *t,
+               int nthreads) {
+    if (n < 4 || (n & 3) || (order != 2 && order != 4)) return -1;
+    Cloud *c = make_cloud(n, x, y, vx, vy, q, m);
+    set_threads(nthreads);
+    ForceArray fa;
+    make_forces(c, forces, par, fa);
+    if (order == 4) {
+        Runge_Kutta4 I(c, fa, dt, *t);
+        for (int64_t s = 0; s < steps; ++s) I.moveParticles(std::nextafter(I.currentTime, INFINITY));
+        *t = I.currentTime;
+    } else {
+        Runge_Kutta2 I(c, fa, dt, *t);
+        for (int64_t s = 0; s < steps; ++s) I.moveParticles(std::nextafter(I.currentTime, INFINITY));
+        *t = I.currentTime;
+    }
+    for (Force *f : fa) delete f;
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = c->x[i];
+        y[i] = c->y[i];
+        vx[i] = c->Vx[i];
+        vy[i] = c->Vy[i];
+    }
+    delete c;
+    return 0;
+}
+
+// The forces of `forces` evaluated `reps` times on one state at time t:
+// repetition k calls force<k % 4 + 1>(t) on the same force objects (every
+// stage reads the given state; the thermal forces draw afresh each call) and
+// writes the sum into fx/fy[k][n].
+int ref_force_all(int64_t n, const double *xs, const double *ys, const double *vxs,
+                  const double *vys, const double *q, const double *m, const double *par,
+                  int forces, double t, int64_t reps, double *fx, double *fy) {
+    if (n < 2 || (n & 1) || reps < 1) return -1;
+    Cloud *c = make_cloud(n, xs, ys, vxs, vys, q, m);
+    for (int64_t i = 0; i < n; i += 2) {
+        c->xCache[i / 2] = _mm_set_pd(xs[i + 1], xs[i]);
+        c->yCache[i / 2] = _mm_set_pd(ys[i + 1], ys[i]);
+        c->VxCache[i / 2] = _mm_set_pd(vxs ? vxs[i + 1] : 0.0, vxs ? vxs[i] : 0.0);
+        c->VyCache[i / 2] = _mm_set_pd(vys ? vys[i + 1] : 0.0, vys ? vys[i] : 0.0);
+    }
+    ForceArray fa;
+    make_forces(c, forces, par, fa);
+    for (int64_t k = 0; k < reps; ++k) {
+        for (int64_t i = 0; i < n; ++i) {
+            c->forceX[i] = 0.0;
+            c->forceY[i] = 0.0;
+        }
+        for (Force *f : fa) {
+            switch (k % 4) {
+                case 0: f->force1(t); break;
+                case 1: f->force2(t); break;
+                case 2: f->force3(t); break;
+                default: f->force4(t); break;
+            }
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            fx[(size_t)k * n + i] = c->forceX[i];
+            fy[(size_t)k * n + i] = c->forceY[i];
+        }
+    }
+    for (Force *f : fa) delete f;
+    delete c;
+    return 0;
+}
 
 int ref_is_openmp(void) {
 #ifdef _OPENMP
@@ -306,6 +418,41 @@ int ref_fits_read(const char *path, int64_t n, double *x, double *y, double *vx,
         q[i] = c->charge[i];
         m[i] = c->mass[i];
     }
+    delete c;
+    return error;
+}
+
+// The reference's readForce of every force in in_path's FORCES (constructed as
+// the driver's -c does, driver.cpp:337-408), then the same objects' writeForce
+// into a new out_path: a file whose keywords the reference reads back
+// unchanged.  Returns the cfitsio status (nonzero when a keyword is missing).
+int ref_fits_forces_roundtrip(const char *in_path, const char *out_path) {
+    int error = 0;
+    fitsfile *file = nullptr;
+    long forces = 0;
+    fits_open_file(&file, in_path, READONLY, &error);
+    fits_read_key_lng(file, const_cast<char *>("FORCES"), &forces, NULL, &error);
+    if (error) return error;
+    double t = 0.0;
+    Cloud *c = Cloud::initializeFromFile(file, error, &t);
+    if (error) {
+        delete c;
+        return error;
+    }
+    fits_movabs_hdu(file, 1, NULL, &error);
+    double par[kNumPar] = {};
+    par[kKappa] = 1.0;
+    ForceArray fa;
+    make_forces(c, (int)forces, par, fa);
+    for (Force *f : fa) f->readForce(file, &error);
+    fits_close_file(file, &error);
+    fitsfile *out = nullptr;
+    const std::string name = std::string("!") + out_path;
+    if (!error) fits_create_file(&out, name.c_str(), &error);
+    if (!error) fits_create_img(out, 16, 0, NULL, &error);
+    for (Force *f : fa) f->writeForce(out, &error);
+    if (out) fits_close_file(out, &error);
+    for (Force *f : fa) delete f;
     delete c;
     return error;
 }

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -82,13 +83,24 @@ int main(int argc, char **argv) {
             ShieldedCoulombForce yuk(cloud.get(), 2000.0);
             ThermalForce therm(cloud.get(), 3e-15);
             TimeVaryingDragForce tvd(cloud.get(), -1.5, 4.0);
+            Cloud *c = cloud.get();
+            ConfinementForceVoid voidf(c, 52018.31, 0.4);
+            DrivingForce drive(c, 1e-5, 1e-13, 0.007);
+            RectConfinementForce rect(c, 100.5, 1000.25);
+            RotationalForce rot(c, 0.015, 0.03, 1e-15);
+            ThermalForceLocalized local(c, 2e-15, 1e-15, 0.001);
+            TimeVaryingThermalForce tvtherm(c, 1e-14, 5e-16);
+            ElectricForce elec(c, 30.0, 1.5);
+            GravitationalForce grav(c, -9.81);
+            VertElectricForce vert(c, 12.5, 0.75);
             ForceArray fa;
-            if (flags & ConfinementForceFlag) fa.push_back(&conf);
-            if ((flags & DragForceFlag) && !(flags & TimeVaryingDragForceFlag)) fa.push_back(&drag);
-            if (flags & MagneticForceFlag) fa.push_back(&mag);
-            if (flags & ShieldedCoulombForceFlag) fa.push_back(&yuk);
-            if (flags & ThermalForceFlag) fa.push_back(&therm);
-            if (flags & TimeVaryingDragForceFlag) fa.push_back(&tvd);
+            // the driver's construction order (driver.cpp:372-402)
+            for (Force *f : std::initializer_list<Force *>{&conf, &voidf, &drag, &drive, &mag, &rect, &rot, &yuk,
+                                                          &therm, &local, &tvd, &tvtherm, &elec, &grav, &vert}) {
+                if (!(flags & f->flag())) continue;
+                if (f == &drag && (flags & TimeVaryingDragForceFlag)) continue;
+                fa.push_back(f);
+            }
             long fl = 0;
             std::vector<FitsKey> keys;
             for (const Force *f : fa) f->writeForce(fl, keys);

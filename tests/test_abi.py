@@ -29,16 +29,22 @@ def test_every_declared_symbol_is_exported_and_bound():
 
 
 def test_versions():
-    assert dm.lib.demon_abi_version() == 2
+    assert dm.lib.demon_abi_version() == 3
     assert b"gfx950" in dm.lib.demon_version()
 
 
 def test_struct_layouts_match_header():
-    # demon_params: 4 doubles, 1 uint64, 4 int32, 2 doubles ; demon_stats: 3 int64, 2 doubles
-    assert ctypes.sizeof(_lib.Params) == 4 * 8 + 8 + 4 * 4 + 2 * 8
+    # demon_params: 4 doubles, 1 uint64, 4 int32, 2 + 19 doubles ; demon_stats: 3 int64, 2 doubles
+    assert ctypes.sizeof(_lib.Params) == 4 * 8 + 8 + 4 * 4 + 21 * 8
     assert ctypes.sizeof(_lib.Stats) == 5 * 8
     assert _lib.Params.forces.offset == 40 and _lib.Params.device.offset == 52
     assert _lib.Params.magnetic_field.offset == 56 and _lib.Params.drag_scale.offset == 64
+    assert _lib.Params.confine_x.offset == 72 and _lib.Params.heat_offset.offset == 216
+    # the header declares the further plugins' constants in this order
+    hdr = (Path(__file__).resolve().parents[1] / "include" / "demon_gpu.h").read_text()
+    body = hdr[hdr.index("typedef struct demon_params"):hdr.index("} demon_params;")]
+    pos = [body.index(name) for name in _lib.PLUGIN_PARAMS]
+    assert pos == sorted(pos)
 
 
 def test_create_rejects_bad_arguments_before_touching_the_device():
@@ -46,8 +52,11 @@ def test_create_rejects_bad_arguments_before_touching_the_device():
     h = ctypes.c_void_p()
     assert dm.lib.demon_create(ctypes.byref(h), 1, ctypes.byref(p)) == _lib.ERR_ARG
     assert b"n must be" in dm.lib.demon_last_error(None)
-    bad = _lib.Params(2000.0, 1.0, 2.0, 0.0, 0, 7 | 2048, 0, 1, 0)  # ConfinementForceVoid flag
+    bad = _lib.Params(2000.0, 1.0, 2.0, 0.0, 0, 7 | 32768, 0, 1, 0)  # no such ForceFlag
     assert dm.lib.demon_create(ctypes.byref(h), 16, ctypes.byref(bad)) == _lib.ERR_UNSUPPORTED
+    nan = _lib.Params(2000.0, 1.0, 2.0, 0.0, 0, 7 | 2048, 0, 1, 0)
+    nan.void_decay = float("nan")
+    assert dm.lib.demon_create(ctypes.byref(h), 16, ctypes.byref(nan)) == _lib.ERR_ARG
     neg = _lib.Params(-1.0, 1.0, 2.0, 0.0, 0, 7, 0, 1, 0)
     assert dm.lib.demon_create(ctypes.byref(h), 16, ctypes.byref(neg)) == _lib.ERR_ARG
     prec = _lib.Params(2000.0, 1.0, 2.0, 0.0, 0, 7, 5, 1, 0)

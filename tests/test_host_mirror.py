@@ -213,3 +213,106 @@ def test_driver_fits_output_and_continue(tmp_path):
     if Reference.available():
         x, y, vx, vy, q, m, rforces, par, rt = Reference().fits_read(tmp_path / "part.fits", n)
         assert rforces == forces and rt == t and np.array_equal(x, long_last[:, 2])
+
+
+def _primary_keys(path):
+    """FORCES and the numeric keywords of a FITS primary header (upper case)."""
+    data = Path(path).read_bytes()
+    keys = {}
+    for off in range(0, len(data), 80):
+        card = data[off:off + 80].decode("ascii")
+        if card.startswith("END"):
+            break
+        body = card[9:] if card.startswith("HIERARCH ") else card
+        if "=" not in body:
+            continue
+        name, rest = body.split("=", 1)
+        value = rest.split("/")[0].strip()
+        try:
+            keys.setdefault(name.strip().upper(), float(value))
+        except ValueError:
+            pass
+    return keys
+
+
+PLUGIN_FLAGS = [
+    2048 | 2 | 4 | 64 | 128 | 32 | 8,              # -V -w -S -L -G
+    8192 | 256 | 4 | 1024 | 4096 | 16384 | 512,    # -R -D -B -E -F -v
+]
+
+
+@needs_ref
+@pytest.mark.parametrize("flags", PLUGIN_FLAGS)
+def test_plugin_keywords_read_by_the_reference(tmp_path, flags):
+    """The further plugins' writeForce keywords (host library) are what the
+    reference's readForce looks for: the reference reads every one of them
+    (cfitsio fails on a missing keyword) and writes back the same values."""
+    path = tmp_path / "ours.fits"
+    run("fits_write", path, 48, 2, flags)
+    n, rows, t, forces, keys, arr = _read_ours(path)
+    # ConfinementForceVoid writes the Confinement bit too, TimeVaryingThermal
+    # the Thermal bit, TimeVaryingDrag the Drag bit (their writeForce)
+    extra = (1 if flags & 2048 else 0) | (16 if flags & 512 else 0) | (2 if flags & 256 else 0)
+    assert forces == flags | extra
+    back = tmp_path / "back.fits"
+    Reference().fits_forces_roundtrip(path, back)
+    ref_keys = _primary_keys(back)
+    assert ref_keys.pop("FORCES") == forces
+    assert {k for k in ref_keys if k not in ("SIMPLE", "BITPIX", "NAXIS", "EXTEND")} == {k.upper() for k in keys}
+    for k, v in keys.items():
+        assert ref_keys[k.upper()] == v, k
+    expect = {
+        2048: {"decay": 0.4, "confineconst": float("%.6E" % 52018.31)},
+        64: {"drivingamplitude": 1e-13, "drivingconst": -1e-5, "drivingshift": 0.007},
+        128: {"rotationalconst": 1e-15, "innerradius": 0.015, "outerradius": 0.03},
+        32: {"heatingvalue1": 2e-15, "heatingvalue2": 1e-15, "heatingradius": 0.001},
+        8: {"gravitationalfieldstrength": -9.81},
+        8192: {"confineconstx": 100.5, "confineconsty": 1000.25},
+        4096: {"electricfieldstrength": 30.0, "plasmaradius": 1.5},
+        16384: {"vertelectricfieldstrength": 12.5, "verticaldecay": 0.75},
+        512: {"heatingvaluescale": 1e-14, "heatingvalueoffset": 5e-16, "heatingvalue": 5e-16},
+    }
+    for bit, want in expect.items():
+        if flags & bit:
+            for k, v in want.items():
+                assert keys[k] == v, k
+
+
+@pytest.mark.gpu
+def test_driver_plugin_options(tmp_path):
+    """The driver's plugin options (-V -S -w -G -R -E -F) build the forces in
+    the reference's order: a run continued from its file (-c) with those
+    forces equals one long run, and the file carries every plugin's bits and
+    keywords; -L and -v (thermal) run and write theirs."""
+    common = ["--hex", "256", "-t", "1e-3", "-o", "0.01", "-V", "300", "-S", "2e-14", "2e-3", "6e-3",
+              "-w", "2e-14", "7e-3", "1e-5", "-G", "-0.5", "-R", "5e4", "1e5", "-E", "-2e4", "5e-3",
+              "-F", "20", "8e-3"]
+
+    def demon(*args, extra=common):
+        r = subprocess.run([str(DEMON), *extra, *map(str, args)], capture_output=True, text=True, timeout=300,
+                           cwd=tmp_path)
+        assert r.returncode == 0, r.stdout + r.stderr
+
+    demon("-e", "0.04", "-O", "long.fits")
+    demon("-e", "0.02", "-O", "part.fits")
+    demon("-c", "part.fits", "-e", "0.04", extra=["--hex", "256", "-t", "1e-3", "-o", "0.01"])
+    n, rows, t, forces, keys, long_last = _read_ours(tmp_path / "long.fits")
+    # ConfinementVoid and Rect replace Confinement; Void writes its bit too
+    assert forces == 2048 | 1 | 2 | 64 | 128 | 8192 | 4 | 4096 | 8 | 16384
+    assert keys["decay"] == 300.0 and keys["rotationalconst"] == 2e-14 and keys["drivingconst"] == -1e-5
+    assert keys["confineconstx"] == 5e4 and keys["electricfieldstrength"] == -2e4 and keys["verticaldecay"] == 8e-3
+    n2, rows2, t2, forces2, keys2, part_last = _read_ours(tmp_path / "part.fits")
+    assert rows2 == rows == 5 and t2 == t and forces2 == forces and keys2 == keys
+    assert np.array_equal(part_last, long_last)
+    assert np.all(np.isfinite(long_last))
+    demon("-e", "0.02", "-O", "heat.fits", extra=["--hex", "256", "-t", "1e-3", "-o", "0.01", "-L", "4e-3", "3e-15",
+                                                  "1e-15", "-B", "0.5"])
+    _, _, _, f3, k3, last3 = _read_ours(tmp_path / "heat.fits")
+    assert f3 == 1 | 2 | 4 | 32 | 1024 and k3["heatingvalue1"] == 3e-15 and k3["heatingradius"] == 4e-3
+    demon("-e", "0.02", "-O", "tvheat.fits", extra=["--hex", "256", "-t", "1e-3", "-o", "0.01", "-v", "1e-13", "1e-15"])
+    _, _, _, f4, k4, last4 = _read_ours(tmp_path / "tvheat.fits")
+    assert f4 == 1 | 2 | 4 | 16 | 512 and k4["heatingvaluescale"] == 1e-13 and k4["heatingvalue"] == 1e-15
+    assert np.all(np.isfinite(last3)) and np.all(np.isfinite(last4))
+    r = subprocess.run([str(DEMON), "-T", "1e-15", "-L", "1e-3", "1e-15", "1e-15"], capture_output=True, text=True,
+                       timeout=60, cwd=tmp_path)
+    assert r.returncode != 0 and "conflicts" in r.stderr

@@ -396,3 +396,167 @@ def test_rk2_step_is_midpoint_polynomial(oracle):
         ex, ey = T @ np.array([x[i], vx[i]]), T @ np.array([y[i], vy[i]])
         assert o[0][i] == pytest.approx(ex[0], rel=1e-13) and o[2][i] == pytest.approx(ex[1], rel=1e-12)
         assert o[1][i] == pytest.approx(ey[0], rel=1e-13) and o[3][i] == pytest.approx(ey[1], rel=1e-12)
+
+
+# --- the further force plugins ---------------------------------------------
+
+RECT, VOID, ROT, DRIVE, ELEC, VERT, GRAV = 8192, 2048, 128, 64, 4096, 16384, 8
+TH_LOCAL, TV_THERMAL = 32, 512
+
+
+def plugin_params(forces, kappa=3000.0, seed=0):
+    """Constants that make each further plugin comparable to the confinement
+    force on a mm-sized cloud of the test particles."""
+    op = OracleParams(kappa, CONFINE, 2.0, 0.0, seed, forces, 1, 0, 0.5, -3.0)
+    op.confine_x, op.confine_y = CONFINE, 3 * CONFINE
+    op.void_decay = 2000.0
+    op.rot_inner, op.rot_outer, op.rot_const = 5e-4, 1.5e-3, 1e-13
+    op.drive_const, op.drive_amplitude, op.drive_shift = 1e-5, 1e-13, 7e-3
+    op.electric_field, op.plasma_radius = -CONFINE, 2e-3
+    op.vert_electric_field, op.vert_decay = 1e2, 3e-3
+    op.gravity = -9.8
+    return op
+
+
+def plugin_cloud(rng, n):
+    x = rng.uniform(-2e-3, 2e-3, n)
+    y = rng.uniform(-2e-3, 2e-3, n)
+    vx, vy = rng.normal(size=(2, n)) * 1e-4
+    q = Q0 * rng.uniform(0.5, 1.5, n)
+    m = M0 * rng.uniform(0.8, 1.2, n)
+    return x, y, vx, vy, q, m
+
+
+@pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("forces", [RECT, VOID, ROT, DRIVE, ELEC, VERT, GRAV, 1 | VOID])
+def test_plugin_force_bitwise_vs_live_reference(oracle, forces):
+    """Each plugin's forceS(t) on one state, the reference's own class against
+    the oracle, bit for bit (the same libm exp/sin/sqrt on both sides)."""
+    ref = Reference()
+    rng = np.random.default_rng(forces)
+    x, y, vx, vy, q, m = plugin_cloud(rng, 48)
+    op = plugin_params(forces)
+    fo = oracle.stage_force(x, y, vx, vy, q, m, op, t=0.37)
+    fx, fy = ref.force_all(x, y, vx, vy, q, m, op, t=0.37, reps=4)
+    assert np.abs(fo[0]).max() + np.abs(fo[1]).max() > 1e-14
+    for k in range(4):
+        assert bits_equal(fo[0], fx[k]) and bits_equal(fo[1], fy[k]), k
+
+
+@pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (needs the reference sources)")
+@pytest.mark.parametrize("order", [4, 2])
+def test_plugin_trajectories_bitwise_vs_live_reference(oracle, order):
+    """All deterministic plugins inside Runge_Kutta4 / Runge_Kutta2 with the dt
+    check, in the driver's construction order, against the reference."""
+    ref = Reference()
+    rng = np.random.default_rng(40 + order)
+    for forces in (VOID | 2 | 4 | DRIVE | ROT | GRAV, RECT | 2 | 4 | ELEC | VERT | MAGNETIC,
+                   1 | VOID | 4 | TV_DRAG | GRAV):
+        x, y, vx, vy, q, m = plugin_cloud(rng, 4 * int(rng.integers(3, 16)))
+        op = plugin_params(forces, kappa=float(rng.uniform(500, 8000)))
+        run = oracle.rk4_steps if order == 4 else oracle.rk2_steps
+        o = run(x, y, vx, vy, q, m, op, 1e-4, 6, t=0.25)
+        r = ref.rk_all(x, y, vx, vy, q, m, op, 1e-4, 6, t=0.25, order=order)
+        for a, b in zip(o[:4], r[:4]):
+            assert bits_equal(a, b), forces
+        assert o[4] == r[4]
+
+
+def test_plugin_closed_forms(oracle):
+    """Rect is the linear field q (cX x, cY y); Void adds the radial term
+    -q d e^{-d r} r_hat to the confinement; Rotational is tangential with
+    magnitude c inside the shear layer and zero outside; Gravitational is
+    m g along y; Driving is A sin(k x - w t) along x; Electric and
+    VertElectric are the exponentially decaying fields."""
+    rng = np.random.default_rng(3)
+    x, y, vx, vy, q, m = plugin_cloud(rng, 64)
+    z = np.zeros(64)
+
+    def force(f, t=0.0):
+        return oracle.stage_force(x, y, vx, vy, q, m, plugin_params(f), t=t)
+
+    r = np.hypot(x, y)
+    fx, fy = force(RECT)
+    np.testing.assert_allclose(fx, q * CONFINE * x, rtol=1e-15)
+    np.testing.assert_allclose(fy, q * 3 * CONFINE * y, rtol=1e-15)
+    fx, fy = force(VOID)
+    d = 2000.0
+    np.testing.assert_allclose(fx, q * CONFINE * x - q * d * np.exp(-d * r) * x / r, rtol=1e-13)
+    np.testing.assert_allclose(fy, q * CONFINE * y - q * d * np.exp(-d * r) * y / r, rtol=1e-13)
+    fx, fy = force(ROT)
+    inside = (r > 5e-4) & (r < 1.5e-3)
+    assert inside.any() and (~inside).any()
+    np.testing.assert_allclose(np.hypot(fx, fy)[inside], 1e-13, rtol=1e-15)
+    np.testing.assert_allclose((fx * x + fy * y)[inside], 0.0, atol=1e-30)
+    assert np.all(fx[~inside] == 0) and np.all(fy[~inside] == 0)
+    assert np.all((x * fy - y * fx)[inside] > 0)   # counter-clockwise for c > 0
+    fx, fy = force(GRAV)
+    assert np.all(fx == 0) and np.array_equal(fy, m * -9.8)
+    fx, fy = force(DRIVE, t=0.013)
+    np.testing.assert_allclose(fx, 1e-13 * np.sin(2 * np.pi / 0.002 * x - 2 * np.pi * 10 * 0.013),
+                               rtol=1e-12, atol=1e-27)
+    assert np.all(fy == 0)
+    fx, fy = force(ELEC)
+    np.testing.assert_allclose(fx, q * -CONFINE * np.exp(-r / 2e-3) * x, rtol=1e-14)
+    fx, fy = force(VERT)
+    assert np.all(fx == 0)
+    np.testing.assert_allclose(fy, q * 1e2 * np.exp(y / 3e-3), rtol=1e-14)
+    # a zero damping constant makes the reference's 0/0 a NaN (DrivingForce.cpp:57)
+    op = plugin_params(DRIVE)
+    op.drive_const = 0.0
+    assert np.isnan(oracle.stage_force(x, y, z, z, q, m, op)[0]).all()
+
+
+@pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_thermal_localized_selection_vs_live_reference(oracle):
+    """ThermalForceLocalized uses heat_in strictly inside the radius (on the
+    stage position) and heat_out elsewhere: with heat_in << heat_out the
+    largest of 16 draws classifies each particle, identically for the
+    reference's classes and the oracle; a particle exactly on the circle is
+    outside.  The magnitude law E|F|^2 = h^2/3 holds for both."""
+    ref = Reference()
+    rng = np.random.default_rng(6)
+    n, R = 400, 1e-3
+    x, y = rng.uniform(-2e-3, 2e-3, (2, n))
+    x[0], y[0] = R, 0.0
+    x[1], y[1] = np.nextafter(R, 0.0), 0.0
+    z = np.zeros(n)
+    q, m = np.full(n, Q0), np.full(n, M0)
+    op = OracleParams(3000.0, 0.0, 0.0, 0.0, 99, TH_LOCAL, 0, 0)
+    op.heat_in, op.heat_out, op.heat_radius = 1e-14, 1e-11, R
+    fx, fy = ref.force_all(x, y, z, z, q, m, op, reps=16)
+    mag_r = np.hypot(fx, fy).max(axis=0)
+    draws = [oracle.stage_force(x, y, z, z, q, m, op, step=s, stage=1 + s % 4) for s in range(16)]
+    mag_o = np.max([np.hypot(a, b) for a, b in draws], axis=0)
+    inside = np.hypot(x, y) < R
+    assert not inside[0] and inside[1]
+    assert np.array_equal(mag_r < 1e-14, inside)
+    assert np.array_equal(mag_o < 1e-14, inside)
+    for f2 in ((fx ** 2 + fy ** 2)[:, ~inside], np.array([a ** 2 + b ** 2 for a, b in draws])[:, ~inside]):
+        assert f2.mean() / 1e-22 == pytest.approx(1 / 3, rel=0.05)
+
+
+@pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (needs the reference sources)")
+def test_tv_thermal_magnitude_at_stage_time_vs_live_reference(oracle):
+    """TimeVaryingThermalForce at time t draws with magnitude h = s t + o:
+    zero where h(t) = 0, bounded by |h| and E|F|^2 = h^2/3 otherwise, for the
+    reference's classes and the oracle alike; RK4 stage times as TimeVaryingDrag."""
+    ref = Reference()
+    rng = np.random.default_rng(7)
+    n = 400
+    x, y = rng.uniform(-2e-3, 2e-3, (2, n))
+    z = np.zeros(n)
+    q, m = np.full(n, Q0), np.full(n, M0)
+    op = OracleParams(3000.0, 0.0, 0.0, 0.0, 5, TV_THERMAL, 0, 0)
+    op.heat_scale, op.heat_offset = 4e-12, -1e-12
+    fx, fy = ref.force_all(x, y, z, z, q, m, op, t=0.25, reps=4)
+    assert np.all(fx == 0) and np.all(fy == 0)
+    fo = oracle.stage_force(x, y, z, z, q, m, op, t=0.25)
+    assert np.all(fo[0] == 0) and np.all(fo[1] == 0)
+    for t in (1.0, 2.5):
+        h = 4e-12 * t - 1e-12
+        fx, fy = ref.force_all(x, y, z, z, q, m, op, t=t, reps=20)
+        draws = np.array([np.hypot(*oracle.stage_force(x, y, z, z, q, m, op, step=s, t=t)) for s in range(20)])
+        for mag in (np.hypot(fx, fy), draws):
+            assert mag.max() <= abs(h)
+            assert (mag ** 2).mean() / h ** 2 == pytest.approx(1 / 3, rel=0.03)
