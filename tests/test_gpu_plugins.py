@@ -92,7 +92,7 @@ def test_elementwise_plugins(oracle, forces):
             fx, fy = oracle.stage_force(x, y, vx, vy, q, m, oracle_params(forces), step=2, stage=stage, t=t)
             assert np.abs(fx).max() + np.abs(fy).max() > 0
             assert force_rel_err(fx, fy, gx, gy, scale) <= ELEM_TOL, stage
-    if forces & dm.ROTATIONAL:
+    if forces == dm.ROTATIONAL:
         r = np.hypot(x, y)
         outside = (r <= PLUGIN["rot_inner"]) | (r >= PLUGIN["rot_outer"])
         assert outside.any() and np.all(gx[outside] == 0) and np.all(gy[outside] == 0)
