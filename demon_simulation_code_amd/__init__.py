@@ -34,11 +34,12 @@ from ._lib import (  # noqa: F401
     Stats,
     hex_lattice,
     lib,
+    trajectory_spectrum,
 )
 from .physics import ELECTRON_CHARGE, EPSILON0, COULOMB, confine_from_k  # noqa: F401
 
 __all__ = [
-    "Batch", "Context", "DemonError", "Params", "Stats", "hex_lattice", "lib", "LIB_PATH",
+    "Batch", "Context", "DemonError", "Params", "Stats", "hex_lattice", "lib", "LIB_PATH", "trajectory_spectrum",
     "CONFINEMENT", "DRAG", "SHIELDED_COULOMB", "THERMAL", "TIME_VARYING_DRAG", "MAGNETIC", "FP64", "FP32_PAIR",
     "BATCH_NEWTON3", "PAIRS_GATHER", "PAIRS_NEWTON3", "GRAVITATIONAL", "THERMAL_LOCALIZED", "DRIVING",
     "ROTATIONAL", "TIME_VARYING_THERMAL", "CONFINEMENT_VOID", "ELECTRIC", "RECT_CONFINEMENT", "VERT_ELECTRIC",

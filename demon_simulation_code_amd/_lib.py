@@ -135,6 +135,9 @@ SIGNATURES = {
     "demon_batch_set_shape": (ctypes.c_int, [_CTX, _I32, _I32, _I32]),
     "demon_batch_get_shape": (ctypes.c_int, [_CTX, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _D]),
     "demon_batch_tune": (ctypes.c_int, [_CTX, _I32]),
+    "demon_kinetic_energy": (ctypes.c_int, [_CTX, _D, _D]),
+    "demon_pair_histogram": (ctypes.c_int, [_CTX, ctypes.c_double, _I32, ctypes.POINTER(ctypes.c_uint64)]),
+    "demon_trajectory_spectrum": (ctypes.c_int, [_D, _I64, _I64, _I64, _D, _I32]),
 }
 
 
@@ -175,6 +178,17 @@ def hex_lattice(n: int, spacing: float = 500e-6, jitter: float = 50e-6, seed: in
     y = np.empty(n)
     _check(lib.demon_hex_lattice(n, spacing, jitter, seed, _ptr(x), _ptr(y)))
     return x, y
+
+
+def trajectory_spectrum(series, nuse: int, device: int = 0) -> np.ndarray:
+    """FFTAnalysis -a: series [rows, stride] of X positions (one row per
+    snapshot); amplitude |DFT of the sum of the first nuse columns| / nuse for
+    the rows // 2 lowest frequencies."""
+    series = np.ascontiguousarray(series, dtype=np.float64)
+    rows, stride = series.shape
+    amp = np.zeros(rows // 2)
+    _check(lib.demon_trajectory_spectrum(_ptr(series), rows, stride, int(nuse), _ptr(amp), device))
+    return amp
 
 
 class Context:
@@ -244,6 +258,21 @@ class Context:
             self._c(lib.demon_force_stage_at(self._h, forces, stage, step, t, _ptr(xs), _ptr(ys), _ptr(vxs),
                                              _ptr(vys), _ptr(fx), _ptr(fy)))
         return fx, fy
+
+    def kinetic_energy(self):
+        """(total, thermal) kinetic energy of the current state [J]; the 2D
+        kinetic temperature is thermal / (n k_B)."""
+        tot, th = ctypes.c_double(), ctypes.c_double()
+        self._c(lib.demon_kinetic_energy(self._h, ctypes.byref(tot), ctypes.byref(th)))
+        return tot.value, th.value
+
+    def pair_histogram(self, r_max: float, nbins: int) -> np.ndarray:
+        """Unordered pair counts of the current positions in nbins bins of
+        squared distance below r_max (uint64)."""
+        out = np.zeros(int(nbins), dtype=np.uint64)
+        self._c(lib.demon_pair_histogram(self._h, float(r_max), int(nbins),
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+        return out
 
     def rk4_steps(self, steps: int, dt: float, t: float = 0.0) -> float:
         tt = ctypes.c_double(t)

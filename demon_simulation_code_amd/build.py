@@ -60,8 +60,8 @@ def cxx() -> str:
 
 
 def build_host(verbose: bool = False) -> list[Path]:
-    """The host C++ mirror (libdemon_host.so) and the `demon` driver, linked
-    against libdemon_gpu.so with an $ORIGIN rpath."""
+    """The host C++ mirror (libdemon_host.so), the `demon` driver and the
+    `demon_fft` analysis tool, linked against libdemon_gpu.so with an $ORIGIN rpath."""
     inc = [f"-I{INCLUDE}", f"-I{HOST}"]
     rpath = "-Wl,-rpath,$ORIGIN"
     lib = PKG / HOST_LIB
@@ -69,6 +69,8 @@ def build_host(verbose: bool = False) -> list[Path]:
         [cxx(), *CXX_FLAGS, *inc, "-shared", "-o", str(lib), str(HOST / "demon_host.cpp"),
          str(HOST / "demon_fits.cpp"), f"-L{PKG}", "-ldemon_gpu", rpath],
         [cxx(), *CXX_FLAGS, *inc, "-o", str(PKG / "demon"), str(HOST / "driver.cpp"),
+         f"-L{PKG}", "-ldemon_host", "-ldemon_gpu", rpath],
+        [cxx(), *CXX_FLAGS, *inc, "-o", str(PKG / "demon_fft"), str(HOST / "fft_analysis.cpp"),
          f"-L{PKG}", "-ldemon_host", "-ldemon_gpu", rpath],
     ]
     test_src = PKG.parent / "tests" / "host" / "test_host.cpp"
@@ -81,7 +83,7 @@ def build_host(verbose: bool = False) -> list[Path]:
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"host build failed:\n{r.stdout}")
-    return [lib, PKG / "demon"]
+    return [lib, PKG / "demon", PKG / "demon_fft"]
 
 
 def build(verbose: bool = False, jobs: int = 4) -> Path:

@@ -66,6 +66,9 @@ struct demon_ctx {
     size_t fit_doubles = 0;
     double last_fwd_ms = 0.0, last_bwd_ms = 0.0;
     hipEvent_t ev_fit[3]{};
+    // analysis scratch (demon_kinetic_energy, demon_pair_histogram), grown on demand
+    double *d_an = nullptr;
+    size_t an_doubles = 0;
 };
 
 namespace {
@@ -599,6 +602,7 @@ void demon_destroy(demon_ctx *c) {
     if (c->d_fit) (void)hipFree(c->d_fit);
     if (c->d_react) (void)hipFree(c->d_react);
     if (c->d_rflag) (void)hipFree(c->d_rflag);
+    if (c->d_an) (void)hipFree(c->d_an);
     for (hipEvent_t e : c->ev_fit)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1014,6 +1018,112 @@ int demon_hex_lattice(int64_t n, double spacing, double jitter, uint64_t seed, d
         x[k] = (x[k] - cx) + jitter * (2.0 * u01(o.v[0], o.v[1]) - 1.0);
         y[k] = (y[k] - cy) + jitter * (2.0 * u01(o.v[2], o.v[3]) - 1.0);
     }
+    return DEMON_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Analysis of the current state.
+
+namespace {
+
+int analysis_scratch(demon_ctx *c, size_t doubles) {
+    if (c->an_doubles >= doubles) return DEMON_OK;
+    if (c->d_an) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipFree(c->d_an));
+        c->d_an = nullptr;
+        c->an_doubles = 0;
+    }
+    HIP_TRY(c, hipMalloc(&c->d_an, sizeof(double) * doubles));
+    c->an_doubles = doubles;
+    return DEMON_OK;
+}
+
+}  // namespace
+
+int demon_kinetic_energy(demon_ctx *c, double *total, double *thermal) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
+    if (!total && !thermal) return fail(c, DEMON_ERR_ARG, "no output");
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    const size_t part = (size_t)(c->npad / 256) * 4;
+    int rc = analysis_scratch(c, part + 8);
+    if (rc) return rc;
+    double *totals = c->d_an + part;
+    HIP_TRY(c, launch_kinetic(c->vx, c->vy, c->m, c->n, c->npad, c->d_an, totals, c->stream));
+    double h[8];
+    HIP_TRY(c, hipMemcpyAsync(h, totals, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (total) *total = 0.5 * h[3];
+    if (thermal) *thermal = 0.5 * h[4];
+    return DEMON_OK;
+}
+
+int demon_pair_histogram(demon_ctx *c, double r_max, int32_t nbins, uint64_t *counts) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
+    if (!counts) return fail(c, DEMON_ERR_ARG, "counts is NULL");
+    if (!(r_max > 0.0) || !std::isfinite(r_max)) return fail(c, DEMON_ERR_ARG, "r_max must be positive and finite");
+    if (nbins < 1 || nbins > kMaxHistBins) return fail(c, DEMON_ERR_ARG, "nbins must be in [1, 2048]");
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    // edges k * (r_max / nbins) with the last one r_max, as numpy.linspace
+    std::vector<double> e2((size_t)nbins + 1);
+    const double w = r_max / (double)nbins;
+    for (int32_t k = 0; k < nbins; ++k) {
+        const double e = (double)k * w;
+        e2[(size_t)k] = e * e;
+    }
+    e2[(size_t)nbins] = r_max * r_max;
+    const size_t need = (size_t)nbins + 1 + (size_t)nbins;  // edges2, counts
+    int rc = analysis_scratch(c, need);
+    if (rc) return rc;
+    double *d_e2 = c->d_an;
+    unsigned long long *d_counts = reinterpret_cast<unsigned long long *>(c->d_an + nbins + 1);
+    HIP_TRY(c, hipMemcpyAsync(d_e2, e2.data(), sizeof(double) * e2.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(d_counts, 0, sizeof(unsigned long long) * (size_t)nbins, c->stream));
+    PairHistLaunch a{};
+    a.x = c->x;
+    a.y = c->y;
+    a.n = c->n;
+    a.nb = (int32_t)((c->n + 255) / 256);
+    a.nbins = nbins;
+    a.edges2 = d_e2;
+    a.r2max = e2[(size_t)nbins];
+    a.inv_width = (float)((double)nbins / r_max);
+    a.counts = d_counts;
+    HIP_TRY(c, launch_pair_hist(a, c->stream));
+    static_assert(sizeof(unsigned long long) == sizeof(uint64_t), "64-bit counts");
+    HIP_TRY(c, hipMemcpyAsync(counts, d_counts, sizeof(uint64_t) * (size_t)nbins, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DEMON_OK;
+}
+
+int demon_trajectory_spectrum(const double *series, int64_t rows, int64_t stride, int64_t nuse, double *amp,
+                              int32_t device) {
+    if (!series || !amp) return fail(nullptr, DEMON_ERR_ARG, "NULL array");
+    if (rows < 1 || nuse < 1 || stride < nuse) return fail(nullptr, DEMON_ERR_ARG, "need rows >= 1, 1 <= nuse <= stride");
+    if (rows > ((int64_t)1 << 31) / 2 || nuse > ((int64_t)1 << 31)) return fail(nullptr, DEMON_ERR_ARG, "series too large");
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipSetDevice");
+    std::vector<double> packed;
+    const double *src = series;
+    if (stride != nuse) {
+        packed.resize((size_t)(rows * nuse));
+        for (int64_t t = 0; t < rows; ++t)
+            std::memcpy(&packed[(size_t)(t * nuse)], series + (size_t)(t * stride), sizeof(double) * (size_t)nuse);
+        src = packed.data();
+    }
+    const size_t nser = (size_t)(rows * nuse), half = (size_t)(rows / 2);
+    double *d = nullptr;
+    e = hipMalloc(&d, sizeof(double) * (nser + 3 * (size_t)rows + half + 1));
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipMalloc");
+    double *d_ser = d, *d_s = d + nser, *d_wre = d_s + rows, *d_wim = d_wre + rows, *d_amp = d_wim + rows;
+    e = hipMemcpy(d_ser, src, sizeof(double) * nser, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_spectrum(d_ser, rows, nuse, d_s, d_wre, d_wim, d_amp, nullptr);
+    if (e == hipSuccess && half) e = hipMemcpy(amp, d_amp, sizeof(double) * half, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "trajectory spectrum");
     return DEMON_OK;
 }
 

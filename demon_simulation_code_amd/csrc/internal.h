@@ -195,4 +195,28 @@ hipError_t launch_adj_stage(double *bar, double *ibar, double *u, double *g, int
 hipError_t launch_observe(const double *x, const double *y, const double *tgt, double *seed,
                           double *lossp, double norm, int64_t n, int64_t npad, hipStream_t s);
 
+// Analysis (analysis.hip).
+// totals[0..3] = {sum m, sum m vx, sum m vy, sum m |v|^2}, totals[4] =
+// sum m |v - v_cm|^2; scratch holds npad/256 * 4 doubles.
+hipError_t launch_kinetic(const double *vx, const double *vy, const double *m, int64_t n, int64_t npad,
+                          double *scratch, double *totals, hipStream_t s);
+// Unordered pairs i < j of the n positions binned by squared distance:
+// bin b holds edges2[b] <= d^2 < edges2[b + 1], d^2 = dx*dx + dy*dy.
+constexpr int32_t kMaxHistBins = 2048;
+struct PairHistLaunch {
+    const double *x, *y;
+    int64_t n;
+    int32_t nb;                   // 256-particle blocks
+    int32_t nbins;                // <= kMaxHistBins
+    const double *edges2;         // nbins + 1 squared bin edges
+    double r2max;                 // edges2[nbins]
+    float inv_width;              // nbins / r_max: the first bin guess
+    unsigned long long *counts;   // nbins, accumulated into
+};
+hipError_t launch_pair_hist(const PairHistLaunch &a, hipStream_t s);
+// amp[k] = |DFT_k(sum_{p<nuse} series[t][p])| / nuse for k < rows/2;
+// s_buf: rows doubles, wre/wim: rows doubles each.
+hipError_t launch_spectrum(const double *series, int64_t rows, int64_t nuse, double *s_buf, double *wre,
+                           double *wim, double *amp, hipStream_t s);
+
 }  // namespace demon

@@ -390,4 +390,35 @@ FitsCloud readFits(const std::string &path, int64_t row) {
     return out;
 }
 
+FitsSeries readFitsSeries(const std::string &path, int column, int64_t nuse) {
+    if (column < 0 || column > 3) throw Error(DEMON_ERR_ARG, "column must be 0..3");
+    std::FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) throw Error(DEMON_ERR_ARG, "cannot open " + path);
+    FitsSeries out;
+    try {
+        const std::vector<Hdu> hdus = read_all(f);
+        const Hdu &ts = find_table(hdus, "TIME_STEP");
+        std::vector<long> off, cnt;
+        columns(ts, off, cnt);
+        if (cnt.size() != 5) throw Error(DEMON_ERR_ARG, "TIME_STEP is not a DEMON table");
+        out.n = cnt[1];
+        out.rows = as_int(ts, "NAXIS2");
+        out.nuse = (nuse < 1 || nuse > out.n) ? out.n : nuse;
+        const long tw = (long)as_int(ts, "NAXIS1");
+        std::vector<unsigned char> rb((size_t)tw);
+        out.data.resize((size_t)(out.rows * out.nuse));
+        std::fseek(f, ts.data_start, SEEK_SET);
+        for (int64_t r = 0; r < out.rows; ++r) {
+            if (std::fread(rb.data(), 1, rb.size(), f) != rb.size()) throw Error(DEMON_ERR_ARG, "truncated TIME_STEP row");
+            for (int64_t i = 0; i < out.nuse; ++i)
+                out.data[(size_t)(r * out.nuse + i)] = get_be(&rb[(size_t)(off[column + 1] + 8 * i)]);
+        }
+    } catch (...) {
+        std::fclose(f);
+        throw;
+    }
+    std::fclose(f);
+    return out;
+}
+
 }  // namespace demon_amd

@@ -76,4 +76,14 @@ struct FitsCloud {
 // Reads the CLOUD table and row `row` of TIME_STEP (-1: the last row).
 FitsCloud readFits(const std::string &path, int64_t row = -1);
 
+// One TIME_STEP column over every row (FFTAnalysis.cpp:245 reads X_POSITION
+// row by row): column 0 X_POSITION, 1 Y_POSITION, 2 X_VELOCITY, 3 Y_VELOCITY;
+// the first nuse particles of each row (all of them when nuse < 1 or > n),
+// row-major [rows][nuse].
+struct FitsSeries {
+    int64_t n = 0, rows = 0, nuse = 0;
+    std::vector<double> data;
+};
+FitsSeries readFitsSeries(const std::string &path, int column, int64_t nuse);
+
 }  // namespace demon_amd

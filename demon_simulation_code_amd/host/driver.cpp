@@ -5,7 +5,7 @@
 //         [-s shielding] [-C confine] [-g gamma] [-T heat] [-B field]
 //         [-D scale offset] [-R X Y] [-V decay] [-S c rmin rmax]
 //         [-w amplitude shift driveConst] [-E field radius] [-F field decay]
-//         [-G g] [-L radius heat_in heat_out] [-v scale offset] [-I]
+//         [-G g] [-L radius heat_in heat_out] [-v scale offset] [-I] [-M v factor]
 //         [-q mean sigma] [-r mean sigma] [-i spacing] [-p x y] [-k vx vy]
 //         [-d density] [-O data.fits | snapshots.csv] [-c file.fits]
 //         [-f file.fits] [--hex N] [--seed S] [--device D] [-h]
@@ -15,14 +15,15 @@
 // unless RectConfinement (-R) or ConfinementVoid (-V) replaces it; Drag
 // unless TimeVaryingDrag (-D) replaces it (driver.cpp:322-326).  -T, -L and
 // -v (the thermal forces) exclude each other (driver.cpp:908-973).  -I
-// selects Runge_Kutta2 (driver.cpp:437-438).  -M (the Mach cone) is not part
-// of this build and is rejected.
+// selects Runge_Kutta2 (driver.cpp:437-438).  -M speed factor fires particle
+// 0 into the cloud (the Mach cone experiment, driver.cpp:425-434).
 // Output (driver.cpp:356-420, 443-450): the reference's data.fits (or the -O
 // name) with a TIME_STEP row every dataTimeStep (demon_fits.h); -c continues
 // a file (its forces, parameters, last row and time) and appends to it; -f
 // starts from a file's last row and writes a new file.  A name ending in .csv
 // writes rows (t, index, x, y, vx, vy) as text instead.
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -45,6 +46,7 @@ void help() {
                  "  -s kappa -C confine -g gamma -T heat -B field -D scale offset -I (RK2)\n"
                  "  -R X Y (rect) -V decay (void) -S c rmin rmax (shear) -w amp shift driveConst\n"
                  "  -E field radius -F field decay (vertical) -G g -L radius in out -v scale offset\n"
+                 "  -M speed massFactor  fire particle 0 into the cloud (Mach cone)\n"
                  "  -q qMean qSigma -r rMean rSigma -i spacing -p x y -k vx vy -d density\n"
                  "  -O data.fits   output file (default data.fits; *.csv writes text rows)\n"
                  "  -c file.fits   continue a run; -f file.fits  start from a run's final state\n"
@@ -60,6 +62,7 @@ bool is_number(const char *s) {
 struct Cli {
     Params P;
     bool rk4 = true;  // driver.cpp:64
+    bool mach = false;  // driver.cpp:63
     cloud_index rowX = 4, rowY = 0;
     int64_t hex = 0;
     uint64_t seed = 0;
@@ -178,6 +181,10 @@ Cli parse(int argc, char **argv) {
         } else if (a == "-G") {
             use_force(P, 'G', GravitationalForceFlag, 0);
             P.gravitationalFieldStrength = take(argc, argv, i, P.gravitationalFieldStrength, "gravitational field const");
+        } else if (a == "-M") {
+            c.mach = true;
+            P.machSpeed = take(argc, argv, i, P.machSpeed, "velocity");
+            P.massFactor = take(argc, argv, i, P.massFactor, "mass");
         } else if (a == "-I") {
             c.rk4 = false;
         } else if (a == "-q") {
@@ -307,6 +314,16 @@ int main(int argc, char **argv) {
             std::fflush(out);
         };
         if (csv) snapshot(P.startTime);
+        if (c.mach) {
+            // driver.cpp:425-434: after the first row is written, particle 0
+            // becomes the bullet, left of the cloud; the distance uses the
+            // driver's own spacing variable, 0.003 m, which no option changes
+            cloud->x[0] = -0.75 * std::sqrt((double)cloud->n) * 0.003;
+            cloud->y[0] = 0.0;
+            cloud->Vx[0] = P.machSpeed;
+            cloud->Vy[0] = 0.0;
+            cloud->mass[0] *= P.massFactor;
+        }
         Runge_Kutta4 I4(cloud.get(), forces, P.simTimeStep, P.startTime);
         Runge_Kutta2 I2(cloud.get(), forces, P.simTimeStep, P.startTime);
         Integrator &I = c.rk4 ? static_cast<Integrator &>(I4) : static_cast<Integrator &>(I2);

@@ -285,6 +285,31 @@ int demon_batch_get_shape(const demon_batch *b, int32_t bucket, int32_t *i_per_l
  * keeps the fastest (reps timed runs each, after one warm-up). */
 int demon_batch_tune(demon_batch *b, int32_t reps);
 
+/* Analysis of the current state (the reference has these only as offline
+ * tools or not at all; no device state is changed).
+ *
+ * Kinetic energy: *total = sum m |v|^2 / 2 and *thermal = sum m |v - v_cm|^2 / 2
+ * with the mass-weighted mean velocity v_cm; the 2D kinetic temperature is
+ * thermal / (n k_B).  Fixed summation order, so repeatable.  Either pointer
+ * may be NULL. */
+int demon_kinetic_energy(demon_ctx *ctx, double *total, double *thermal);
+
+/* Pair-distance histogram of the current positions (the counts behind g(r)):
+ * counts[b], b < nbins, is the number of unordered pairs i < j with
+ * e_b^2 <= d^2 < e_{b+1}^2, where d^2 = dx*dx + dy*dy and e_k = k*(r_max/nbins)
+ * (e_nbins = r_max), all in double; pairs with d >= r_max are not counted.
+ * nbins in [1, 2048].  Integer counts, exact. */
+int demon_pair_histogram(demon_ctx *ctx, double r_max, int32_t nbins, uint64_t *counts);
+
+/* FFTAnalysis -a (FFTAnalysis.cpp:213-283) on device `device`: series is a
+ * row-major [rows][stride] host array (row t = one snapshot's X positions,
+ * the TIME_STEP column X_POSITION), of which the first nuse <= stride
+ * particles are used.  amp[k] = |sum_p sum_t series[t][p] e^{-2 pi i k t/rows}|
+ * / nuse for k < rows/2 (the reference sums the particles' transforms; the
+ * transform of the sum is the same).  No context needed. */
+int demon_trajectory_spectrum(const double *series, int64_t rows, int64_t stride, int64_t nuse, double *amp,
+                              int32_t device);
+
 /* Host-only helper (no device needed): the synthetic hexagonal-lattice input
  * used by the benchmarks -- the n sites of a triangular lattice of spacing
  * `spacing` nearest the origin (ordered by lattice radius, then row, then

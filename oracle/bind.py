@@ -114,6 +114,8 @@ class Oracle:
                                            ctypes.c_int, _I64, _I64, _D, _D, _D, _D, _D, _D, _D,
                                            _D, _D, _D, _D]
         L.oracle_fit_loss_grad.restype = ctypes.c_int
+        L.oracle_average_fft.argtypes = [_D, _I64, _I64, _I64, _D]
+        L.oracle_average_fft.restype = None
         self.coulomb = L.oracle_coulomb_const()
 
     # --- forces -----------------------------------------------------------
@@ -165,6 +167,14 @@ class Oracle:
         fx, fy = np.zeros(n), np.zeros(n)
         self.L.oracle_thermal_add(n, heat, seed, step, stage, p(fx), p(fy))
         return fx, fy
+
+    def average_fft(self, series, nuse):
+        """FFTAnalysis -a on series [rows, stride]: amplitudes for k < rows // 2."""
+        series = np.ascontiguousarray(series, dtype=np.float64)
+        rows, stride = series.shape
+        amp = np.zeros(rows // 2)
+        self.L.oracle_average_fft(p(series), rows, stride, nuse, p(amp))
+        return amp
 
     def philox(self, ctr, key):
         c = (ctypes.c_uint32 * 4)(*ctr)

@@ -298,6 +298,43 @@ void oracle_tvthermal_add(int64_t n, double scale, double offset, double t, uint
 }
 
 /* ------------------------------------------------------------------------ */
+/* FFTAnalysis -a                                                           */
+/* ------------------------------------------------------------------------ */
+
+void oracle_average_fft(const double *series, int64_t rows, int64_t stride, int64_t nuse, double *amp)
+{
+    const int64_t half = rows / 2;
+    double *re = (double *)calloc((size_t)(half > 0 ? half : 1), sizeof(double));
+    double *im = (double *)calloc((size_t)(half > 0 ? half : 1), sizeof(double));
+    if (!re || !im) {
+        free(re);
+        free(im);
+        return;
+    }
+    /* FFTAnalysis.cpp:241-257: one transform per particle, summed */
+    for (int64_t p = 0; p < nuse; ++p) {
+        for (int64_t k = 0; k < half; ++k) {
+            double sr = 0.0, si = 0.0;
+            for (int64_t t = 0; t < rows; ++t) {
+                const int64_t m = (k * t) % rows;
+                const double a = (2.0 * M_PI) * (double)m / (double)rows;
+                const double x = series[t * stride + p];      /* :245-247 */
+                sr += x * cos(a);
+                si -= x * sin(a);
+            }
+            re[k] += sr;                                      /* :254-255 */
+            im[k] += si;
+        }
+    }
+    /* FFTAnalysis.cpp:270-279: the printed amplitude is |sum| / nuse (the
+     * zeroing at :274-277 comes after amp is formed and does not change it) */
+    for (int64_t k = 0; k < half; ++k)
+        amp[k] = sqrt(re[k] * re[k] + im[k] * im[k]) / (double)nuse;
+    free(re);
+    free(im);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Integrator::modifyTimeStep                                               */
 /* ------------------------------------------------------------------------ */
 

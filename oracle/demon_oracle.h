@@ -39,6 +39,8 @@
  *                            its RK4 / RK2 trajectories; closed forms (linear
  *                            rect field, radial void field, the tangential
  *                            rotation of the shear layer, gravity = m g)
+ *   oracle_average_fft       numpy's FFT (a textbook transform) and the
+ *                            closed-form spectrum of a pure cosine
  *   oracle_thermal_localized_add, oracle_tvthermal_add
  *                            the reference's own draws: per-particle magnitude
  *                            bounds inside / outside the radius, and |F|^2
@@ -202,6 +204,15 @@ void oracle_thermal_localized_add(int64_t n, const double *xs, const double *ys,
  * counter domain 4. */
 void oracle_tvthermal_add(int64_t n, double scale, double offset, double t, uint64_t seed,
                           uint64_t step, int stage, double *fx, double *fy);
+
+/* FFTAnalysis averageFFT (FFTAnalysis.cpp:213-283): for each of the first
+ * nuse particles the discrete Fourier transform of its X_POSITION series
+ * (rows snapshots; series is row-major [rows][stride]), the transforms
+ * summed, and amp[i] = |sum_i| / nuse for i < rows/2.  The transform is the
+ * definition sum_t x_t e^{-2 pi i k t / rows} (FFTW's FFTW_FORWARD), with the
+ * angle reduced exactly through (k t) mod rows.  O(nuse rows^2): small
+ * cases only. */
+void oracle_average_fft(const double *series, int64_t rows, int64_t stride, int64_t nuse, double *amp);
 
 /* Integrator::modifyTimeStep (Integrator.cpp:68-134, 144-182), serial, with
  * the reference's exact pair coverage.  n must be a multiple of 4. */

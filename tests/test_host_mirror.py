@@ -316,3 +316,12 @@ def test_driver_plugin_options(tmp_path):
     r = subprocess.run([str(DEMON), "-T", "1e-15", "-L", "1e-3", "1e-15", "1e-15"], capture_output=True, text=True,
                        timeout=60, cwd=tmp_path)
     assert r.returncode != 0 and "conflicts" in r.stderr
+    # -M: particle 0 is fired into the cloud after the first row is written
+    # (driver.cpp:425-434); drag slows it at the rate gamma = 10 whatever its mass
+    demon("-e", "0.01", "-O", "mach.fits", "-M", "0.2", "100", extra=["--hex", "256", "-t", "1e-3", "-o", "0.01"])
+    first = _read_ours(tmp_path / "mach.fits", 0)
+    last = _read_ours(tmp_path / "mach.fits")
+    assert first[5][0, 4] == 0.0 and first[5][0, 2] != -0.75 * 16 * 0.003
+    assert last[5][0, 0] == first[5][0, 0]          # the CLOUD table keeps the original mass
+    x0, vx0 = last[5][0, 2], last[5][0, 4]
+    assert -0.036 + 0.0017 < x0 < -0.036 + 0.0021 and 0.17 < vx0 < 0.19

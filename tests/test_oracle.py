@@ -560,3 +560,32 @@ def test_tv_thermal_magnitude_at_stage_time_vs_live_reference(oracle):
         for mag in (np.hypot(fx, fy), draws):
             assert mag.max() <= abs(h)
             assert (mag ** 2).mean() / h ** 2 == pytest.approx(1 / 3, rel=0.03)
+
+
+# --- FFTAnalysis -a -----------------------------------------------------------
+
+def test_average_fft_is_the_summed_dft(oracle):
+    """The restated averageFFT equals numpy's FFT (a textbook transform) of
+    each particle's series, summed over particles, |.| / nuse for k < rows/2;
+    odd and even row counts; a stride wider than nuse."""
+    rng = np.random.default_rng(8)
+    for rows, stride, nuse in ((64, 5, 3), (77, 4, 4), (2, 1, 1), (1, 3, 2)):
+        series = rng.normal(size=(rows, stride)) * 1e-3
+        amp = oracle.average_fft(series, nuse)
+        want = np.abs(np.fft.fft(series[:, :nuse], axis=0).sum(axis=1))[: rows // 2] / nuse
+        assert amp.shape == (rows // 2,)
+        np.testing.assert_allclose(amp, want, rtol=0, atol=1e-15 * np.abs(series).sum())
+
+
+def test_average_fft_of_a_cosine(oracle):
+    """A pure cosine of frequency f0 = k0 / (rows dt) puts rows/2 * A into bin
+    k0 and nothing elsewhere (every particle in phase); the reference labels
+    bin i with i / (rows * 0.01) Hz."""
+    rows, k0, amp0 = 200, 13, 2e-4
+    t = np.arange(rows)
+    x = amp0 * np.cos(2 * np.pi * k0 * t / rows)
+    series = np.stack([x, x, x], axis=1)
+    amp = oracle.average_fft(series, 3)
+    assert amp[k0] == pytest.approx(rows / 2 * amp0, rel=1e-12)
+    others = np.delete(amp, k0)
+    assert others.max() < 1e-12 * amp[k0]
