@@ -135,6 +135,12 @@ __global__ __launch_bounds__(256) void k_pair_hist(PairHistLaunch a) {
     uint32_t *h = hist + wave * nbins;
     const float inv_w = a.inv_width;
     const int jstart = d == 0 ? tid + 1 : 0;  // the diagonal tile: j > i only
+    // Consecutive j are spatial neighbours (Morton order), so a lane's bin
+    // changes rarely: count runs in registers and add a run when it ends.
+    // This takes the lanes of a wave off the same LDS word, where coarse bins
+    // would otherwise serialise their atomics.
+    int32_t run_bin = -1;
+    uint32_t run = 0u;
     for (int jj = jstart; jj < 256; ++jj) {
         const double dx = xi - xj[jj];
         const double dy = yi - yj[jj];
@@ -144,9 +150,16 @@ __global__ __launch_bounds__(256) void k_pair_hist(PairHistLaunch a) {
             b = b < 0 ? 0 : (b >= nbins ? nbins - 1 : b);
             while (b > 0 && d2 < e2[b]) --b;
             while (b + 1 < nbins && d2 >= e2[b + 1]) ++b;
-            atomicAdd(&h[b], 1u);
+            if (b == run_bin) {
+                ++run;
+            } else {
+                if (run) atomicAdd(&h[run_bin], run);
+                run_bin = b;
+                run = 1u;
+            }
         }
     }
+    if (run) atomicAdd(&h[run_bin], run);
     __syncthreads();
     for (int32_t b = tid; b < nbins; b += 256) {
         const uint32_t c = hist[b] + hist[nbins + b] + hist[2 * nbins + b] + hist[3 * nbins + b];
