@@ -427,6 +427,29 @@ def plugin_cloud(rng, n):
     return x, y, vx, vy, q, m
 
 
+def test_plugins_match_reference_golden_bitwise(oracle):
+    """The reference's own plugin classes' forceS(t) and 6-step RK4 / RK2
+    runs, recorded in tests/golden/plugins.npz, against the oracle."""
+    d = np.load(GOLD / "plugins.npz")
+    op = OracleParams()
+    for name, v in zip(d["par_names"], d["par"]):
+        setattr(op, str(name), float(v))
+    op.dt_check = 1
+    st = [d[k] for k in ("x", "y", "vx", "vy", "q", "m")]
+    for f in d["force_sets"]:
+        op.forces = int(f)
+        fx, fy = oracle.stage_force(*st, op, t=0.37)
+        assert bits_equal(fx, d[f"force_{f}_x"]) and bits_equal(fy, d[f"force_{f}_y"]), f
+    for order in (4, 2):
+        run = oracle.rk4_steps if order == 4 else oracle.rk2_steps
+        for f in d["runs"]:
+            op.forces = int(f)
+            o = run(*st, op, 1e-4, 6, t=0.25)
+            for name, arr in zip(("x", "y", "vx", "vy"), o[:4]):
+                assert bits_equal(arr, d[f"rk{order}_{f}_{name}"]), (order, f, name)
+            assert o[4] == float(d[f"rk{order}_{f}_t"])
+
+
 @pytest.mark.skipif(not Reference.available(), reason="oracle/_ref not built (needs the reference sources)")
 @pytest.mark.parametrize("forces", [RECT, VOID, ROT, DRIVE, ELEC, VERT, GRAV, 1 | VOID])
 def test_plugin_force_bitwise_vs_live_reference(oracle, forces):
