@@ -132,25 +132,30 @@ def test_time_dependent_plugins_need_the_stage_time():
         c.force_stage(dm.CONFINEMENT, 1, 0, x, y, vx, vy)
 
 
+RUN_A = (dm.CONFINEMENT_VOID | dm.DRAG | dm.DRIVING | dm.ROTATIONAL | dm.SHIELDED_COULOMB | dm.THERMAL_LOCALIZED
+         | dm.GRAVITATIONAL)
+RUN_B = (dm.MAGNETIC | dm.RECT_CONFINEMENT | dm.SHIELDED_COULOMB | dm.TIME_VARYING_DRAG | dm.TIME_VARYING_THERMAL
+         | dm.ELECTRIC | dm.VERT_ELECTRIC)
 RUNS = [
     # -V -w -S -G -L: the void replaces the confinement; the localized heating
-    (dm.CONFINEMENT_VOID | dm.DRAG | dm.DRIVING | dm.ROTATIONAL | dm.SHIELDED_COULOMB | dm.THERMAL_LOCALIZED
-     | dm.GRAVITATIONAL, 4),
+    (RUN_A, 4, False),
     # -R -B -E -F -v -D: rectangular trap with the fields and the time-varying heat and drag
-    (dm.MAGNETIC | dm.RECT_CONFINEMENT | dm.SHIELDED_COULOMB | dm.TIME_VARYING_DRAG | dm.TIME_VARYING_THERMAL
-     | dm.ELECTRIC | dm.VERT_ELECTRIC, 4),
-    (dm.CONFINEMENT_VOID | dm.DRAG | dm.DRIVING | dm.ROTATIONAL | dm.SHIELDED_COULOMB | dm.THERMAL_LOCALIZED
-     | dm.GRAVITATIONAL, 2),
+    (RUN_B, 4, False),
+    (RUN_A, 2, False),
+    # the captured step (hipGraph) reads the stage times on the device
+    (RUN_B, 4, True),
+    (RUN_A, 2, True),
 ]
 
 
-@pytest.mark.parametrize("forces,order", RUNS)
-def test_plugin_trajectories(oracle, forces, order):
+@pytest.mark.parametrize("forces,order,graph", RUNS)
+def test_plugin_trajectories(oracle, forces, order, graph):
     n = 1024
     x, y, vx, vy, q, m = state(n, 25)
     steps, t0 = 60, 0.5
     with context(n, forces) as c:
         c.upload(x, y, vx, vy, q, m)
+        c.use_graph(graph)
         t = (c.rk4_steps if order == 4 else c.rk2_steps)(steps, 1e-3, t0)
         g = c.download()
         assert c.stats().steps == steps
