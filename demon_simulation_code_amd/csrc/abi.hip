@@ -1102,7 +1102,9 @@ int demon_trajectory_spectrum(const double *series, int64_t rows, int64_t stride
                               int32_t device) {
     if (!series || !amp) return fail(nullptr, DEMON_ERR_ARG, "NULL array");
     if (rows < 1 || nuse < 1 || stride < nuse) return fail(nullptr, DEMON_ERR_ARG, "need rows >= 1, 1 <= nuse <= stride");
-    if (rows > ((int64_t)1 << 31) / 2 || nuse > ((int64_t)1 << 31)) return fail(nullptr, DEMON_ERR_ARG, "series too large");
+    // the transform is O(rows^2): 2^22 snapshots is about 10^13 operations
+    if (rows > ((int64_t)1 << 22) || nuse > ((int64_t)1 << 31))
+        return fail(nullptr, DEMON_ERR_ARG, "at most 2^22 snapshots and 2^31 particles");
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(nullptr, e, "hipSetDevice");
     std::vector<double> packed;

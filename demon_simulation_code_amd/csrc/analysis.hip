@@ -32,7 +32,7 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *red) {
     __syncthreads();
 }
 
-// Pass 1 (center = nullptr): per block {sum m, sum m vx, sum m vy, sum m |v|^2}.
+// Pass 1 (totals = nullptr): per block {sum m, sum m vx, sum m vy, sum m |v|^2}.
 // Pass 2: per block {sum m |v - v_cm|^2} with v_cm from the pass-1 totals.
 __global__ __launch_bounds__(256) void k_kinetic_part(const double *vx, const double *vy, const double *m,
                                                       int64_t n, const double *totals, double *part) {
@@ -126,8 +126,8 @@ __global__ __launch_bounds__(256) void k_pair_hist(PairHistLaunch a) {
         dst[2] = fmin(fmin(src[2], src[6]), fmin(src[10], src[14]));
         dst[3] = fmax(fmax(src[3], src[7]), fmax(src[11], src[15]));
     }
-    // gap between the boxes per axis (0 when they overlap); NaN boxes (tiles
-    // of padding only) compare false and are skipped
+    // gap between the boxes per axis (0 when they overlap); every block holds
+    // a real particle (nb = ceil(n / 256)), so the boxes are finite
     const double gx = fmax(fmax(ib[0] - jb[1], jb[0] - ib[1]), 0.0);
     const double gy = fmax(fmax(ib[2] - jb[3], jb[2] - ib[3]), 0.0);
     if (!(gx * gx + gy * gy < a.r2max)) return;  // uniform over the block
@@ -146,8 +146,9 @@ __global__ __launch_bounds__(256) void k_pair_hist(PairHistLaunch a) {
         const double dy = yi - yj[jj];
         const double d2 = dx * dx + dy * dy;
         if (d2 < a.r2max) {
-            int32_t b = (int32_t)(__fsqrt_rn((float)d2) * inv_w);
-            b = b < 0 ? 0 : (b >= nbins ? nbins - 1 : b);
+            // first guess, clamped in FP32 (the product may overflow for a tiny r_max)
+            const float g = __fsqrt_rn((float)d2) * inv_w;
+            int32_t b = g < (float)nbins ? (int32_t)g : nbins - 1;
             while (b > 0 && d2 < e2[b]) --b;
             while (b + 1 < nbins && d2 >= e2[b + 1]) ++b;
             if (b == run_bin) {

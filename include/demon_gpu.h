@@ -306,7 +306,8 @@ int demon_pair_histogram(demon_ctx *ctx, double r_max, int32_t nbins, uint64_t *
  * the TIME_STEP column X_POSITION), of which the first nuse <= stride
  * particles are used.  amp[k] = |sum_p sum_t series[t][p] e^{-2 pi i k t/rows}|
  * / nuse for k < rows/2 (the reference sums the particles' transforms; the
- * transform of the sum is the same).  No context needed. */
+ * transform of the sum is the same).  rows <= 2^22 (a direct transform,
+ * O(rows^2)).  No context needed. */
 int demon_trajectory_spectrum(const double *series, int64_t rows, int64_t stride, int64_t nuse, double *amp,
                               int32_t device);
 
