@@ -167,3 +167,19 @@ def test_plugin_trajectories(oracle, forces, order, graph):
     assert np.max(np.hypot(o[2] - g[2], o[3] - g[3])) / vscale <= 1e-8
     # the run moved the cloud: the plugins were not no-ops
     assert np.max(np.hypot(o[0] - x, o[1] - y)) > 1e-6
+
+
+def test_elementwise_plugins_full_size(oracle):
+    """Every elementwise plugin at N = 131072 (the benchmark's size), one
+    stage force against the oracle."""
+    n = 131072
+    x, y, vx, vy, q, m = state(n, 26)
+    forces = (dm.CONFINEMENT_VOID | dm.DRAG | dm.DRIVING | dm.MAGNETIC | dm.RECT_CONFINEMENT | dm.ROTATIONAL
+              | dm.TIME_VARYING_DRAG | dm.ELECTRIC | dm.GRAVITATIONAL | dm.VERT_ELECTRIC | dm.THERMAL_LOCALIZED
+              | dm.TIME_VARYING_THERMAL)
+    with context(n, forces) as c:
+        c.upload(x, y, vx, vy, q, m)
+        gx, gy = c.force_stage(forces, 3, 11, x, y, vx, vy, t=0.75)
+    fx, fy = oracle.stage_force(x, y, vx, vy, q, m, oracle_params(forces), step=11, stage=3, t=0.75)
+    scale = term_scale(x, y, vx, vy, q, m) + PLUGIN["heat_in"] + PLUGIN["heat_scale"] * 0.75 + PLUGIN["heat_offset"]
+    assert force_rel_err(fx, fy, gx, gy, scale) <= ELEM_TOL
