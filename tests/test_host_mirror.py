@@ -325,3 +325,29 @@ def test_driver_plugin_options(tmp_path):
     assert last[5][0, 0] == first[5][0, 0]          # the CLOUD table keeps the original mass
     x0, vx0 = last[5][0, 2], last[5][0, 4]
     assert -0.036 + 0.0017 < x0 < -0.036 + 0.0021 and 0.17 < vx0 < 0.19
+
+
+@pytest.mark.gpu
+def test_driver_runs_params_cfg(tmp_path):
+    """The reference's shipped Params.cfg (forceFlags B,D,) and a Params file
+    selecting seven further plugins run through the driver and write their
+    forces' bits and keywords."""
+    def demon(*args):
+        r = subprocess.run([str(DEMON), *map(str, args)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+        assert r.returncode == 0, r.stdout + r.stderr
+
+    demon("-P", GOLD / "Params.cfg", "-e", "0.03", "-O", "shipped.fits")
+    n, rows, t, forces, keys, last = _read_ours(tmp_path / "shipped.fits")
+    # Confinement, ShieldedCoulomb, Magnetic and TimeVaryingDrag (which writes the Drag bit too)
+    assert n == 100 and rows == 4 and forces == 1 | 2 | 4 | 1024 | 256
+    assert keys["magneticfield"] == 1.0 and keys["tvdragscaleconst"] == -1.0 and np.all(np.isfinite(last))
+    cfg = tmp_path / "plugins.cfg"
+    cfg.write_text((GOLD / "Params.cfg").read_text().replace("forceFlags", "unusedKey")
+                   + "\nnumParticles 64\nforceFlags R,S,w,G,E,F,L,\n")
+    demon("-P", cfg, "-e", "0.02", "-O", "plugins.fits")
+    n, rows, t, forces, keys, last = _read_ours(tmp_path / "plugins.fits")
+    assert n == 64 and rows == 3 and np.all(np.isfinite(last))
+    assert forces == 2 | 4 | 8192 | 128 | 64 | 8 | 4096 | 16384 | 32   # Rect replaces Confinement
+    for k in ("confineconstx", "rotationalconst", "drivingamplitude", "gravitationalfieldstrength",
+              "electricfieldstrength", "vertelectricfieldstrength", "heatingvalue1"):
+        assert k in keys, k
