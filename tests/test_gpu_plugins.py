@@ -183,3 +183,28 @@ def test_elementwise_plugins_full_size(oracle):
     fx, fy = oracle.stage_force(x, y, vx, vy, q, m, oracle_params(forces), step=11, stage=3, t=0.75)
     scale = term_scale(x, y, vx, vy, q, m) + PLUGIN["heat_in"] + PLUGIN["heat_scale"] * 0.75 + PLUGIN["heat_offset"]
     assert force_rel_err(fx, fy, gx, gy, scale) <= ELEM_TOL
+
+
+def test_plugins_ragged_sizes(oracle):
+    """Sizes that fill no block evenly: the elementwise plugins at n = 1001 and
+    a 20-step RK4 run of all of them at n = 1000 (the dt check needs n % 4 = 0
+    in the reference)."""
+    forces = (dm.CONFINEMENT_VOID | dm.DRAG | dm.DRIVING | dm.RECT_CONFINEMENT | dm.ROTATIONAL | dm.ELECTRIC
+              | dm.GRAVITATIONAL | dm.VERT_ELECTRIC | dm.THERMAL_LOCALIZED)
+    x, y, vx, vy, q, m = state(1001, 27)
+    with context(1001, forces) as c:
+        c.upload(x, y, vx, vy, q, m)
+        gx, gy = c.force_stage(forces, 2, 4, x, y, vx, vy, t=0.1)
+    fx, fy = oracle.stage_force(x, y, vx, vy, q, m, oracle_params(forces), step=4, stage=2, t=0.1)
+    scale = term_scale(x, y, vx, vy, q, m) + PLUGIN["heat_in"]
+    assert force_rel_err(fx, fy, gx, gy, scale) <= ELEM_TOL
+    n = 1000
+    x, y, vx, vy, q, m = state(n, 28)
+    full = forces | dm.SHIELDED_COULOMB
+    with context(n, full) as c:
+        c.upload(x, y, vx, vy, q, m)
+        t = c.rk4_steps(20, 1e-3, 0.0)
+        g = c.download()
+    o = oracle.rk4_steps(x, y, vx, vy, q, m, oracle_params(full), 1e-3, 20)
+    assert t == o[4]
+    assert pos_rel_err(o[0], o[1], g[0], g[1]) <= POS_TOL
