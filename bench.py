@@ -80,7 +80,16 @@ def setup(wl):
     return x, y, q, m
 
 
-def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mode=None):
+# Every per-particle force plugin with zero strength: the trajectory is the
+# headline one bit for bit, so a profile of the stage-update kernels with and
+# without them measures what the plugins cost.
+ZERO_PLUGINS = dict(confine_x=0.0, confine_y=0.0, void_decay=0.0, rot_inner=1e-3, rot_outer=1.0, rot_const=0.0,
+                    drive_const=1e-5, drive_amplitude=0.0, drive_shift=7e-3, electric_field=0.0, plasma_radius=1.0,
+                    vert_electric_field=0.0, vert_decay=1.0, gravity=0.0, heat_in=0.0, heat_out=0.0,
+                    heat_radius=1e-3, heat_scale=0.0, heat_offset=0.0)
+
+
+def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mode=None, plugins_zero=False):
     """Warm up on the workload, re-upload the initial state, then time exactly
     `steps` RK4 steps from it (one demon_rk4_steps call); with `start`, from
     the state after that many untimed steps instead.  With `snapshots` (step
@@ -90,8 +99,15 @@ def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mod
     x, y, q, m = setup(wl)
     n = wl.n
     forces = dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB
+    extra = {}
+    if plugins_zero:
+        # ConfinementVoid with decay 0 is the confinement force plus exactly 0
+        forces = (dm.CONFINEMENT_VOID | dm.DRAG | dm.DRIVING | dm.MAGNETIC | dm.RECT_CONFINEMENT | dm.ROTATIONAL
+                  | dm.SHIELDED_COULOMB | dm.THERMAL_LOCALIZED | dm.TIME_VARYING_THERMAL | dm.ELECTRIC
+                  | dm.GRAVITATIONAL | dm.VERT_ELECTRIC)
+        extra = ZERO_PLUGINS
     c = dm.Context(n, shielding=wl.kappa, confine=wl.confine, gamma=wl.gamma, forces=forces,
-                   precision=precision)
+                   precision=precision, **extra)
     if mode is not None:
         c.set_pair_mode(mode)
     if shape:
@@ -274,6 +290,9 @@ def main():
                     help="all-pairs kernel")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--plugins-zero", action="store_true",
+                    help="add every per-particle force plugin at zero strength (same trajectory; "
+                         "profiles what the plugins cost in the stage updates)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -290,7 +309,8 @@ def main():
     snap_steps = [K // 8, 3 * K // 8, 5 * K // 8, 7 * K // 8] if not args.no_cpu else []
     log(f"FP64 run: N={n} warmup={W} steps={K}")
     mode = dm.PAIRS_NEWTON3 if args.mode == "newton3" else dm.PAIRS_GATHER
-    r = run_gpu(wl, K, W, dm.FP64, shape, snapshots=snap_steps, start=args.start_step, mode=mode)
+    r = run_gpu(wl, K, W, dm.FP64, shape, snapshots=snap_steps, start=args.start_step, mode=mode,
+                plugins_zero=args.plugins_zero)
     pairs_total = 4.0 * n * (n - 1) * K
     value = pairs_total / r["wall"]
     log(f"FP64 done: {r['wall']:.3f} s, {value:.4g} pair interactions/s, shape={r['shape']}")
@@ -346,6 +366,7 @@ def main():
             "pair_kernel": args.mode,
             "pair_shape": list(r["shape"]),
             "start_step": args.start_step,
+            "plugins_zero": args.plugins_zero,
         },
         "rk4_steps_per_s": K / r["wall"],
         "roofline": roofline,
