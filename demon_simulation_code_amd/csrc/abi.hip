@@ -827,8 +827,13 @@ int demon_pair_counts(const demon_ctx *c, int64_t *near_pairs, int64_t *tested_p
     Scalars s{};
     hipError_t e = hipMemcpy(&s, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(const_cast<demon_ctx *>(c), e, "hipMemcpy pair counts");
-    if (near_pairs) *near_pairs = (int64_t)s.pairs_near;
-    if (tested_pairs) *tested_pairs = (int64_t)s.pairs_tested;
+    unsigned long long near = 0, tested = 0;
+    for (int k = 0; k < kCountSlots; ++k) {
+        near += s.pair_counts[k][0];
+        tested += s.pair_counts[k][1];
+    }
+    if (near_pairs) *near_pairs = (int64_t)near;
+    if (tested_pairs) *tested_pairs = (int64_t)tested;
     return DEMON_OK;
 }
 

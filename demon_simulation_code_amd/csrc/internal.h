@@ -71,6 +71,8 @@ struct PairSums {
 };
 
 // Device scalars of the stepping loop.
+constexpr int kCountSlots = 64;
+
 struct Scalars {
     double t;                 // integrator currentTime
     double t_end;             // moveParticles(endTime)
@@ -81,12 +83,15 @@ struct Scalars {
     unsigned long long remaining;   // steps still allowed in this call
     unsigned long long dt_reductions;
     unsigned long long rmax2_bits;  // max |r|^2 of the stage-1 positions (double bits)
-    unsigned long long pairs_near;    // ordered pairs inside the cutoff, summed over launches
-    unsigned long long pairs_tested;  // ordered pairs given a distance test, summed over launches
     unsigned int dtmin_bits;        // min FP32 squared distance of tested pairs (float bits)
     int active;                     // 1 while the current step runs
     int err;                        // sticky device-side error code
     int pad_;
+    // ordered pairs inside the cutoff [0] and given a distance test [1],
+    // summed over launches; a workgroup adds to slot (row + slice) mod
+    // kCountSlots, so that the atomics of many workgroups do not serialise
+    // on one address
+    unsigned long long pair_counts[kCountSlots][2];
 };
 
 struct PairLaunch {

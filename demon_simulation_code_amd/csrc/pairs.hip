@@ -60,8 +60,9 @@ __global__ __launch_bounds__(256) void k_pairs(const double *__restrict__ xs,
     uint64_t n_near = 0, n_tested = 0;
     wave_tile<R, F32>(xs, ys, nc, q, orig, box, (int32_t)n, ibase, j_begin, j_end, flags, kappa,
                       rc2p, rad, ec, &sc->dtmin_bits, ax, ay, n_near, n_tested);
-    // one atomic per counter per workgroup (device-wide atomics on one
-    // address serialise); every thread reaches this point
+    // one atomic per counter per workgroup, to one of kCountSlots addresses
+    // (device-wide atomics on one address serialise); every thread reaches
+    // this point
     __shared__ unsigned long long cnt[4][2];
     if (lane == 0) {
         cnt[wave][0] = n_near;
@@ -69,8 +70,9 @@ __global__ __launch_bounds__(256) void k_pairs(const double *__restrict__ xs,
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        atomicAdd(&sc->pairs_near, ((cnt[0][0] + cnt[1][0]) + cnt[2][0]) + cnt[3][0]);
-        atomicAdd(&sc->pairs_tested, ((cnt[0][1] + cnt[1][1]) + cnt[2][1]) + cnt[3][1]);
+        unsigned long long *slot = sc->pair_counts[(blockIdx.x + blockIdx.y) & (kCountSlots - 1)];
+        atomicAdd(&slot[0], ((cnt[0][0] + cnt[1][0]) + cnt[2][0]) + cnt[3][0]);
+        atomicAdd(&slot[1], ((cnt[0][1] + cnt[1][1]) + cnt[2][1]) + cnt[3][1]);
     }
     double *px = part + (size_t)(2 * blockIdx.y) * npad;
     double *py = px + npad;

@@ -287,9 +287,9 @@ __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_
     px[npad + ibase] = ay[0];
     // The pair counters: the four waves' counts meet in LDS (the reaction
     // buffer, free once every tile is done) and the workgroup adds them with
-    // one atomic each.  Device-wide atomics on one address serialise, and a
-    // launch whose tiles are mostly culled would otherwise spend most of its
-    // time on them.
+    // one atomic each, to one of kCountSlots addresses.  Device-wide atomics
+    // on one address serialise, and a launch whose tiles are mostly culled
+    // would otherwise spend most of its time on them.
     __syncthreads();
     unsigned long long *cnt = reinterpret_cast<unsigned long long *>(&red[0][0]);
     if (lane == 0) {
@@ -298,8 +298,9 @@ __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        atomicAdd(&a.sc->pairs_near, ((cnt[0] + cnt[2]) + cnt[4]) + cnt[6]);
-        atomicAdd(&a.sc->pairs_tested, ((cnt[1] + cnt[3]) + cnt[5]) + cnt[7]);
+        unsigned long long *slot = a.sc->pair_counts[(S + slice) & (kCountSlots - 1)];
+        atomicAdd(&slot[0], ((cnt[0] + cnt[2]) + cnt[4]) + cnt[6]);
+        atomicAdd(&slot[1], ((cnt[1] + cnt[3]) + cnt[5]) + cnt[7]);
     }
     // red is reused by the next row of a persistent caller
     __syncthreads();
