@@ -112,6 +112,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     pair::ExpCoef ec) {
     __shared__ n3::RedLds red;
     __shared__ n3::ScrLds scr;
+    __shared__ n3::TBoxLds tbox;
     __shared__ int32_t slot;
     for (;;) {
         // the previous item is done with the LDS sums and with slot
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const n3::N3Row row{xs + cl.off, ys + cl.off, nc + cl.off, q + cl.off, qf + cl.off, orig + cl.off,
                             box + cl.off / 16, part + cl.part_off, react + cl.react_off,
                             rflag + cl.rflag_off, s, cl.n, cl.npad, base_flags, cl.kappa, cl.rc2p};
-        n3::n3_row<F32, DTC>(row, ec, item.ibase, d0, d1, item.slice, red, scr);
+        n3::n3_row<F32, DTC>(row, ec, item.ibase, d0, d1, item.slice, red, scr, tbox);
     }
 }
 

@@ -180,12 +180,17 @@ struct N3Row {
 
 using RedLds = double[4][2 * kSuper];  // [wave][2 jj + component]
 using ScrLds = double[4][kScrDoubles]; // per-wave transpose scratch
+// Bounding boxes of the superblocks T a row visits, staged in LDS a chunk at
+// a time: one round trip to memory per chunk instead of one per offset,
+// which is what a launch of mostly culled tiles waits on.
+constexpr int kBoxChunk = 128;
+using TBoxLds = Box[kBoxChunk];
 
 // One workgroup's share: row S against superblocks S + d for d in [d0, d1),
 // its i-side sums written to part[slice].  Called by all 256 threads.
 template <bool F32, bool DTC>
 __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_t S, int32_t d0, int32_t d1,
-                                       int32_t slice, RedLds &red, ScrLds &scr) {
+                                       int32_t slice, RedLds &red, ScrLds &scr, TBoxLds &tbox) {
     const double *__restrict__ xs = a.xs;
     const double *__restrict__ ys = a.ys;
     const double *__restrict__ nc = a.nc;
@@ -209,6 +214,7 @@ __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_
     const bool ireal = t.whi <= n;
     double ax[1] = {0.0}, ay[1] = {0.0};
     uint64_t n_near = 0, n_tested = 0;
+    int32_t tb0 = 0;  // offset d of tbox[0]
     for (int32_t d = d0; d < d1; ++d) {
         if (d == 0) {
             // the superblock with itself, every ordered pair, j == i skipped
@@ -216,11 +222,25 @@ __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_
                                             e, ax, ay, n_near, n_tested, smin);
             continue;
         }
+        if ((d - d0) % kBoxChunk == 0 || d == 1) {
+            // stage the boxes of T = S + d .. S + min(d1, d + kBoxChunk) - 1;
+            // the barriers are uniform (d is), the first one lets the
+            // previous chunk's readers finish
+            const int32_t dc = d, dn = d1 < d + kBoxChunk ? d1 : d + kBoxChunk;
+            __syncthreads();
+            for (int32_t k = (int32_t)threadIdx.x; k < dn - dc; k += 256) {
+                const int32_t Tk = S + dc + k < nsb ? S + dc + k : S + dc + k - nsb;
+                Box b = empty_box();
+#pragma unroll
+                for (int m = 0; m < 4; ++m) b = box_union(b, box + 4 * (Tk * 4 + m));
+                tbox[k] = b;
+            }
+            __syncthreads();
+            tb0 = dc;
+        }
         const int32_t T = S + d < nsb ? S + d : S + d - nsb;
         uint32_t *fl = a.rflag + (size_t)T * half + (d - 1);
-        Box sbT = empty_box();
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sbT = box_union(sbT, box + 4 * (T * 4 + k));
+        const Box sbT = tbox[d - tb0];
         // for even nsb the pair (S, S + nsb/2) is reached from both rows and
         // belongs to the lower one; for odd nsb every offset is distinct
         if ((d == half && !(nsb & 1) && S >= half) || box_gap2(sbS, sbT) > rad.rf2) {

@@ -15,6 +15,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                                                    ExpCoef ec) {
     __shared__ RedLds red;
     __shared__ ScrLds scr;
+    __shared__ TBoxLds tbox;
     if ((a.flags & kFlagGated) && !a.sc->active) return;
     const int32_t nsb = a.npad / kSuper;
     const int32_t half = nsb / 2;
@@ -26,7 +27,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                 : (int32_t)(blockIdx.x & 7) * (nsb >> 3) + (int32_t)(blockIdx.x >> 3);
     const int32_t d0 = (int32_t)blockIdx.y * span;
     const int32_t d1 = d0 + span < half + 1 ? d0 + span : half + 1;
-    n3_row<F32, DTC>(a, ec, S, d0, d1, (int32_t)blockIdx.y, red, scr);
+    n3_row<F32, DTC>(a, ec, S, d0, d1, (int32_t)blockIdx.y, red, scr, tbox);
 }
 
 }  // namespace
