@@ -235,3 +235,23 @@ def test_magnetic_and_time_varying_drag_stage_forces(oracle):
             assert force_rel_err(fx, fy, gx, gy, scale) <= FP64_TOL, stage
         with pytest.raises(dm.DemonError):
             c.force_stage(dm.TIME_VARYING_DRAG, 1, 0, x, y, vx, vy)  # needs the stage time
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_near_pair_count_is_exact(oracle, mode):
+    """The in-kernel count of ordered pairs inside the cutoff (the P_near of
+    the roofline's FLOP count) equals the oracle's count under the
+    reference's rounding, on a sparse lattice where most tiles are culled and
+    on two evaluations in a row (the counts add up)."""
+    n = 4096
+    x, y = lattice(n, seed=21)
+    q = charges(n)
+    want = oracle.near_pairs(x, y, KAPPA)
+    assert 0 < want < n * (n - 1) // 4
+    with _ctx(n, q, x=x, y=y, mode=mode) as c:
+        c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y)
+        near1, tested1 = c.pair_counts()
+        c.force_stage(dm.SHIELDED_COULOMB, 2, 0, x, y)
+        near2, tested2 = c.pair_counts()
+    assert near1 == want and near2 == 2 * want
+    assert tested2 == 2 * tested1 and want <= tested1 < n * n
