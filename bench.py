@@ -177,6 +177,31 @@ def run_ensemble(steps, warmup):
     return out
 
 
+def run_small(wl, steps, heat=0.0, graph=True):
+    """A smaller single-cloud configuration: the hex lattice of wl, FP64, dt
+    check on, `steps` RK4 steps of wl.dt from the lattice (after a warm-up and
+    a re-upload), the step captured as a hipGraph; heat > 0 adds the thermal
+    force (counter-based draws)."""
+    import demon_simulation_code_amd as dm
+
+    x, y, q, m = setup(wl)
+    n = wl.n
+    forces = dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB | (dm.THERMAL if heat else 0)
+    with dm.Context(n, shielding=wl.kappa, confine=wl.confine, gamma=wl.gamma, heat=heat, seed=3,
+                    forces=forces) as c:
+        c.upload(x, y, None, None, q, m)
+        c.use_graph(graph)
+        c.rk4_steps(5, wl.dt, 0.0)
+        c.upload(x, y, None, None, q, m)
+        c.synchronize()
+        t0 = time.perf_counter()
+        c.rk4_steps(steps, wl.dt, 0.0)
+        c.synchronize()
+        wall = time.perf_counter() - t0
+    return {"value": 4.0 * n * (n - 1) * steps / wall, "unit": "pair interactions/s", "n": n, "steps": steps,
+            "rk4_steps_per_s": steps / wall, "wall_s": wall, "thermal_heat_N": heat, "graph": graph}
+
+
 def run_adjoint(reps=3):
     """The differentiable fit: N = 4096 on the 500 um lattice, 200 RK4 steps
     of 1 ms with the dt check, loss on positions every 10th step, checkpoints
@@ -290,6 +315,7 @@ def main():
                     help="all-pairs kernel")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the N=1024 and N=16384 thermal legs")
     ap.add_argument("--plugins-zero", action="store_true",
                     help="add every per-particle force plugin at zero strength (same trajectory; "
                          "profiles what the plugins cost in the stage updates)")
@@ -390,6 +416,15 @@ def main():
         }
         log(f"FP32-pair done: {r32['wall']:.3f} s")
 
+    if not args.no_configs:
+        from demon_simulation_code_amd.physics import PLAIN_1K, THERMAL_16K
+        # the thermal magnitude for about 300 K with the reference's per-stage
+        # model: heatVal = sqrt(43.2 gamma k_B T m / dt)
+        heat = float(np.sqrt(43.2 * THERMAL_16K.gamma * 1.380649e-23 * 300.0 * THERMAL_16K.mass / THERMAL_16K.dt))
+        log("N=1024, 1000 steps; N=16384 thermal, 1000 steps")
+        out["configs"] = {"hex1024_fp64_rk4": run_small(PLAIN_1K, 1000),
+                          "hex16384_fp64_rk4_thermal": run_small(THERMAL_16K, 1000, heat=heat)}
+        log("configs done")
     if not args.no_ensemble:
         log("batched ensemble (256 clouds, 500 steps)")
         out["ensemble"] = run_ensemble(500, 5)
