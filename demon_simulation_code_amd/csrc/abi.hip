@@ -344,11 +344,18 @@ void drop_graph(demon_ctx *c) {
     }
 }
 
+// Steps per captured graph.  Steps beyond a call's count are no-ops (the
+// kernels are gated on Scalars::active, which k_boundary clears when no step
+// is left or t >= t_end), so a call of s steps launches ceil(s / kGraphSteps)
+// graphs; for small clouds the per-launch host cost is the step's cost.
+constexpr int kGraphSteps = 8;
+
 int ensure_graph(demon_ctx *c) {
     if (c->graph) return DEMON_OK;
     hipGraph_t g = nullptr;
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_step(c);
+    int rc = DEMON_OK;
+    for (int k = 0; k < kGraphSteps && !rc; ++k) rc = enqueue_step(c);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
@@ -379,7 +386,7 @@ int run_steps(demon_ctx *c, int64_t steps, Scalars *out) {
     if (c->use_graph) {
         const int rc = ensure_graph(c);
         if (rc) return rc;
-        for (int64_t s = 0; s < steps; ++s) HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+        for (int64_t s = 0; s < steps; s += kGraphSteps) HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
     } else {
         for (int64_t s = 0; s < steps; ++s) {
             const int rc = enqueue_step(c);
