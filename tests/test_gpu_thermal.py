@@ -102,23 +102,36 @@ def test_thermal_statistics_vs_reference():
     """N = 512 from the relaxed cloud, 2000 steps: kinetic temperature and
     g(r) over the second half against the reference's own Runge_Kutta4 with
     its ThermalForce (single-threaded: faster than OpenMP at this size).
-    Temperatures are sampled every 20 steps; each run's standard error comes
-    from five block means, and the two runs must agree within four combined
-    standard errors (and each within 15% of the 300 K heatVal was set for)."""
+    A single run's temperature wanders by more than its block standard error
+    suggests (the velocity autocorrelation time, 1/(2 gamma) = 250 steps, is
+    longer than the blocks), so the device runs six seeds and the reference
+    two (its wall-clock seed changes each second): the means must agree
+    within three standard errors taken from the seed-to-seed spread, and
+    every run must lie within 15% of the 300 K heatVal was set for."""
+    import time
+
     n, steps, every = 512, 2000, 20
     x, y = _relaxed(n, seed=3)
-    q, m, gs = _gpu_snapshots(x, y, steps, every, seed=3)
-    rec = Reference().rk4_record(x, y, q, m, KAPPA, CONFINE, GAMMA, HEAT, 7 | 16, DT, steps, every)
-    late = slice(len(gs) // 2, None)
-    Tg, eg = _block_mean([kinetic_T(s[2], s[3]) for s in gs[late]])
-    Tr, er = _block_mean([kinetic_T(r[2], r[3]) for r in rec[late]])
-    hg = np.mean([pair_hist(s[0], s[1]) for s in gs[late][::5]], axis=0)
-    hr = np.mean([pair_hist(r[0], r[1]) for r in rec[late][::5]], axis=0)
-    l1 = float(np.abs(hg - hr).sum())
-    print(f"N={n}: T_gpu={Tg:.1f}+-{eg:.1f} K, T_ref={Tr:.1f}+-{er:.1f} K (target {T_TARGET} K), "
-          f"g(r) L1 distance {l1:.4f}")
-    assert abs(Tg - Tr) <= 4.0 * np.hypot(eg, er)
-    assert abs(Tg - T_TARGET) / T_TARGET < 0.15 and abs(Tr - T_TARGET) / T_TARGET < 0.15
+    late = slice(steps // every // 2, None)
+    tg, hg = [], []
+    for seed in range(3, 9):
+        q, m, gs = _gpu_snapshots(x, y, steps, every, seed=seed)
+        tg.append(np.mean([kinetic_T(s[2], s[3]) for s in gs[late]]))
+        hg.append(np.mean([pair_hist(s[0], s[1]) for s in gs[late][::5]], axis=0))
+    tr, hr = [], []
+    for k in range(2):
+        if k:
+            time.sleep(1.1)
+        rec = Reference().rk4_record(x, y, q, m, KAPPA, CONFINE, GAMMA, HEAT, 7 | 16, DT, steps, every)
+        tr.append(np.mean([kinetic_T(r[2], r[3]) for r in rec[late]]))
+        hr.append(np.mean([pair_hist(r[0], r[1]) for r in rec[late][::5]], axis=0))
+    Tg, Tr = float(np.mean(tg)), float(np.mean(tr))
+    sd = float(np.std(tg, ddof=1))
+    l1 = float(np.abs(np.mean(hg, axis=0) - np.mean(hr, axis=0)).sum())
+    print(f"N={n}: T_gpu={Tg:.1f} K (seeds {np.round(tg, 1)}), T_ref={Tr:.1f} K (runs {np.round(tr, 1)}), "
+          f"target {T_TARGET} K, g(r) L1 distance {l1:.4f}")
+    assert abs(Tg - Tr) <= 3.0 * sd * np.sqrt(1 / len(tg) + 1 / len(tr))
+    assert all(abs(T - T_TARGET) / T_TARGET < 0.15 for T in tg + tr)
     assert l1 < 0.05
 
 
