@@ -163,8 +163,13 @@ size_t n3_bytes(int64_t npad) {
     return sizeof(double) * half * 2 * (size_t)npad + sizeof(uint32_t) * half * nsb;
 }
 
-// Newton-3 is the default while its reaction buffers stay under this size.
+// Newton-3 is the default while its reaction buffers stay under this size,
+// from this many padded particles on: below it a cloud has too few
+// superblock rows to fill the chip, and the gather kernel, with its j
+// slices, is faster (measured with graphs on one MI355X: 2.1x at N = 1024
+// and 2048, 1.35x at 4096; Newton-3 1.27x faster at 8192, 1.75x at 16384).
 constexpr size_t kN3DefaultBytes = (size_t)16 << 30;
+constexpr int64_t kN3MinNpad = 6144;
 
 int ensure_n3(demon_ctx *c) {
     if (c->d_react) return DEMON_OK;
@@ -587,7 +592,7 @@ int demon_create(demon_ctx **out, int64_t n, const demon_params *p) {
     }
     c->h_tmp.assign(P * 4, 0.0);
     choose_slices(c);
-    if (n3_bytes(c->npad) <= kN3DefaultBytes && c->npad / kSuper / 2 <= kMaxHalf) {
+    if (c->npad >= kN3MinNpad && n3_bytes(c->npad) <= kN3DefaultBytes && c->npad / kSuper / 2 <= kMaxHalf) {
         if (ensure_n3(c)) return bail(DEMON_ERR_ALLOC);
         c->pair_mode = DEMON_PAIRS_NEWTON3;
     }

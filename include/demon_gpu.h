@@ -211,9 +211,10 @@ int demon_last_timing(const demon_ctx *ctx, double *pair_ms, double *total_ms);
 /* All-pairs kernel (no counterpart in the reference's interface, whose
  * ShieldedCoulombForce always uses Newton's third law,
  * ShieldedCoulombForce.cpp:51-56).  NEWTON3 evaluates every unordered pair
- * once and applies it to both particles (the default while its reaction
- * buffers, about 8 * npad^2 / 256 bytes, stay under 16 GiB); GATHER
- * evaluates every ordered pair and keeps no reaction buffers.  Both give
+ * once and applies it to both particles (the default from npad = 6144 on,
+ * while its reaction buffers, about 8 * npad^2 / 256 bytes, stay under
+ * 16 GiB); GATHER evaluates every ordered pair and keeps no reaction
+ * buffers (the default for smaller clouds, where it is faster).  Both give
  * the reference's forces to the same tolerance. */
 #define DEMON_PAIRS_GATHER 0
 #define DEMON_PAIRS_NEWTON3 1

@@ -34,8 +34,12 @@ def _ctx(n, q, forces=dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB, precision=
     return c
 
 
-def test_default_mode_is_newton3():
+def test_default_mode_by_size():
+    """Gather for small clouds (too few superblock rows for the Newton-3
+    kernel to fill the chip), Newton-3 from about 6000 particles on."""
     with _ctx(1024, charges(1024)) as c:
+        assert c.pair_mode() == dm.PAIRS_GATHER
+    with _ctx(8192, charges(8192)) as c:
         assert c.pair_mode() == dm.PAIRS_NEWTON3
 
 

@@ -37,9 +37,9 @@ def _run_both(oracle, x, y, q, m, steps, dt=1e-3, shape=None, graph=False, force
     return g, t, st, o
 
 
-@pytest.mark.parametrize("n,graph,mode", [(256, False, None), (1024, False, None), (1024, True, None),
-                                          (1000, False, None), (1024, False, dm.PAIRS_GATHER),
-                                          (1024, True, dm.PAIRS_GATHER)])
+@pytest.mark.parametrize("n,graph,mode", [(256, False, dm.PAIRS_NEWTON3), (1024, False, dm.PAIRS_NEWTON3),
+                                          (1024, True, dm.PAIRS_NEWTON3), (1000, False, dm.PAIRS_NEWTON3),
+                                          (1024, False, dm.PAIRS_GATHER), (1024, True, dm.PAIRS_GATHER)])
 def test_rk4_100_steps(oracle, n, graph, mode):
     x, y = lattice(n, seed=n)
     q = charges(n)
@@ -113,7 +113,7 @@ def test_rk4_state_is_finite_at_full_size():
     assert all(np.isfinite(a).all() for a in g)
 
 
-@pytest.mark.parametrize("mode", [None, dm.PAIRS_GATHER])
+@pytest.mark.parametrize("mode", [dm.PAIRS_NEWTON3, dm.PAIRS_GATHER])
 def test_rk4_magnetic_time_varying_drag(oracle, mode):
     """forceFlags B,D as the reference's driver builds them: Confinement,
     Magnetic, ShieldedCoulomb, TimeVaryingDrag (no Drag), 100 steps from
@@ -141,7 +141,7 @@ def test_rk4_magnetic_time_varying_drag(oracle, mode):
     assert np.max(np.hypot(o[2] - g[2], o[3] - g[3])) / vscale <= 1e-8
 
 
-@pytest.mark.parametrize("graph,mode", [(False, None), (True, None), (False, dm.PAIRS_GATHER)])
+@pytest.mark.parametrize("graph,mode", [(False, dm.PAIRS_NEWTON3), (True, dm.PAIRS_NEWTON3), (False, dm.PAIRS_GATHER)])
 def test_rk2_100_steps(oracle, graph, mode):
     """Runge_Kutta2 (the driver's -I): the midpoint rule with the dt check,
     against the oracle's restatement; then RK4 again on the same context (the
