@@ -58,6 +58,7 @@ def build(ref: bool = True) -> None:
     targets = ["oracle"]
     if ref and Path("/root/reference/Cloud.cpp").exists():
         targets.append("ref")
+        targets.append("driver")
         if (HERE.parent / "demon_simulation_code_amd" / "libdemon_gpu.so").exists():
             targets.append("dropin")
     subprocess.run(["make", "-s", "-C", str(HERE), *targets], check=True)

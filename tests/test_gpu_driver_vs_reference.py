@@ -1,0 +1,86 @@
+"""The `demon` command against the reference's own driver (its DEMON
+executable, compiled from the reference's sources by `make -C oracle driver`):
+the same options, the same output file -- FORCES, every parameter keyword,
+the CLOUD table, and every TIME_STEP row (times equal; positions within 1e-9
+and velocities within 1e-8 of the largest).  Charges are fixed (-q 6000 0):
+the reference draws them from a wall-clock-seeded generator otherwise.
+Thermal forces are left out for the same reason (their statistics are
+compared in test_gpu_thermal.py)."""
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = Path(__file__).resolve().parent
+REF = HERE.parent / "oracle" / "_ref" / "DEMON"
+OURS = HERE.parent / "demon_simulation_code_amd" / "demon"
+HOST = HERE / "host" / "test_host"
+GOLD = HERE / "golden"
+
+needs_ref = pytest.mark.skipif(not REF.exists(), reason="oracle/_ref/DEMON not built (needs the reference sources)")
+
+
+def read(path, row):
+    lines = subprocess.run([str(HOST), "fits_read", str(path), str(row)], capture_output=True, text=True,
+                           check=True).stdout.splitlines()
+    n, rows, t, forces = lines[0].split()
+    nk = int(lines[1])
+    keys = {ln.split()[0].lower(): float(ln.split()[1]) for ln in lines[2:2 + nk]}
+    arr = np.array([[float(v) for v in ln.split()] for ln in lines[2 + nk:]])
+    return int(n), int(rows), float(t), int(forces), keys, arr
+
+
+def run(exe, args, cwd):
+    r = subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, timeout=300, cwd=cwd)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+def compare(a, b):
+    n, rows, t, forces, keys, _ = read(a, 0)
+    n2, rows2, t2, forces2, keys2, _ = read(b, 0)
+    assert (n, rows, t, forces) == (n2, rows2, t2, forces2)
+    assert keys == keys2
+    for row in range(rows):
+        ta, arr_a = read(a, row)[2], read(a, row)[5]
+        tb, arr_b = read(b, row)[2], read(b, row)[5]
+        assert ta == tb, row
+        np.testing.assert_allclose(arr_a[:, :2], arr_b[:, :2], rtol=1e-15)      # mass, charge
+        rs = max(np.abs(arr_b[:, 2:4]).max(), 1e-30)
+        vs = max(np.abs(arr_b[:, 4:6]).max(), 1e-30)
+        assert np.abs(arr_a[:, 2:4] - arr_b[:, 2:4]).max() <= 1e-9 * rs, row
+        assert np.abs(arr_a[:, 4:6] - arr_b[:, 4:6]).max() <= 1e-8 * vs, row
+
+
+CASES = {
+    "default": ["-n", 64, "-q", 6000, 0, "-e", 0.05],
+    "B,D": ["-n", 100, "-q", 6000, 0, "-e", 0.04, "-B", 0.5, "-D", -1.0, 10.0],
+    "RK2": ["-n", 64, "-q", 6000, 0, "-e", 0.04, "-I"],
+    "plugins": ["-n", 64, "-q", 6000, 0, "-e", 0.03, "-R", 100, 1000, "-S", 1e-15, 0.005, 0.02,
+                "-w", 1e-13, 0.007, 1e-5, "-E", 100, 0.01, "-F", 10, 0.01, "-G", -9.8, "-V", 0.4],
+    "Mach": ["-n", 64, "-q", 6000, 0, "-e", 0.03, "-M", 0.2, 100],
+    "Params.cfg": ["-P", GOLD / "Params.cfg", "-q", 6000, 0, "-e", 0.03],
+}
+
+
+@needs_ref
+@pytest.mark.parametrize("case", list(CASES))
+def test_demon_matches_reference_driver(tmp_path, case):
+    args = CASES[case]
+    run(REF, [*args, "-O", "ref.fits"], tmp_path)
+    run(OURS, [*args, "-O", "ours.fits"], tmp_path)
+    compare(tmp_path / "ours.fits", tmp_path / "ref.fits")
+
+
+@needs_ref
+def test_continue_a_reference_file(tmp_path):
+    """-c on a file the reference's driver wrote: `demon` appends rows equal
+    to those of one uninterrupted reference run."""
+    base = ["-n", 64, "-q", 6000, 0]
+    run(REF, [*base, "-e", 0.04, "-O", "long.fits"], tmp_path)
+    run(REF, [*base, "-e", 0.02, "-O", "part.fits"], tmp_path)
+    run(OURS, ["-c", "part.fits", "-e", 0.04], tmp_path)
+    compare(tmp_path / "part.fits", tmp_path / "long.fits")
