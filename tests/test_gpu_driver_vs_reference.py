@@ -77,10 +77,14 @@ def test_demon_matches_reference_driver(tmp_path, case):
 
 @needs_ref
 def test_continue_a_reference_file(tmp_path):
-    """-c on a file the reference's driver wrote: `demon` appends rows equal
-    to those of one uninterrupted reference run."""
+    """-c on a file the reference's driver wrote: `demon` appends the rows the
+    reference's own -c appends (both restart the output schedule from the
+    file's last TIME, driver.cpp:343,443-448, so these differ from the rows of
+    one uninterrupted run)."""
     base = ["-n", 64, "-q", 6000, 0]
-    run(REF, [*base, "-e", 0.04, "-O", "long.fits"], tmp_path)
     run(REF, [*base, "-e", 0.02, "-O", "part.fits"], tmp_path)
+    shutil.copy(tmp_path / "part.fits", tmp_path / "part_ref.fits")
+    run(REF, ["-c", "part_ref.fits", "-e", 0.04], tmp_path)
     run(OURS, ["-c", "part.fits", "-e", 0.04], tmp_path)
-    compare(tmp_path / "part.fits", tmp_path / "long.fits")
+    assert read(tmp_path / "part.fits", 0)[1] > 3
+    compare(tmp_path / "part.fits", tmp_path / "part_ref.fits")
