@@ -1133,15 +1133,24 @@ int demon_trajectory_spectrum(const double *series, int64_t rows, int64_t stride
         src = packed.data();
     }
     const size_t nser = (size_t)(rows * nuse), half = (size_t)(rows / 2);
+    // a stream of its own, so that the call waits for nothing else on the device
+    hipStream_t st = nullptr;
+    e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipStreamCreateWithFlags");
     double *d = nullptr;
     e = hipMalloc(&d, sizeof(double) * (nser + 3 * (size_t)rows + half + 1));
-    if (e != hipSuccess) return hip_fail(nullptr, e, "hipMalloc");
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(st);
+        return hip_fail(nullptr, e, "hipMalloc");
+    }
     double *d_ser = d, *d_s = d + nser, *d_wre = d_s + rows, *d_wim = d_wre + rows, *d_amp = d_wim + rows;
-    e = hipMemcpy(d_ser, src, sizeof(double) * nser, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = launch_spectrum(d_ser, rows, nuse, d_s, d_wre, d_wim, d_amp, nullptr);
-    if (e == hipSuccess && half) e = hipMemcpy(amp, d_amp, sizeof(double) * half, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    e = hipMemcpyAsync(d_ser, src, sizeof(double) * nser, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_spectrum(d_ser, rows, nuse, d_s, d_wre, d_wim, d_amp, st);
+    if (e == hipSuccess && half) e = hipMemcpyAsync(amp, d_amp, sizeof(double) * half, hipMemcpyDeviceToHost, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = es;
     (void)hipFree(d);
+    (void)hipStreamDestroy(st);
     if (e != hipSuccess) return hip_fail(nullptr, e, "trajectory spectrum");
     return DEMON_OK;
 }
