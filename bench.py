@@ -322,7 +322,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+              "(the single-cloud kernel would shard by superblock rows; that is design only).", file=sys.stderr)
         sys.exit(2)
 
     import demon_simulation_code_amd as dm
