@@ -909,9 +909,13 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     c->ev_used = 0;
     const bool timing = c->timing;
     c->timing = false;
+    // the last segment's stage records are kept as the forward makes them,
+    // so the backward sweep starts without recomputing that segment
+    const int64_t last0 = (n_ckpt - 1) * ckpt_every;
     for (int64_t s = 0; s < steps; ++s) {
         if (s % ckpt_every == 0 && (rc = save_state(c, A + off_ck + (size_t)(s / ckpt_every) * 4 * P))) break;
-        if ((rc = enqueue_step_ex(c, c->p.dt_check, nullptr))) break;
+        double *store = (want && s >= last0) ? A + off_st + (size_t)(s - last0) * 4 * SD : nullptr;
+        if ((rc = enqueue_step_ex(c, c->p.dt_check, store))) break;
         // the step's dt (after the dt check), replayed by the adjoint sweep
         HIP_TRY(c, hipMemcpyAsync(A + off_dt + s, &c->d_sc->dt_step, sizeof(double),
                                   hipMemcpyDeviceToDevice, c->stream));
@@ -943,13 +947,15 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
         const double rc2p = (10.0 / c->p.shielding) * (10.0 / c->p.shielding) * (1.0 + kCutoffSlack);
         for (int64_t seg = n_ckpt - 1; seg >= 0 && !rc; --seg) {
             const int64_t s0 = seg * ckpt_every, s1 = std::min(steps, s0 + ckpt_every);
-            if ((rc = restore_state(c, A + off_ck + (size_t)seg * 4 * P))) break;
-            hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, 0.0, inf, dt,
-                               (unsigned long long)(s1 - s0));
-            for (int64_t s = s0; s < s1 && !rc; ++s)
-                rc = enqueue_step_ex(c, c->p.dt_check, A + off_st + (size_t)(s - s0) * 4 * SD);
-            if (rc) break;
-            HIP_TRY(c, launch_boundary(c->d_sc, 1, c->stream));
+            if (seg != n_ckpt - 1) {  // the last segment's records are still in the store
+                if ((rc = restore_state(c, A + off_ck + (size_t)seg * 4 * P))) break;
+                hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, 0.0, inf, dt,
+                                   (unsigned long long)(s1 - s0));
+                for (int64_t s = s0; s < s1 && !rc; ++s)
+                    rc = enqueue_step_ex(c, c->p.dt_check, A + off_st + (size_t)(s - s0) * 4 * SD);
+                if (rc) break;
+                HIP_TRY(c, launch_boundary(c->d_sc, 1, c->stream));
+            }
             for (int64_t s = s1 - 1; s >= s0; --s) {
                 const bool obs = (s + 1) % obs_every == 0 && (s + 1) / obs_every <= n_obs;
                 const double *seed = obs ? A + off_sd + (size_t)(2 * ((s + 1) / obs_every - 1)) * P : nullptr;
