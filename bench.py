@@ -421,9 +421,9 @@ def main():
         # the thermal magnitude for about 300 K with the reference's per-stage
         # model: heatVal = sqrt(43.2 gamma k_B T m / dt)
         heat = float(np.sqrt(43.2 * THERMAL_16K.gamma * 1.380649e-23 * 300.0 * THERMAL_16K.mass / THERMAL_16K.dt))
-        log("N=1024, 1000 steps; N=16384 thermal, 1000 steps")
+        log("N=1024, 1000 steps; N=16384 thermal, 5000 steps")
         out["configs"] = {"hex1024_fp64_rk4": run_small(PLAIN_1K, 1000),
-                          "hex16384_fp64_rk4_thermal": run_small(THERMAL_16K, 1000, heat=heat)}
+                          "hex16384_fp64_rk4_thermal": run_small(THERMAL_16K, 5000, heat=heat)}
         log("configs done")
     if not args.no_ensemble:
         log("batched ensemble (256 clouds, 500 steps)")
