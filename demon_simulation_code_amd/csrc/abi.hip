@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <new>
@@ -354,6 +355,10 @@ void drop_graph(demon_ctx *c) {
 // is left or t >= t_end), so a call of s steps launches ceil(s / kGraphSteps)
 // graphs; for small clouds the per-launch host cost is the step's cost.
 constexpr int kGraphSteps = 8;
+
+// Device bytes the fit may spend on stage records kept from its forward sweep
+// (the environment variable DEMON_FIT_KEEP_BYTES overrides it).
+constexpr size_t kFitKeepBytes = (size_t)16 << 30;
 
 int ensure_graph(demon_ctx *c) {
     if (c->graph) return DEMON_OK;
@@ -876,7 +881,23 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     const size_t off_ck = 0, off_tg = off_ck + (size_t)n_ckpt * 4 * P;
     const size_t off_sd = off_tg + (size_t)n_obs * 2 * P, off_lp = off_sd + (size_t)n_obs * 2 * P;
     const size_t SD = store_doubles(c->npad);  // one stored stage record
-    const size_t off_st = off_lp + P, off_bar = off_st + (size_t)ckpt_every * 4 * SD;
+    // The forward sweep keeps the stage records of the last kseg segments, as
+    // many as fit in kFitKeepBytes and a quarter of the free device memory (at
+    // least one); the backward sweep recomputes only the segments before them,
+    // into the first segment's slots, which it has consumed by then.
+    int64_t kseg = 1;
+    if (want) {
+        size_t free_b = 0, total_b = 0;
+        HIP_TRY(c, hipMemGetInfo(&free_b, &total_b));
+        const size_t seg_bytes = (size_t)ckpt_every * 4 * SD * sizeof(double);
+        size_t cap = kFitKeepBytes;
+        if (const char *env = std::getenv("DEMON_FIT_KEEP_BYTES")) cap = (size_t)std::strtoull(env, nullptr, 10);
+        const size_t fit = std::min(cap, free_b / 4) / seg_bytes;
+        kseg = std::max<int64_t>(1, std::min<int64_t>(n_ckpt, (int64_t)std::min<size_t>(fit, (size_t)n_ckpt)));
+    }
+    const int64_t kept0 = (n_ckpt - kseg) * ckpt_every;  // first step whose records are kept
+    const int64_t store_steps = std::max(std::min(ckpt_every, steps), steps - kept0);
+    const size_t off_st = off_lp + P, off_bar = off_st + (size_t)store_steps * 4 * SD;
     const size_t off_ib = off_bar + 4 * P, off_u = off_ib + 16 * P, off_g = off_u + 2 * P;
     const size_t off_ap = off_g + 4 * P, off_dt = off_ap + (size_t)c->slices * 4 * P;
     const size_t total = off_dt + (size_t)steps;
@@ -894,10 +915,7 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     }
     HIP_TRY(c, hipMemsetAsync(A + off_lp, 0, sizeof(double) * P, c->stream));
     // store slots need valid padding (nc = -inf) for the adjoint prefilter
-    for (int64_t k = 0; k < ckpt_every * 4; ++k) {
-        HIP_TRY(c, launch_prepare_stage(store_view(A + off_st + (size_t)k * SD, c->npad), 0, c->npad,
-                                        c->stream));
-    }
+    if (want) HIP_TRY(c, launch_prepare_records(A + off_st, store_steps * 4, c->npad, c->stream));
     const double norm = 1.0 / (2.0 * (double)c->n * (double)(n_obs > 0 ? n_obs : 1));
     const double inf = std::numeric_limits<double>::infinity();
 
@@ -909,12 +927,9 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     c->ev_used = 0;
     const bool timing = c->timing;
     c->timing = false;
-    // the last segment's stage records are kept as the forward makes them,
-    // so the backward sweep starts without recomputing that segment
-    const int64_t last0 = (n_ckpt - 1) * ckpt_every;
     for (int64_t s = 0; s < steps; ++s) {
         if (s % ckpt_every == 0 && (rc = save_state(c, A + off_ck + (size_t)(s / ckpt_every) * 4 * P))) break;
-        double *store = (want && s >= last0) ? A + off_st + (size_t)(s - last0) * 4 * SD : nullptr;
+        double *store = (want && s >= kept0) ? A + off_st + (size_t)(s - kept0) * 4 * SD : nullptr;
         if ((rc = enqueue_step_ex(c, c->p.dt_check, store))) break;
         // the step's dt (after the dt check), replayed by the adjoint sweep
         HIP_TRY(c, hipMemcpyAsync(A + off_dt + s, &c->d_sc->dt_step, sizeof(double),
@@ -947,7 +962,9 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
         const double rc2p = (10.0 / c->p.shielding) * (10.0 / c->p.shielding) * (1.0 + kCutoffSlack);
         for (int64_t seg = n_ckpt - 1; seg >= 0 && !rc; --seg) {
             const int64_t s0 = seg * ckpt_every, s1 = std::min(steps, s0 + ckpt_every);
-            if (seg != n_ckpt - 1) {  // the last segment's records are still in the store
+            const bool kept = s0 >= kept0;  // records stored by the forward sweep
+            const int64_t base = kept ? s0 - kept0 : 0;
+            if (!kept) {
                 if ((rc = restore_state(c, A + off_ck + (size_t)seg * 4 * P))) break;
                 hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, 0.0, inf, dt,
                                    (unsigned long long)(s1 - s0));
@@ -961,7 +978,7 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
                 const double *seed = obs ? A + off_sd + (size_t)(2 * ((s + 1) / obs_every - 1)) * P : nullptr;
                 HIP_TRY(c, launch_adj_begin(bar, ibar, u, g, seed, c->m, c->n, c->npad, c->stream));
                 for (int k = 4; k >= 1; --k) {
-                    const StageBuf st = store_view(A + off_st + ((size_t)(s - s0) * 4 + (k - 1)) * SD, c->npad);
+                    const StageBuf st = store_view(A + off_st + ((size_t)(s - s0 + base) * 4 + (k - 1)) * SD, c->npad);
                     HIP_TRY(c, launch_adj_pairs(st, c->q, u, u + P, ap, c->n, c->npad, c->jslice, c->slices,
                                                 c->p.shielding, rc2p, c->stream));
                     HIP_TRY(c, launch_adj_stage(bar, ibar, u, g, k, st, c->q, c->m, ap, c->slices,

@@ -157,7 +157,7 @@ hipError_t launch_update(int stage, const UpdateLaunch &a, hipStream_t s);
 
 // Doubles in one stored stage record (UpdateLaunch::store) and its view.
 constexpr size_t store_doubles(int64_t npad) { return 5 * (size_t)npad + (size_t)npad / 16; }
-inline StageBuf store_view(double *slot, int64_t npad) {
+__host__ __device__ inline StageBuf store_view(double *slot, int64_t npad) {
     const size_t P = (size_t)npad;
     StageBuf b{slot, slot + P, slot + 2 * P, slot + 3 * P, slot + 4 * P};
     b.box = slot + 5 * P;
@@ -183,6 +183,9 @@ hipError_t launch_force_out(const ForceOutLaunch &a, hipStream_t s);
 
 // nc = -(xs^2+ys^2)*kNormShrink for i < n; padding gets xs=ys=NaN, v=0, nc=-inf.
 hipError_t launch_prepare_stage(StageBuf st, int64_t n, int64_t npad, hipStream_t s);
+// `count` consecutive stored stage records from base, every particle slot
+// emptied (padding form) so that only what a step writes is live.
+hipError_t launch_prepare_records(double *base, int64_t count, int64_t npad, hipStream_t s);
 // rmax2 = max |r|^2 over the stage positions (into sc->rmax2_bits, after reset).
 hipError_t launch_rmax(StageBuf st, int64_t n, Scalars *sc, hipStream_t s);
 hipError_t launch_round_fp32(const double *src, float *dst, int64_t npad, hipStream_t s);

@@ -345,8 +345,7 @@ __global__ __launch_bounds__(256) void k_force_out(ForceOutLaunch a) {
     a.fy[i] = fy;
 }
 
-__global__ __launch_bounds__(256) void k_prepare(StageBuf st, int64_t n, int64_t npad) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;  // npad is a multiple of 256
+__device__ inline void prepare_at(const StageBuf &st, int64_t i, int64_t n) {
     double x = 0.0, y = 0.0;
     if (i < n) {
         x = st.xs[i];
@@ -361,6 +360,18 @@ __global__ __launch_bounds__(256) void k_prepare(StageBuf st, int64_t n, int64_t
         st.nc[i] = -__builtin_inf();
     }
     if (st.box) wave_box(i < n, x, y, st.box + 4 * (i / kBoxBlock));
+}
+
+__global__ __launch_bounds__(256) void k_prepare(StageBuf st, int64_t n, int64_t npad) {
+    prepare_at(st, (int64_t)blockIdx.x * 256 + threadIdx.x, n);  // npad is a multiple of 256
+}
+
+// Empties `count` consecutive stored stage records (stride store_doubles(npad)):
+// block b covers particles [256 (b mod bps), +256) of record b / bps.
+__global__ __launch_bounds__(256) void k_prepare_records(double *base, int64_t npad) {
+    const int64_t bps = npad / 256, b = blockIdx.x;
+    const StageBuf st = store_view(base + (size_t)(b / bps) * store_doubles(npad), npad);
+    prepare_at(st, (b % bps) * 256 + threadIdx.x, 0);
 }
 
 __global__ __launch_bounds__(256) void k_rmax(StageBuf st, int64_t n, Scalars *sc) {
@@ -414,6 +425,15 @@ hipError_t launch_prepare_stage(StageBuf st, int64_t n, int64_t npad, hipStream_
 __global__ __launch_bounds__(256) void k_round_fp32(const double *__restrict__ src, float *__restrict__ dst) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     dst[i] = (float)src[i];
+}
+
+hipError_t launch_prepare_records(double *base, int64_t count, int64_t npad, hipStream_t s) {
+    if (npad % 256 || count < 0) return hipErrorInvalidValue;
+    const int64_t blocks = count * (npad / 256);
+    if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prepare_records, dim3((unsigned)blocks), dim3(256), 0, s, base, npad);
+    return hipGetLastError();
 }
 
 hipError_t launch_round_fp32(const double *src, float *dst, int64_t npad, hipStream_t s) {

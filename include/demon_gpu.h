@@ -240,7 +240,11 @@ int demon_pair_counts(const demon_ctx *ctx, int64_t *near_pairs, int64_t *tested
  * steps and recompute, its gradient with respect to the charges (n values),
  * the shielding constant, the confinement constant, gamma and the initial
  * state.  Gradient pointers may be NULL (all NULL: loss only).  The context's
- * state is the initial state again on return.  FP64 only. */
+ * state is the initial state again on return.  FP64 only.
+ * The forward sweep keeps the stage records of the last checkpoint segments,
+ * as many as fit in 16 GiB (or DEMON_FIT_KEEP_BYTES, in bytes) and a quarter
+ * of the free device memory, at least one; only the segments before them are
+ * recomputed.  The result does not depend on how many are kept. */
 int demon_fit_loss_grad(demon_ctx *ctx, int64_t steps, double dt, int64_t obs_every,
                         int64_t ckpt_every, const double *target_x, const double *target_y,
                         double *loss, double *grad_q, double *grad_kappa, double *grad_confine,

@@ -100,6 +100,34 @@ def test_adjoint_matches_cpu_reference_gradient_config5(oracle):
     _vs_cpu(oracle, 4096, 200, 10, 20, seed=11, tol=1e-8, conditioning=True)
 
 
+def test_kept_segments_do_not_change_the_gradient(monkeypatch):
+    """The forward sweep keeps the stage records of as many trailing checkpoint
+    segments as DEMON_FIT_KEEP_BYTES allows and the backward sweep recomputes
+    the others: loss and gradient are the same to the bit whether 1, 2 or all
+    6 segments (40 steps, checkpoints every 7; the last one partial) are kept."""
+    n, steps, obs, ckpt = 256, 40, 10, 7
+    x, y, m, q, tx, ty = _problem(n, steps, obs, seed=3)
+    npad = 1024                                   # n rounded up to 1024
+    seg = ckpt * 4 * (5 * npad + npad // 16) * 8  # bytes of one segment's records
+    runs = []
+    with _ctx(n, x, y, m, q) as c:
+        for keep in (0, 2 * seg + seg // 2, None):
+            if keep is None:
+                monkeypatch.delenv("DEMON_FIT_KEEP_BYTES", raising=False)
+            else:
+                monkeypatch.setenv("DEMON_FIT_KEEP_BYTES", str(keep))
+            runs.append(c.fit_loss_grad(steps, 1e-3, obs, ckpt, tx, ty, state_grad=True))
+            s = c.download()
+            assert np.array_equal(s[0], x) and np.array_equal(s[1], y)
+    (l0, g0), rest = runs[0], runs[1:]
+    assert g0["q"].any() and g0["kappa"] != 0.0
+    for l1, g1 in rest:
+        assert l1 == l0
+        assert g1.keys() == g0.keys()
+        for k in g0:
+            assert np.array_equal(np.asarray(g1[k]), np.asarray(g0[k])), k
+
+
 def test_loss_call_restores_state_and_is_deterministic():
     n, steps, obs = 512, 30, 10
     x, y, m, q, tx, ty = _problem(n, steps, obs, seed=5)
