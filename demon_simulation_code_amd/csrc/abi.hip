@@ -893,7 +893,7 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
         size_t cap = kFitKeepBytes;
         if (const char *env = std::getenv("DEMON_FIT_KEEP_BYTES")) cap = (size_t)std::strtoull(env, nullptr, 10);
         const size_t fit = std::min(cap, free_b / 4) / seg_bytes;
-        kseg = std::max<int64_t>(1, std::min<int64_t>(n_ckpt, (int64_t)std::min<size_t>(fit, (size_t)n_ckpt)));
+        kseg = (int64_t)std::max<size_t>(1, std::min<size_t>(fit, (size_t)n_ckpt));
     }
     const int64_t kept0 = (n_ckpt - kseg) * ckpt_every;  // first step whose records are kept
     const int64_t store_steps = std::max(std::min(ckpt_every, steps), steps - kept0);
