@@ -100,23 +100,28 @@ def test_adjoint_matches_cpu_reference_gradient_config5(oracle):
     _vs_cpu(oracle, 4096, 200, 10, 20, seed=11, tol=1e-8, conditioning=True)
 
 
-def test_kept_segments_do_not_change_the_gradient(monkeypatch):
+@pytest.mark.parametrize("n", [256, 8192])
+def test_kept_segments_do_not_change_the_gradient(monkeypatch, n):
     """The forward sweep keeps the stage records of as many trailing checkpoint
     segments as DEMON_FIT_KEEP_BYTES allows and the backward sweep recomputes
     the others: loss and gradient are the same to the bit whether 1, 2 or all
-    6 segments (40 steps, checkpoints every 7; the last one partial) are kept."""
-    n, steps, obs, ckpt = 256, 40, 10, 7
+    6 segments (40 steps, checkpoints every 7; the last one partial) are kept.
+    N = 8192 runs the forward with the Newton-3 pair kernel, N = 256 with the
+    gather kernel."""
+    steps, obs, ckpt = 40, 10, 7
     x, y, m, q, tx, ty = _problem(n, steps, obs, seed=3)
-    npad = 1024                                   # n rounded up to 1024
+    npad = -(-n // 1024) * 1024                   # n rounded up to 1024
     seg = ckpt * 4 * (5 * npad + npad // 16) * 8  # bytes of one segment's records
     runs = []
     with _ctx(n, x, y, m, q) as c:
+        assert c.pair_mode() == (dm.PAIRS_NEWTON3 if n == 8192 else dm.PAIRS_GATHER)
         for keep in (0, 2 * seg + seg // 2, None):
             if keep is None:
                 monkeypatch.delenv("DEMON_FIT_KEEP_BYTES", raising=False)
             else:
                 monkeypatch.setenv("DEMON_FIT_KEEP_BYTES", str(keep))
             runs.append(c.fit_loss_grad(steps, 1e-3, obs, ckpt, tx, ty, state_grad=True))
+            assert np.isfinite(runs[-1][0]) and np.all(np.isfinite(runs[-1][1]["q"]))
             s = c.download()
             assert np.array_equal(s[0], x) and np.array_equal(s[1], y)
     (l0, g0), rest = runs[0], runs[1:]
