@@ -6,22 +6,29 @@ particles on a hexagonal lattice (500 um spacing, +-50 um jitter), charge
 6000 e, mass 2.8e-14 kg, lambda = 500 um (kappa = 2000 /m), confinement
 k = 5e-11 N/m, drag gamma = 2 /s, thermal off, RK4 with dt = 1e-3 s and the
 reference's dt check, FP64.  A "step" is one RK4 step (4 all-pairs force
-evaluations + the dt check).
+evaluations + the dt check) from the lattice; the timed `--steps K` are the
+first K steps of the 200-step trajectory (after a warm-up and a re-upload).
 
 Prints ONE JSON line:
-  value        pair interactions/s = 4 N (N-1) steps / t   (ordered pairs)
+  value        pair interactions/s = 4 N (N-1) K / t   (ordered pairs)
   rk4_steps_per_s, ms_per_step
-  roofline     the all-pairs kernel: algorithmic FLOP per launch
-               (7 per far ordered pair, 20 per near one; the near count is
-               exact, counted in the kernel) / its mean launch time (HIP
-               events around every launch on the kernel's stream), against
-               the FP64 vector peak of 78.6 TFLOP/s
-  fp32_pair    the same run with FP32 pair arithmetic (speed and error)
-  cpu_baseline the reference's own SSE/OpenMP code (compiled from its
-               sources into oracle/_ref) timed on the host cores on a bounded
-               sample: one force evaluation on a random 32768-particle subset
-               of each of four states spread over the same trajectory, scaled
-               by the pair count
+  roofline     the all-pairs kernel over the timed launches: the FP64 FLOP of
+               the pairs it actually evaluated -- 7 per ordered pair given a
+               distance test, 20 per one inside the cutoff (counted exactly in
+               the kernel) -- over its mean launch time (HIP events around
+               every launch on the kernel's stream), against the FP64 vector
+               peak of 78.6 TFLOP/s.  `effective` prices every ordered pair of
+               N (N-1), the box-culled ones included, at 7 FLOP (the rate the
+               culling buys, not a hardware throughput).
+  update       the fused stage-update kernel: bytes per launch (its loads and
+               stores per particle, plus the all-pairs partial sums and
+               Newton-3 reactions it reads) over its mean launch time
+  c3_200       the whole 200-step trajectory (the cloud implodes: after about
+               100 steps every pair is inside the cutoff), FP64 and FP32-pair
+  fp32_pair    the timed run with FP32 pair arithmetic (speed and error)
+  cpu_baseline the reference's own SSE/OpenMP code (compiled from its sources
+               into oracle/_ref) on the host cores: whole Runge_Kutta4 steps
+               (moveParticles with modifyTimeStep) from the same lattice
 """
 from __future__ import annotations
 
@@ -29,6 +36,8 @@ import argparse
 import json
 import math
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -38,11 +47,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK = 78.6e12       # MI355X FP64 vector, FLOP/s
+FP32_PEAK = 157.3e12      # MI355X FP32 vector, FLOP/s
+HBM_PEAK = 8.0e12         # MI355X HBM3E, B/s
 FLOP_FAR, FLOP_NEAR = 7, 20
-# HBM traffic of the FP64 all-pairs kernel per launch, averaged over the
-# timed launches of the default run: FETCH_SIZE and WRITE_SIZE from two
-# rocprofv3 --pmc passes, scaled by what they read for a plain copy of known
-# size (the summary file gives the commands, the counts and the scale).
+# HBM traffic of the FP64 all-pairs kernel per launch: FETCH_SIZE and
+# WRITE_SIZE from two rocprofv3 --pmc passes over a run of the same
+# command, scaled by what they read for a plain copy of known size (the
+# summary file gives the commands, the counts and the scale).
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pairs_n3_hex131072_fp64_traffic.md")
 
 
@@ -51,24 +62,22 @@ def log(msg):
 
 
 def measured_traffic(workload, steps, warmup, start, mode):
-    """Bytes per launch from the committed counter summary, when it was taken
-    on the same workload, step count, start step and pair kernel as this run;
-    None otherwise."""
+    """Bytes per launch from the committed counter summary, when a run on the
+    same workload, step count, warm-up, start step and pair kernel was
+    counted there; None otherwise."""
     try:
         text = open(TRAFFIC_FILE).read()
     except OSError:
         return None
-    fields = {}
-    for line in text.splitlines():
-        if line.startswith("traffic:"):
-            for kv in line[len("traffic:"):].split():
-                k, _, v = kv.partition("=")
-                fields[k] = v
     want = {"workload": workload, "steps": str(steps), "warmup": str(warmup), "start_step": str(start),
             "pair_kernel": mode}
-    if any(fields.get(k) != v for k, v in want.items()) or "bytes_per_launch" not in fields:
-        return None
-    return float(fields["bytes_per_launch"])
+    for line in text.splitlines():
+        if not line.startswith("traffic:"):
+            continue
+        fields = dict(kv.partition("=")[::2] for kv in line[len("traffic:"):].split())
+        if all(fields.get(k) == v for k, v in want.items()) and "bytes_per_launch" in fields:
+            return float(fields["bytes_per_launch"])
+    return None
 
 
 def setup(wl):
@@ -118,17 +127,23 @@ def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mod
     c.upload(x, y, None, None, q, m)
     t = c.rk4_steps(start, wl.dt, 0.0) if start else 0.0
     n0, t0_ = c.pair_counts()
+    tiles0, items0 = c.pair_tiles()
     c.synchronize()
     t0 = time.perf_counter()
     t = c.rk4_steps(steps, wl.dt, t)
     c.synchronize()
     wall = time.perf_counter() - t0
     pair_ms, total_ms = c.last_timing()
+    kt = c.kernel_timing()
     near, tested = c.pair_counts()
     near, tested = near - n0, tested - t0_
+    tiles, items = c.pair_tiles()
+    tiles, items = tiles - tiles0, items - items0
     xs1, ys1, vx1, vy1 = c.download()
     st = c.stats()
     shape_used = c.pair_shape()
+    mode_used = c.pair_mode()
+    slices = shape_used[1]
     snaps = []
     if snapshots:
         c.upload(x, y, None, None, q, m)
@@ -139,11 +154,71 @@ def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mod
                 done = s
             snaps.append(c.download()[:2])
     c.close()
-    return dict(wall=wall, pair_ms=pair_ms, total_ms=total_ms, t=t, stats=st, near=near,
-                tested=tested, final=(xs1, ys1, vx1, vy1), shape=shape_used, snaps=snaps)
+    return dict(wall=wall, pair_ms=pair_ms, total_ms=total_ms, kt=kt, t=t, stats=st, near=near, tested=tested,
+                tiles=tiles, items=items, final=(xs1, ys1, vx1, vy1), shape=shape_used, mode=mode_used,
+                slices=slices, snaps=snaps)
 
 
-def run_ensemble(steps, warmup):
+def roofline(r, n, launches, peak, kernel):
+    """The all-pairs kernel's FLOP rate over the run's launches: executed
+    (the pairs it distance-tested or evaluated) and effective (every ordered
+    pair of the cloud, culled ones at 7 FLOP)."""
+    ordered = n * (n - 1)
+    near, tested = r["near"], r["tested"]
+    pair_s = r["kt"]["pair_ms"] / 1e3
+    executed = FLOP_FAR * (tested - near) + FLOP_NEAR * near
+    effective = FLOP_FAR * (launches * ordered - near) + FLOP_NEAR * near
+    return {
+        "bound": "valu",
+        "achieved": executed / pair_s / 1e12,
+        "peak": peak / 1e12,
+        "unit": "TFLOP/s",
+        "frac": executed / pair_s / peak,
+        "kernel": kernel,
+        "launches": r["kt"]["pair_launches"],
+        "mean_launch_ms": r["kt"]["pair_ms"] / max(1, r["kt"]["pair_launches"]),
+        "flop_per_launch": executed / launches,
+        "flop_counting": "executed: 7 per ordered pair distance-tested, 20 per ordered pair inside the cutoff",
+        "near_fraction": near / (launches * ordered),
+        "tested_fraction": tested / (launches * ordered),
+        "effective": {"achieved": effective / pair_s / 1e12, "frac": effective / pair_s / peak,
+                      "flop_counting": "every ordered pair of N(N-1), box-culled ones at 7 FLOP"},
+        "list_kernel_ms_per_launch": r["kt"]["list_ms"] / max(1, r["kt"]["pair_launches"]),
+        "pair_kernel_share": r["kt"]["pair_ms"] / r["total_ms"] if r["total_ms"] else None,
+    }
+
+
+# Bytes per particle of one stage update that do not depend on the pair
+# kernel: stage record in (x, y, vx, vy), charge, mass, caller index, base
+# state, stage record out (x, y, vx, vy, nc) and its box table, and the RK
+# increments (written at stages 1-3, read at 3 and 4, base state written at 4;
+# averaged over the four stages).
+UPDATE_FIXED_B = 32 + 16 + 4 + 32 + 40 + 0.5 + (0 + 0 + 32 + 64) / 4 + (32 + 32 + 32 + 0) / 4 + 32 / 4
+
+
+def update_traffic(r, n):
+    """The stage-update kernel's bytes per launch and GB/s: the fixed bytes
+    per particle, plus the pair kernel's i-side sums (one 16-byte pair per
+    particle per j slice or Newton-3 work item) and reactions (256 per
+    evaluated tile) it reads."""
+    import demon_simulation_code_amd as dm
+
+    kt = r["kt"]
+    launches = max(1, kt["update_launches"])
+    if r["mode"] == dm.PAIRS_NEWTON3:
+        partial_b = 16.0 * 256 * (r["items"] + r["tiles"]) / launches
+    else:
+        partial_b = 16.0 * n * r["slices"]
+    total_b = n * UPDATE_FIXED_B + partial_b
+    ms = kt["update_ms"] / launches
+    return {"kernel": "k_update<1..4>", "bytes_per_launch": total_b, "partial_sum_bytes_per_launch": partial_b,
+            "mean_launch_ms": ms, "achieved_GBps": total_b / (ms / 1e3) / 1e9 if ms else None,
+            "peak_GBps": HBM_PEAK / 1e9, "frac": total_b / (ms / 1e3) / HBM_PEAK if ms else None,
+            "bytes_counting": f"{UPDATE_FIXED_B:g} B per particle of state, records and increments, plus 16 B per "
+                              "particle per partial sum and per reaction read"}
+
+
+def run_ensemble(steps, warmup, graph=True):
     """The batched ensemble: 256 clouds, N log-uniform in [64, 8192] (multiples
     of 4 for the dt check), lambda, charge and k per cloud, gamma = 2/s, hex
     lattice (500 um, +-50 um), FP64, dt check on, 500 RK4 steps of 1 ms; one
@@ -162,7 +237,7 @@ def run_ensemble(steps, warmup):
 
     upload()
     b.tune(2)
-    b.use_graph(True)
+    b.use_graph(graph)
     if warmup:
         b.steps(warmup, 1e-3)
     upload()
@@ -170,9 +245,22 @@ def run_ensemble(steps, warmup):
     b.steps(steps, 1e-3)
     wall = time.perf_counter() - t0
     pairs = sum(4.0 * n * (n - 1) for n in sizes) * steps
+    # the N buckets the clouds occupy (floor(log2 npad), npad = N rounded up
+    # to 256) and the form and shape the autotuner chose for each
+    buckets = {}
+    for n in sizes:
+        npad = -(-n // 256) * 256
+        bk = npad.bit_length() - 1
+        buckets[bk] = buckets.get(bk, 0) + 1
+    shapes = {}
+    for bk in sorted(buckets):
+        R, js, us = b.shape(bk)
+        shapes[f"npad_{1 << bk}_{(1 << (bk + 1)) - 1}"] = {
+            "clouds": buckets[bk], "form": "newton3" if R == 0 else "gather",
+            "i_per_lane": None if R == 0 else R,
+            ("superblock_offsets_per_item" if R == 0 else "j_slice"): js, "tuned_us_per_pass": us}
     out = {"value": pairs / wall, "unit": "pair interactions/s", "clouds": len(sizes), "steps": steps,
-           "n_total": int(sum(sizes)), "wall_s": wall, "device_ms": b.last_ms(),
-           "shapes": [list(b.shape(k)[:2]) for k in range(3)]}
+           "n_total": int(sum(sizes)), "wall_s": wall, "device_ms": b.last_ms(), "graph": graph, "shapes": shapes}
     b.close()
     return out
 
@@ -245,58 +333,140 @@ def run_adjoint(reps=3):
             "gradient_over_forward": (fwd + bwd) / fwd, "loss_only_s": loss_only, "wall_s": wall}
 
 
-CPU_SUBSET = 32768
+def fp32_force_error(wl, state):
+    """FP32-pair against FP64 Yukawa forces on one state (the FP64 path is
+    within 1e-12 of the oracle): max over 256 sampled particles of
+    |dF_i| / sum_j |F_ij| (condition-aware)."""
+    import demon_simulation_code_amd as dm
+    from demon_simulation_code_amd.physics import COULOMB
+
+    x, y = state
+    n = wl.n
+    q = np.full(n, wl.charge)
+    m = np.full(n, wl.mass)
+    out = []
+    for prec in (dm.FP64, dm.FP32_PAIR):
+        with dm.Context(n, shielding=wl.kappa, confine=wl.confine, gamma=wl.gamma,
+                        forces=dm.SHIELDED_COULOMB, precision=prec) as c:
+            c.upload(x, y, None, None, q, m)
+            out.append(c.force_stage(dm.SHIELDED_COULOMB, 1, 0, x, y))
+    rows = np.random.default_rng(5).choice(n, 256, replace=False)
+    dx = x[rows, None] - x[None, :]
+    dy = y[rows, None] - y[None, :]
+    d = np.hypot(dx, dy)
+    u = d * wl.kappa
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f = np.abs(COULOMB * wl.charge * wl.charge * (1 + u) * np.exp(-u) / d ** 2)
+    f[(u >= 10.0) | (d == 0)] = 0.0
+    scale = f.sum(axis=1)
+    err = np.hypot(out[0][0][rows] - out[1][0][rows], out[0][1][rows] - out[1][1][rows]) / scale
+    return float(err.max())
 
 
-def cpu_baseline(wl, snaps, snap_steps):
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+CPU_STEPS = 3
+CPU_SUBSET = 16384
+
+
+def cpu_baseline(wl, snaps, snap_steps, threads):
     """The reference's own code (OpenMP build) on the host cores, on a bounded
-    sample of the same workload: ShieldedCoulombForce::force1 at four states
-    spread over the FP64 trajectory.  The per-pair cost grows as the cloud
-    contracts (pairs inside the cutoff need the exp and the division; on the
-    imploded cloud one full N = 131072 evaluation takes minutes), so each
-    evaluation runs on a random subset of CPU_SUBSET particles of that state:
-    its pairs are a uniform sample of the state's pairs, with the same
-    fraction inside the cutoff, and the time is scaled by the pair count
-    N(N-1) / (M(M-1)).  A smaller working set only helps the CPU's caches."""
-    from oracle.bind import Oracle, Reference, default_threads
+    sample of the same workload: whole Runge_Kutta4::moveParticles steps
+    (the dt check and four force evaluations of Confinement, Drag and
+    ShieldedCoulomb) from the same 131072-particle lattice as the timed GPU
+    steps.  Beside it, for the imploded phase of the 200-step trajectory,
+    where one force evaluation of the full cloud takes minutes,
+    ShieldedCoulombForce::force1 on a random CPU_SUBSET-particle subset of
+    the GPU trajectory's state, scaled by the pair count N(N-1)/(M(M-1)): the
+    subset's pairs are a uniform sample of the state's pairs."""
+    from oracle.bind import Oracle, Reference
 
     n = wl.n
-    _, _, q, _ = setup(wl)
-    m_sub = min(n, CPU_SUBSET)
+    x, y, q, m = setup(wl)
+    info = cpu_info()
     if Reference.available(openmp=True):
-        ref, kind, threads = Reference(openmp=True), "reference", default_threads()
+        ref, kind = Reference(openmp=True), "reference"
+        z = np.zeros(n)
+        forces = 1 | 2 | 4  # Confinement, Drag, ShieldedCoulomb (Force.h:75-91)
+        t0 = time.perf_counter()
+        ref.rk4_steps(x, y, z, z, q, m, wl.kappa, wl.confine, wl.gamma, 0.0, forces, wl.dt, CPU_STEPS,
+                      threads=threads)
+        step_s = (time.perf_counter() - t0) / CPU_STEPS
+        log(f"CPU reference: {CPU_STEPS} RK4 steps of N={n} from the lattice on {threads} threads: "
+            f"{step_s:.2f} s per step")
 
-        def evaluate(x, y, qq):
-            ref.yukawa(x, y, qq, wl.kappa, 1, threads=threads)
+        def evaluate(xx, yy, qq):
+            ref.yukawa(xx, yy, qq, wl.kappa, 1, threads=threads)
     else:
         orc, kind, threads = Oracle(), "port", 1
+        step_s = None
 
-        def evaluate(x, y, qq):
-            orc.yukawa(x, y, qq, wl.kappa)
+        def evaluate(xx, yy, qq):
+            orc.yukawa(xx, yy, qq, wl.kappa)
 
     rng = np.random.default_rng(0)
+    m_sub = min(n, CPU_SUBSET)
     scale = n * (n - 1) / (m_sub * (m_sub - 1))
-    times = []
-    for (x, y), s in zip(snaps, snap_steps):
+    imploded = []
+    for (xx, yy), s in zip(snaps, snap_steps):
         idx = np.sort(rng.choice(n, m_sub, replace=False))
-        xs, ys, qs = np.ascontiguousarray(x[idx]), np.ascontiguousarray(y[idx]), q[idx]
+        xs, ys, qs = np.ascontiguousarray(xx[idx]), np.ascontiguousarray(yy[idx]), q[idx]
         t0 = time.perf_counter()
         evaluate(xs, ys, qs)
-        times.append((time.perf_counter() - t0) * scale)
-        log(f"CPU force1 at step {s} on {m_sub} particles: {times[-1] / scale:.2f} s "
-            f"(scaled to N={n}: {times[-1]:.1f} s)")
-    t_eval = sum(times) / len(times)
-    return {
-        "value": n * (n - 1) / t_eval,
+        imploded.append((time.perf_counter() - t0) * scale)
+        log(f"CPU force1 at step {s} on {m_sub} particles: {imploded[-1] / scale:.2f} s "
+            f"(scaled to N={n}: {imploded[-1]:.1f} s)")
+    out = {
         "unit": "pair interactions/s",
         "cores": threads,
         "kind": kind,
-        "rk4_steps_per_s": 1.0 / (4.0 * t_eval),
-        "sample": (f"ShieldedCoulombForce::force1 on a random {m_sub}-particle subset of the FP64 "
-                   f"trajectory's state after steps {list(snap_steps)}, each time scaled by the pair "
-                   f"count to N={n}: " + ", ".join(f"{t:.1f} s" for t in times)
-                   + "; RK4 steps/s counts 4 evaluations per step"),
+        "cpu": info,
+        "threads_note": "OpenMP threads = OMP_NUM_THREADS when set (the CPU share given to this GPU), else the "
+                        "CPUs this process may run on; the reference itself starts omp_get_num_procs() threads "
+                        "(Cloud.cpp:34-36)",
     }
+    if step_s is not None:
+        out.update({
+            "value": 4.0 * n * (n - 1) / step_s,
+            "rk4_steps_per_s": 1.0 / step_s,
+            "s_per_step": step_s,
+            "sample": (f"{CPU_STEPS} whole Runge_Kutta4 steps (modifyTimeStep + 4 x Confinement, Drag, "
+                       f"ShieldedCoulomb) of the N={n} lattice from step 0, the phase the timed GPU steps run"),
+        })
+    if imploded:
+        t_eval = sum(imploded) / len(imploded)
+        out["imploded"] = {
+            "value": n * (n - 1) / t_eval,
+            "rk4_steps_per_s": 1.0 / (4.0 * t_eval),
+            "sample": (f"ShieldedCoulombForce::force1 on a random {m_sub}-particle subset of the GPU FP64 "
+                       f"trajectory's state after steps {list(snap_steps)}, scaled by the pair count to "
+                       f"N={n}: " + ", ".join(f"{t:.1f} s" for t in imploded) + " per evaluation"),
+        }
+    return out
+
+
+def cpu_threads():
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.strip().isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def main():
@@ -307,6 +477,7 @@ def main():
     ap.add_argument("--n", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the whole-trajectory (200-step) leg")
     ap.add_argument("--shape", type=str, default="", help="i_per_lane,j_slices")
     ap.add_argument("--start-step", type=int, default=0,
                     help="time the steps that follow this many untimed ones (to profile a later "
@@ -316,6 +487,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the N=1024 and N=16384 thermal legs")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="plain launches instead of captured graphs in the N=1024/16384 and ensemble legs")
     ap.add_argument("--plugins-zero", action="store_true",
                     help="add every per-particle force plugin at zero strength (same trajectory; "
                          "profiles what the plugins cost in the stage updates)")
@@ -331,40 +504,23 @@ def main():
     wl = LARGE_131K if args.n == LARGE_131K.n else Workload(f"hex{args.n}_fp64", n=args.n, steps=args.steps)
     shape = tuple(int(v) for v in args.shape.split(",")) if args.shape else None
     n, K, W = wl.n, args.steps, args.warmup
-
-    snap_steps = [K // 8, 3 * K // 8, 5 * K // 8, 7 * K // 8] if not args.no_cpu else []
-    log(f"FP64 run: N={n} warmup={W} steps={K}")
+    full = wl.steps  # the whole trajectory of the configuration (200 steps)
     mode = dm.PAIRS_NEWTON3 if args.mode == "newton3" else dm.PAIRS_GATHER
-    r = run_gpu(wl, K, W, dm.FP64, shape, snapshots=snap_steps, start=args.start_step, mode=mode,
-                plugins_zero=args.plugins_zero)
+    kernel = "k_pairs_n3" if mode == dm.PAIRS_NEWTON3 else "k_pairs"
+    run_c3 = not args.no_c3 and (K != full or args.start_step)
+    # states of the imploded phase for the CPU sample (from the whole run)
+    snap_steps = [] if args.no_cpu else [full // 2, 7 * full // 8]
+
+    log(f"FP64 run: N={n} warmup={W} steps={K}")
+    r = run_gpu(wl, K, W, dm.FP64, shape, snapshots=[] if run_c3 else snap_steps, start=args.start_step,
+                mode=mode, plugins_zero=args.plugins_zero)
     pairs_total = 4.0 * n * (n - 1) * K
     value = pairs_total / r["wall"]
     log(f"FP64 done: {r['wall']:.3f} s, {value:.4g} pair interactions/s, shape={r['shape']}")
-
-    ordered = n * (n - 1)
     launches = 4 * K
-    near = r["near"]  # exact count over the timed launches (in-kernel)
-    flop_total = FLOP_FAR * (launches * ordered - near) + FLOP_NEAR * near
-    pair_s = r["pair_ms"] / 1e3
-    achieved = flop_total / pair_s
-    R = r["shape"][0]
-    kernel = "k_pairs_n3<false>" if mode == dm.PAIRS_NEWTON3 else f"k_pairs<{R},false>"
-    roofline = {
-        "bound": "valu",
-        "achieved": achieved / 1e12,
-        "peak": FP64_PEAK / 1e12,
-        "unit": "TFLOP/s",
-        "frac": achieved / FP64_PEAK,
-        "traffic": measured_traffic(wl.name + "_rk4", K, W, args.start_step, args.mode),
-        "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
-        "kernel": kernel + " (FP64 all-pairs)",
-        "mean_launch_ms": pair_s * 1e3 / launches,
-        "flop_per_launch": flop_total / launches,
-        "near_fraction": near / (launches * ordered),
-        "tested_fraction": r["tested"] / (launches * ordered),
-        "pair_kernel_share": r["pair_ms"] / r["total_ms"] if r["total_ms"] else None,
-        "near_pair_rate": near / pair_s,
-    }
+    rl = roofline(r, n, launches, FP64_PEAK, f"{kernel}<false> (FP64 all-pairs)")
+    rl["traffic"] = measured_traffic(wl.name + "_rk4", K, W, args.start_step, args.mode)
+    rl["traffic_unit"] = "bytes per launch (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)"
 
     out = {
         "metric": "pair interactions/s + RK4 steps/s, N=131k, FP64 vs FP32-pair; adjoint grads/s",
@@ -395,7 +551,8 @@ def main():
             "plugins_zero": args.plugins_zero,
         },
         "rk4_steps_per_s": K / r["wall"],
-        "roofline": roofline,
+        "roofline": rl,
+        "update": update_traffic(r, n),
         "dt_reductions": r["stats"].dt_reductions,
     }
 
@@ -412,22 +569,52 @@ def main():
             "speedup_vs_fp64": r["wall"] / r32["wall"],
             "pos_rel_err_vs_fp64_after_steps": perr,
             "steps": K,
-            "mean_launch_ms": r32["pair_ms"] / launches,
+            "roofline": roofline(r32, n, launches, FP32_PEAK, f"{kernel}<true> (FP32-pair)"),
         }
         log(f"FP32-pair done: {r32['wall']:.3f} s")
 
+    c3 = r if not run_c3 else None
+    if run_c3:
+        log(f"whole trajectory: {full} steps, FP64")
+        c3 = run_gpu(wl, full, W, dm.FP64, shape, snapshots=snap_steps, mode=mode)
+    if not args.no_c3:
+        v3 = 4.0 * n * (n - 1) * full / c3["wall"]
+        out["c3_200"] = {
+            "value": v3, "unit": "pair interactions/s", "steps": full, "ms_per_step": c3["wall"] / full * 1e3,
+            "rk4_steps_per_s": full / c3["wall"], "dt_reductions": c3["stats"].dt_reductions,
+            "roofline": roofline(c3, n, 4 * full, FP64_PEAK, f"{kernel}<false> (FP64 all-pairs)"),
+            "update": update_traffic(c3, n),
+        }
+        out["c3_200"]["roofline"]["traffic"] = measured_traffic(wl.name + "_rk4", full, W, 0, args.mode)
+        log(f"whole trajectory done: {c3['wall']:.3f} s, {v3:.4g} pair interactions/s")
+        if not args.no_fp32:
+            log(f"whole trajectory: {full} steps, FP32-pair")
+            c32 = run_gpu(wl, full, W, dm.FP32_PAIR, shape, mode=mode)
+            x64, y64 = c3["final"][0], c3["final"][1]
+            out["c3_200"]["fp32_pair"] = {
+                "value": 4.0 * n * (n - 1) * full / c32["wall"],
+                "speedup_vs_fp64": c3["wall"] / c32["wall"],
+                "roofline": roofline(c32, n, 4 * full, FP32_PEAK, f"{kernel}<true> (FP32-pair)"),
+                "pos_rel_err_vs_fp64_after_200_steps": float(
+                    np.max(np.hypot(x64 - c32["final"][0], y64 - c32["final"][1])) / np.max(np.hypot(x64, y64))),
+                "pos_note": "the imploded cloud is chaotic: FP32 and FP64 trajectories separate",
+                "force_rel_err_vs_fp64_on_final_state": fp32_force_error(wl, (x64, y64)),
+            }
+            log(f"FP32-pair whole trajectory done: {c32['wall']:.3f} s")
+
+    graph = not args.no_graphs
     if not args.no_configs:
         from demon_simulation_code_amd.physics import PLAIN_1K, THERMAL_16K
         # the thermal magnitude for about 300 K with the reference's per-stage
         # model: heatVal = sqrt(43.2 gamma k_B T m / dt)
         heat = float(np.sqrt(43.2 * THERMAL_16K.gamma * 1.380649e-23 * 300.0 * THERMAL_16K.mass / THERMAL_16K.dt))
         log("N=1024, 1000 steps; N=16384 thermal, 5000 steps")
-        out["configs"] = {"hex1024_fp64_rk4": run_small(PLAIN_1K, 1000),
-                          "hex16384_fp64_rk4_thermal": run_small(THERMAL_16K, 5000, heat=heat)}
+        out["configs"] = {"hex1024_fp64_rk4": run_small(PLAIN_1K, 1000, graph=graph),
+                          "hex16384_fp64_rk4_thermal": run_small(THERMAL_16K, 5000, heat=heat, graph=graph)}
         log("configs done")
     if not args.no_ensemble:
         log("batched ensemble (256 clouds, 500 steps)")
-        out["ensemble"] = run_ensemble(500, 5)
+        out["ensemble"] = run_ensemble(500, 5, graph=graph)
         log(f"ensemble done: {out['ensemble']['wall_s']:.3f} s")
     if not args.no_adjoint:
         log("trajectory-fit gradient (N=4096, 200 steps)")
@@ -437,8 +624,12 @@ def main():
     if not args.no_cpu:
         log("CPU baseline (reference code on host cores)")
         try:
-            out["cpu_baseline"] = cpu_baseline(wl, r["snaps"], snap_steps)
-            out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
+            out["cpu_baseline"] = cpu_baseline(wl, c3["snaps"] if c3 else [], snap_steps, cpu_threads())
+            if "value" in out["cpu_baseline"]:
+                out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
+            if "imploded" in out["cpu_baseline"] and "c3_200" in out:
+                out["c3_200"]["gpu_vs_cpu_imploded_sample"] = (
+                    out["c3_200"]["value"] / out["cpu_baseline"]["imploded"]["value"])
         except Exception as e:  # the GPU result stands without it
             out["cpu_baseline"] = {"error": repr(e)}
         log("CPU baseline done")

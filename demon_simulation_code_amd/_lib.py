@@ -116,6 +116,8 @@ SIGNATURES = {
     "demon_get_pair_shape": (ctypes.c_int, [_CTX, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "demon_last_timing": (ctypes.c_int, [_CTX, _D, _D]),
     "demon_pair_counts": (ctypes.c_int, [_CTX, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "demon_pair_tiles": (ctypes.c_int, [_CTX, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "demon_last_kernel_ms": (ctypes.c_int, [_CTX, _D, ctypes.POINTER(_I64), _D, _D, ctypes.POINTER(_I64)]),
     "demon_set_pair_mode": (ctypes.c_int, [_CTX, _I32]),
     "demon_get_pair_mode": (ctypes.c_int, [_CTX, ctypes.POINTER(_I32)]),
     "demon_hex_lattice": (ctypes.c_int, [_I64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _D, _D]),
@@ -349,6 +351,24 @@ class Context:
     def last_timing(self):
         a, b = ctypes.c_double(0), ctypes.c_double(0)
         self._c(lib.demon_last_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def kernel_timing(self):
+        """Device ms of the last rk call by kernel, from HIP events around every
+        plain launch: dict(pair_ms, pair_launches, list_ms, update_ms,
+        update_launches)."""
+        pm, lm, um = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+        pl, ul = _I64(0), _I64(0)
+        self._c(lib.demon_last_kernel_ms(self._h, ctypes.byref(pm), ctypes.byref(pl), ctypes.byref(lm),
+                                         ctypes.byref(um), ctypes.byref(ul)))
+        return dict(pair_ms=pm.value, pair_launches=pl.value, list_ms=lm.value, update_ms=um.value,
+                    update_launches=ul.value)
+
+    def pair_tiles(self):
+        """(Newton-3 superblock tiles evaluated off the diagonal, work items)
+        since upload."""
+        a, b = _I64(0), _I64(0)
+        self._c(lib.demon_pair_tiles(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
     def set_pair_mode(self, mode: int):

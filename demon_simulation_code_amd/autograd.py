@@ -47,10 +47,17 @@ class _FitLoss(torch.autograd.Function):
 
 class TrajectoryFit:
     """Loss L(q, kappa, C, gamma) of a trajectory against targets at every
-    obs_every-th step, L = sum |r - r*|^2 / (2 n n_obs), on one GPU."""
+    obs_every-th step, L = sum |r - r*|^2 / (2 n n_obs), on one GPU.
+
+    dt_check (default True) keeps the reference integrator's adaptive time
+    step: every step's dt comes from Integrator::modifyTimeStep
+    (Integrator.cpp:68-155), as in Runge_Kutta4::moveParticles
+    (Runge_Kutta4.cpp:25), and the adjoint replays the dt of each step.
+    dt_check=False steps with the fixed dt instead, which differs from the
+    reference whenever a tested pair comes within 100 um."""
 
     def __init__(self, x0, y0, mass, targets_x, targets_y, *, steps, dt, obs_every, ckpt_every,
-                 vx0=None, vy0=None, device: int = 0):
+                 vx0=None, vy0=None, device: int = 0, dt_check: bool = True):
         self.n = len(x0)
         self.x0 = np.ascontiguousarray(x0, dtype=np.float64)
         self.y0 = np.ascontiguousarray(y0, dtype=np.float64)
@@ -61,7 +68,7 @@ class TrajectoryFit:
         self.ty = np.ascontiguousarray(targets_y, dtype=np.float64)
         self.steps, self.dt, self.obs_every, self.ckpt_every = int(steps), float(dt), int(obs_every), int(ckpt_every)
         self.sim = Context(self.n, shielding=1.0, forces=CONFINEMENT | DRAG | SHIELDED_COULOMB,
-                           dt_check=False, device=device)
+                           dt_check=bool(dt_check), device=device)
 
     def close(self):
         self.sim.close()

@@ -10,8 +10,11 @@
 //   sx                         scratch stage record for demon_force_stage
 //   acc1[4], acc23[4]          RK increments (k l m n) of stage 1, of 2 (+3)
 //   fout[2]                    force output of demon_force_stage
-//   part[slices][2]            all-pairs partial sums per slice (i side)
-//   react[nsb/2][2], rflag     Newton-3 reactions and tile flags (own buffers)
+//   part                       all-pairs partial sums of the i side: per j
+//                              slice [slices][2][npad] (gather kernel) or per
+//                              work item [items][2][256] (Newton-3 kernel)
+//   react[nsb/2][2], rflag     Newton-3 reactions and tile flags (own buffers,
+//                              with the tile lists and the work plan)
 // plus orig[npad] (caller index of each device slot) and the Scalars block.
 #include <algorithm>
 #include <cmath>
@@ -35,7 +38,7 @@ struct demon_ctx {
     int64_t n = 0, npad = 0;
     hipStream_t stream = nullptr;
     double *d_arena = nullptr;   // 32 arrays of npad doubles
-    double *d_part = nullptr;    // kMaxSlices * 2 * npad
+    double *d_part = nullptr;    // part_doubles(npad)
     int32_t *d_orig = nullptr;
     Scalars *d_sc = nullptr;
     double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *q = nullptr, *m = nullptr;
@@ -49,17 +52,23 @@ struct demon_ctx {
     int32_t slices_req = 0;      // 0 = automatic
     int32_t slices = 1, jslice = 0;  // gather kernel (and the adjoint pair kernel)
     int32_t pair_mode = DEMON_PAIRS_GATHER;
-    int32_t n3_slices = 1, n3_span = 1;  // Newton-3 kernel: superblock offsets per slice
+    int32_t n3_slices = 1;       // Newton-3 kernel: work items per superblock row when every tile is live
     double *d_react = nullptr;   // Newton-3 reactions [nsb/2][2][npad]
     uint32_t *d_rflag = nullptr; // Newton-3 tile flags [nsb][nsb/2]
+    int32_t *d_n3i = nullptr;    // Newton-3 tile lists and item table (n3_ints)
+    N3Plan *d_plan = nullptr;    // Newton-3 work plan
+    int32_t cus = 256;           // compute units of the device
     bool use_graph = false;
     int32_t order = 4;           // integrator of the captured step: 4 (RK4) or 2 (RK2)
     hipGraphExec_t graph = nullptr;
     bool timing = true;
-    std::vector<hipEvent_t> ev;  // pairs of events around each all-pairs launch
+    std::vector<hipEvent_t> ev;  // pairs of events around timed launches (plain launches only)
+    std::vector<uint8_t> ev_kind;// what each pair brackets (TimedKind)
     size_t ev_used = 0;
     hipEvent_t ev_call[2]{};
     double last_pair_ms = 0.0, last_total_ms = 0.0;
+    double last_update_ms = 0.0, last_list_ms = 0.0;
+    int64_t last_pair_launches = 0, last_update_launches = 0;
     int64_t force_evals = 0;
     uint64_t rng_step = 0;       // thermal RNG step counter; survives re-uploads
     // trajectory fit (demon_fit_loss_grad): one arena, grown on demand
@@ -148,20 +157,36 @@ void choose_slices(demon_ctx *c) {
     s = (c->npad + js - 1) / js;
     c->slices = (int32_t)s;
     c->jslice = (int32_t)js;
-    // Newton-3: nsb/2 + 1 superblock offsets (0 = the diagonal) over the
-    // slices, one workgroup of four waves per (superblock row, slice)
+    // Newton-3: a launch is cut into about n3_slices * nsb work items of
+    // consecutive live tiles (one workgroup of four waves each), i.e.
+    // n3_slices items per row when every one of its nsb/2 + 1 tiles is live
     const int64_t nsb = c->npad / kSuper, offsets = nsb / 2 + 1;
     int64_t s3 = c->slices_req > 0 ? c->slices_req : (32768 + 4 * nsb - 1) / (4 * nsb);
-    s3 = std::max<int64_t>(1, std::min<int64_t>({s3, (int64_t)kMaxSlices, offsets}));
-    const int64_t span = (offsets + s3 - 1) / s3;
-    c->n3_span = (int32_t)span;
-    c->n3_slices = (int32_t)((offsets + span - 1) / span);
+    c->n3_slices = (int32_t)std::max<int64_t>(1, std::min<int64_t>({s3, (int64_t)kMaxSlices, offsets}));
 }
 
-// Device memory of the Newton-3 reaction buffers.
+// Newton-3 items of one launch: at most target + nsb (every row's last item
+// may be short), target = n3_slices * nsb <= kMaxSlices * nsb.
+int64_t n3_max_items(int64_t npad) { return (int64_t)(kMaxSlices + 1) * (npad / kSuper); }
+
+// Doubles of the partial-sum buffer: j slices of the gather kernel or work
+// items of the Newton-3 kernel.
+size_t part_doubles(int64_t npad) {
+    return std::max<size_t>((size_t)npad * 2 * kMaxSlices, (size_t)n3_max_items(npad) * 2 * kSuper);
+}
+
+// int32 entries of the Newton-3 tile lists and item table: list
+// [nsb][nsb/2 + 1], cnt [nsb], first [nsb + 1], itemrow [max items].
+size_t n3_ints(int64_t npad) {
+    const size_t nsb = (size_t)(npad / kSuper);
+    return nsb * (nsb / 2 + 1) + nsb + (nsb + 1) + (size_t)n3_max_items(npad);
+}
+
+// Device memory of the Newton-3 reaction buffers, tile lists and plan.
 size_t n3_bytes(int64_t npad) {
     const size_t nsb = (size_t)(npad / kSuper), half = nsb / 2;
-    return sizeof(double) * half * 2 * (size_t)npad + sizeof(uint32_t) * half * nsb;
+    return sizeof(double) * half * 2 * (size_t)npad + sizeof(uint32_t) * half * nsb + sizeof(int32_t) * n3_ints(npad) +
+           sizeof(N3Plan);
 }
 
 // Newton-3 is the default while its reaction buffers stay under this size,
@@ -182,25 +207,49 @@ int ensure_n3(demon_ctx *c) {
         c->d_react = nullptr;
         return fail(c, DEMON_ERR_ALLOC, "hipMalloc of the Newton-3 reaction buffers failed");
     }
-    if (hipMalloc((void **)&c->d_rflag, sizeof(uint32_t) * half * nsb) != hipSuccess) {
-        (void)hipFree(c->d_react);
+    if (hipMalloc((void **)&c->d_rflag, sizeof(uint32_t) * half * nsb) != hipSuccess ||
+        hipMalloc((void **)&c->d_n3i, sizeof(int32_t) * n3_ints(c->npad)) != hipSuccess ||
+        hipMalloc((void **)&c->d_plan, sizeof(N3Plan)) != hipSuccess) {
+        for (void *ptr : {(void *)c->d_react, (void *)c->d_rflag, (void *)c->d_n3i, (void *)c->d_plan})
+            if (ptr) (void)hipFree(ptr);
         c->d_react = nullptr;
         c->d_rflag = nullptr;
-        return fail(c, DEMON_ERR_ALLOC, "hipMalloc of the Newton-3 tile flags failed");
+        c->d_n3i = nullptr;
+        c->d_plan = nullptr;
+        return fail(c, DEMON_ERR_ALLOC, "hipMalloc of the Newton-3 tile flags, lists or plan failed");
     }
-    // no tile evaluated yet: the stage kernels must read no reaction
+    // no tile evaluated yet: the stage kernels must read no reaction and no
+    // item; the plan's counters start at zero
     HIP_TRY(c, hipMemset(c->d_rflag, 0, sizeof(uint32_t) * half * nsb));
+    HIP_TRY(c, hipMemset(c->d_n3i, 0, sizeof(int32_t) * n3_ints(c->npad)));
+    HIP_TRY(c, hipMemset(c->d_plan, 0, sizeof(N3Plan)));
     return DEMON_OK;
+}
+
+N3List n3_list(const demon_ctx *c) {
+    const int64_t nsb = c->npad / kSuper;
+    N3List l{};
+    l.list = c->d_n3i;
+    l.cnt = l.list + nsb * (nsb / 2 + 1);
+    l.first = l.cnt + nsb;
+    l.itemrow = l.first + nsb + 1;
+    l.plan = c->d_plan;
+    l.target = (int32_t)(c->n3_slices * nsb);
+    l.max_items = (int32_t)n3_max_items(c->npad);
+    // a persistent grid of four workgroups per CU (the kernel's occupancy)
+    l.grid = 4 * c->cus;
+    return l;
 }
 
 PairSums pair_sums(const demon_ctx *c) {
     PairSums ps{};
     ps.part = c->d_part;
     if (c->pair_mode == DEMON_PAIRS_NEWTON3) {
-        ps.slices = c->n3_slices;
+        ps.slices = 0;
         ps.react = c->d_react;
         ps.rflag = c->d_rflag;
         ps.nsb = (int32_t)(c->npad / kSuper);
+        ps.first = n3_list(c).first;
     } else {
         ps.slices = c->slices;
     }
@@ -254,6 +303,38 @@ int put(demon_ctx *c, double *dst, const double *src, double fill, size_t slot) 
     return DEMON_OK;
 }
 
+// Launches timed with HIP events on the context stream: the all-pairs
+// kernel, the Newton-3 tile-list kernel and the stage update.  Events are
+// recorded around plain launches only, not into a captured graph.
+enum TimedKind : uint8_t { kTimedPairs = 0, kTimedList = 1, kTimedUpdate = 2 };
+
+void ev_reset(demon_ctx *c) {
+    c->ev_used = 0;
+    c->ev_kind.clear();
+}
+
+int ev_open(demon_ctx *c, bool *on) {
+    *on = c->timing && !c->use_graph;
+    if (!*on) return DEMON_OK;
+    if (c->ev_used + 2 > c->ev.size()) {
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            HIP_TRY(c, hipEventCreate(&e));
+            c->ev.push_back(e);
+        }
+    }
+    HIP_TRY(c, hipEventRecord(c->ev[c->ev_used], c->stream));
+    return DEMON_OK;
+}
+
+int ev_close(demon_ctx *c, bool on, TimedKind kind) {
+    if (!on) return DEMON_OK;
+    HIP_TRY(c, hipEventRecord(c->ev[c->ev_used + 1], c->stream));
+    c->ev_used += 2;
+    c->ev_kind.push_back((uint8_t)kind);
+    return DEMON_OK;
+}
+
 int enqueue_pairs(demon_ctx *c, const StageBuf &st, int gated, int dtcheck) {
     PairLaunch a{};
     a.xs = st.xs; a.ys = st.ys; a.nc = st.nc; a.q = c->q; a.qf = c->qf; a.orig = c->d_orig; a.box = st.box;
@@ -264,31 +345,29 @@ int enqueue_pairs(demon_ctx *c, const StageBuf &st, int gated, int dtcheck) {
     a.kappa = c->p.shielding;
     const double rc = 10.0 / c->p.shielding;
     a.rc2p = rc * rc * (1.0 + kCutoffSlack);
-    const bool ev = c->timing && !c->use_graph;
-    if (ev) {
-        if (c->ev_used + 2 > c->ev.size()) {
-            for (int k = 0; k < 64; ++k) {
-                hipEvent_t e;
-                HIP_TRY(c, hipEventCreate(&e));
-                c->ev.push_back(e);
-            }
-        }
-        HIP_TRY(c, hipEventRecord(c->ev[c->ev_used], c->stream));
-    }
+    bool on = false;
+    int err;
     if (c->pair_mode == DEMON_PAIRS_NEWTON3) {
         a.react = c->d_react;
         a.rflag = c->d_rflag;
-        a.slices = c->n3_slices;
-        a.span = c->n3_span;
+        a.lst = n3_list(c);
+        if ((err = ev_open(c, &on))) return err;
+        HIP_TRY(c, launch_n3_tiles(a, c->stream));
+        if ((err = ev_close(c, on, kTimedList)) || (err = ev_open(c, &on))) return err;
         HIP_TRY(c, launch_pairs_n3(a, c->stream));
     } else {
+        if ((err = ev_open(c, &on))) return err;
         HIP_TRY(c, launch_pairs(a, c->stream));
     }
-    if (ev) {
-        HIP_TRY(c, hipEventRecord(c->ev[c->ev_used + 1], c->stream));
-        c->ev_used += 2;
-    }
-    return DEMON_OK;
+    return ev_close(c, on, kTimedPairs);
+}
+
+int timed_update(demon_ctx *c, int stage, const UpdateLaunch &u) {
+    bool on = false;
+    int rc = ev_open(c, &on);
+    if (rc) return rc;
+    HIP_TRY(c, launch_update(stage, u, c->stream));
+    return ev_close(c, on, kTimedUpdate);
 }
 
 UpdateLaunch make_update(demon_ctx *c, int stage, int dtcheck, double *store) {
@@ -322,7 +401,8 @@ int enqueue_step_ex(demon_ctx *c, int dtcheck, double *store) {
                 return fail(c, DEMON_ERR_UNSUPPORTED,
                             "the dt check runs inside the all-pairs pass; enable ShieldedCoulomb");
             }
-            HIP_TRY(c, launch_update(s, make_update(c, s, dtcheck, nullptr), c->stream));
+            const int rc = timed_update(c, s, make_update(c, s, dtcheck, nullptr));
+            if (rc) return rc;
         }
         return DEMON_OK;
     }
@@ -336,7 +416,8 @@ int enqueue_step_ex(demon_ctx *c, int dtcheck, double *store) {
                         "the dt check runs inside the all-pairs pass; enable ShieldedCoulomb");
         }
         double *slot = store ? store + (size_t)(s - 1) * store_doubles(c->npad) : nullptr;
-        HIP_TRY(c, launch_update(s, make_update(c, s, dtcheck, slot), c->stream));
+        const int rc = timed_update(c, s, make_update(c, s, dtcheck, slot));
+        if (rc) return rc;
     }
     return DEMON_OK;
 }
@@ -391,7 +472,7 @@ __global__ void k_call_setup(Scalars *sc, double t, double t_end, double init_dt
 
 // Queues `steps` steps plus the closing boundary, then reads the scalars back.
 int run_steps(demon_ctx *c, int64_t steps, Scalars *out) {
-    c->ev_used = 0;
+    ev_reset(c);
     HIP_TRY(c, hipEventRecord(c->ev_call[0], c->stream));
     if (c->use_graph) {
         const int rc = ensure_graph(c);
@@ -413,7 +494,11 @@ int run_steps(demon_ctx *c, int64_t steps, Scalars *out) {
     for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
         float pm = 0.0f;
         HIP_TRY(c, hipEventElapsedTime(&pm, c->ev[k], c->ev[k + 1]));
-        c->last_pair_ms += pm;
+        switch (c->ev_kind[k / 2]) {
+            case kTimedPairs: c->last_pair_ms += pm; c->last_pair_launches += 1; break;
+            case kTimedList: c->last_list_ms += pm; break;
+            default: c->last_update_ms += pm; c->last_update_launches += 1; break;
+        }
     }
     c->force_evals += c->order * steps;
     c->rng_step = out->step;
@@ -482,13 +567,18 @@ void set_order(demon_ctx *c, int32_t order) {
     }
 }
 
+void reset_timing(demon_ctx *c) {
+    c->last_pair_ms = c->last_total_ms = c->last_update_ms = c->last_list_ms = 0.0;
+    c->last_pair_launches = c->last_update_launches = 0;
+}
+
 int rk_steps(demon_ctx *c, int32_t order, int64_t steps, double init_dt, double *current_time) {
     if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
     if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
     if (!current_time || steps < 0 || !(init_dt > 0.0)) return fail(c, DEMON_ERR_ARG, "bad steps/dt/time");
     HIP_TRY(c, hipSetDevice(c->p.device));
     set_order(c, order);
-    c->last_pair_ms = c->last_total_ms = 0.0;
+    reset_timing(c);
     hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, *current_time,
                        std::numeric_limits<double>::infinity(), init_dt,
                        (unsigned long long)steps);
@@ -507,7 +597,7 @@ int rk_move(demon_ctx *c, int32_t order, double *current_time, double end_time, 
     if (!current_time || !(init_dt > 0.0)) return fail(c, DEMON_ERR_ARG, "bad dt/time");
     HIP_TRY(c, hipSetDevice(c->p.device));
     set_order(c, order);
-    c->last_pair_ms = c->last_total_ms = 0.0;
+    reset_timing(c);
     Scalars s{};
     HIP_TRY(c, hipMemcpy(&s, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost));
     const unsigned long long before = s.steps_done;
@@ -559,19 +649,21 @@ int demon_create(demon_ctx **out, int64_t n, const demon_params *p) {
     };
     hipError_t e = hipSetDevice(p->device);
     if (e != hipSuccess) return bail(hip_fail(c, e, "hipSetDevice"));
+    e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, p->device);
+    if (e != hipSuccess) return bail(hip_fail(c, e, "hipDeviceGetAttribute"));
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(hip_fail(c, e, "hipStreamCreate"));
     const size_t P = (size_t)c->npad;
     e = hipMalloc((void **)&c->d_arena, sizeof(double) * P * 32);
     if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc state arena failed"));
-    e = hipMalloc((void **)&c->d_part, sizeof(double) * P * 2 * kMaxSlices);
+    e = hipMalloc((void **)&c->d_part, sizeof(double) * part_doubles(c->npad));
     if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc partial sums failed"));
     e = hipMalloc((void **)&c->d_orig, sizeof(int32_t) * P);
     if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc index map failed"));
     e = hipMalloc((void **)&c->d_sc, sizeof(Scalars));
     if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc scalars failed"));
     e = hipMemset(c->d_arena, 0, sizeof(double) * P * 32);
-    if (e == hipSuccess) e = hipMemset(c->d_part, 0, sizeof(double) * P * 2 * kMaxSlices);
+    if (e == hipSuccess) e = hipMemset(c->d_part, 0, sizeof(double) * part_doubles(c->npad));
     if (e == hipSuccess) e = hipMemset(c->d_sc, 0, sizeof(Scalars));
     if (e != hipSuccess) return bail(hip_fail(c, e, "hipMemset"));
     double *b = c->d_arena;
@@ -619,6 +711,8 @@ void demon_destroy(demon_ctx *c) {
     if (c->d_fit) (void)hipFree(c->d_fit);
     if (c->d_react) (void)hipFree(c->d_react);
     if (c->d_rflag) (void)hipFree(c->d_rflag);
+    if (c->d_n3i) (void)hipFree(c->d_n3i);
+    if (c->d_plan) (void)hipFree(c->d_plan);
     if (c->d_an) (void)hipFree(c->d_an);
     for (hipEvent_t e : c->ev_fit)
         if (e) (void)hipEventDestroy(e);
@@ -718,7 +812,7 @@ int demon_force_stage_at(demon_ctx *c, int32_t forces, int32_t stage, uint64_t s
         return rc;
     HIP_TRY(c, launch_prepare_stage(c->sx, c->n, c->npad, c->stream));
     const bool yk = forces & DEMON_FORCE_SHIELDED_COULOMB;
-    c->ev_used = 0;
+    ev_reset(c);
     if (yk && (rc = enqueue_pairs(c, c->sx, 0, 0))) return rc;
     ForceOutLaunch f{};
     f.st = c->sx; f.q = c->q; f.m = c->m; f.orig = c->d_orig; f.pairs = pair_sums(c);
@@ -839,6 +933,27 @@ int demon_last_timing(const demon_ctx *c, double *pair_ms, double *total_ms) {
     return DEMON_OK;
 }
 
+int demon_last_kernel_ms(const demon_ctx *c, double *pair_ms, int64_t *pair_launches, double *list_ms,
+                         double *update_ms, int64_t *update_launches) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (pair_ms) *pair_ms = c->last_pair_ms;
+    if (pair_launches) *pair_launches = c->last_pair_launches;
+    if (list_ms) *list_ms = c->last_list_ms;
+    if (update_ms) *update_ms = c->last_update_ms;
+    if (update_launches) *update_launches = c->last_update_launches;
+    return DEMON_OK;
+}
+
+int demon_pair_tiles(const demon_ctx *c, int64_t *tiles, int64_t *items) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    Scalars s{};
+    hipError_t e = hipMemcpy(&s, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(const_cast<demon_ctx *>(c), e, "hipMemcpy tile count");
+    if (tiles) *tiles = (int64_t)s.n3_tiles;
+    if (items) *items = (int64_t)s.n3_items;
+    return DEMON_OK;
+}
+
 int demon_pair_counts(const demon_ctx *c, int64_t *near_pairs, int64_t *tested_pairs) {
     if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
     Scalars s{};
@@ -924,7 +1039,7 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, 0.0, inf, dt,
                        (unsigned long long)steps);
     HIP_TRY(c, hipGetLastError());
-    c->ev_used = 0;
+    ev_reset(c);
     const bool timing = c->timing;
     c->timing = false;
     for (int64_t s = 0; s < steps; ++s) {

@@ -57,17 +57,41 @@ __device__ __forceinline__ void wave_box(bool real, double x, double y, double *
     }
 }
 
-// Where the all-pairs kernels leave a stage's Yukawa forces (device order):
-// part[s][2][npad] (s < slices), partial sums of the i side; with the
-// Newton-3 kernel also react[d - 1][2][npad] (d = 1 .. nsb/2), reactions
-// from superblock row T - d (mod nsb), valid where rflag[T nsb/2 + d - 1]
-// is nonzero (T = the particle's superblock).
+// Where the all-pairs kernels leave a stage's Yukawa forces (device order).
+// Gather kernel: part[s][2][npad] (s < slices), partial sums per j slice.
+// Newton-3 kernel: the i-side sums of superblock X in the work items
+// first[X] .. first[X + 1] - 1, part[item][2][256] (item order), and the
+// reactions react[d - 1][2][npad] (d = 1 .. nsb/2) from superblock row
+// X - d (mod nsb), valid where rflag[X nsb/2 + d - 1] is nonzero.
 struct PairSums {
     const double *part;
     int32_t slices;
     const double *react;       // null for the gather kernel
     const uint32_t *rflag;
     int32_t nsb;
+    const int32_t *first;      // Newton-3 item ranges per superblock (null: gather)
+};
+
+// Newton-3 work plan of one launch, written by the tile-list kernel: the
+// live tiles of every superblock row, cut into items of `span` consecutive
+// live tiles of one row, each item one workgroup's share.
+struct N3Plan {
+    int32_t span;              // live tiles per item (the last item of a row may hold fewer)
+    int32_t nitems;            // items of the launch
+    uint32_t done;             // row workgroups finished (the last one resets it)
+    uint32_t pad_;
+    uint32_t next[8];          // item counters of the pair kernel, one per XCD
+};
+
+struct N3List {
+    int32_t *list;             // [nsb][nsb/2 + 1] live offsets d of each row, increasing (0: the row itself)
+    int32_t *cnt;              // [nsb] live tiles per row
+    int32_t *first;            // [nsb + 1] first item of each row (first[nsb] = nitems)
+    int32_t *itemrow;          // [max_items] row of each item
+    N3Plan *plan;
+    int32_t target;            // items wanted per launch: span = ceil(live tiles / target)
+    int32_t max_items;         // >= target + nsb
+    int32_t grid;              // workgroups of the persistent pair kernel
 };
 
 // Device scalars of the stepping loop.
@@ -92,6 +116,10 @@ struct Scalars {
     // kCountSlots, so that the atomics of many workgroups do not serialise
     // on one address
     unsigned long long pair_counts[kCountSlots][2];
+    // off-diagonal superblock tiles the Newton-3 kernel evaluated (each
+    // writes 256 reactions) and its work items (each writes 256 i-side
+    // sums), summed over launches
+    unsigned long long n3_tiles, n3_items;
 };
 
 struct PairLaunch {
@@ -100,7 +128,7 @@ struct PairLaunch {
     const double *box;       // per-block bounding boxes of xs/ys, or null
     double *react;           // Newton-3 kernel: reactions [nsb/2][2][npad]
     uint32_t *rflag;         // Newton-3 kernel: tile flags [nsb][nsb/2]
-    int32_t span;            // Newton-3 kernel: superblock offsets per slice
+    N3List lst;              // Newton-3 kernel: tile lists and work plan
     const int32_t *orig;
     double *part;            // [slices][2][npad] partial force sums
     Scalars *sc;
@@ -115,6 +143,9 @@ struct PairLaunch {
 };
 
 hipError_t launch_pairs(const PairLaunch &a, hipStream_t s);
+// The Newton-3 pass: the tile-list kernel, then the pair kernel (in this
+// order on one stream, for every evaluation).
+hipError_t launch_n3_tiles(const PairLaunch &a, hipStream_t s);
 hipError_t launch_pairs_n3(const PairLaunch &a, hipStream_t s);
 
 // Constants of the per-particle force plugins (everything but the pairs),

@@ -186,16 +186,107 @@ using ScrLds = double[4][kScrDoubles]; // per-wave transpose scratch
 constexpr int kBoxChunk = 128;
 using TBoxLds = Box[kBoxChunk];
 
-// One workgroup's share: row S against superblocks S + d for d in [d0, d1),
-// its i-side sums written to part[slice].  Called by all 256 threads.
+// Tile (S, T): the four 64-j blocks of superblock T against the wave's i
+// block; adds the i-side forces into ax/ay and writes the reactions on T's
+// 256 particles to react[d - 1] (negated sums over the four waves in a fixed
+// order).  Called by all 256 threads; ib: the wave's i-block box.
 template <bool F32, bool DTC>
-__device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_t S, int32_t d0, int32_t d1,
-                                       int32_t slice, RedLds &red, ScrLds &scr, TBoxLds &tbox) {
+__device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const ITile<1> &t, const Box &ib,
+                                        bool ireal, int32_t T, int32_t d, double &ax, double &ay,
+                                        uint64_t &n_near, uint64_t &n_tested, float &smin, RedLds &red,
+                                        ScrLds &scr) {
     const double *__restrict__ xs = a.xs;
     const double *__restrict__ ys = a.ys;
     const double *__restrict__ nc = a.nc;
     const double *__restrict__ q = a.q;
     const float *__restrict__ qf = a.qf;
+    const double *__restrict__ box = a.box;
+    const int32_t n = a.n, npad = a.npad;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // the previous tile's sums have been read by all threads
+    __syncthreads();
+    double *r = red[wave];
+    for (int k = lane; k < 2 * kSuper; k += 64) r[k] = 0.0;
+    for (int jb4 = 0; jb4 < 4; ++jb4) {
+        const int32_t jb = T * kSuper + jb4 * kBoxBlock;
+        const BlockClass cls = classify(ib, box + 4 * (jb / kBoxBlock), ireal, jb + kBoxBlock <= n, e.rad, e.cut);
+        if (cls == kSkip) continue;
+        n_tested += 2 * 64 * kBoxBlock;
+        if (cls == kDense) {
+            n_near += 2 * 64 * kBoxBlock;
+            for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
+                double cx[kChunkN3], cy[kChunkN3];
+                PairVal<F32> cq[kChunkN3];
+#pragma unroll
+                for (int u = 0; u < kChunkN3; ++u) {
+                    cx[u] = xs[j0 + u];
+                    cy[u] = ys[j0 + u];
+                    if constexpr (F32) cq[u] = qf[j0 + u];
+                    else cq[u] = q[j0 + u];
+                }
+                chunk_n3<F32, false, DTC>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr[wave], r,
+                                          j0 - T * kSuper);
+            }
+            continue;
+        }
+        for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
+            double cx[kChunkN3], cy[kChunkN3];
+            PairVal<F32> cq[kChunkN3];
+#pragma unroll
+            for (int u = 0; u < kChunkN3; ++u) {
+                cx[u] = xs[j0 + u];
+                cy[u] = ys[j0 + u];
+            }
+            if (!chunk_hit<1, kChunkN3>(t, cx, cy, nc, j0)) continue;
+#pragma unroll
+            for (int u = 0; u < kChunkN3; ++u) {
+                if constexpr (F32) cq[u] = qf[j0 + u];
+                else cq[u] = q[j0 + u];
+            }
+            chunk_n3<F32, true, DTC>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr[wave], r,
+                                     j0 - T * kSuper);
+        }
+    }
+    __syncthreads();
+    // the four waves' sums in a fixed order; the reaction is their negation
+    const int jj = threadIdx.x;
+    const double rx = ((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj];
+    const double ry = ((red[0][2 * jj + 1] + red[1][2 * jj + 1]) + red[2][2 * jj + 1]) + red[3][2 * jj + 1];
+    double *rr = a.react + (size_t)(d - 1) * 2 * npad;
+    rr[T * kSuper + jj] = -rx;
+    rr[npad + T * kSuper + jj] = -ry;
+}
+
+// The pair counts of a workgroup's waves, added with one atomic each to one
+// of kCountSlots counter addresses (device-wide atomics on one address
+// serialise).  Uses red as scratch; called by all 256 threads.
+__device__ __forceinline__ void n3_commit_counts(Scalars *sc, int32_t slot_index, uint64_t n_near,
+                                                 uint64_t n_tested, RedLds &red) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(&red[0][0]);
+    if (lane == 0) {
+        cnt[2 * wave] = n_near;
+        cnt[2 * wave + 1] = n_tested;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long *slot = sc->pair_counts[slot_index & (kCountSlots - 1)];
+        atomicAdd(&slot[0], ((cnt[0] + cnt[2]) + cnt[4]) + cnt[6]);
+        atomicAdd(&slot[1], ((cnt[1] + cnt[3]) + cnt[5]) + cnt[7]);
+    }
+    // red is reused by the caller
+    __syncthreads();
+}
+
+// One workgroup's share in the batched ensemble: row S against superblocks
+// S + d for d in [d0, d1), its i-side sums written to part[slice].  Called by
+// all 256 threads.
+template <bool F32, bool DTC>
+__device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_t S, int32_t d0, int32_t d1,
+                                       int32_t slice, RedLds &red, ScrLds &scr, TBoxLds &tbox) {
     const double *__restrict__ box = a.box;
     const int32_t n = a.n, npad = a.npad;
     const Radii rad = launch_radii(a.rc2p, a.flags, a.sc->rmax2_bits);
@@ -206,7 +297,7 @@ __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_
     const int32_t nsb = npad / kSuper;
     const int32_t half = nsb / 2;
     const int32_t ibase = S * kSuper + wave * 64 + lane;
-    const ITile<1> t = make_itile<1>(xs, ys, nc, q, n, ibase, rad.rf2);
+    const ITile<1> t = make_itile<1>(a.xs, a.ys, a.nc, a.q, n, ibase, rad.rf2);
     const Box ib = box_union(empty_box(), box + 4 * (t.wlo / kBoxBlock));
     Box sbS = empty_box();
 #pragma unroll
@@ -218,7 +309,7 @@ __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_
     for (int32_t d = d0; d < d1; ++d) {
         if (d == 0) {
             // the superblock with itself, every ordered pair, j == i skipped
-            gather_blocks<1, kChunkN3, F32>(t, xs, ys, nc, q, box, ib, n, S * kSuper, S * kSuper + kSuper,
+            gather_blocks<1, kChunkN3, F32>(t, a.xs, a.ys, a.nc, a.q, box, ib, n, S * kSuper, S * kSuper + kSuper,
                                             e, ax, ay, n_near, n_tested, smin);
             continue;
         }
@@ -247,83 +338,14 @@ __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_
             if (threadIdx.x == 0) *fl = 0u;
             continue;
         }
-        // the previous tile's sums have been read by all threads
-        __syncthreads();
-        double *r = red[wave];
-        for (int k = lane; k < 2 * kSuper; k += 64) r[k] = 0.0;
-        for (int jb4 = 0; jb4 < 4; ++jb4) {
-            const int32_t jb = T * kSuper + jb4 * kBoxBlock;
-            const BlockClass cls = classify(ib, box + 4 * (jb / kBoxBlock), ireal, jb + kBoxBlock <= n, rad, e.cut);
-            if (cls == kSkip) continue;
-            n_tested += 2 * 64 * kBoxBlock;
-            if (cls == kDense) {
-                n_near += 2 * 64 * kBoxBlock;
-                for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
-                    double cx[kChunkN3], cy[kChunkN3];
-                    PairVal<F32> cq[kChunkN3];
-#pragma unroll
-                    for (int u = 0; u < kChunkN3; ++u) {
-                        cx[u] = xs[j0 + u];
-                        cy[u] = ys[j0 + u];
-                        if constexpr (F32) cq[u] = qf[j0 + u];
-                        else cq[u] = q[j0 + u];
-                    }
-                    chunk_n3<F32, false, DTC>(t, cx, cy, cq, j0, e, n_near, smin, ax[0], ay[0], scr[wave], r,
-                                              j0 - T * kSuper);
-                }
-                continue;
-            }
-            for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
-                double cx[kChunkN3], cy[kChunkN3];
-                PairVal<F32> cq[kChunkN3];
-#pragma unroll
-                for (int u = 0; u < kChunkN3; ++u) {
-                    cx[u] = xs[j0 + u];
-                    cy[u] = ys[j0 + u];
-                }
-                if (!chunk_hit<1, kChunkN3>(t, cx, cy, nc, j0)) continue;
-#pragma unroll
-                for (int u = 0; u < kChunkN3; ++u) {
-                    if constexpr (F32) cq[u] = qf[j0 + u];
-                    else cq[u] = q[j0 + u];
-                }
-                chunk_n3<F32, true, DTC>(t, cx, cy, cq, j0, e, n_near, smin, ax[0], ay[0], scr[wave], r,
-                                         j0 - T * kSuper);
-            }
-        }
-        __syncthreads();
-        // the four waves' sums in a fixed order; the reaction is their negation
-        const int jj = threadIdx.x;
-        const double rx = ((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj];
-        const double ry = ((red[0][2 * jj + 1] + red[1][2 * jj + 1]) + red[2][2 * jj + 1]) + red[3][2 * jj + 1];
-        double *rr = a.react + (size_t)(d - 1) * 2 * npad;
-        rr[T * kSuper + jj] = -rx;
-        rr[npad + T * kSuper + jj] = -ry;
+        n3_tile<F32, DTC>(a, e, t, ib, ireal, T, d, ax[0], ay[0], n_near, n_tested, smin, red, scr);
         if (threadIdx.x == 0) *fl = 1u;
     }
     if (DTC) commit_dtmin(smin, &a.sc->dtmin_bits);
     double *px = a.part + (size_t)(2 * slice) * npad;
     px[ibase] = ax[0];
     px[npad + ibase] = ay[0];
-    // The pair counters: the four waves' counts meet in LDS (the reaction
-    // buffer, free once every tile is done) and the workgroup adds them with
-    // one atomic each, to one of kCountSlots addresses.  Device-wide atomics
-    // on one address serialise, and a launch whose tiles are mostly culled
-    // would otherwise spend most of its time on them.
-    __syncthreads();
-    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(&red[0][0]);
-    if (lane == 0) {
-        cnt[2 * wave] = n_near;
-        cnt[2 * wave + 1] = n_tested;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long *slot = a.sc->pair_counts[(S + slice) & (kCountSlots - 1)];
-        atomicAdd(&slot[0], ((cnt[0] + cnt[2]) + cnt[4]) + cnt[6]);
-        atomicAdd(&slot[1], ((cnt[1] + cnt[3]) + cnt[5]) + cnt[7]);
-    }
-    // red is reused by the next row of a persistent caller
-    __syncthreads();
+    n3_commit_counts(a.sc, S + slice, n_near, n_tested, red);
 }
 
 }  // namespace n3

@@ -1,6 +1,25 @@
-// pairs_n3.hip -- the single-cloud Newton-3 all-pairs kernel: one workgroup
-// per (row S, slice of superblock offsets); the tile code and the work
-// decomposition are in pairs_n3.h.
+// pairs_n3.hip -- the single-cloud Newton-3 all-pairs pass (the tile code is
+// in pairs_n3.h).  Two launches per force evaluation:
+//
+//  k_n3_tiles   one workgroup per superblock row S: tests the bounding box of
+//               S against the box of every superblock T = S + d it owns
+//               (d = 1 .. nsb/2), writes the tile flags rflag[T][d - 1] and
+//               the row's live offsets, in increasing d (d = 0, the row with
+//               itself, first), into list[S].  The last workgroup to finish
+//               cuts the rows into work items of `span` consecutive live
+//               tiles, span = ceil(live tiles / target), and writes the item
+//               table (first item of each row, row of each item).
+//  k_pairs_n3   a persistent grid: every workgroup takes items from an
+//               atomic counter until none is left; one item = the row's i
+//               particles in registers against its span of live tiles, the
+//               i-side sums written once, to part[item].
+//
+// Only live tiles cost anything: on the benchmark lattice about 1% of the
+// superblock pairs are within the cutoff, and a launch takes as long as
+// those take.  Items are queued per XCD (rows k nsb/8 .. (k+1) nsb/8 - 1 on
+// XCD k, whose workgroups are those with blockIdx.x = k mod 8), which keeps
+// the j superblocks an XCD visits to 5/8 of the cloud, within its own L2;
+// a workgroup whose queue is empty takes items from the other XCDs' queues.
 #include "pairs_n3.h"
 
 namespace demon {
@@ -10,42 +29,221 @@ namespace {
 using namespace pair;
 using namespace n3;
 
+__device__ __forceinline__ Box superblock_box(const double *__restrict__ box, int32_t S) {
+    Box b = empty_box();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b = box_union(b, box + 4 * (S * 4 + k));
+    return b;
+}
+
+__device__ __forceinline__ int32_t load_agent(const int32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Block-wide exclusive prefix sum of one value per thread (256 threads);
+// *total receives the sum.  Ends with a barrier.
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t *lds, int32_t *total) {
+    const int tid = threadIdx.x;
+    lds[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int32_t add = tid >= off ? lds[tid - off] : 0;
+        __syncthreads();
+        lds[tid] += add;
+        __syncthreads();
+    }
+    const int32_t incl = lds[tid];
+    *total = lds[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {
+    __shared__ int32_t wcnt[4];
+    __shared__ int32_t base_s, last_s;
+    __shared__ int32_t scan[256];
+    if ((a.flags & kFlagGated) && !a.sc->active) return;
+    const int32_t nsb = a.npad / kSuper, half = nsb / 2;
+    const int32_t S = (int32_t)blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (S == 0 && threadIdx.x < 8) l.plan->next[threadIdx.x] = 0u;
+    const Radii rad = launch_radii(a.rc2p, a.flags, a.sc->rmax2_bits);
+    const Box sbS = superblock_box(a.box, S);
+    const bool sreal = sbS.x0 <= sbS.x1;  // the row holds real particles
+    int32_t *lst = l.list + (size_t)S * (half + 1);
+    if (threadIdx.x == 0) {
+        base_s = sreal ? 1 : 0;
+        if (sreal) lst[0] = 0;
+    }
+    __syncthreads();
+    for (int32_t d0 = 1; d0 <= half; d0 += 256) {
+        const int32_t d = d0 + (int32_t)threadIdx.x;
+        bool live = false;
+        if (d <= half) {
+            const int32_t T = S + d < nsb ? S + d : S + d - nsb;
+            // for even nsb the pair (S, S + nsb/2) is reached from both rows
+            // and belongs to the lower one
+            const bool owned = !(d == half && !(nsb & 1) && S >= half);
+            if (owned && sreal) live = box_gap2(sbS, superblock_box(a.box, T)) <= rad.rf2;
+            a.rflag[(size_t)T * half + (d - 1)] = live ? 1u : 0u;
+        }
+        const unsigned long long m = __ballot(live);
+        if (lane == 0) wcnt[wave] = __builtin_popcountll(m);
+        __syncthreads();
+        int32_t off = base_s;
+        for (int w = 0; w < wave; ++w) off += wcnt[w];
+        if (live) {
+            const int32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            lst[off + below] = d;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) base_s += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        l.cnt[S] = base_s;
+        atomicAdd(&a.sc->n3_tiles, (unsigned long long)(base_s - (sreal ? 1 : 0)));
+        __threadfence();
+        last_s = atomicAdd(&l.plan->done, 1u) == (uint32_t)(nsb - 1);
+    }
+    __syncthreads();
+    if (!last_s) return;
+    // The last row workgroup: every row's count is in memory.
+    __threadfence();
+    const int32_t per = (nsb + 255) / 256;
+    const int32_t r0 = min((int32_t)threadIdx.x * per, nsb), r1 = min(r0 + per, nsb);
+    int32_t tiles = 0;
+    for (int32_t r = r0; r < r1; ++r) tiles += load_agent(l.cnt + r);
+    int32_t total = 0;
+    (void)block_exclusive_scan(tiles, scan, &total);
+    const int32_t span = max(1, (total + l.target - 1) / l.target);
+    int32_t items = 0;
+    for (int32_t r = r0; r < r1; ++r) items += (load_agent(l.cnt + r) + span - 1) / span;
+    int32_t nitems = 0;
+    int32_t f = block_exclusive_scan(items, scan, &nitems);
+    for (int32_t r = r0; r < r1; ++r) {
+        const int32_t ni = (load_agent(l.cnt + r) + span - 1) / span;
+        l.first[r] = f;
+        for (int32_t k = 0; k < ni; ++k) l.itemrow[f + k] = r;
+        f += ni;
+    }
+    if (threadIdx.x == 0) {
+        l.first[nsb] = nitems;
+        l.plan->span = span;
+        l.plan->nitems = nitems;
+        l.plan->done = 0u;
+        atomicAdd(&a.sc->n3_items, (unsigned long long)nitems);
+    }
+}
+
+// One item: row S's i particles (wave w: block 4 S + w) against the live
+// tiles lst[k0 .. k1); the i-side sums go to part[it].
 template <bool F32, bool DTC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pairs_n3(N3Row a, int32_t span,
+__device__ __forceinline__ void n3_item(const N3Row &a, const PairEnv &e, int32_t S, const int32_t *lst,
+                                        int32_t k0, int32_t k1, int32_t it, uint64_t &n_near, uint64_t &n_tested,
+                                        float &smin, RedLds &red, ScrLds &scr) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int32_t n = a.n, nsb = a.npad / kSuper;
+    const int32_t ibase = S * kSuper + wave * 64 + lane;
+    const ITile<1> t = make_itile<1>(a.xs, a.ys, a.nc, a.q, n, ibase, e.rad.rf2);
+    const Box ib = box_union(empty_box(), a.box + 4 * (t.wlo / kBoxBlock));
+    const bool ireal = t.whi <= n;
+    double ax[1] = {0.0}, ay[1] = {0.0};
+    for (int32_t k = k0; k < k1; ++k) {
+        const int32_t d = __builtin_amdgcn_readfirstlane(lst[k]);
+        if (d == 0) {
+            // the superblock with itself, every ordered pair, j == i skipped
+            gather_blocks<1, kChunkN3, F32>(t, a.xs, a.ys, a.nc, a.q, a.box, ib, n, S * kSuper,
+                                            S * kSuper + kSuper, e, ax, ay, n_near, n_tested, smin);
+            continue;
+        }
+        const int32_t T = S + d < nsb ? S + d : S + d - nsb;
+        n3_tile<F32, DTC>(a, e, t, ib, ireal, T, d, ax[0], ay[0], n_near, n_tested, smin, red, scr);
+    }
+    double *px = a.part + (size_t)it * (2 * kSuper);
+    px[wave * 64 + lane] = ax[0];
+    px[kSuper + wave * 64 + lane] = ay[0];
+}
+
+template <bool F32, bool DTC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pairs_n3(N3Row a, N3List l,
                                                                                    ExpCoef ec) {
     __shared__ RedLds red;
     __shared__ ScrLds scr;
-    __shared__ TBoxLds tbox;
+    __shared__ int32_t slot;
     if ((a.flags & kFlagGated) && !a.sc->active) return;
-    const int32_t nsb = a.npad / kSuper;
-    const int32_t half = nsb / 2;
-    // Workgroups go to the 8 XCDs round-robin by linear id, i.e. by
-    // blockIdx.x mod 8 (nsb is a multiple of 8).  Giving XCD k the rows
-    // [k nsb/8, (k+1) nsb/8) keeps the j superblocks its workgroups visit
-    // (rows + 0 .. nsb/2) to 5/8 of the cloud, which its own L2 then holds.
-    const int32_t S = (nsb & 7) ? (int32_t)blockIdx.x
-                                : (int32_t)(blockIdx.x & 7) * (nsb >> 3) + (int32_t)(blockIdx.x >> 3);
-    const int32_t d0 = (int32_t)blockIdx.y * span;
-    const int32_t d1 = d0 + span < half + 1 ? d0 + span : half + 1;
-    n3_row<F32, DTC>(a, ec, S, d0, d1, (int32_t)blockIdx.y, red, scr, tbox);
+    const int32_t nsb = a.npad / kSuper, half = nsb / 2;
+    const Radii rad = launch_radii(a.rc2p, a.flags, a.sc->rmax2_bits);
+    const PairEnv e{make_cutoff(a.kappa, a.rc2p), rad, fp32_kl2(a.kappa), DTC, a.orig, ec};
+    const int32_t span = __builtin_amdgcn_readfirstlane(l.plan->span);
+    float smin = __builtin_inff();
+    uint64_t n_near = 0, n_tested = 0;
+    const int32_t home = (int32_t)(blockIdx.x & 7);
+    for (int32_t qi = 0; qi < 8; ++qi) {
+        // this XCD's queue first, then the others'
+        const int32_t k = (home + qi) & 7;
+        const int32_t i0 = __builtin_amdgcn_readfirstlane(l.first[k * nsb / 8]);
+        const int32_t i1 = __builtin_amdgcn_readfirstlane(l.first[(k + 1) * nsb / 8]);
+        if (i0 >= i1) continue;
+        for (;;) {
+            __syncthreads();  // the previous item is done with slot and the LDS sums
+            if (threadIdx.x == 0) slot = (int32_t)atomicAdd(&l.plan->next[k], 1u);
+            __syncthreads();
+            const int32_t it = i0 + __builtin_amdgcn_readfirstlane(slot);
+            if (it >= i1) break;
+            const int32_t S = __builtin_amdgcn_readfirstlane(l.itemrow[it]);
+            const int32_t k0 = (it - __builtin_amdgcn_readfirstlane(l.first[S])) * span;
+            const int32_t nl = __builtin_amdgcn_readfirstlane(l.cnt[S]);
+            const int32_t k1 = k0 + span < nl ? k0 + span : nl;
+            n3_item<F32, DTC>(a, e, S, l.list + (size_t)S * (half + 1), k0, k1, it, n_near, n_tested, smin, red,
+                              scr);
+        }
+    }
+    if (DTC) commit_dtmin(smin, &a.sc->dtmin_bits);
+    n3_commit_counts(a.sc, (int32_t)blockIdx.x, n_near, n_tested, red);
 }
 
 }  // namespace
 
-hipError_t launch_pairs_n3(const PairLaunch &a, hipStream_t s) {
+namespace {
+
+// The launch constants of both kernels, after checking that the buffers fit
+// the sizes the kernels index with.
+hipError_t n3_row_args(const PairLaunch &a, N3Row *row) {
     const int64_t nsb = a.npad / kSuper;
-    if (a.npad % kPad || !a.box || !a.react || !a.rflag || a.span < 1 || a.n > a.npad || (a.fp32pair && !a.qf))
+    const N3List &l = a.lst;
+    if (a.npad % kPad || !a.box || !a.react || !a.rflag || a.n > a.npad || (a.fp32pair && !a.qf))
         return hipErrorInvalidValue;
-    if ((int64_t)a.slices * a.span < nsb / 2 + 1) return hipErrorInvalidValue;
+    if (!l.list || !l.cnt || !l.first || !l.itemrow || !l.plan || l.target < 1 || l.grid < 1 ||
+        (int64_t)l.max_items < (int64_t)l.target + nsb || nsb / 2 > kMaxHalf)
+        return hipErrorInvalidValue;
     int32_t flags = 0;
     if (a.dtcheck) flags |= kFlagDtCheck;
     if (a.gated) flags |= kFlagGated;
-    const dim3 grid((unsigned)nsb, (unsigned)a.slices);
-    const N3Row row{a.xs, a.ys, a.nc, a.q, a.qf, a.orig, a.box, a.part, a.react, a.rflag, a.sc,
-                    (int32_t)a.n, (int32_t)a.npad, flags, a.kappa, a.rc2p};
+    *row = N3Row{a.xs, a.ys, a.nc, a.q, a.qf, a.orig, a.box, a.part, a.react, a.rflag, a.sc,
+                 (int32_t)a.n, (int32_t)a.npad, flags, a.kappa, a.rc2p};
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_n3_tiles(const PairLaunch &a, hipStream_t s) {
+    N3Row row;
+    const hipError_t e = n3_row_args(a, &row);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_n3_tiles, dim3((unsigned)(a.npad / kSuper)), dim3(256), 0, s, row, a.lst);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs_n3(const PairLaunch &a, hipStream_t s) {
+    N3Row row;
+    const hipError_t e = n3_row_args(a, &row);
+    if (e != hipSuccess) return e;
     auto kern = a.fp32pair ? (a.dtcheck ? k_pairs_n3<true, true> : k_pairs_n3<true, false>)
                            : (a.dtcheck ? k_pairs_n3<false, true> : k_pairs_n3<false, false>);
-    hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, row, a.span, exp_coef());
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.lst.grid), dim3(256), 0, s, row, a.lst, exp_coef());
     return hipGetLastError();
 }
 

@@ -71,7 +71,7 @@ __device__ __forceinline__ double da(double a1, double a23, double a4) {
 __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys, double vxs,
                                             double vys, double qi, double mi,
                                             const PairSums &ps, const unsigned long long *fmask,
-                                            int64_t npad, int64_t i, const ForceParams &fp,
+                                            int64_t npad, int64_t i, int32_t sb, const ForceParams &fp,
                                             double t_stage, uint64_t step, int32_t stage,
                                             int32_t oi, double &fx, double &fy) {
     fx = 0.0;
@@ -119,6 +119,14 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
         }
     }
     if (forces & 4) {
+        if (ps.first) {
+            // Newton-3: the work items of superblock sb, in item order
+            const int32_t it1 = ps.first[sb + 1], li = (int32_t)(i - (int64_t)sb * kSuper);
+            for (int32_t it = ps.first[sb]; it < it1; ++it) {
+                fx = fx + ps.part[(size_t)it * (2 * kSuper) + li];
+                fy = fy + ps.part[(size_t)it * (2 * kSuper) + kSuper + li];
+            }
+        }
         for (int32_t s = 0; s < ps.slices; ++s) {
             fx = fx + ps.part[(size_t)(2 * s) * npad + i];
             fy = fy + ps.part[(size_t)(2 * s + 1) * npad + i];
@@ -246,7 +254,7 @@ __global__ __launch_bounds__(256) void k_update(UpdateLaunch a) {
         double fx, fy;
         // Runge_Kutta4.cpp:31,50,69,88 (Runge_Kutta2.cpp:31,50): the stage times
         const double ts = FS == 1 ? sc->t : FS == 4 ? sc->t + dt : sc->t + dt / 2.0;
-        stage_force(a.forces, xs, ys, vxs, vys, qi, mi, a.pairs, fmask, a.npad, i, a.fp, ts,
+        stage_force(a.forces, xs, ys, vxs, vys, qi, mi, a.pairs, fmask, a.npad, i, (int32_t)blockIdx.x, a.fp, ts,
                     sc->step, FS, a.orig[i], fx, fy);
         const double k = (dt * fx) / mi;
         const double l = dt * vxs;
@@ -340,7 +348,7 @@ __global__ __launch_bounds__(256) void k_force_out(ForceOutLaunch a) {
     if (i >= a.n) return;
     double fx, fy;
     stage_force(a.forces, a.st.xs[i], a.st.ys[i], a.st.vxs[i], a.st.vys[i], a.q[i], a.m[i],
-                a.pairs, fmask, a.npad, i, a.fp, a.t, a.step, a.stage, a.orig[i], fx, fy);
+                a.pairs, fmask, a.npad, i, (int32_t)blockIdx.x, a.fp, a.t, a.step, a.stage, a.orig[i], fx, fy);
     a.fx[i] = fx;
     a.fy[i] = fy;
 }

@@ -208,6 +208,16 @@ int demon_get_pair_shape(const demon_ctx *ctx, int32_t *i_per_lane, int32_t *j_s
  * all-pairs kernels and everything else (HIP events on the context stream). */
 int demon_last_timing(const demon_ctx *ctx, double *pair_ms, double *total_ms);
 
+/* The same split further (instrumentation; no counterpart in the reference):
+ * device milliseconds of the all-pairs kernel and its launch count, of the
+ * Newton-3 tile-list kernel, and of the stage-update kernels (the fused
+ * CacheOperator / force epilogue / RK increment pass) and their launch
+ * count, from HIP events around each launch of the last demon_rk4_* /
+ * demon_rk2_* call.  Steps replayed from a captured graph are not split
+ * (zero). */
+int demon_last_kernel_ms(const demon_ctx *ctx, double *pair_ms, int64_t *pair_launches, double *list_ms,
+                         double *update_ms, int64_t *update_launches);
+
 /* All-pairs kernel (no counterpart in the reference's interface, whose
  * ShieldedCoulombForce always uses Newton's third law,
  * ShieldedCoulombForce.cpp:51-56).  NEWTON3 evaluates every unordered pair
@@ -227,6 +237,14 @@ int demon_get_pair_mode(const demon_ctx *ctx, int32_t *mode);
  * pairs that reached a distance test at all (pairs in 64-particle blocks
  * whose bounding boxes are farther apart than the cutoff are never tested). */
 int demon_pair_counts(const demon_ctx *ctx, int64_t *near_pairs, int64_t *tested_pairs);
+
+/* Newton-3 superblock tiles (256 x 256 particles, off the diagonal) whose
+ * bounding boxes came within the cutoff, and the work items (runs of live
+ * tiles of one superblock row) they were cut into, summed over the
+ * evaluations since the last upload (instrumentation; no counterpart in the
+ * reference).  Each tile leaves 256 reactions and each item 256 partial sums
+ * for the stage update.  Either pointer may be NULL. */
+int demon_pair_tiles(const demon_ctx *ctx, int64_t *tiles, int64_t *items);
 
 /* Differentiable trajectory fit (no counterpart in the reference, which has
  * no differentiation; it differentiates Runge_Kutta4::moveParticles with

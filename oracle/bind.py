@@ -88,6 +88,13 @@ class Oracle:
         L.oracle_yukawa_rows.argtypes = [_I64, _D, _D, _D, ctypes.c_double,
                                          ctypes.POINTER(ctypes.c_int64), _I64, _D, _D]
         L.oracle_yukawa_rows.restype = None
+        L.oracle_yukawa_cells_add.argtypes = [_I64, _D, _D, _D, ctypes.c_double, _D, _D]
+        L.oracle_yukawa_cells_add.restype = ctypes.c_int
+        L.oracle_dtcheck_min_s_cells.argtypes = [_I64, _D, _D, ctypes.c_double]
+        L.oracle_dtcheck_min_s_cells.restype = ctypes.c_float
+        L.oracle_modify_time_step_cells.argtypes = [_I64, _D, _D, ctypes.c_float, ctypes.c_double,
+                                                    ctypes.c_double]
+        L.oracle_modify_time_step_cells.restype = ctypes.c_double
         L.oracle_near_pairs.argtypes = [_I64, _D, _D, ctypes.c_double]
         L.oracle_near_pairs.restype = ctypes.c_int64
         L.oracle_confinement_add.argtypes = [_I64, _D, _D, _D, ctypes.c_double, _D, _D]
@@ -130,6 +137,15 @@ class Oracle:
             assert rc == 0
         else:
             self.L.oracle_yukawa_gather_add(n, p(x), p(y), p(q), kappa, p(fx), p(fy))
+        return fx, fy
+
+    def yukawa_cells(self, x, y, q, kappa, fx=None, fy=None):
+        """oracle_yukawa_add's result from a cell list (pairs in range only)."""
+        n = len(x)
+        fx = np.zeros(n) if fx is None else fx
+        fy = np.zeros(n) if fy is None else fy
+        rc = self.L.oracle_yukawa_cells_add(n, p(_f64(x)), p(_f64(y)), p(_f64(q)), kappa, p(fx), p(fy))
+        assert rc == 0, "oracle_yukawa_cells_add rejected the input (odd n?)"
         return fx, fy
 
     def yukawa_gather(self, x, y, q, kappa):
@@ -209,6 +225,12 @@ class Oracle:
 
     def dtcheck_min_s(self, x, y) -> float:
         return self.L.oracle_dtcheck_min_s(len(x), p(_f64(x)), p(_f64(y)))
+
+    def dtcheck_min_s_cells(self, x, y, cell=5e-3) -> float:
+        return self.L.oracle_dtcheck_min_s_cells(len(x), p(_f64(x)), p(_f64(y)), cell)
+
+    def modify_time_step_cells(self, x, y, dist=1.0e-4, dt=1.0e-3, cell=5e-3) -> float:
+        return self.L.oracle_modify_time_step_cells(len(x), p(_f64(x)), p(_f64(y)), dist, dt, cell)
 
     def covers(self, i, j) -> bool:
         return bool(self.L.oracle_dtcheck_covers(i, j))

@@ -96,6 +96,241 @@ int oracle_yukawa_add(int64_t n, const double *xs, const double *ys,
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Cell-list form of oracle_yukawa_add (same results, pairs in range only)  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct cell_grid {
+    double x0, y0, h;
+    int64_t nx, ny;
+    int64_t *start;  /* [nx*ny + 1] */
+    int64_t *idx;    /* particles sorted by cell, ascending index within a cell */
+} cell_grid;
+
+static void cell_of(const cell_grid *g, double x, double y, int64_t *cx, int64_t *cy)
+{
+    int64_t a = (int64_t)floor((x - g->x0) / g->h), b = (int64_t)floor((y - g->y0) / g->h);
+    *cx = a < 0 ? 0 : a >= g->nx ? g->nx - 1 : a;
+    *cy = b < 0 ? 0 : b >= g->ny ? g->ny - 1 : b;
+}
+
+/* Cells of side h over the bounding box (at most about 4 n cells). */
+static int cell_build(cell_grid *g, int64_t n, const double *x, const double *y, double h)
+{
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int64_t i = 0; i < n; ++i) {
+        if (x[i] < x0) x0 = x[i];
+        if (x[i] > x1) x1 = x[i];
+        if (y[i] < y0) y0 = y[i];
+        if (y[i] > y1) y1 = y[i];
+    }
+    const double lim = 2.0 * sqrt((double)n) + 2.0;
+    if ((x1 - x0) / h > lim || (y1 - y0) / h > lim)
+        h = fmax(x1 - x0, y1 - y0) / lim;
+    g->x0 = x0; g->y0 = y0; g->h = h;
+    g->nx = (int64_t)((x1 - x0) / h) + 1;
+    g->ny = (int64_t)((y1 - y0) / h) + 1;
+    const int64_t nc = g->nx * g->ny;
+    g->start = (int64_t *)calloc((size_t)nc + 1, sizeof(int64_t));
+    g->idx = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    int64_t *cell = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    if (!g->start || !g->idx || !cell) {
+        free(g->start); free(g->idx); free(cell);
+        return -1;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t cx, cy;
+        cell_of(g, x[i], y[i], &cx, &cy);
+        cell[i] = cy * g->nx + cx;
+        g->start[cell[i] + 1] += 1;
+    }
+    for (int64_t c = 0; c < nc; ++c) g->start[c + 1] += g->start[c];
+    int64_t *fill = (int64_t *)malloc((size_t)(nc > 0 ? nc : 1) * sizeof(int64_t));
+    if (!fill) {
+        free(g->start); free(g->idx); free(cell);
+        return -1;
+    }
+    memcpy(fill, g->start, (size_t)nc * sizeof(int64_t));
+    for (int64_t i = 0; i < n; ++i) g->idx[fill[cell[i]]++] = i;
+    free(fill);
+    free(cell);
+    return 0;
+}
+
+static void cell_free(cell_grid *g)
+{
+    free(g->start);
+    free(g->idx);
+}
+
+/* One live pair (a < b) as the reference computes it: the intra-pair formula
+ * for a pair (p, p+1) of one load (ShieldedCoulombForce.cpp:138-159), the
+ * cross-pair one otherwise (:172-198); dX = x_a - x_b, added to a and
+ * subtracted from b.  Returns 0 if the pair is beyond the cutoff. */
+static int yk_pair(int64_t a, int64_t b, const double *xs, const double *ys, const double *q,
+                   double kappa, double coul, double *fX, double *fY)
+{
+    const double dX = xs[a] - xs[b];
+    const double dY = ys[a] - ys[b];
+    const double d = sqrt(dX * dX + dY * dY);
+    const double u = d * kappa;
+    if (!(u < 10.0))
+        return 0;
+    const double qq = q[a] * q[b];
+    double fc;
+    if ((a >> 1) == (b >> 1))
+        fc = coul * qq * (1.0 + u) / (d * d * d * exp(u));
+    else
+        fc = coul * qq * (1.0 + u) * exp(-u) / (d * d * d);
+    *fX = fc * dX;
+    *fY = fc * dY;
+    return 1;
+}
+
+/* Order of particle k's contributions in the reference's serial loop:
+ * by the partner's pair index j/2, and within one partner pair the straight
+ * lane (j % 2 == k % 2) before the swapped one (ShieldedCoulombForce.cpp:52-55). */
+static int64_t yk_order(int64_t k, int64_t j) { return 2 * (j >> 1) + ((j & 1) != (k & 1)); }
+
+static int cmp_i64(const void *a, const void *b)
+{
+    const int64_t u = *(const int64_t *)a, v = *(const int64_t *)b;
+    return (u > v) - (u < v);
+}
+
+int oracle_yukawa_cells_add(int64_t n, const double *xs, const double *ys,
+                            const double *q, double kappa, double *fx, double *fy)
+{
+    if (n < 2 || (n & 1))
+        return -1;
+    const double coul = oracle_coulomb_const();
+    cell_grid g;
+    /* cells wider than the cutoff radius 10/kappa (the margin covers the
+       rounding of u): every live pair lies in neighbouring cells */
+    if (cell_build(&g, n, xs, ys, (10.0 / kappa) * (1.0 + 1e-6)))
+        return -1;
+    int failed = 0;
+#pragma omp parallel
+    {
+        int64_t cap = 1024, len = 0;
+        int64_t *key = (int64_t *)malloc((size_t)cap * sizeof(int64_t));
+        if (!key) {
+#pragma omp atomic write
+            failed = 1;
+        }
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t k = 0; k < n; ++k) {
+            if (!key)
+                continue;
+            int64_t cx, cy;
+            cell_of(&g, xs[k], ys[k], &cx, &cy);
+            len = 0;
+            for (int64_t by = cy - 1; by <= cy + 1; ++by) {
+                if (by < 0 || by >= g.ny)
+                    continue;
+                for (int64_t bx = cx - 1; bx <= cx + 1; ++bx) {
+                    if (bx < 0 || bx >= g.nx)
+                        continue;
+                    const int64_t c = by * g.nx + bx;
+                    for (int64_t s = g.start[c]; s < g.start[c + 1]; ++s) {
+                        const int64_t j = g.idx[s];
+                        if (j == k)
+                            continue;
+                        double fX, fY;
+                        if (!yk_pair(k < j ? k : j, k < j ? j : k, xs, ys, q, kappa, coul, &fX, &fY))
+                            continue;
+                        if (len == cap) {
+                            int64_t *nk = (int64_t *)realloc(key, (size_t)(2 * cap) * sizeof(int64_t));
+                            if (!nk) {
+#pragma omp atomic write
+                                failed = 1;
+                                break;
+                            }
+                            key = nk;
+                            cap *= 2;
+                        }
+                        /* order key in the high bits, partner index in the low 32 */
+                        key[len++] = (yk_order(k, j) << 32) | j;
+                    }
+                }
+            }
+            qsort(key, (size_t)len, sizeof(int64_t), cmp_i64);
+            double ax = fx[k], ay = fy[k];
+            for (int64_t e = 0; e < len; ++e) {
+                const int64_t j = key[e] & 0xFFFFFFFFll;
+                double fX = 0.0, fY = 0.0;
+                if (k < j) {
+                    yk_pair(k, j, xs, ys, q, kappa, coul, &fX, &fY);
+                    ax += fX;
+                    ay += fY;
+                } else {
+                    yk_pair(j, k, xs, ys, q, kappa, coul, &fX, &fY);
+                    ax -= fX;
+                    ay -= fY;
+                }
+            }
+            fx[k] = ax;
+            fy[k] = ay;
+        }
+        free(key);
+    }
+    cell_free(&g);
+    return failed ? -1 : 0;
+}
+
+float oracle_dtcheck_min_s_cells(int64_t n, const double *x, const double *y, double cell)
+{
+    cell_grid g;
+    if (cell_build(&g, n, x, y, cell))
+        return NAN;
+    /* the double distance of any pair the FP32 test can find within 1e-4 m
+       is below 1.01e-4 m (coordinate rounding is at most 2^-24 |x|) */
+    const double r2 = 1.01e-4 * 1.01e-4;
+    float best = INFINITY;
+#pragma omp parallel for schedule(dynamic, 256) reduction(min : best)
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t cx, cy;
+        cell_of(&g, x[i], y[i], &cx, &cy);
+        for (int64_t by = cy - 1; by <= cy + 1; ++by) {
+            if (by < 0 || by >= g.ny)
+                continue;
+            for (int64_t bx = cx - 1; bx <= cx + 1; ++bx) {
+                if (bx < 0 || bx >= g.nx)
+                    continue;
+                const int64_t c = by * g.nx + bx;
+                for (int64_t s = g.start[c]; s < g.start[c + 1]; ++s) {
+                    const int64_t j = g.idx[s];
+                    if (j <= i)
+                        continue;
+                    const double ex = x[i] - x[j], ey = y[i] - y[j];
+                    if (ex * ex + ey * ey > r2 || !oracle_dtcheck_covers(i, j))
+                        continue;
+                    const float sx = (float)x[i] - (float)x[j];
+                    const float sy = (float)y[i] - (float)y[j];
+                    const float s2 = sx * sx + sy * sy;
+                    if (s2 < best)
+                        best = s2;
+                }
+            }
+        }
+    }
+    cell_free(&g);
+    return best;
+}
+
+double oracle_modify_time_step_cells(int64_t n, const double *x, const double *y, float dist, double dt,
+                                     double cell)
+{
+    const float s = oracle_dtcheck_min_s_cells(n, x, y, cell);
+    for (int k = 0; sqrtf(s) <= dist; ++k) {
+        if (k > 60)
+            return NAN; /* coincident particles: the reference loops forever */
+        dist /= 10.0f;
+        dt /= 10.0f;
+    }
+    return dt;
+}
+
 void oracle_yukawa_gather_add(int64_t n, const double *xs, const double *ys,
                               const double *q, double kappa, double *fx, double *fy)
 {
@@ -565,7 +800,10 @@ int oracle_stage_force(int64_t n, const double *xs, const double *ys,
     if (f & ORACLE_ROTATIONAL)
         oracle_rotational_add(n, xs, ys, p->rot_inner, p->rot_outer, p->rot_const, fx, fy);
     if (p->forces & ORACLE_SHIELDED_COULOMB) {
-        if ((n & 1) || p->gather)
+        if (p->gather == 2 && !(n & 1)) {
+            if (oracle_yukawa_cells_add(n, xs, ys, q, p->shielding, fx, fy))
+                return -1;
+        } else if ((n & 1) || p->gather)
             oracle_yukawa_gather_add(n, xs, ys, q, p->shielding, fx, fy);
         else if (oracle_yukawa_add(n, xs, ys, q, p->shielding, fx, fy))
             return -1;
@@ -626,7 +864,12 @@ static int rk4_one_step(int64_t n, double *x, double *y, double *vx, double *vy,
                         double init_dt, double *t, uint64_t *step_counter, rk_work *w)
 {
     /* Runge_Kutta4.cpp:25 */
-    const double dt = p->dt_check ? oracle_modify_time_step(n, x, y, 1.0e-4f, init_dt) : init_dt;
+    double dt = init_dt;
+    if (p->dt_check)
+        dt = p->gather == 2 ? oracle_modify_time_step_cells(n, x, y, 1.0e-4f, init_dt, 10.0 / p->shielding)
+                            : oracle_modify_time_step(n, x, y, 1.0e-4f, init_dt);
+    if (!(dt > 0.0))
+        return -1;
     const uint64_t step = *step_counter;
     for (int s = 0; s < 4; ++s) {
         const double *xs, *ys, *vxs, *vys;

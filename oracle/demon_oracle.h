@@ -31,6 +31,14 @@
  *   oracle_modify_time_step  constructed cases incl. the coverage hole;
  *                            bitwise vs compiled reference
  *   oracle_dtcheck_min_s     brute force over the coverage predicate
+ *   oracle_yukawa_cells_add  bitwise equal to oracle_yukawa_add (sparse,
+ *                            dense and mixed-charge clouds, pairs at +-1 ulp
+ *                            of the cutoff); its RK4 trajectories (gather = 2)
+ *                            bitwise equal to the all-pairs ones
+ *   oracle_dtcheck_min_s_cells, oracle_modify_time_step_cells
+ *                            equal to oracle_dtcheck_min_s and
+ *                            oracle_modify_time_step on constructed close
+ *                            pairs, the coverage hole included
  *   oracle_philox4x32        published Random123 known-answer vectors
  *   oracle_thermal_add       moments of the published uniform/angle model
  *   oracle_rect_add, oracle_void_add, oracle_rotational_add,
@@ -86,7 +94,10 @@ typedef struct oracle_params {
     int32_t forces;     /* bitmask of ORACLE_* flags */
     int32_t dt_check;   /* 1: run Integrator::modifyTimeStep each step */
     int32_t gather;     /* 1: Yukawa in gather form (j ascending per i), OpenMP over i;
-                           deterministic, differs from the reference order by rounding */
+                           deterministic, differs from the reference order by rounding;
+                           2: oracle_yukawa_cells_add and the cell-list dt check (the
+                           reference's serial results, for clouds whose pairs are
+                           mostly beyond the cutoff) */
     double magnetic_field; /* B [T]                             (MagneticForce.h:29) */
     double drag_scale;  /* TimeVaryingDragForce scaleConst [Hz/s]; its offset is gamma
                            (driver.cpp:394, TimeVaryingDragForce.h:16-17) */
@@ -109,6 +120,29 @@ typedef struct oracle_params {
  * particles in pairs).  Returns 0, or -1 if n is odd. */
 int oracle_yukawa_add(int64_t n, const double *xs, const double *ys,
                       const double *q, double kappa, double *fx, double *fy);
+
+/* oracle_yukawa_add's result, bit for bit, from the pairs inside the
+ * cutoff only (a pair with u >= 10 adds exactly +-0 in the reference,
+ * ShieldedCoulombForce.cpp:279-292): candidates from a cell list of cells
+ * at least 10/kappa wide, the reference's exact test u = sqrt(dX*dX+dY*dY)*kappa
+ * < 10, and each particle's contributions added in the order the
+ * reference's serial loop adds them (ShieldedCoulombForce.cpp:43-56: by
+ * partner pair index, the pair's own intra term at its place, the straight
+ * lane before the swapped one), with the reference's intra-pair or
+ * cross-pair formula.  OpenMP over particles.  n even; returns -1 if odd or
+ * out of memory. */
+int oracle_yukawa_cells_add(int64_t n, const double *xs, const double *ys,
+                            const double *q, double kappa, double *fx, double *fy);
+
+/* oracle_dtcheck_min_s from a cell list: pairs within about 1e-4 m only
+ * (the only ones the dt check can act on), +inf if there are none. */
+float oracle_dtcheck_min_s_cells(int64_t n, const double *x, const double *y, double cell);
+
+/* oracle_modify_time_step's result from the smallest tested squared
+ * distance: while sqrtf(s_min) <= dist, dist /= 10 and dt /= 10 -- the
+ * divisions the reference makes (Integrator.cpp:144-155) in any pair order. */
+double oracle_modify_time_step_cells(int64_t n, const double *x, const double *y, float dist, double dt,
+                                     double cell);
 
 /* Same physics for any n as a per-particle gather, j ascending, with the
  * cross-pair formula (ShieldedCoulombForce.cpp:183-184) for every pair. */

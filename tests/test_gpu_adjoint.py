@@ -231,12 +231,25 @@ def test_adjoint_vs_central_differences_config5():
         assert rel_err <= b, (name, v, ad, fd, rel_err, b)
 
 
-def test_torch_autograd_binding():
+@pytest.mark.parametrize("close_pair", [False, True])
+def test_torch_autograd_binding(close_pair):
+    """The torch.autograd wrapper returns the C ABI's loss and gradients to
+    the bit, with the reference's dt check on by default.  close_pair puts
+    particles 0 and 1 (one group of four, so a pair the reference's check
+    tests) 60 um apart: the first steps' dt is then cut tenfold until the
+    pair has separated beyond 100 um, and a wrapper stepping with the fixed
+    dt would differ."""
     torch = pytest.importorskip("torch")
     from demon_simulation_code_amd.autograd import TrajectoryFit
 
     n, steps, obs = 256, 20, 10
     x, y, m, q, tx, ty = _problem(n, steps, obs, seed=7)
+    if close_pair:
+        x, y = x.copy(), y.copy()
+        x[1], y[1] = x[0] + 60e-6, y[0]
+        with _ctx(n, x, y, m, q) as c:
+            c.rk4_steps(steps, 1e-3, 0.0)
+            assert c.stats().dt_reductions > 0
     fit = TrajectoryFit(x, y, m, tx, ty, steps=steps, dt=1e-3, obs_every=obs, ckpt_every=10)
     qt = torch.tensor(q, dtype=torch.float64, requires_grad=True)
     kt = torch.tensor(KAPPA * 1.02, dtype=torch.float64, requires_grad=True)
@@ -248,5 +261,13 @@ def test_torch_autograd_binding():
         L, g = c.fit_loss_grad(steps, 1e-3, obs, 10, tx, ty)
     fit.close()
     assert loss.item() == L
-    assert np.allclose(qt.grad.numpy(), 2 * g["q"], rtol=0, atol=0)
+    assert np.array_equal(qt.grad.numpy(), 2 * g["q"])
     assert kt.grad.item() == 2 * g["kappa"] and gt.grad.item() == 2 * g["gamma"]
+    assert ct.grad.item() == 2 * g["confine"]
+    if close_pair:
+        # the fixed-dt fit is a different function here
+        fixed = TrajectoryFit(x, y, m, tx, ty, steps=steps, dt=1e-3, obs_every=obs, ckpt_every=10,
+                              dt_check=False)
+        lf = fixed(qt.detach(), kt.detach(), ct.detach(), gt.detach()).item()
+        fixed.close()
+        assert lf != L
