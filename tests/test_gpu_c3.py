@@ -1,0 +1,121 @@
+"""The large-cloud configuration at its own size: N = 131072 on the benchmark
+lattice (500 um spacing, +-50 um jitter), lambda = 500 um, k = 5e-11 N/m,
+gamma = 2/s, RK4 with dt = 1e-3 s and the reference's dt check
+(Runge_Kutta4.cpp:23-137, Integrator.cpp:68-155).
+
+* Trajectory: the device against the oracle's trajectory of the same cloud
+  (tests/golden/c3_trajectory.npz, stepped on the CPU by
+  tests/golden/make_c3_trajectory.py with the oracle's cell-list Yukawa, bit
+  for bit its all-pairs restatement of ShieldedCoulombForce): the integrator
+  time after every call identical, the dt of every step identical, and
+  positions within 1e-9 and velocities within 1e-8 relative
+  (max_i |d_i| / max_i |v_i| over a fixed sample of 4096 particles) at
+  every snapshot, for both all-pairs kernels.
+* dt check on the imploded cloud: along the device trajectory, where close
+  pairs cut dt, the device's dt for the next step equals the oracle's
+  modifyTimeStep replay on the downloaded state.
+"""
+import hashlib
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import demon_simulation_code_amd as dm
+from demon_simulation_code_amd.physics import LARGE_131K as WL
+
+pytestmark = pytest.mark.gpu
+FIX = Path(__file__).resolve().parent / "golden" / "c3_trajectory.npz"
+
+
+def _lattice():
+    x, y = dm.hex_lattice(WL.n, WL.spacing, WL.jitter, WL.seed)
+    return x, y, np.full(WL.n, WL.charge), np.full(WL.n, WL.mass)
+
+
+def _ctx(mode, precision=dm.FP64):
+    c = dm.Context(WL.n, shielding=WL.kappa, confine=WL.confine, gamma=WL.gamma,
+                   forces=dm.CONFINEMENT | dm.DRAG | dm.SHIELDED_COULOMB, precision=precision)
+    c.set_pair_mode(mode)
+    return c
+
+
+def _digest(x, y):
+    h = hashlib.sha256()
+    for a in (x, y):
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return h.hexdigest()
+
+
+def _rel(a, b):
+    return float(np.max(np.hypot(a[0] - b[0], a[1] - b[1])) / np.max(np.hypot(b[0], b[1])))
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("mode", [dm.PAIRS_NEWTON3, dm.PAIRS_GATHER])
+def test_trajectory_vs_oracle_full_size(mode):
+    f = np.load(FIX)
+    x, y, q, m = _lattice()
+    assert _digest(x, y) == str(f["lattice_digest"])
+    every, rows, dt = int(f["snap_every"]), f["rows"], float(f["dt"])
+    times, dts, snaps = f["times"], f["dts"], f["snaps"]
+    with _ctx(mode) as c:
+        c.upload(x, y, None, None, q, m)
+        t = 0.0
+        for k, want in enumerate(snaps):
+            t = c.rk4_steps(every, dt, t)
+            assert t == times[(k + 1) * every - 1]
+            st = c.stats()
+            assert st.steps == (k + 1) * every
+            assert st.dt_reductions == int(np.sum(dts[: (k + 1) * every] != dt))
+            assert st.last_dt == dts[(k + 1) * every - 1]
+            gx, gy, gvx, gvy = c.download()
+            pos = _rel((gx[rows], gy[rows]), (want[0], want[1]))
+            vel = _rel((gvx[rows], gvy[rows]), (want[2], want[3]))
+            print(f"mode {mode} step {(k + 1) * every}: position error {pos:.2e}, velocity error {vel:.2e}")
+            assert pos <= 1e-9 and vel <= 1e-8
+
+
+@pytest.mark.timeout(300)
+def test_fp32_pair_trajectory_vs_oracle_full_size():
+    """The FP32-pair variant (FP64 state and RK4 combine) on the same
+    trajectory: its position error against the FP64 oracle stays far below
+    the cloud's scale while the lattice contracts."""
+    f = np.load(FIX)
+    x, y, q, m = _lattice()
+    every, rows, dt = int(f["snap_every"]), f["rows"], float(f["dt"])
+    with _ctx(dm.PAIRS_NEWTON3, dm.FP32_PAIR) as c:
+        c.upload(x, y, None, None, q, m)
+        t = 0.0
+        for k, want in enumerate(f["snaps"]):
+            t = c.rk4_steps(every, dt, t)
+            gx, gy, gvx, gvy = c.download()
+            pos = _rel((gx[rows], gy[rows]), (want[0], want[1]))
+            vel = _rel((gvx[rows], gvy[rows]), (want[2], want[3]))
+            print(f"FP32-pair step {(k + 1) * every}: position error {pos:.2e}, velocity error {vel:.2e}")
+            assert pos <= 1e-6 and vel <= 1e-5
+            assert pos > 0.0  # it really is the FP32 path
+
+
+@pytest.mark.timeout(300)
+def test_dt_check_on_imploded_cloud_full_size(oracle):
+    """Steps 120 .. 160 of the benchmark trajectory: the cloud has imploded to
+    a few mm and the dt check cuts most steps.  At states along the way, the
+    device's dt for the next step equals the oracle's replay of
+    modifyTimeStep (the FP32 distances of the pairs the reference's lane
+    shuffles cover) on the downloaded positions."""
+    x, y, q, m = _lattice()
+    with _ctx(dm.PAIRS_NEWTON3) as c:
+        c.upload(x, y, None, None, q, m)
+        t = c.rk4_steps(120, WL.dt, 0.0)
+        cut = 0
+        for _ in range(4):
+            t = c.rk4_steps(9, WL.dt, t)
+            gx, gy, _, _ = c.download()
+            want = oracle.modify_time_step_cells(gx, gy, 1e-4, WL.dt, cell=2e-4)
+            t = c.rk4_steps(1, WL.dt, t)
+            got = c.stats().last_dt
+            print(f"step {c.stats().steps}: device dt {got:.3e}, oracle dt {want:.3e}")
+            assert got == want
+            cut += got < WL.dt
+        assert c.stats().dt_reductions > 0 and cut > 0
