@@ -72,14 +72,12 @@ struct PairSums {
     const int32_t *first;      // Newton-3 item ranges per superblock (null: gather)
 };
 
-// Newton-3 work plan of one launch, written by the tile-list kernel: the
-// live tiles of every superblock row, cut into items of `span` consecutive
-// live tiles of one row, each item one workgroup's share.
+// Newton-3 work plan of one launch, written by the plan kernel: the live
+// tiles of every superblock row, cut into items of `span` consecutive live
+// tiles of one row, each item one workgroup's share.
 struct N3Plan {
     int32_t span;              // live tiles per item (the last item of a row may hold fewer)
     int32_t nitems;            // items of the launch
-    uint32_t done;             // row workgroups finished (the last one resets it)
-    uint32_t pad_;
     uint32_t next[8];          // item counters of the pair kernel, one per XCD
 };
 
@@ -143,8 +141,8 @@ struct PairLaunch {
 };
 
 hipError_t launch_pairs(const PairLaunch &a, hipStream_t s);
-// The Newton-3 pass: the tile-list kernel, then the pair kernel (in this
-// order on one stream, for every evaluation).
+// The Newton-3 pass: the tile-list and plan kernels (launch_n3_tiles), then
+// the pair kernel (in this order on one stream, for every evaluation).
 hipError_t launch_n3_tiles(const PairLaunch &a, hipStream_t s);
 hipError_t launch_pairs_n3(const PairLaunch &a, hipStream_t s);
 

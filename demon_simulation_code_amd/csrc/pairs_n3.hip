@@ -1,14 +1,15 @@
 // pairs_n3.hip -- the single-cloud Newton-3 all-pairs pass (the tile code is
-// in pairs_n3.h).  Two launches per force evaluation:
+// in pairs_n3.h).  Three launches per force evaluation:
 //
 //  k_n3_tiles   one workgroup per superblock row S: tests the bounding box of
 //               S against the box of every superblock T = S + d it owns
 //               (d = 1 .. nsb/2), writes the tile flags rflag[T][d - 1] and
 //               the row's live offsets, in increasing d (d = 0, the row with
-//               itself, first), into list[S].  The last workgroup to finish
-//               cuts the rows into work items of `span` consecutive live
-//               tiles, span = ceil(live tiles / target), and writes the item
-//               table (first item of each row, row of each item).
+//               itself, first), into list[S].
+//  k_n3_plan    one workgroup: cuts the rows into work items of `span`
+//               consecutive live tiles, span = ceil(live tiles / target), and
+//               writes the item table (first item of each row, row of each
+//               item).
 //  k_pairs_n3   a persistent grid: every workgroup takes items from an
 //               atomic counter until none is left; one item = the row's i
 //               particles in registers against its span of live tiles, the
@@ -36,10 +37,6 @@ __device__ __forceinline__ Box superblock_box(const double *__restrict__ box, in
     return b;
 }
 
-__device__ __forceinline__ int32_t load_agent(const int32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Block-wide exclusive prefix sum of one value per thread (256 threads);
 // *total receives the sum.  Ends with a barrier.
 __device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t *lds, int32_t *total) {
@@ -60,13 +57,11 @@ __device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t *lds,
 
 __global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {
     __shared__ int32_t wcnt[4];
-    __shared__ int32_t base_s, last_s;
-    __shared__ int32_t scan[256];
+    __shared__ int32_t base_s;
     if ((a.flags & kFlagGated) && !a.sc->active) return;
     const int32_t nsb = a.npad / kSuper, half = nsb / 2;
     const int32_t S = (int32_t)blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (S == 0 && threadIdx.x < 8) l.plan->next[threadIdx.x] = 0u;
     const Radii rad = launch_radii(a.rc2p, a.flags, a.sc->rmax2_bits);
     const Box sbS = superblock_box(a.box, S);
     const bool sreal = sbS.x0 <= sbS.x1;  // the row holds real particles
@@ -101,29 +96,35 @@ __global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {
         if (threadIdx.x == 0) base_s += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        l.cnt[S] = base_s;
-        atomicAdd(&a.sc->n3_tiles, (unsigned long long)(base_s - (sreal ? 1 : 0)));
-        __threadfence();
-        last_s = atomicAdd(&l.plan->done, 1u) == (uint32_t)(nsb - 1);
-    }
-    __syncthreads();
-    if (!last_s) return;
-    // The last row workgroup: every row's count is in memory.
-    __threadfence();
+    if (threadIdx.x == 0) l.cnt[S] = base_s;
+}
+
+// One workgroup: cuts the rows' live tiles into items (span = ceil(live /
+// target) consecutive tiles of one row), writes the item table and resets
+// the pair kernel's queue counters.
+__global__ __launch_bounds__(256) void k_n3_plan(N3Row a, N3List l) {
+    __shared__ int32_t scan[256];
+    if ((a.flags & kFlagGated) && !a.sc->active) return;
+    const int32_t nsb = a.npad / kSuper;
+    if (threadIdx.x < 8) l.plan->next[threadIdx.x] = 0u;
     const int32_t per = (nsb + 255) / 256;
     const int32_t r0 = min((int32_t)threadIdx.x * per, nsb), r1 = min(r0 + per, nsb);
-    int32_t tiles = 0;
-    for (int32_t r = r0; r < r1; ++r) tiles += load_agent(l.cnt + r);
-    int32_t total = 0;
+    int32_t tiles = 0, rows = 0;
+    for (int32_t r = r0; r < r1; ++r) {
+        const int32_t c = l.cnt[r];
+        tiles += c;
+        rows += c > 0;
+    }
+    int32_t total = 0, real_rows = 0;
     (void)block_exclusive_scan(tiles, scan, &total);
+    (void)block_exclusive_scan(rows, scan, &real_rows);
     const int32_t span = max(1, (total + l.target - 1) / l.target);
     int32_t items = 0;
-    for (int32_t r = r0; r < r1; ++r) items += (load_agent(l.cnt + r) + span - 1) / span;
+    for (int32_t r = r0; r < r1; ++r) items += (l.cnt[r] + span - 1) / span;
     int32_t nitems = 0;
     int32_t f = block_exclusive_scan(items, scan, &nitems);
     for (int32_t r = r0; r < r1; ++r) {
-        const int32_t ni = (load_agent(l.cnt + r) + span - 1) / span;
+        const int32_t ni = (l.cnt[r] + span - 1) / span;
         l.first[r] = f;
         for (int32_t k = 0; k < ni; ++k) l.itemrow[f + k] = r;
         f += ni;
@@ -132,8 +133,9 @@ __global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {
         l.first[nsb] = nitems;
         l.plan->span = span;
         l.plan->nitems = nitems;
-        l.plan->done = 0u;
-        atomicAdd(&a.sc->n3_items, (unsigned long long)nitems);
+        // the diagonal tile of every row with real particles is not counted
+        a.sc->n3_tiles += (unsigned long long)(total - real_rows);
+        a.sc->n3_items += (unsigned long long)nitems;
     }
 }
 
@@ -234,6 +236,7 @@ hipError_t launch_n3_tiles(const PairLaunch &a, hipStream_t s) {
     const hipError_t e = n3_row_args(a, &row);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_n3_tiles, dim3((unsigned)(a.npad / kSuper)), dim3(256), 0, s, row, a.lst);
+    hipLaunchKernelGGL(k_n3_plan, dim3(1), dim3(256), 0, s, row, a.lst);
     return hipGetLastError();
 }
 
