@@ -138,6 +138,7 @@ SIGNATURES = {
     "demon_batch_get_shape": (ctypes.c_int, [_CTX, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _D]),
     "demon_batch_tune": (ctypes.c_int, [_CTX, _I32]),
     "demon_kinetic_energy": (ctypes.c_int, [_CTX, _D, _D]),
+    "demon_kinetic_energy_noalloc": (ctypes.c_int, [_CTX, _D, _D]),
     "demon_pair_histogram": (ctypes.c_int, [_CTX, ctypes.c_double, _I32, ctypes.POINTER(ctypes.c_uint64)]),
     "demon_trajectory_spectrum": (ctypes.c_int, [_D, _I64, _I64, _I64, _D, _I32]),
 }

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1195,6 +1196,64 @@ int analysis_scratch(demon_ctx *c, size_t doubles) {
 }
 
 }  // namespace
+
+// The scratch of demon_kinetic_energy_noalloc: the per-block sums and the
+// totals of the kinetic-energy reduction for clouds of npad <= 2^22, in a
+// file-scope device array instead of the context's analysis scratch.  Every
+// GPU the library runs on holds its own copy (512 KB) until the process
+// ends; calls that overlap on one GPU share it.
+constexpr int64_t kNoAllocMaxNpad = (int64_t)1 << 22;
+constexpr size_t kNoAllocDoubles = (size_t)(kNoAllocMaxNpad / 256) * 4 + 8;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+// Device address of `scratch` per device ordinal, looked up (with that device
+// current) on the first call on the device.  In host code `scratch` names a
+// host-side stand-in, never the device memory.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+int scratch_address(demon_ctx *c, double **out) {
+    const int dev = c->p.device;
+    if (dev < 0 || dev >= kMaxDevices) return fail(c, DEMON_ERR_UNSUPPORTED, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(c, e, "hipGetSymbolAddress of the kinetic-energy scratch");
+        g_scratch[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch[dev];
+    return DEMON_OK;
+}
+
+}  // namespace
+
+int demon_kinetic_energy_noalloc(demon_ctx *c, double *total, double *thermal) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    if (!c->uploaded) return fail(c, DEMON_ERR_ARG, "no state uploaded");
+    if (!total && !thermal) return fail(c, DEMON_ERR_ARG, "no output");
+    if (c->npad > kNoAllocMaxNpad)
+        return fail(c, DEMON_ERR_UNSUPPORTED,
+                    "demon_kinetic_energy_noalloc: n beyond its fixed scratch (at most 4194304 particles); "
+                    "use demon_kinetic_energy");
+    HIP_TRY(c, hipSetDevice(c->p.device));
+    double *base = nullptr;
+    int rc = scratch_address(c, &base);
+    if (rc) return rc;
+    const size_t part = (size_t)(c->npad / 256) * 4;
+    double *totals = base + part;
+    // every block sum and total is written before it is read: nothing to clear
+    HIP_TRY(c, launch_kinetic(c->vx, c->vy, c->m, c->n, c->npad, base, totals, c->stream));
+    double h[8];
+    HIP_TRY(c, hipMemcpyAsync(h, totals, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (total) *total = 0.5 * h[3];
+    if (thermal) *thermal = 0.5 * h[4];
+    return DEMON_OK;
+}
 
 int demon_kinetic_energy(demon_ctx *c, double *total, double *thermal) {
     if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");

@@ -317,6 +317,16 @@ int demon_batch_tune(demon_batch *b, int32_t reps);
  * may be NULL. */
 int demon_kinetic_energy(demon_ctx *ctx, double *total, double *thermal);
 
+/* demon_kinetic_energy without any device allocation (no counterpart in the
+ * reference; for callers that cannot allocate: code under HIP graph capture,
+ * tight serving loops).  Same kernels and the same results, bit for bit;
+ * the block sums live in a fixed 512 KB device array that every GPU the
+ * library runs on holds until the process ends, instead of in scratch the
+ * context allocates on first use.  Limit: n <= 4194304 (larger clouds get
+ * DEMON_ERR_UNSUPPORTED).  Calls that overlap on one GPU (several contexts
+ * or threads at once) share the array and race; serialise them. */
+int demon_kinetic_energy_noalloc(demon_ctx *ctx, double *total, double *thermal);
+
 /* Pair-distance histogram of the current positions (the counts behind g(r)):
  * counts[b], b < nbins, is the number of unordered pairs i < j with
  * e_b^2 <= d^2 < e_{b+1}^2, where d^2 = dx*dx + dy*dy and e_k = k*(r_max/nbins)
