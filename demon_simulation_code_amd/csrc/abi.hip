@@ -58,6 +58,7 @@ struct demon_ctx {
     uint32_t *d_rflag = nullptr; // Newton-3 tile flags [nsb][nsb/2]
     int32_t *d_n3i = nullptr;    // Newton-3 tile lists and item table (n3_ints)
     N3Plan *d_plan = nullptr;    // Newton-3 work plan
+    double *d_exptab = nullptr;  // Newton-3 kernel's exp table e^{-k/64}
     int32_t cus = 256;           // compute units of the device
     bool use_graph = false;
     int32_t order = 4;           // integrator of the captured step: 4 (RK4) or 2 (RK2)
@@ -210,13 +211,16 @@ int ensure_n3(demon_ctx *c) {
     }
     if (hipMalloc((void **)&c->d_rflag, sizeof(uint32_t) * half * nsb) != hipSuccess ||
         hipMalloc((void **)&c->d_n3i, sizeof(int32_t) * n3_ints(c->npad)) != hipSuccess ||
-        hipMalloc((void **)&c->d_plan, sizeof(N3Plan)) != hipSuccess) {
-        for (void *ptr : {(void *)c->d_react, (void *)c->d_rflag, (void *)c->d_n3i, (void *)c->d_plan})
+        hipMalloc((void **)&c->d_plan, sizeof(N3Plan)) != hipSuccess ||
+        hipMalloc((void **)&c->d_exptab, sizeof(double) * kExpTabN) != hipSuccess) {
+        for (void *ptr : {(void *)c->d_react, (void *)c->d_rflag, (void *)c->d_n3i, (void *)c->d_plan,
+                          (void *)c->d_exptab})
             if (ptr) (void)hipFree(ptr);
         c->d_react = nullptr;
         c->d_rflag = nullptr;
         c->d_n3i = nullptr;
         c->d_plan = nullptr;
+        c->d_exptab = nullptr;
         return fail(c, DEMON_ERR_ALLOC, "hipMalloc of the Newton-3 tile flags, lists or plan failed");
     }
     // no tile evaluated yet: the stage kernels must read no reaction and no
@@ -224,6 +228,10 @@ int ensure_n3(demon_ctx *c) {
     HIP_TRY(c, hipMemset(c->d_rflag, 0, sizeof(uint32_t) * half * nsb));
     HIP_TRY(c, hipMemset(c->d_n3i, 0, sizeof(int32_t) * n3_ints(c->npad)));
     HIP_TRY(c, hipMemset(c->d_plan, 0, sizeof(N3Plan)));
+    // e^{-k/64}: the double nearest to a long-double evaluation
+    double tab[kExpTabN];
+    for (int k = 0; k < kExpTabN; ++k) tab[k] = (double)std::exp(-(long double)k / 64.0L);
+    HIP_TRY(c, hipMemcpy(c->d_exptab, tab, sizeof tab, hipMemcpyHostToDevice));
     return DEMON_OK;
 }
 
@@ -239,6 +247,7 @@ N3List n3_list(const demon_ctx *c) {
     l.max_items = (int32_t)n3_max_items(c->npad);
     // a persistent grid of four workgroups per CU (the kernel's occupancy)
     l.grid = 4 * c->cus;
+    l.exptab = c->d_exptab;
     return l;
 }
 
@@ -714,6 +723,7 @@ void demon_destroy(demon_ctx *c) {
     if (c->d_rflag) (void)hipFree(c->d_rflag);
     if (c->d_n3i) (void)hipFree(c->d_n3i);
     if (c->d_plan) (void)hipFree(c->d_plan);
+    if (c->d_exptab) (void)hipFree(c->d_exptab);
     if (c->d_an) (void)hipFree(c->d_an);
     for (hipEvent_t e : c->ev_fit)
         if (e) (void)hipEventDestroy(e);

@@ -81,6 +81,10 @@ struct N3Plan {
     uint32_t next[8];          // item counters of the pair kernel, one per XCD
 };
 
+// Entries of the Newton-3 pair kernel's exp table, e^{-k/64} for k < 642
+// (u < 10.03).
+constexpr int kExpTabN = 642;
+
 struct N3List {
     int32_t *list;             // [nsb][nsb/2 + 1] live offsets d of each row, increasing (0: the row itself)
     int32_t *cnt;              // [nsb] live tiles per row
@@ -90,6 +94,7 @@ struct N3List {
     int32_t target;            // items wanted per launch: span = ceil(live tiles / target)
     int32_t max_items;         // >= target + nsb
     int32_t grid;              // workgroups of the persistent pair kernel
+    const double *exptab;      // e^{-k/64}, k < 642: the pair kernel's exp table (copied to LDS)
 };
 
 // Device scalars of the stepping loop.

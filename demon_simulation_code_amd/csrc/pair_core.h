@@ -132,6 +132,36 @@ __device__ __forceinline__ double exp_neg(double u, const ExpCoef &ec) {
     return __builtin_ldexp(p, ni);
 }
 
+// e^-u for 0 <= u < 10.03 (callers pass u < 10) from a table in LDS,
+// tab[k] = e^{-k/64} (k < kExpTabN, each the double nearest to a
+// long-double evaluation on the host): n = rint(64 u) by the same
+// magic-number rounding, d = u - n/64 (exact: n/64 is exact and u - n/64
+// needs at most 43 significant bits), and e^-d by its degree-6 Taylor
+// polynomial, |d| <= 1/128, whose truncation error is below 4e-19 relative;
+// about 1 ulp in all.  The Taylor coefficients tc[0..5] = 1/720, -1/120,
+// 1/24, -1/6, 1/2, -1 and the reduction's constants tc[6..8] = 64, 1.5 2^52,
+// -1/64 arrive as kernel arguments (SGPRs), as exp_neg's do: as literals the
+// rounding constant would be copied into a VGPR pair for every evaluation.
+// Three FMA-class operations and an add fewer than exp_neg, and no ldexp:
+// the table replaces the reduction's ln 2 split and the high-degree tail.
+constexpr double kExpTabScale = 64.0;
+
+inline ExpCoef exp_tab_coef() {
+    return ExpCoef{{1.0 / 720.0, -1.0 / 120.0, 1.0 / 24.0, -1.0 / 6.0, 0.5, -1.0, kExpTabScale, 0x1.8p52,
+                    -1.0 / kExpTabScale, 0.0}};
+}
+
+__device__ __forceinline__ double exp_neg_tab(double u, const double *__restrict__ tab, const ExpCoef &tc) {
+    const double m = fma(u, tc.c[6], tc.c[7]);
+    const int ni = __double2loint(m);
+    const double d = fma(m - tc.c[7], tc.c[8], u);
+    double p = tc.c[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k) p = fma(p, d, tc.c[k]);
+    p = fma(p, d, 1.0);
+    return tab[ni] * p;
+}
+
 // Pair scale for a contributing pair (i <- j):
 //   F_i += s (r_i - r_j),  s = C q_i q_j (1 + u) e^-u / r^3,  u = kappa r,
 // the reference's cross-pair formula (ShieldedCoulombForce.cpp:183-184);
@@ -297,8 +327,23 @@ struct PairEnv {
     float kl2_f;  // -kappa log2(e), FP32 pair arithmetic
     bool dtc;
     const int32_t *__restrict__ orig;
-    ExpCoef ec;
+    ExpCoef ec;   // exp_neg's coefficients, or exp_neg_tab's with etab
+    const double *etab = nullptr;  // e^{-k/64} in LDS (kernels built with TAB)
 };
+
+// The pair scale with the launch's exp: exp_neg_tab on the LDS table (TAB)
+// or exp_neg.
+template <bool TAB>
+__device__ __forceinline__ double pair_scale_env(double r2, double cqi, double qj, const PairEnv &e) {
+    if constexpr (TAB) {
+        const InvSqrt s = inv_sqrt_fp64(r2);
+        const double u = s.r * e.cut.kappa;
+        const double y3 = (s.y * s.y) * s.y;
+        return ((cqi * qj) * fma(u, y3, y3)) * exp_neg_tab(u, e.etab, e.ec);
+    } else {
+        return pair_scale_fp64(r2, cqi, qj, e.cut.kappa, e.ec);
+    }
+}
 
 // Distance and contribution test of one pair: outside the band r2 decides;
 // inside it, the reference's own test (CHECK; without it the pair is known
@@ -329,7 +374,7 @@ __device__ __forceinline__ void pair_dtcheck(double xi, double yi, double xj, do
 // Gather form over all (u, r) pairs of one chunk: accumulates F_i into
 // ax/ay.  SELF: the chunk overlaps the wave's own i range, so j == i is
 // skipped.
-template <int R, int C, bool F32, bool CHECK, bool SELF>
+template <int R, int C, bool F32, bool CHECK, bool SELF, bool TAB = false>
 __device__ __forceinline__ void chunk_gather(const ITile<R> &t, const double (&cx)[C],
                                              const double (&cy)[C], const double (&cq)[C],
                                              int32_t j0, const PairEnv &e, uint64_t &n_near,
@@ -352,7 +397,7 @@ __device__ __forceinline__ void chunk_gather(const ITile<R> &t, const double (&c
                     ax[r] = ax[r] + fx;
                     ay[r] = ay[r] + fy;
                 } else {
-                    const double s = pair_scale_fp64(r2, t.cqi[r], cq[u], e.cut.kappa, e.ec);
+                    const double s = pair_scale_env<TAB>(r2, t.cqi[r], cq[u], e);
                     ax[r] = fma(s, dx, ax[r]);
                     ay[r] = fma(s, dy, ay[r]);
                 }
@@ -367,7 +412,7 @@ __device__ __forceinline__ void chunk_gather(const ITile<R> &t, const double (&c
 // given.
 // Adds the wave's counts to n_near (ordered pairs inside the cutoff) and
 // n_tested (ordered pairs not culled by the boxes).
-template <int R, int C, bool F32>
+template <int R, int C, bool F32, bool TAB = false>
 __device__ __forceinline__ void gather_blocks(const ITile<R> &t, const double *__restrict__ xs,
                                               const double *__restrict__ ys, const double *__restrict__ nc,
                                               const double *__restrict__ q, const double *__restrict__ box,
@@ -395,11 +440,11 @@ __device__ __forceinline__ void gather_blocks(const ITile<R> &t, const double *_
             for (int u = 0; u < C; ++u) cq[u] = q[j0 + u];
             const bool own = j0 < t.whi && j0 + C > t.wlo;
             if (own)
-                chunk_gather<R, C, F32, true, true>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);
+                chunk_gather<R, C, F32, true, true, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);
             else if (cls == kDense)
-                chunk_gather<R, C, F32, false, false>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);
+                chunk_gather<R, C, F32, false, false, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);
             else
-                chunk_gather<R, C, F32, true, false>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);
+                chunk_gather<R, C, F32, true, false, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);
         }
     }
 }

@@ -118,7 +118,7 @@ static_assert(2 * kChunkN3 * Scr<float>::kRow <= 2 * kScrDoubles, "FP32 rows fit
 template <bool F32>
 using PairVal = typename std::conditional<F32, float, double>::type;
 
-template <bool F32, bool CHECK, bool DTC>
+template <bool F32, bool CHECK, bool DTC, bool TAB = false>
 __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[kChunkN3],
                                          const double (&cy)[kChunkN3], const PairVal<F32> (&cq)[kChunkN3],
                                          int32_t j0, const PairEnv &e, uint64_t &n_near, float &smin,
@@ -138,7 +138,7 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
             if constexpr (F32) {
                 pair_force_fp32f(dx, dy, t.cqf[0], cq[u], e.kl2_f, fx, fy);
             } else {
-                const double s = pair_scale_fp64(r2, t.cqi[0], cq[u], e.cut.kappa, e.ec);
+                const double s = pair_scale_env<TAB>(r2, t.cqi[0], cq[u], e);
                 fx = s * dx;
                 fy = s * dy;
             }
@@ -190,7 +190,7 @@ using TBoxLds = Box[kBoxChunk];
 // block; adds the i-side forces into ax/ay and writes the reactions on T's
 // 256 particles to react[d - 1] (negated sums over the four waves in a fixed
 // order).  Called by all 256 threads; ib: the wave's i-block box.
-template <bool F32, bool DTC>
+template <bool F32, bool DTC, bool TAB = false>
 __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const ITile<1> &t, const Box &ib,
                                         bool ireal, int32_t T, int32_t d, double &ax, double &ay,
                                         uint64_t &n_near, uint64_t &n_tested, float &smin, RedLds &red,
@@ -225,7 +225,7 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
                     if constexpr (F32) cq[u] = qf[j0 + u];
                     else cq[u] = q[j0 + u];
                 }
-                chunk_n3<F32, false, DTC>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr[wave], r,
+                chunk_n3<F32, false, DTC, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr[wave], r,
                                           j0 - T * kSuper);
             }
             continue;
@@ -244,7 +244,7 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
                 if constexpr (F32) cq[u] = qf[j0 + u];
                 else cq[u] = q[j0 + u];
             }
-            chunk_n3<F32, true, DTC>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr[wave], r,
+            chunk_n3<F32, true, DTC, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr[wave], r,
                                      j0 - T * kSuper);
         }
     }

@@ -3,7 +3,9 @@
 //
 //  k_n3_tiles   one workgroup per superblock row S: tests the bounding box of
 //               S against the box of every superblock T = S + d it owns
-//               (d = 1 .. nsb/2), writes the tile flags rflag[T][d - 1] and
+//               (d = 1 .. nsb/2), and then S's four 64-particle block boxes
+//               against T's (a tile is live when some block pair may hold a
+//               pair within reach), writes the tile flags rflag[T][d - 1] and
 //               the row's live offsets, in increasing d (d = 0, the row with
 //               itself, first), into list[S].
 //  k_n3_plan    one workgroup: cuts the rows into work items of `span`
@@ -65,6 +67,9 @@ __global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {
     const Radii rad = launch_radii(a.rc2p, a.flags, a.sc->rmax2_bits);
     const Box sbS = superblock_box(a.box, S);
     const bool sreal = sbS.x0 <= sbS.x1;  // the row holds real particles
+    Box ibS[4];  // the row's four 64-particle blocks
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ibS[k] = box_union(empty_box(), a.box + 4 * (S * 4 + k));
     int32_t *lst = l.list + (size_t)S * (half + 1);
     if (threadIdx.x == 0) {
         base_s = sreal ? 1 : 0;
@@ -79,7 +84,16 @@ __global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {
             // for even nsb the pair (S, S + nsb/2) is reached from both rows
             // and belongs to the lower one
             const bool owned = !(d == half && !(nsb & 1) && S >= half);
-            if (owned && sreal) live = box_gap2(sbS, superblock_box(a.box, T)) <= rad.rf2;
+            if (owned && sreal && box_gap2(sbS, superblock_box(a.box, T)) <= rad.rf2) {
+                // live when some block pair is within reach: otherwise the
+                // pair kernel would skip all sixteen (the same box test)
+#pragma unroll
+                for (int kj = 0; kj < 4; ++kj) {
+                    const Box jb = box_union(empty_box(), a.box + 4 * (T * 4 + kj));
+#pragma unroll
+                    for (int ki = 0; ki < 4; ++ki) live = live || box_gap2(ibS[ki], jb) <= rad.rf2;
+                }
+            }
             a.rflag[(size_t)T * half + (d - 1)] = live ? 1u : 0u;
         }
         const unsigned long long m = __ballot(live);
@@ -157,12 +171,12 @@ __device__ __forceinline__ void n3_item(const N3Row &a, const PairEnv &e, int32_
         const int32_t d = __builtin_amdgcn_readfirstlane(lst[k]);
         if (d == 0) {
             // the superblock with itself, every ordered pair, j == i skipped
-            gather_blocks<1, kChunkN3, F32>(t, a.xs, a.ys, a.nc, a.q, a.box, ib, n, S * kSuper,
-                                            S * kSuper + kSuper, e, ax, ay, n_near, n_tested, smin);
+            gather_blocks<1, kChunkN3, F32, !DTC>(t, a.xs, a.ys, a.nc, a.q, a.box, ib, n, S * kSuper,
+                                                  S * kSuper + kSuper, e, ax, ay, n_near, n_tested, smin);
             continue;
         }
         const int32_t T = S + d < nsb ? S + d : S + d - nsb;
-        n3_tile<F32, DTC>(a, e, t, ib, ireal, T, d, ax[0], ay[0], n_near, n_tested, smin, red, scr);
+        n3_tile<F32, DTC, !DTC>(a, e, t, ib, ireal, T, d, ax[0], ay[0], n_near, n_tested, smin, red, scr);
     }
     double *px = a.part + (size_t)it * (2 * kSuper);
     px[wave * 64 + lane] = ax[0];
@@ -174,11 +188,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                                                    ExpCoef ec) {
     __shared__ RedLds red;
     __shared__ ScrLds scr;
+    __shared__ double etab[kExpTabN];
     __shared__ int32_t slot;
     if ((a.flags & kFlagGated) && !a.sc->active) return;
+    // The FP64 exp is exp_neg_tab on this table, except in the stage-1
+    // launch (the dt check's FP32 replay leaves no registers for it there:
+    // it keeps exp_neg).  The item loop's first barrier orders these stores
+    // before any read.
+    if (!DTC)
+        for (int k = (int)threadIdx.x; k < kExpTabN; k += 256) etab[k] = l.exptab[k];
     const int32_t nsb = a.npad / kSuper, half = nsb / 2;
     const Radii rad = launch_radii(a.rc2p, a.flags, a.sc->rmax2_bits);
-    const PairEnv e{make_cutoff(a.kappa, a.rc2p), rad, fp32_kl2(a.kappa), DTC, a.orig, ec};
+    const PairEnv e{make_cutoff(a.kappa, a.rc2p), rad, fp32_kl2(a.kappa), DTC, a.orig, ec, etab};
     const int32_t span = __builtin_amdgcn_readfirstlane(l.plan->span);
     float smin = __builtin_inff();
     uint64_t n_near = 0, n_tested = 0;
@@ -218,7 +239,7 @@ hipError_t n3_row_args(const PairLaunch &a, N3Row *row) {
     const N3List &l = a.lst;
     if (a.npad % kPad || !a.box || !a.react || !a.rflag || a.n > a.npad || (a.fp32pair && !a.qf))
         return hipErrorInvalidValue;
-    if (!l.list || !l.cnt || !l.first || !l.itemrow || !l.plan || l.target < 1 || l.grid < 1 ||
+    if (!l.list || !l.cnt || !l.first || !l.itemrow || !l.plan || !l.exptab || l.target < 1 || l.grid < 1 ||
         (int64_t)l.max_items < (int64_t)l.target + nsb || nsb / 2 > kMaxHalf)
         return hipErrorInvalidValue;
     int32_t flags = 0;
@@ -246,7 +267,8 @@ hipError_t launch_pairs_n3(const PairLaunch &a, hipStream_t s) {
     if (e != hipSuccess) return e;
     auto kern = a.fp32pair ? (a.dtcheck ? k_pairs_n3<true, true> : k_pairs_n3<true, false>)
                            : (a.dtcheck ? k_pairs_n3<false, true> : k_pairs_n3<false, false>);
-    hipLaunchKernelGGL(kern, dim3((unsigned)a.lst.grid), dim3(256), 0, s, row, a.lst, exp_coef());
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.lst.grid), dim3(256), 0, s, row, a.lst,
+                       a.dtcheck ? exp_coef() : exp_tab_coef());
     return hipGetLastError();
 }
 
