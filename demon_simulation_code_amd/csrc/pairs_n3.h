@@ -186,6 +186,57 @@ using ScrLds = double[4][kScrDoubles]; // per-wave transpose scratch
 constexpr int kBoxChunk = 128;
 using TBoxLds = Box[kBoxChunk];
 
+// The wave's i block against one 64-j block (first particle jb) in the
+// Newton-3 form: box classification, then the straight-line path (dense) or
+// the prefiltered one (mixed); i-side forces into ax/ay, the wave's sums of
+// the forces on each j into red[2 (j - t0) + c].
+template <bool F32, bool DTC, bool TAB>
+__device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, const ITile<1> &t, const Box &ib,
+                                          bool ireal, int32_t jb, int32_t t0, double &ax, double &ay,
+                                          uint64_t &n_near, uint64_t &n_tested, float &smin, double *scr,
+                                          double *red) {
+    const double *__restrict__ xs = a.xs;
+    const double *__restrict__ ys = a.ys;
+    const double *__restrict__ nc = a.nc;
+    const double *__restrict__ q = a.q;
+    const float *__restrict__ qf = a.qf;
+    const BlockClass cls = classify(ib, a.box + 4 * (jb / kBoxBlock), ireal, jb + kBoxBlock <= a.n, e.rad, e.cut);
+    if (cls == kSkip) return;
+    n_tested += 2 * 64 * kBoxBlock;
+    if (cls == kDense) {
+        n_near += 2 * 64 * kBoxBlock;
+        for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
+            double cx[kChunkN3], cy[kChunkN3];
+            PairVal<F32> cq[kChunkN3];
+#pragma unroll
+            for (int u = 0; u < kChunkN3; ++u) {
+                cx[u] = xs[j0 + u];
+                cy[u] = ys[j0 + u];
+                if constexpr (F32) cq[u] = qf[j0 + u];
+                else cq[u] = q[j0 + u];
+            }
+            chunk_n3<F32, false, DTC, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr, red, j0 - t0);
+        }
+        return;
+    }
+    for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
+        double cx[kChunkN3], cy[kChunkN3];
+        PairVal<F32> cq[kChunkN3];
+#pragma unroll
+        for (int u = 0; u < kChunkN3; ++u) {
+            cx[u] = xs[j0 + u];
+            cy[u] = ys[j0 + u];
+        }
+        if (!chunk_hit<1, kChunkN3>(t, cx, cy, nc, j0)) continue;
+#pragma unroll
+        for (int u = 0; u < kChunkN3; ++u) {
+            if constexpr (F32) cq[u] = qf[j0 + u];
+            else cq[u] = q[j0 + u];
+        }
+        chunk_n3<F32, true, DTC, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr, red, j0 - t0);
+    }
+}
+
 // Tile (S, T): the four 64-j blocks of superblock T against the wave's i
 // block; adds the i-side forces into ax/ay and writes the reactions on T's
 // 256 particles to react[d - 1] (negated sums over the four waves in a fixed
@@ -195,59 +246,16 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
                                         bool ireal, int32_t T, int32_t d, double &ax, double &ay,
                                         uint64_t &n_near, uint64_t &n_tested, float &smin, RedLds &red,
                                         ScrLds &scr) {
-    const double *__restrict__ xs = a.xs;
-    const double *__restrict__ ys = a.ys;
-    const double *__restrict__ nc = a.nc;
-    const double *__restrict__ q = a.q;
-    const float *__restrict__ qf = a.qf;
-    const double *__restrict__ box = a.box;
-    const int32_t n = a.n, npad = a.npad;
+    const int32_t npad = a.npad;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // the previous tile's sums have been read by all threads
     __syncthreads();
     double *r = red[wave];
     for (int k = lane; k < 2 * kSuper; k += 64) r[k] = 0.0;
-    for (int jb4 = 0; jb4 < 4; ++jb4) {
-        const int32_t jb = T * kSuper + jb4 * kBoxBlock;
-        const BlockClass cls = classify(ib, box + 4 * (jb / kBoxBlock), ireal, jb + kBoxBlock <= n, e.rad, e.cut);
-        if (cls == kSkip) continue;
-        n_tested += 2 * 64 * kBoxBlock;
-        if (cls == kDense) {
-            n_near += 2 * 64 * kBoxBlock;
-            for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
-                double cx[kChunkN3], cy[kChunkN3];
-                PairVal<F32> cq[kChunkN3];
-#pragma unroll
-                for (int u = 0; u < kChunkN3; ++u) {
-                    cx[u] = xs[j0 + u];
-                    cy[u] = ys[j0 + u];
-                    if constexpr (F32) cq[u] = qf[j0 + u];
-                    else cq[u] = q[j0 + u];
-                }
-                chunk_n3<F32, false, DTC, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr[wave], r,
-                                          j0 - T * kSuper);
-            }
-            continue;
-        }
-        for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
-            double cx[kChunkN3], cy[kChunkN3];
-            PairVal<F32> cq[kChunkN3];
-#pragma unroll
-            for (int u = 0; u < kChunkN3; ++u) {
-                cx[u] = xs[j0 + u];
-                cy[u] = ys[j0 + u];
-            }
-            if (!chunk_hit<1, kChunkN3>(t, cx, cy, nc, j0)) continue;
-#pragma unroll
-            for (int u = 0; u < kChunkN3; ++u) {
-                if constexpr (F32) cq[u] = qf[j0 + u];
-                else cq[u] = q[j0 + u];
-            }
-            chunk_n3<F32, true, DTC, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr[wave], r,
-                                     j0 - T * kSuper);
-        }
-    }
+    for (int jb4 = 0; jb4 < 4; ++jb4)
+        n3_jblock<F32, DTC, TAB>(a, e, t, ib, ireal, T * kSuper + jb4 * kBoxBlock, T * kSuper, ax, ay, n_near,
+                                 n_tested, smin, scr[wave], r);
     __syncthreads();
     // the four waves' sums in a fixed order; the reaction is their negation
     const int jj = threadIdx.x;
@@ -256,6 +264,39 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
     double *rr = a.react + (size_t)(d - 1) * 2 * npad;
     rr[T * kSuper + jj] = -rx;
     rr[npad + T * kSuper + jj] = -ry;
+}
+
+// The superblock S with itself in the Newton-3 form.  Each wave takes its
+// own block in the gather form (64 x 63 ordered pairs) and then the block
+// pairs (w, w + 1 mod 4) and, for waves 0 and 1, (w, w + 2): each of the six
+// block pairs once.  The reactions meet in LDS and go to the thread that
+// holds the particle as its i (thread jj holds particle 256 S + jj), in a
+// fixed order.  Called by all 256 threads.
+template <bool F32, bool DTC, bool TAB>
+__device__ __forceinline__ void n3_diag(const N3Row &a, const PairEnv &e, const ITile<1> &t, const Box &ib,
+                                        bool ireal, int32_t S, double &ax, double &ay, uint64_t &n_near,
+                                        uint64_t &n_tested, float &smin, RedLds &red, ScrLds &scr) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // the previous tile's sums have been read by all threads
+    __syncthreads();
+    double *r = red[wave];
+    for (int k = lane; k < 2 * kSuper; k += 64) r[k] = 0.0;
+    const int32_t own = S * kSuper + wave * kBoxBlock;
+    double gx[1] = {0.0}, gy[1] = {0.0};
+    gather_blocks<1, kChunkN3, F32, TAB>(t, a.xs, a.ys, a.nc, a.q, a.box, ib, a.n, own, own + kBoxBlock, e, gx, gy,
+                                         n_near, n_tested, smin);
+    ax = ax + gx[0];
+    ay = ay + gy[0];
+    n3_jblock<F32, DTC, TAB>(a, e, t, ib, ireal, S * kSuper + ((wave + 1) & 3) * kBoxBlock, S * kSuper, ax, ay,
+                             n_near, n_tested, smin, scr[wave], r);
+    if (wave < 2)
+        n3_jblock<F32, DTC, TAB>(a, e, t, ib, ireal, S * kSuper + (wave + 2) * kBoxBlock, S * kSuper, ax, ay,
+                                 n_near, n_tested, smin, scr[wave], r);
+    __syncthreads();
+    const int jj = threadIdx.x;
+    ax = ax - (((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj]);
+    ay = ay - (((red[0][2 * jj + 1] + red[1][2 * jj + 1]) + red[2][2 * jj + 1]) + red[3][2 * jj + 1]);
 }
 
 // The pair counts of a workgroup's waves, added with one atomic each to one

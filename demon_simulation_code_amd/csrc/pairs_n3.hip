@@ -170,9 +170,8 @@ __device__ __forceinline__ void n3_item(const N3Row &a, const PairEnv &e, int32_
     for (int32_t k = k0; k < k1; ++k) {
         const int32_t d = __builtin_amdgcn_readfirstlane(lst[k]);
         if (d == 0) {
-            // the superblock with itself, every ordered pair, j == i skipped
-            gather_blocks<1, kChunkN3, F32, !DTC>(t, a.xs, a.ys, a.nc, a.q, a.box, ib, n, S * kSuper,
-                                                  S * kSuper + kSuper, e, ax, ay, n_near, n_tested, smin);
+            // the superblock with itself, each unordered pair once
+            n3_diag<F32, DTC, !DTC>(a, e, t, ib, ireal, S, ax[0], ay[0], n_near, n_tested, smin, red, scr);
             continue;
         }
         const int32_t T = S + d < nsb ? S + d : S + d - nsb;
