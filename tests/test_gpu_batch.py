@@ -27,10 +27,12 @@ def _make(clouds, seed, multiple_of=4, n_max=1500, dt_check=True, forces=7, spac
     return b, ens, states
 
 
-def _check(oracle, b, ens, states, steps, dt_check=True, tol=1e-9):
+def _check(oracle, b, ens, states, steps, dt_check=True, tol=1e-9, clouds=None, gather=0):
     for c, (e, (x, y, q, m)) in enumerate(zip(ens, states)):
+        if clouds is not None and c not in clouds:
+            continue
         z = np.zeros(e["n"])
-        op = OracleParams(e["kappa"], e["confine"], e["gamma"], 0.0, 0, 7, 1 if dt_check else 0)
+        op = OracleParams(e["kappa"], e["confine"], e["gamma"], 0.0, 0, 7, 1 if dt_check else 0, gather)
         o = oracle.rk4_steps(x, y, z, z, q, m, op, 1e-3, steps)
         g = b.download(c)
         st = b.stats(c)
@@ -127,3 +129,53 @@ def test_batch_newton3_equals_gather_fp32():
         for c in range(len(ens)):
             u, v = b1.download(c), b2.download(c)
             assert pos_rel_err(u[0], u[1], v[0], v[1]) <= 1e-5, c
+
+
+def test_batch_newton3_bucket13_matches_oracle(oracle):
+    """Clouds of npad = 8192 (N in (7936, 8192], the ensemble's top size
+    bucket) in the Newton-3 form with 1 and 3 superblock offsets per item,
+    20 steps, against the oracle (its cell-list form, bit for bit its
+    all-pairs restatement of ShieldedCoulombForce)."""
+    sizes = [8192, 8000, 7940, 8188]
+    rng = np.random.default_rng(13)
+    ens = [dict(n=n, kappa=1.0 / rng.uniform(250e-6, 1000e-6), confine=5e-11 / (6000 * 1.602e-19), gamma=2.0,
+                charge=rng.uniform(3000, 9000) * -1.602e-19) for n in sizes]
+    for span in (1, 3):
+        b = dm.Batch([e["n"] for e in ens], [e["kappa"] for e in ens], [e["confine"] for e in ens],
+                     [e["gamma"] for e in ens])
+        states = []
+        with b:
+            b.set_shape(13, dm.BATCH_NEWTON3, span)
+            assert b.shape(13)[:2] == (dm.BATCH_NEWTON3, span)
+            for c, e in enumerate(ens):
+                x, y = dm.hex_lattice(e["n"], 500e-6, 50e-6, 300 + c)
+                q, m = np.full(e["n"], e["charge"]), np.full(e["n"], M0)
+                b.upload(c, x, y, None, None, q, m)
+                states.append((x, y, q, m))
+            b.steps(20, 1e-3)
+            _check(oracle, b, ens, states, 20, gather=2)
+
+
+def test_batch_real_ensemble_sample_matches_oracle(oracle):
+    """The benchmark ensemble itself (256 clouds, N log-uniform in
+    [64, 8192], autotuned forms and shapes, one captured graph per step),
+    25 steps: the smallest, the largest and ten more clouds spread over the
+    sizes against the oracle (cell-list form)."""
+    ens = ensemble(256, seed=4, multiple_of=4)
+    b = dm.Batch([e["n"] for e in ens], [e["kappa"] for e in ens], [e["confine"] for e in ens],
+                 [e["gamma"] for e in ens])
+    states = []
+    with b:
+        for c, e in enumerate(ens):
+            x, y = dm.hex_lattice(e["n"], 500e-6, 50e-6, 100 + c)
+            q, m = np.full(e["n"], e["charge"]), np.full(e["n"], 2.8e-14)
+            b.upload(c, x, y, None, None, q, m)
+            states.append((x, y, q, m))
+        b.tune(1)
+        b.use_graph(True)
+        b.steps(25, 1e-3)
+        order = np.argsort([e["n"] for e in ens])
+        sample = {int(order[k]) for k in np.linspace(0, 255, 12).astype(int)}
+        forms = {b.shape(bk)[0] for bk in range(8, 14)}
+        assert dm.BATCH_NEWTON3 in forms  # the tuner picks the Newton-3 form somewhere
+        _check(oracle, b, ens, states, 25, clouds=sample, gather=2)
