@@ -434,8 +434,11 @@ int enqueue_step_ex(demon_ctx *c, int dtcheck, double *store) {
 
 int enqueue_step(demon_ctx *c) { return enqueue_step_ex(c, c->p.dt_check, nullptr); }
 
+// The captured step is destroyed only once the stream has drained: a replay
+// still in flight must not lose its executable graph.
 void drop_graph(demon_ctx *c) {
     if (c->graph) {
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
     }

@@ -704,8 +704,9 @@ int demon_batch_create(demon_batch **out, int32_t nclouds, const int64_t *n, con
 
 void demon_batch_destroy(demon_batch *b) {
     if (!b) return;
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    // drain the stream before the captured step goes away
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (hipEvent_t e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (void *ptr : {(void *)b->arena, (void *)b->part, (void *)b->react, (void *)b->rflag, (void *)b->orig,
@@ -867,6 +868,7 @@ int demon_batch_use_graph(demon_batch *b, int32_t enable) {
     if (!b) return bfail(nullptr, DEMON_ERR_ARG, "batch is NULL");
     b->use_graph = enable != 0;
     if (b->graph) {
+        (void)hipStreamSynchronize(b->stream);
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
