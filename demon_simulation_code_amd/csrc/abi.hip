@@ -225,13 +225,16 @@ int ensure_n3(demon_ctx *c) {
     }
     // no tile evaluated yet: the stage kernels must read no reaction and no
     // item; the plan's counters start at zero
-    HIP_TRY(c, hipMemset(c->d_rflag, 0, sizeof(uint32_t) * half * nsb));
-    HIP_TRY(c, hipMemset(c->d_n3i, 0, sizeof(int32_t) * n3_ints(c->npad)));
-    HIP_TRY(c, hipMemset(c->d_plan, 0, sizeof(N3Plan)));
+    // (on the context's stream, which is non-blocking: null-stream fills would
+    // not be ordered before its kernels)
+    HIP_TRY(c, hipMemsetAsync(c->d_rflag, 0, sizeof(uint32_t) * half * nsb, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_n3i, 0, sizeof(int32_t) * n3_ints(c->npad), c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_plan, 0, sizeof(N3Plan), c->stream));
     // e^{-k/64}: the double nearest to a long-double evaluation
     double tab[kExpTabN];
     for (int k = 0; k < kExpTabN; ++k) tab[k] = (double)std::exp(-(long double)k / 64.0L);
-    HIP_TRY(c, hipMemcpy(c->d_exptab, tab, sizeof tab, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpyAsync(c->d_exptab, tab, sizeof tab, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));  // tab is a local
     return DEMON_OK;
 }
 
@@ -675,9 +678,11 @@ int demon_create(demon_ctx **out, int64_t n, const demon_params *p) {
     if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc index map failed"));
     e = hipMalloc((void **)&c->d_sc, sizeof(Scalars));
     if (e != hipSuccess) return bail(fail(c, DEMON_ERR_ALLOC, "hipMalloc scalars failed"));
-    e = hipMemset(c->d_arena, 0, sizeof(double) * P * 32);
-    if (e == hipSuccess) e = hipMemset(c->d_part, 0, sizeof(double) * part_doubles(c->npad));
-    if (e == hipSuccess) e = hipMemset(c->d_sc, 0, sizeof(Scalars));
+    // fills on the context's own (non-blocking) stream, ordered before its kernels
+    e = hipMemsetAsync(c->d_arena, 0, sizeof(double) * P * 32, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_part, 0, sizeof(double) * part_doubles(c->npad), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->d_sc, 0, sizeof(Scalars), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return bail(hip_fail(c, e, "hipMemset"));
     double *b = c->d_arena;
     auto take = [&]() { double *r = b; b += P; return r; };
@@ -761,7 +766,7 @@ int demon_upload(demon_ctx *c, const double *x, const double *y, const double *v
     std::vector<int32_t> orig((size_t)c->npad);
     for (int64_t k = 0; k < c->npad; ++k)
         orig[(size_t)k] = k < c->n ? (int32_t)c->perm[(size_t)k] : (int32_t)(0x40000000 + k);
-    HIP_TRY(c, hipMemcpy(c->d_orig, orig.data(), sizeof(int32_t) * c->npad, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpyAsync(c->d_orig, orig.data(), sizeof(int32_t) * c->npad, hipMemcpyHostToDevice, c->stream));
     int rc;
     // each put uses its own staging slot until the stream is synchronised
     if ((rc = put(c, c->x, x, 0.0, 0)) || (rc = put(c, c->y, y, 0.0, 1)) ||

@@ -508,8 +508,11 @@ int rebuild_items(demon_batch *b) {
             return bfail(b, DEMON_ERR_ALLOC, "hipMalloc tile flags failed");
         b->rflag_words = cnt;
     }
-    BHIP(b, hipMemcpy(b->d_items, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice));
-    BHIP(b, hipMemcpy(b->d_clouds, b->hc.data(), sizeof(BCloud) * b->hc.size(), hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpyAsync(b->d_items, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice,
+                           b->stream));
+    BHIP(b, hipMemcpyAsync(b->d_clouds, b->hc.data(), sizeof(BCloud) * b->hc.size(), hipMemcpyHostToDevice,
+                           b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));  // items is a local
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -669,9 +672,11 @@ int demon_batch_create(demon_batch **out, int32_t nclouds, const int64_t *n, con
         hipMalloc((void **)&b->d_sc, sizeof(Scalars) * (size_t)nclouds) != hipSuccess ||
         hipMalloc((void **)&b->d_counters, sizeof(unsigned int) * kGroups) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_ALLOC, "hipMalloc failed"));
-    if (hipMemset(b->arena, 0, sizeof(double) * P * 25) != hipSuccess ||
-        hipMemset(b->d_sc, 0, sizeof(Scalars) * (size_t)nclouds) != hipSuccess ||
-        hipMemset(b->d_counters, 0, sizeof(unsigned int) * kGroups) != hipSuccess)
+    // every fill and copy goes on the batch's own stream: it is non-blocking,
+    // so work on the null stream would not be ordered before its kernels
+    if (hipMemsetAsync(b->arena, 0, sizeof(double) * P * 25, b->stream) != hipSuccess ||
+        hipMemsetAsync(b->d_sc, 0, sizeof(Scalars) * (size_t)nclouds, b->stream) != hipSuccess ||
+        hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * kGroups, b->stream) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_HIP, "hipMemset failed"));
     double *a = b->arena;
     auto take = [&]() { double *r = a; a += P; return r; };
@@ -687,8 +692,8 @@ int demon_batch_create(demon_batch **out, int32_t nclouds, const int64_t *n, con
     }
     b->qf = reinterpret_cast<float *>(a);  // P floats
     a += P / 2;
-    if (hipMemcpy(b->d_block_cloud, b->block_cloud.data(), sizeof(int32_t) * b->block_cloud.size(),
-                  hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpyAsync(b->d_block_cloud, b->block_cloud.data(), sizeof(int32_t) * b->block_cloud.size(),
+                       hipMemcpyHostToDevice, b->stream) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_HIP, "hipMemcpy failed"));
     for (int k = 0; k < 2; ++k)
         if (hipEventCreate(&b->ev[k]) != hipSuccess) return bail(bfail(b, DEMON_ERR_HIP, "hipEventCreate failed"));
@@ -759,16 +764,23 @@ int demon_batch_upload(demon_batch *b, int32_t cloud, const double *x, const dou
         for (int64_t k = 0; k < n; ++k) h[(size_t)(a * np + k)] = src[a] ? src[a][perm[(size_t)k]] : 0.0;
     for (int64_t k = 0; k < np; ++k) o[(size_t)k] = k < n ? (int32_t)perm[(size_t)k] : (int32_t)(0x40000000 + k);
     BHIP(b, hipStreamSynchronize(b->stream));
+    // Every copy is ordered on the batch's stream.  A device-to-device
+    // hipMemcpy runs on the null stream and returns before it completes; the
+    // batch's stream is non-blocking, so the box pass below could otherwise
+    // read the stage record before it lands (boxes of stale or zero
+    // coordinates make far block pairs look fully inside the cutoff).
     double *dst[6] = {b->x, b->y, b->vx, b->vy, b->q, b->m};
     for (int a = 0; a < 6; ++a)
-        BHIP(b, hipMemcpy(dst[a] + cl.off, h.data() + (size_t)a * np, sizeof(double) * np, hipMemcpyHostToDevice));
-    BHIP(b, hipMemcpy(b->orig + cl.off, o.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+        BHIP(b, hipMemcpyAsync(dst[a] + cl.off, h.data() + (size_t)a * np, sizeof(double) * np,
+                               hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->orig + cl.off, o.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, b->stream));
     BHIP(b, launch_round_fp32(b->q + cl.off, b->qf + cl.off, np, b->stream));
     // stage-1 record of this cloud
     const double *bs[4] = {b->x, b->y, b->vx, b->vy};
     double *rec[4] = {b->sb[0].xs, b->sb[0].ys, b->sb[0].vxs, b->sb[0].vys};
     for (int a = 0; a < 4; ++a)
-        BHIP(b, hipMemcpy(rec[a] + cl.off, bs[a] + cl.off, sizeof(double) * np, hipMemcpyDeviceToDevice));
+        BHIP(b, hipMemcpyAsync(rec[a] + cl.off, bs[a] + cl.off, sizeof(double) * np, hipMemcpyDeviceToDevice,
+                               b->stream));
     StageBuf view{b->sb[0].xs + cl.off, b->sb[0].ys + cl.off, b->sb[0].vxs + cl.off, b->sb[0].vys + cl.off,
                   b->sb[0].nc + cl.off};
     view.box = b->sb[0].box + cl.off / 16;  // 4 doubles per 64-particle block
@@ -947,7 +959,8 @@ int demon_batch_tune(demon_batch *b, int32_t reps) {
             for (auto &e : all) items.push_back(e.second);
             BItem *d_it = nullptr;
             BHIP(b, hipMalloc((void **)&d_it, sizeof(BItem) * items.size()));
-            BHIP(b, hipMemcpy(d_it, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice));
+            BHIP(b, hipMemcpyAsync(d_it, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice,
+                                   b->stream));
             float tot = 0.0f;
             int ok = DEMON_OK;
             for (int r = 0; r <= reps && !ok; ++r) {
@@ -973,7 +986,7 @@ int demon_batch_tune(demon_batch *b, int32_t reps) {
         b->bucket_js[bk] = bestJ;
         b->bucket_us[bk] = best;
     }
-    BHIP(b, hipMemset(b->d_counters, 0, sizeof(unsigned int) * kGroups));
+    BHIP(b, hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * kGroups, b->stream));
     return rebuild_items(b);
 }
 
