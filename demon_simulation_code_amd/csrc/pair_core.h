@@ -136,19 +136,20 @@ __device__ __forceinline__ double exp_neg(double u, const ExpCoef &ec) {
 // tab[k] = e^{-k/64} (k < kExpTabN, each the double nearest to a
 // long-double evaluation on the host): n = rint(64 u) by the same
 // magic-number rounding, d = u - n/64 (exact: n/64 is exact and u - n/64
-// needs at most 43 significant bits), and e^-d by its degree-6 Taylor
-// polynomial, |d| <= 1/128, whose truncation error is below 4e-19 relative;
-// about 1 ulp in all.  The Taylor coefficients tc[0..5] = 1/720, -1/120,
-// 1/24, -1/6, 1/2, -1 and the reduction's constants tc[6..8] = 64, 1.5 2^52,
-// -1/64 arrive as kernel arguments (SGPRs), as exp_neg's do: as literals the
-// rounding constant would be copied into a VGPR pair for every evaluation.
+// needs at most 43 significant bits), and e^-d by the degree-5 Chebyshev
+// interpolant of e^-d on |d| <= 1/128 (gen_exp_coef.py: 2.0e-17 relative
+// with the coefficients rounded to double); about 1 ulp in all.  Its
+// coefficients tc[0..4] (highest power first; the constant term is 1) and
+// the reduction's constants tc[6..8] = 64, 1.5 2^52, -1/64 arrive as kernel
+// arguments (SGPRs), as exp_neg's do: as literals the rounding constant
+// would be copied into a VGPR pair for every evaluation.
 // Three FMA-class operations and an add fewer than exp_neg, and no ldexp:
 // the table replaces the reduction's ln 2 split and the high-degree tail.
 constexpr double kExpTabScale = 64.0;
 
 inline ExpCoef exp_tab_coef() {
-    return ExpCoef{{1.0 / 720.0, -1.0 / 120.0, 1.0 / 24.0, -1.0 / 6.0, 0.5, -1.0, kExpTabScale, 0x1.8p52,
-                    -1.0 / kExpTabScale, 0.0}};
+    return ExpCoef{{-0.008333351498575732, 0.0416667938233981, -0.1666666666662509, 0.4999999999970896, -1.0, 0.0,
+                    kExpTabScale, 0x1.8p52, -1.0 / kExpTabScale, 0.0}};
 }
 
 __device__ __forceinline__ double exp_neg_tab(double u, const double *__restrict__ tab, const ExpCoef &tc) {
@@ -157,7 +158,7 @@ __device__ __forceinline__ double exp_neg_tab(double u, const double *__restrict
     const double d = fma(m - tc.c[7], tc.c[8], u);
     double p = tc.c[0];
 #pragma unroll
-    for (int k = 1; k < 6; ++k) p = fma(p, d, tc.c[k]);
+    for (int k = 1; k < 5; ++k) p = fma(p, d, tc.c[k]);
     p = fma(p, d, 1.0);
     return tab[ni] * p;
 }
@@ -177,16 +178,19 @@ __device__ __forceinline__ double pair_scale_fp64(double r2, double cqi, double 
 
 // FP32 pair force: distance, exp and force in FP32 (hardware reciprocal
 // square root and 2^x) from the FP64 difference rounded once.  kl2 =
-// -kappa log2(e), so a = kl2 r gives e^-u = 2^a and 1 + u = 1 - a ln 2.
-__device__ __forceinline__ void pair_force_fp32f(double dx, double dy, float cqi, float qj, float kl2,
+// -kappa log2(e), so a = kl2 r gives e^-u = 2^a; kf = kappa, and
+// (1 + u) / r^3 = y^3 + kappa r y^3 = y^2 (y + kappa) with y = 1/r, which
+// takes one operation fewer than forming 1 + u (the FP32 rounding of r y = 1
+// is far below the FP32 path's error).
+__device__ __forceinline__ void pair_force_fp32f(double dx, double dy, float cqi, float qj, float kl2, float kf,
                                                  float &fx, float &fy) {
     const float gx = (float)dx, gy = (float)dy;
     const float r2 = fmaf(gx, gx, gy * gy);
     const float y = __builtin_amdgcn_rsqf(r2);
     const float a = (r2 * y) * kl2;
     const float e = __builtin_amdgcn_exp2f(a);
-    const float b = fmaf(a, -0.693147182f, 1.0f);
-    const float s = ((cqi * qj) * (b * e)) * ((y * y) * y);
+    const float g = (y * y) * (y + kf);
+    const float s = ((cqi * qj) * e) * g;
     fx = s * gx;
     fy = s * gy;
 }
@@ -197,10 +201,10 @@ __host__ __device__ __forceinline__ float fp32_kl2(double kappa) {
 }
 
 // The same, with the components returned in FP64 for FP64 accumulation.
-__device__ __forceinline__ void pair_force_fp32(double dx, double dy, float cqi, float qj, float kl2,
+__device__ __forceinline__ void pair_force_fp32(double dx, double dy, float cqi, float qj, float kl2, float kf,
                                                 double &fx, double &fy) {
     float gx, gy;
-    pair_force_fp32f(dx, dy, cqi, qj, kl2, gx, gy);
+    pair_force_fp32f(dx, dy, cqi, qj, kl2, kf, gx, gy);
     fx = (double)gx;
     fy = (double)gy;
 }
@@ -393,7 +397,7 @@ __device__ __forceinline__ void chunk_gather(const ITile<R> &t, const double (&c
             if (in) {
                 if (F32) {
                     double fx, fy;
-                    pair_force_fp32(dx, dy, t.cqf[r], (float)cq[u], e.kl2_f, fx, fy);
+                    pair_force_fp32(dx, dy, t.cqf[r], (float)cq[u], e.kl2_f, (float)e.cut.kappa, fx, fy);
                     ax[r] = ax[r] + fx;
                     ay[r] = ay[r] + fy;
                 } else {

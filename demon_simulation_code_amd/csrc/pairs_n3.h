@@ -136,7 +136,7 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
         V fx = 0, fy = 0;
         if (in) {
             if constexpr (F32) {
-                pair_force_fp32f(dx, dy, t.cqf[0], cq[u], e.kl2_f, fx, fy);
+                pair_force_fp32f(dx, dy, t.cqf[0], cq[u], e.kl2_f, (float)e.cut.kappa, fx, fy);
             } else {
                 const double s = pair_scale_env<TAB>(r2, t.cqi[0], cq[u], e);
                 fx = s * dx;

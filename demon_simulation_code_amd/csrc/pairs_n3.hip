@@ -171,11 +171,11 @@ __device__ __forceinline__ void n3_item(const N3Row &a, const PairEnv &e, int32_
         const int32_t d = __builtin_amdgcn_readfirstlane(lst[k]);
         if (d == 0) {
             // the superblock with itself, each unordered pair once
-            n3_diag<F32, DTC, !DTC>(a, e, t, ib, ireal, S, ax[0], ay[0], n_near, n_tested, smin, red, scr);
+            n3_diag<F32, DTC, true>(a, e, t, ib, ireal, S, ax[0], ay[0], n_near, n_tested, smin, red, scr);
             continue;
         }
         const int32_t T = S + d < nsb ? S + d : S + d - nsb;
-        n3_tile<F32, DTC, !DTC>(a, e, t, ib, ireal, T, d, ax[0], ay[0], n_near, n_tested, smin, red, scr);
+        n3_tile<F32, DTC, true>(a, e, t, ib, ireal, T, d, ax[0], ay[0], n_near, n_tested, smin, red, scr);
     }
     double *px = a.part + (size_t)it * (2 * kSuper);
     px[wave * 64 + lane] = ax[0];
@@ -190,11 +190,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ double etab[kExpTabN];
     __shared__ int32_t slot;
     if ((a.flags & kFlagGated) && !a.sc->active) return;
-    // The FP64 exp is exp_neg_tab on this table, except in the stage-1
-    // launch (the dt check's FP32 replay leaves no registers for it there:
-    // it keeps exp_neg).  The item loop's first barrier orders these stores
-    // before any read.
-    if (!DTC)
+    // The FP64 exp is exp_neg_tab on this table.  The item loop's first
+    // barrier orders these stores before any read.
+    if (!F32)
         for (int k = (int)threadIdx.x; k < kExpTabN; k += 256) etab[k] = l.exptab[k];
     const int32_t nsb = a.npad / kSuper, half = nsb / 2;
     const Radii rad = launch_radii(a.rc2p, a.flags, a.sc->rmax2_bits);
@@ -266,8 +264,7 @@ hipError_t launch_pairs_n3(const PairLaunch &a, hipStream_t s) {
     if (e != hipSuccess) return e;
     auto kern = a.fp32pair ? (a.dtcheck ? k_pairs_n3<true, true> : k_pairs_n3<true, false>)
                            : (a.dtcheck ? k_pairs_n3<false, true> : k_pairs_n3<false, false>);
-    hipLaunchKernelGGL(kern, dim3((unsigned)a.lst.grid), dim3(256), 0, s, row, a.lst,
-                       a.dtcheck ? exp_coef() : exp_tab_coef());
+    hipLaunchKernelGGL(kern, dim3((unsigned)a.lst.grid), dim3(256), 0, s, row, a.lst, exp_tab_coef());
     return hipGetLastError();
 }
 
