@@ -50,6 +50,13 @@ FP64_PEAK = 78.6e12       # MI355X FP64 vector, FLOP/s
 FP32_PEAK = 157.3e12      # MI355X FP32 vector, FLOP/s
 HBM_PEAK = 8.0e12         # MI355X HBM3E, B/s
 FLOP_FAR, FLOP_NEAR = 7, 20
+# The adjoint pair sum per ordered pair inside the cutoff, in the same
+# arithmetic (+ - x / sqrt exp each 1, FMA 2): the 7 of the distance test
+# (which forms r and u = kappa r), y = 1/r (1), y^2, y^3 (2), C q_j (1),
+# exp (1), phi/q_i = C q_j e^-u (1 + u) y^3 (4), C q_i q_j e^-u (2), phi (1),
+# phi'/r (6), u_i - u_j (2), its dot with d (3), w (1), the two
+# force-adjoint sums (8), the charge sum (2), the kappa sum (7).
+FLOP_ADJ_NEAR = 48
 # HBM traffic of the FP64 all-pairs kernel per launch: FETCH_SIZE and
 # WRITE_SIZE from two rocprofv3 --pmc passes over a run of the same
 # command, scaled by what they read for a plain copy of known size (the
@@ -188,6 +195,30 @@ def roofline(r, n, launches, peak, kernel):
     }
 
 
+def counts_roofline(near, tested, ordered, pair_ms, launches, peak, kernel, flop_near=FLOP_NEAR):
+    """A pair kernel's executed FP64 FLOP rate from its in-kernel counts and
+    its HIP-event time: FLOP_FAR per ordered pair distance-tested, flop_near
+    per ordered pair inside the cutoff; `ordered` is the ordered pairs one
+    launch covers (culled or not)."""
+    pair_s = pair_ms / 1e3
+    executed = FLOP_FAR * (tested - near) + flop_near * near
+    return {
+        "bound": "valu",
+        "achieved": executed / pair_s / 1e12 if pair_s else None,
+        "peak": peak / 1e12,
+        "unit": "TFLOP/s",
+        "frac": executed / pair_s / peak if pair_s else None,
+        "kernel": kernel,
+        "launches": launches,
+        "mean_launch_ms": pair_ms / max(1, launches),
+        "flop_per_launch": executed / max(1, launches),
+        "flop_counting": f"executed: {FLOP_FAR} per ordered pair distance-tested, {flop_near} per ordered pair "
+                         "inside the cutoff",
+        "near_fraction": near / (launches * ordered) if launches else None,
+        "tested_fraction": tested / (launches * ordered) if launches else None,
+    }
+
+
 # Bytes per particle of one stage update that do not depend on the pair
 # kernel: stage record in (x, y, vx, vy), charge, mass, caller index, base
 # state, stage record out (x, y, vx, vy, nc) and its box table, and the RK
@@ -261,6 +292,23 @@ def run_ensemble(steps, warmup, graph=True):
             ("superblock_offsets_per_item" if R == 0 else "j_slice"): js, "tuned_us_per_pass": us}
     out = {"value": pairs / wall, "unit": "pair interactions/s", "clouds": len(sizes), "steps": steps,
            "n_total": int(sum(sizes)), "wall_s": wall, "device_ms": b.last_ms(), "graph": graph, "shapes": shapes}
+    # an instrumented pass over the same steps: plain launches with HIP
+    # events around every all-pairs pass and stage update, and the pair
+    # kernels' own counts
+    upload()
+    b.use_graph(False)
+    b.set_timing(True)
+    b.steps(steps, 1e-3)
+    kt = b.kernel_timing()
+    near, tested = b.pair_counts()
+    b.set_timing(False)
+    ordered = sum(n * (n - 1) for n in sizes)
+    rl = counts_roofline(near, tested, ordered, kt["pair_ms"], kt["pair_launches"], FP64_PEAK,
+                         "k_bpairs / k_bpairs_n3 (FP64 all-pairs, every cloud in one launch)")
+    rl["instrumented_pass_ms"] = b.last_ms()
+    rl["pair_kernel_share"] = kt["pair_ms"] / b.last_ms() if b.last_ms() else None
+    rl["update_mean_launch_ms"] = kt["update_ms"] / max(1, kt["update_launches"])
+    out["roofline"] = rl
     b.close()
     return out
 
@@ -314,6 +362,7 @@ def run_adjoint(reps=3):
     tx, ty = np.array(tx), np.array(ty)
     q = wl.charge * (1 + 0.1 * np.random.default_rng(12).uniform(-1, 1, n))
     with dm.Context(n, shielding=wl.kappa * 1.02, confine=wl.confine * 0.97, gamma=wl.gamma * 1.05) as c:
+        c.set_timing(False)  # the timed gradients run without per-launch events
         c.upload(x, y, None, None, q, m)
         c.fit_loss_grad(steps, wl.dt, obs, ckpt, tx, ty)  # warm-up (allocates the fit arena)
         fwd = bwd = 0.0
@@ -328,9 +377,18 @@ def run_adjoint(reps=3):
         for _ in range(reps):
             c.fit_loss_grad(steps, wl.dt, obs, ckpt, tx, ty, grad=False)
         loss_only = (time.perf_counter() - t0) / reps
+        # one instrumented gradient: HIP events around every adjoint pair
+        # launch, and the kernel's own pair counts
+        c.set_timing(True)
+        c.fit_loss_grad(steps, wl.dt, obs, ckpt, tx, ty)
+        ak = c.fit_adjoint_kernel()
+    rl = counts_roofline(ak["near_pairs"], ak["tested_pairs"], n * (n - 1), ak["pair_ms"], ak["launches"],
+                         FP64_PEAK, "k_adj_pairs (FP64 adjoint pair sums)", flop_near=FLOP_ADJ_NEAR)
+    rl["share_of_recompute_adjoint"] = ak["pair_ms"] / (bwd / reps) if bwd else None
     return {"grads_per_s": 1.0 / wall, "unit": "full gradients/s", "n": n, "steps": steps,
             "params": n + 3, "forward_ms": fwd / reps, "recompute_adjoint_ms": bwd / reps,
-            "gradient_over_forward": (fwd + bwd) / fwd, "loss_only_s": loss_only, "wall_s": wall}
+            "gradient_over_forward": (fwd + bwd) / fwd, "loss_only_s": loss_only, "wall_s": wall,
+            "roofline": rl}
 
 
 def fp32_force_error(wl, state):

@@ -115,6 +115,7 @@ SIGNATURES = {
     "demon_set_pair_shape": (ctypes.c_int, [_CTX, _I32, _I32]),
     "demon_get_pair_shape": (ctypes.c_int, [_CTX, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "demon_last_timing": (ctypes.c_int, [_CTX, _D, _D]),
+    "demon_set_timing": (ctypes.c_int, [_CTX, _I32]),
     "demon_pair_counts": (ctypes.c_int, [_CTX, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "demon_pair_tiles": (ctypes.c_int, [_CTX, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "demon_last_kernel_ms": (ctypes.c_int, [_CTX, _D, ctypes.POINTER(_I64), _D, _D, ctypes.POINTER(_I64)]),
@@ -124,6 +125,8 @@ SIGNATURES = {
     "demon_fit_loss_grad": (ctypes.c_int, [_CTX, _I64, ctypes.c_double, _I64, _I64, _D, _D, _D, _D, _D,
                                            _D, _D, _D, _D, _D, _D]),
     "demon_fit_timing": (ctypes.c_int, [_CTX, _D, _D]),
+    "demon_fit_adjoint_kernel": (ctypes.c_int, [_CTX, _D, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                                                ctypes.POINTER(_I64)]),
     "demon_batch_last_error": (ctypes.c_char_p, [_CTX]),
     "demon_batch_create": (ctypes.c_int, [ctypes.POINTER(_CTX), _I32, ctypes.POINTER(_I64), _D, _D, _D,
                                           ctypes.POINTER(Params)]),
@@ -134,6 +137,9 @@ SIGNATURES = {
     "demon_batch_cloud_stats": (ctypes.c_int, [_CTX, _I32, ctypes.POINTER(Stats)]),
     "demon_batch_use_graph": (ctypes.c_int, [_CTX, _I32]),
     "demon_batch_last_ms": (ctypes.c_int, [_CTX, _D]),
+    "demon_batch_set_timing": (ctypes.c_int, [_CTX, _I32]),
+    "demon_batch_kernel_ms": (ctypes.c_int, [_CTX, _D, ctypes.POINTER(_I64), _D, ctypes.POINTER(_I64)]),
+    "demon_batch_pair_counts": (ctypes.c_int, [_CTX, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "demon_batch_set_shape": (ctypes.c_int, [_CTX, _I32, _I32, _I32]),
     "demon_batch_get_shape": (ctypes.c_int, [_CTX, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _D]),
     "demon_batch_tune": (ctypes.c_int, [_CTX, _I32]),
@@ -349,6 +355,19 @@ class Context:
         self._c(lib.demon_fit_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def set_timing(self, on: bool):
+        """Per-launch HIP events around plain launches (on by default)."""
+        self._c(lib.demon_set_timing(self._h, 1 if on else 0))
+
+    def fit_adjoint_kernel(self):
+        """The last fit's adjoint pair kernel: dict(pair_ms, launches,
+        near_pairs, tested_pairs); the time comes from HIP events (0 with
+        timing off)."""
+        ms, nl, nn, nt = ctypes.c_double(0), _I64(0), _I64(0), _I64(0)
+        self._c(lib.demon_fit_adjoint_kernel(self._h, ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(nn),
+                                             ctypes.byref(nt)))
+        return dict(pair_ms=ms.value, launches=nl.value, near_pairs=nn.value, tested_pairs=nt.value)
+
     def last_timing(self):
         a, b = ctypes.c_double(0), ctypes.c_double(0)
         self._c(lib.demon_last_timing(self._h, ctypes.byref(a), ctypes.byref(b)))
@@ -451,6 +470,26 @@ class Batch:
         v = ctypes.c_double(0)
         self._c(lib.demon_batch_last_ms(self._h, ctypes.byref(v)))
         return v.value
+
+    def set_timing(self, on: bool):
+        """HIP events around every pair pass and stage update of plain
+        (uncaptured) steps."""
+        self._c(lib.demon_batch_set_timing(self._h, 1 if on else 0))
+
+    def kernel_timing(self):
+        """Device ms of the last steps call by kernel: dict(pair_ms,
+        pair_launches, update_ms, update_launches)."""
+        pm, um, pl, ul = ctypes.c_double(0), ctypes.c_double(0), _I64(0), _I64(0)
+        self._c(lib.demon_batch_kernel_ms(self._h, ctypes.byref(pm), ctypes.byref(pl), ctypes.byref(um),
+                                          ctypes.byref(ul)))
+        return dict(pair_ms=pm.value, pair_launches=pl.value, update_ms=um.value, update_launches=ul.value)
+
+    def pair_counts(self):
+        """(ordered pairs inside the cutoff, ordered pairs distance-tested),
+        summed over the clouds since their uploads."""
+        a, b = _I64(0), _I64(0)
+        self._c(lib.demon_batch_pair_counts(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def set_shape(self, bucket, i_per_lane, jslice):
         """i_per_lane 1, 2, 4: gather form, j slices of jslice particles;

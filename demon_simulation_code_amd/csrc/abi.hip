@@ -77,6 +77,8 @@ struct demon_ctx {
     double *d_fit = nullptr;
     size_t fit_doubles = 0;
     double last_fwd_ms = 0.0, last_bwd_ms = 0.0;
+    double last_adj_ms = 0.0;         // the last fit's adjoint pair launches (HIP events)
+    int64_t last_adj_launches = 0;
     hipEvent_t ev_fit[3]{};
     // analysis scratch (demon_kinetic_energy, demon_pair_histogram), grown on demand
     double *d_an = nullptr;
@@ -319,15 +321,16 @@ int put(demon_ctx *c, double *dst, const double *src, double fill, size_t slot) 
 // Launches timed with HIP events on the context stream: the all-pairs
 // kernel, the Newton-3 tile-list kernel and the stage update.  Events are
 // recorded around plain launches only, not into a captured graph.
-enum TimedKind : uint8_t { kTimedPairs = 0, kTimedList = 1, kTimedUpdate = 2 };
+enum TimedKind : uint8_t { kTimedPairs = 0, kTimedList = 1, kTimedUpdate = 2, kTimedAdjPairs = 3 };
 
 void ev_reset(demon_ctx *c) {
     c->ev_used = 0;
     c->ev_kind.clear();
 }
 
-int ev_open(demon_ctx *c, bool *on) {
-    *on = c->timing && !c->use_graph;
+// Opens a timed launch when want is set (the event pair is closed by ev_close).
+int ev_open_if(demon_ctx *c, bool want, bool *on) {
+    *on = want;
     if (!*on) return DEMON_OK;
     if (c->ev_used + 2 > c->ev.size()) {
         for (int k = 0; k < 64; ++k) {
@@ -339,6 +342,8 @@ int ev_open(demon_ctx *c, bool *on) {
     HIP_TRY(c, hipEventRecord(c->ev[c->ev_used], c->stream));
     return DEMON_OK;
 }
+
+int ev_open(demon_ctx *c, bool *on) { return ev_open_if(c, c->timing && !c->use_graph, on); }
 
 int ev_close(demon_ctx *c, bool on, TimedKind kind) {
     if (!on) return DEMON_OK;
@@ -945,6 +950,12 @@ int demon_get_pair_mode(const demon_ctx *c, int32_t *mode) {
     return DEMON_OK;
 }
 
+int demon_set_timing(demon_ctx *c, int32_t enable) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    c->timing = enable != 0;
+    return DEMON_OK;
+}
+
 int demon_last_timing(const demon_ctx *c, double *pair_ms, double *total_ms) {
     if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
     if (pair_ms) *pair_ms = c->last_pair_ms;
@@ -1053,6 +1064,7 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     const double norm = 1.0 / (2.0 * (double)c->n * (double)(n_obs > 0 ? n_obs : 1));
     const double inf = std::numeric_limits<double>::infinity();
 
+    HIP_TRY(c, hipMemsetAsync(&c->d_sc->adj_counts, 0, sizeof(c->d_sc->adj_counts), c->stream));
     // forward sweep with checkpoints and observations
     HIP_TRY(c, hipEventRecord(c->ev_fit[0], c->stream));
     hipLaunchKernelGGL(k_call_setup, dim3(1), dim3(64), 0, c->stream, c->d_sc, 0.0, inf, dt,
@@ -1107,14 +1119,18 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
                 if (rc) break;
                 HIP_TRY(c, launch_boundary(c->d_sc, 1, c->stream));
             }
-            for (int64_t s = s1 - 1; s >= s0; --s) {
+            for (int64_t s = s1 - 1; s >= s0 && !rc; --s) {
                 const bool obs = (s + 1) % obs_every == 0 && (s + 1) / obs_every <= n_obs;
                 const double *seed = obs ? A + off_sd + (size_t)(2 * ((s + 1) / obs_every - 1)) * P : nullptr;
                 HIP_TRY(c, launch_adj_begin(bar, ibar, u, g, seed, c->m, c->n, c->npad, c->stream));
                 for (int k = 4; k >= 1; --k) {
                     const StageBuf st = store_view(A + off_st + ((size_t)(s - s0 + base) * 4 + (k - 1)) * SD, c->npad);
+                    // timed with events when the context times its launches
+                    bool on = false;
+                    if ((rc = ev_open_if(c, timing, &on))) break;
                     HIP_TRY(c, launch_adj_pairs(st, c->q, u, u + P, ap, c->n, c->npad, c->jslice, c->slices,
-                                                c->p.shielding, rc2p, c->stream));
+                                                c->p.shielding, rc2p, c->d_sc, c->stream));
+                    if ((rc = ev_close(c, on, kTimedAdjPairs))) break;
                     HIP_TRY(c, launch_adj_stage(bar, ibar, u, g, k, st, c->q, c->m, ap, c->slices,
                                                 c->p.forces, dts[(size_t)s], c->p.confine,
                                                 c->p.drag_gamma, c->n, c->npad, c->stream));
@@ -1150,7 +1166,35 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     HIP_TRY(c, hipEventElapsedTime(&f1, c->ev_fit[1], c->ev_fit[2]));
     c->last_fwd_ms = f0;
     c->last_bwd_ms = f1;
+    c->last_adj_ms = 0.0;
+    c->last_adj_launches = 0;
+    for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
+        if (c->ev_kind[k / 2] != kTimedAdjPairs) continue;
+        float pm = 0.0f;
+        HIP_TRY(c, hipEventElapsedTime(&pm, c->ev[k], c->ev[k + 1]));
+        c->last_adj_ms += pm;
+        c->last_adj_launches += 1;
+    }
+    ev_reset(c);
     return rc;
+}
+
+int demon_fit_adjoint_kernel(const demon_ctx *c, double *pair_ms, int64_t *launches, int64_t *near_pairs,
+                             int64_t *tested_pairs) {
+    if (!c) return fail(nullptr, DEMON_ERR_ARG, "ctx is NULL");
+    Scalars s{};
+    hipError_t e = hipMemcpy(&s, c->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(const_cast<demon_ctx *>(c), e, "hipMemcpy adjoint pair counts");
+    unsigned long long near = 0, tested = 0;
+    for (int k = 0; k < kCountSlots; ++k) {
+        near += s.adj_counts[k][0];
+        tested += s.adj_counts[k][1];
+    }
+    if (pair_ms) *pair_ms = c->last_adj_ms;
+    if (launches) *launches = c->last_adj_launches;
+    if (near_pairs) *near_pairs = (int64_t)near;
+    if (tested_pairs) *tested_pairs = (int64_t)tested;
+    return DEMON_OK;
 }
 
 int demon_fit_timing(const demon_ctx *c, double *forward_ms, double *backward_ms) {

@@ -39,7 +39,7 @@ __device__ __forceinline__ void chunk_adj(const ITile<1> &t, double uxi, double 
                                           const double (&cx)[kChunkAdj], const double (&cy)[kChunkAdj],
                                           const double (&cq)[kChunkAdj], const double (&cux)[kChunkAdj],
                                           const double (&cuy)[kChunkAdj], int32_t j0, const PairEnv &e,
-                                          double &rx, double &ry, double &dq, double &dk) {
+                                          uint64_t &n_near, double &rx, double &ry, double &dq, double &dk) {
 #pragma unroll
     for (int u = 0; u < kChunkAdj; ++u) {
         const double dx = t.xi[0] - cx[u];
@@ -47,6 +47,7 @@ __device__ __forceinline__ void chunk_adj(const ITile<1> &t, double uxi, double 
         const double r2 = fma(dx, dx, dy * dy);
         bool in = pair_in<CHECK>(dx, dy, r2, e);
         if (SELF) in = in && j0 + u != t.ibase;
+        n_near += (uint64_t)__builtin_popcountll(__ballot(in));
         if (in) {
             const double y = rsqrt_fp64(r2);
             const double y2 = y * y;
@@ -72,7 +73,9 @@ __device__ __forceinline__ void chunk_adj(const ITile<1> &t, double uxi, double 
 // Pair sums of the stage adjoint: for each i the sums above over all j of
 // slice blockIdx.y, with the forward kernel's box culling and contribution
 // test (the same cutoff set; the derivative ignores the jump at u = 10,
-// which has measure zero).  Gather form, deterministic.
+// which has measure zero).  Gather form, deterministic.  The wave's pair
+// counts (ordered pairs inside the cutoff, and not culled by the boxes) go to
+// sc->adj_counts with one atomic per counter per workgroup.
 __global__ __launch_bounds__(256) void k_adj_pairs(const double *__restrict__ xs,
                                                    const double *__restrict__ ys,
                                                    const double *__restrict__ nc,
@@ -81,7 +84,8 @@ __global__ __launch_bounds__(256) void k_adj_pairs(const double *__restrict__ xs
                                                    const double *__restrict__ ux,
                                                    const double *__restrict__ uy,
                                                    double *__restrict__ part, int32_t n, int32_t npad,
-                                                   int32_t jslice, double kappa, double rc2p, ExpCoef ec) {
+                                                   int32_t jslice, double kappa, double rc2p, ExpCoef ec,
+                                                   Scalars *__restrict__ sc) {
     const Radii rad{rc2p, -1.0};
     const PairEnv e{make_cutoff(kappa, rc2p), rad, fp32_kl2(kappa), false, nullptr, ec};
     const int32_t ibase = (int32_t)blockIdx.x * 256 + (int32_t)threadIdx.x;
@@ -90,11 +94,13 @@ __global__ __launch_bounds__(256) void k_adj_pairs(const double *__restrict__ xs
     const Box ib = box_union(empty_box(), box + 4 * (t.wlo / kBoxBlock));
     const bool ireal = t.whi <= n;
     double rx = 0.0, ry = 0.0, dq = 0.0, dk = 0.0;
+    uint64_t n_near = 0, n_tested = 0;
     const int32_t j_begin = (int32_t)blockIdx.y * jslice;
     const int32_t j_end = j_begin + jslice < npad ? j_begin + jslice : npad;
     for (int32_t jb = j_begin; jb < j_end; jb += kBoxBlock) {
         const BlockClass cls = classify(ib, box + 4 * (jb / kBoxBlock), ireal, jb + kBoxBlock <= n, rad, e.cut);
         if (cls == kSkip) continue;
+        n_tested += 64 * kBoxBlock;
         for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkAdj) {
             double cx[kChunkAdj], cy[kChunkAdj], cq[kChunkAdj], cux[kChunkAdj], cuy[kChunkAdj];
 #pragma unroll
@@ -110,12 +116,25 @@ __global__ __launch_bounds__(256) void k_adj_pairs(const double *__restrict__ xs
                 cuy[u] = uy[j0 + u];
             }
             if (j0 < t.whi && j0 + kChunkAdj > t.wlo)
-                chunk_adj<true, true>(t, uxi, uyi, qi, cx, cy, cq, cux, cuy, j0, e, rx, ry, dq, dk);
+                chunk_adj<true, true>(t, uxi, uyi, qi, cx, cy, cq, cux, cuy, j0, e, n_near, rx, ry, dq, dk);
             else if (cls == kDense)
-                chunk_adj<false, false>(t, uxi, uyi, qi, cx, cy, cq, cux, cuy, j0, e, rx, ry, dq, dk);
+                chunk_adj<false, false>(t, uxi, uyi, qi, cx, cy, cq, cux, cuy, j0, e, n_near, rx, ry, dq, dk);
             else
-                chunk_adj<true, false>(t, uxi, uyi, qi, cx, cy, cq, cux, cuy, j0, e, rx, ry, dq, dk);
+                chunk_adj<true, false>(t, uxi, uyi, qi, cx, cy, cq, cux, cuy, j0, e, n_near, rx, ry, dq, dk);
         }
+    }
+    // every thread reaches this point
+    __shared__ unsigned long long cnt[4][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        cnt[wave][0] = n_near;
+        cnt[wave][1] = n_tested;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long *slot = sc->adj_counts[(blockIdx.x + blockIdx.y) & (kCountSlots - 1)];
+        atomicAdd(&slot[0], ((cnt[0][0] + cnt[1][0]) + cnt[2][0]) + cnt[3][0]);
+        atomicAdd(&slot[1], ((cnt[0][1] + cnt[1][1]) + cnt[2][1]) + cnt[3][1]);
     }
     double *p0 = part + (size_t)(4 * blockIdx.y) * npad;
     p0[ibase] = rx;
@@ -226,12 +245,12 @@ unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 hipError_t launch_adj_pairs(const StageBuf &st, const double *q, const double *ux, const double *uy,
                             double *part, int64_t n, int64_t npad, int32_t jslice, int32_t slices,
-                            double kappa, double rc2p, hipStream_t s) {
-    if (npad % kPad || jslice % kBoxBlock || (int64_t)slices * jslice < npad || !st.box || n > npad)
+                            double kappa, double rc2p, Scalars *sc, hipStream_t s) {
+    if (npad % kPad || jslice % kBoxBlock || (int64_t)slices * jslice < npad || !st.box || n > npad || !sc)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_adj_pairs, dim3((unsigned)(npad / 256), (unsigned)slices), dim3(256), 0, s, st.xs,
                        st.ys, st.nc, st.box, q, ux, uy, part, (int32_t)n, (int32_t)npad, jslice, kappa, rc2p,
-                       exp_coef());
+                       exp_coef(), sc);
     return hipGetLastError();
 }
 

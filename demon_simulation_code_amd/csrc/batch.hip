@@ -85,11 +85,17 @@ __global__ __launch_bounds__(256) void k_bpairs(const double *__restrict__ xs,
         double ax[R], ay[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) ax[r] = ay[r] = 0.0;
-        uint64_t n_near = 0, n_tested = 0;  // not reported for ensembles
+        uint64_t n_near = 0, n_tested = 0;
         pair::wave_tile<R, F32>(xs + cl.off, ys + cl.off, nc + cl.off, q + cl.off, orig + cl.off,
                                 box + cl.off / 16,
                                 cl.n, ibase, j_begin, j_end, base_flags, cl.kappa, cl.rc2p, rad,
                                 ec, &s->dtmin_bits, ax, ay, n_near, n_tested);
+        // the wave's counts (wave-uniform), one atomic each per item
+        if (lane == 0) {
+            unsigned long long *slot = s->pair_counts[(it + (int32_t)(threadIdx.x >> 6)) & (kCountSlots - 1)];
+            atomicAdd(&slot[0], (unsigned long long)n_near);
+            atomicAdd(&slot[1], (unsigned long long)n_tested);
+        }
         double *px = part + cl.part_off + (size_t)(2 * item.slice) * cl.npad;
         double *py = px + cl.npad;
 #pragma unroll
@@ -392,6 +398,14 @@ struct demon_batch {
     double last_ms = 0.0;
     bool uploaded_all = false;
     std::vector<char> uploaded;
+    // per-launch HIP events around the pair and update launches of plain
+    // (uncaptured) steps, when timing is on
+    bool timing = false;
+    std::vector<hipEvent_t> tev;
+    std::vector<uint8_t> tkind;  // 0 pairs, 1 update, per event pair
+    size_t tev_used = 0;
+    double pair_ms = 0.0, update_ms = 0.0;
+    int64_t pair_launches = 0, update_launches = 0;
 };
 
 namespace {
@@ -577,19 +591,43 @@ BUpdate bupdate(demon_batch *b, int stage) {
     return u;
 }
 
-int enqueue_bstep(demon_batch *b) {
+// Event pair around one plain launch (timed: not under capture).
+int tev_open(demon_batch *b, bool timed) {
+    if (!timed) return DEMON_OK;
+    if (b->tev_used + 2 > b->tev.size())
+        for (int k = 0; k < 64; ++k) {
+            hipEvent_t e;
+            BHIP(b, hipEventCreate(&e));
+            b->tev.push_back(e);
+        }
+    BHIP(b, hipEventRecord(b->tev[b->tev_used], b->stream));
+    return DEMON_OK;
+}
+
+int tev_close(demon_batch *b, bool timed, uint8_t kind) {
+    if (!timed) return DEMON_OK;
+    BHIP(b, hipEventRecord(b->tev[b->tev_used + 1], b->stream));
+    b->tev_used += 2;
+    b->tkind.push_back(kind);
+    return DEMON_OK;
+}
+
+int enqueue_bstep(demon_batch *b, bool timed) {
     hipLaunchKernelGGL(k_bboundary, dim3((b->nclouds + 255) / 256), dim3(256), 0, b->stream, b->d_sc,
                        b->nclouds, 0, b->d_counters);
     BHIP(b, hipGetLastError());
     const dim3 grid((unsigned)(b->P / 256)), blk(256);
+    int rc;
     for (int s = 1; s <= 4; ++s) {
         int32_t flags = pair::kFlagGated;
         if (s == 1 && b->dt_check) flags |= pair::kFlagDtCheck;
         if (b->forces & DEMON_FORCE_SHIELDED_COULOMB) {
-            const int rc = enqueue_bpairs(b, b->sb[(s - 1) & 1], flags);
-            if (rc) return rc;
+            if ((rc = tev_open(b, timed)) || (rc = enqueue_bpairs(b, b->sb[(s - 1) & 1], flags)) ||
+                (rc = tev_close(b, timed, 0)))
+                return rc;
         }
         const BUpdate u = bupdate(b, s);
+        if ((rc = tev_open(b, timed))) return rc;
         switch (s) {
             case 1: hipLaunchKernelGGL(k_bupdate<1>, grid, blk, 0, b->stream, u); break;
             case 2: hipLaunchKernelGGL(k_bupdate<2>, grid, blk, 0, b->stream, u); break;
@@ -597,6 +635,7 @@ int enqueue_bstep(demon_batch *b) {
             default: hipLaunchKernelGGL(k_bupdate<4>, grid, blk, 0, b->stream, u); break;
         }
         BHIP(b, hipGetLastError());
+        if ((rc = tev_close(b, timed, 1))) return rc;
     }
     return DEMON_OK;
 }
@@ -714,6 +753,7 @@ void demon_batch_destroy(demon_batch *b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (hipEvent_t e : b->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->tev) (void)hipEventDestroy(e);
     for (void *ptr : {(void *)b->arena, (void *)b->part, (void *)b->react, (void *)b->rflag, (void *)b->orig,
                       (void *)b->d_block_cloud,
                       (void *)b->d_clouds, (void *)b->d_sc, (void *)b->d_items, (void *)b->d_counters})
@@ -821,6 +861,10 @@ int demon_batch_steps(demon_batch *b, int64_t steps, double dt) {
     if (!b || steps < 0 || !(dt > 0.0)) return bfail(b, DEMON_ERR_ARG, "bad steps/dt");
     if (!b->uploaded_all) return bfail(b, DEMON_ERR_ARG, "upload every cloud first");
     BHIP(b, hipSetDevice(b->device));
+    b->tev_used = 0;
+    b->tkind.clear();
+    b->pair_ms = b->update_ms = 0.0;
+    b->pair_launches = b->update_launches = 0;
     hipLaunchKernelGGL(k_bsetup, dim3((b->nclouds + 255) / 256), dim3(256), 0, b->stream, b->d_sc,
                        b->nclouds, dt, (unsigned long long)steps);
     BHIP(b, hipGetLastError());
@@ -829,7 +873,7 @@ int demon_batch_steps(demon_batch *b, int64_t steps, double dt) {
         if (!b->graph) {
             hipGraph_t g = nullptr;
             BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            const int rc = enqueue_bstep(b);
+            const int rc = enqueue_bstep(b, false);
             const hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) {
                 if (g) (void)hipGraphDestroy(g);
@@ -843,7 +887,7 @@ int demon_batch_steps(demon_batch *b, int64_t steps, double dt) {
         for (int64_t s = 0; s < steps; ++s) BHIP(b, hipGraphLaunch(b->graph, b->stream));
     } else {
         for (int64_t s = 0; s < steps; ++s) {
-            const int rc = enqueue_bstep(b);
+            const int rc = enqueue_bstep(b, b->timing);
             if (rc) return rc;
         }
     }
@@ -855,6 +899,17 @@ int demon_batch_steps(demon_batch *b, int64_t steps, double dt) {
     float ms = 0.0f;
     BHIP(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     b->last_ms = ms;
+    for (size_t k = 0; k + 1 < b->tev_used; k += 2) {
+        float km = 0.0f;
+        BHIP(b, hipEventElapsedTime(&km, b->tev[k], b->tev[k + 1]));
+        if (b->tkind[k / 2] == 0) {
+            b->pair_ms += km;
+            b->pair_launches += 1;
+        } else {
+            b->update_ms += km;
+            b->update_launches += 1;
+        }
+    }
     std::vector<Scalars> s((size_t)b->nclouds);
     BHIP(b, hipMemcpy(s.data(), b->d_sc, sizeof(Scalars) * s.size(), hipMemcpyDeviceToHost));
     for (const Scalars &v : s)
@@ -884,6 +939,38 @@ int demon_batch_use_graph(demon_batch *b, int32_t enable) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
+    return DEMON_OK;
+}
+
+int demon_batch_set_timing(demon_batch *b, int32_t enable) {
+    if (!b) return bfail(nullptr, DEMON_ERR_ARG, "batch is NULL");
+    b->timing = enable != 0;
+    return DEMON_OK;
+}
+
+int demon_batch_kernel_ms(const demon_batch *b, double *pair_ms, int64_t *pair_launches, double *update_ms,
+                          int64_t *update_launches) {
+    if (!b) return bfail(nullptr, DEMON_ERR_ARG, "batch is NULL");
+    if (pair_ms) *pair_ms = b->pair_ms;
+    if (pair_launches) *pair_launches = b->pair_launches;
+    if (update_ms) *update_ms = b->update_ms;
+    if (update_launches) *update_launches = b->update_launches;
+    return DEMON_OK;
+}
+
+int demon_batch_pair_counts(const demon_batch *b, int64_t *near_pairs, int64_t *tested_pairs) {
+    if (!b) return bfail(nullptr, DEMON_ERR_ARG, "batch is NULL");
+    std::vector<Scalars> s((size_t)b->nclouds);
+    if (hipMemcpy(s.data(), b->d_sc, sizeof(Scalars) * s.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return bfail(const_cast<demon_batch *>(b), DEMON_ERR_HIP, "hipMemcpy pair counts failed");
+    unsigned long long near = 0, tested = 0;
+    for (const Scalars &v : s)
+        for (int k = 0; k < kCountSlots; ++k) {
+            near += v.pair_counts[k][0];
+            tested += v.pair_counts[k][1];
+        }
+    if (near_pairs) *near_pairs = (int64_t)near;
+    if (tested_pairs) *tested_pairs = (int64_t)tested;
     return DEMON_OK;
 }
 

@@ -123,6 +123,9 @@ struct Scalars {
     // writes 256 reactions) and its work items (each writes 256 i-side
     // sums), summed over launches
     unsigned long long n3_tiles, n3_items;
+    // the adjoint pair kernel's counts, as pair_counts (a fit call starts
+    // them at zero)
+    unsigned long long adj_counts[kCountSlots][2];
 };
 
 struct PairLaunch {
@@ -227,7 +230,7 @@ hipError_t launch_round_fp32(const double *src, float *dst, int64_t npad, hipStr
 // Discrete adjoint (adjoint.hip).  st: a stored stage record (with boxes).
 hipError_t launch_adj_pairs(const StageBuf &st, const double *q, const double *ux, const double *uy,
                             double *part, int64_t n, int64_t npad, int32_t jslice, int32_t slices,
-                            double kappa, double rc2p, hipStream_t s);
+                            double kappa, double rc2p, Scalars *sc, hipStream_t s);
 hipError_t launch_adj_begin(double *bar, double *ibar, double *u, double *g, const double *seed,
                             const double *m, int64_t n, int64_t npad, hipStream_t s);
 hipError_t launch_adj_stage(double *bar, double *ibar, double *u, double *g, int stage,

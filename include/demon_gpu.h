@@ -204,6 +204,11 @@ int demon_synchronize(demon_ctx *ctx);
 int demon_set_pair_shape(demon_ctx *ctx, int32_t i_per_lane, int32_t j_slices);
 int demon_get_pair_shape(const demon_ctx *ctx, int32_t *i_per_lane, int32_t *j_slices);
 
+/* Per-launch HIP events around the plain (uncaptured) launches of the
+ * context: on by default; off, the kernel splits below read zero
+ * (instrumentation; no counterpart in the reference). */
+int demon_set_timing(demon_ctx *ctx, int32_t enable);
+
 /* Milliseconds of device time of the last demon_rk4_* call, split into the
  * all-pairs kernels and everything else (HIP events on the context stream). */
 int demon_last_timing(const demon_ctx *ctx, double *pair_ms, double *total_ms);
@@ -273,6 +278,14 @@ int demon_fit_loss_grad(demon_ctx *ctx, int64_t steps, double dt, int64_t obs_ev
  * recompute + adjoint sweep. */
 int demon_fit_timing(const demon_ctx *ctx, double *forward_ms, double *backward_ms);
 
+/* The adjoint pair kernel of the last demon_fit_loss_grad (instrumentation;
+ * no counterpart in the reference): its device milliseconds and launches from
+ * HIP events (0 when timing is off), and the ordered pairs it found inside
+ * the cutoff and gave a distance test (not box-culled).  NULL pointers are
+ * skipped. */
+int demon_fit_adjoint_kernel(const demon_ctx *ctx, double *pair_ms, int64_t *launches,
+                             int64_t *near_pairs, int64_t *tested_pairs);
+
 /* ---- Batched ensemble: many independent clouds in one device arena ----
  * Each cloud is its own Cloud + Runge_Kutta4 system (own kappa, C, gamma,
  * dt check and time); all clouds step together: per stage one persistent
@@ -294,6 +307,17 @@ int demon_batch_steps(demon_batch *b, int64_t steps, double init_dt);
 int demon_batch_cloud_stats(const demon_batch *b, int32_t cloud, demon_stats *s);
 int demon_batch_use_graph(demon_batch *b, int32_t enable);
 int demon_batch_last_ms(const demon_batch *b, double *ms);
+/* Instrumentation (no counterpart in the reference).  With timing on, plain
+ * (uncaptured) steps record HIP events around every all-pairs pass and every
+ * stage update; demon_batch_kernel_ms returns their device milliseconds and
+ * launch counts for the last demon_batch_steps (zero launches when the step
+ * ran as a captured graph).  demon_batch_pair_counts sums, over the clouds,
+ * the ordered pairs found inside the cutoff and given a distance test since
+ * each cloud's upload. */
+int demon_batch_set_timing(demon_batch *b, int32_t enable);
+int demon_batch_kernel_ms(const demon_batch *b, double *pair_ms, int64_t *pair_launches,
+                          double *update_ms, int64_t *update_launches);
+int demon_batch_pair_counts(const demon_batch *b, int64_t *near_pairs, int64_t *tested_pairs);
 /* Form and tile shape of the clouds whose padded size falls in
  * [2^bucket, 2^(bucket+1)): i_per_lane 1, 2 or 4 is the gather form (each
  * ordered pair evaluated for its i; j slices of jslice particles, a multiple

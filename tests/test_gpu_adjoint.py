@@ -271,3 +271,25 @@ def test_torch_autograd_binding(close_pair):
         lf = fixed(qt.detach(), kt.detach(), ct.detach(), gt.detach()).item()
         fixed.close()
         assert lf != L
+
+
+def test_adjoint_pair_counts_equal_forward_counts():
+    """The adjoint pair kernel visits the forward kernel's cutoff set: over a
+    fit, its counts equal the forward counts of the same trajectory (the
+    forward count of pairs inside the cutoff is the oracle's exact count);
+    with timing on its launches (4 per step) are timed, with timing off not."""
+    n, steps, obs, ckpt = 1024, 20, 10, 5
+    x, y, m, q, tx, ty = _problem(n, steps, obs)
+    with _ctx(n, x, y, m, q) as c:
+        c.rk4_steps(steps, 1e-3, 0.0)
+        want = c.pair_counts()
+        c.upload(x, y, None, None, q, m)
+        c.fit_loss_grad(steps, 1e-3, obs, ckpt, tx, ty)
+        ak = c.fit_adjoint_kernel()
+        c.set_timing(False)
+        c.fit_loss_grad(steps, 1e-3, obs, ckpt, tx, ty)
+        ak2 = c.fit_adjoint_kernel()
+    assert 0 < want[0] < want[1]
+    assert (ak["near_pairs"], ak["tested_pairs"]) == want
+    assert ak["launches"] == 4 * steps and ak["pair_ms"] > 0
+    assert ak2["launches"] == 0 and (ak2["near_pairs"], ak2["tested_pairs"]) == want

@@ -179,3 +179,39 @@ def test_batch_real_ensemble_sample_matches_oracle(oracle):
         forms = {b.shape(bk)[0] for bk in range(8, 14)}
         assert dm.BATCH_NEWTON3 in forms  # the tuner picks the Newton-3 form somewhere
         _check(oracle, b, ens, states, 25, clouds=sample, gather=2)
+
+
+@pytest.mark.parametrize("form", ["gather", "newton3"])
+def test_batch_pair_counts_and_kernel_timing(form):
+    """The ensemble kernels' pair counts equal the single-cloud context's over
+    the same steps (the same box culling and contribution test on the same
+    stage states; the context's count of pairs inside the cutoff is the
+    oracle's exact count), and plain steps carry one event pair per all-pairs
+    pass and per stage update, captured steps none."""
+    b, ens, states = _make(5, seed=31, n_max=3000)
+    steps = 3
+    with b:
+        if form == "newton3":
+            for bk in range(8, 13):
+                b.set_shape(bk, dm.BATCH_NEWTON3, 2)
+        b.use_graph(False)
+        b.set_timing(True)
+        b.steps(steps, 1e-3)
+        kt = b.kernel_timing()
+        near, tested = b.pair_counts()
+        b.use_graph(True)
+        b.steps(1, 1e-3)
+        kt_graph = b.kernel_timing()
+    assert kt["pair_launches"] == 4 * steps and kt["update_launches"] == 4 * steps
+    assert kt["pair_ms"] > 0 and kt["update_ms"] > 0
+    assert kt_graph["pair_launches"] == 0 and kt_graph["update_launches"] == 0
+    want_near = want_tested = 0
+    for e, (x, y, q, m) in zip(ens, states):
+        with dm.Context(e["n"], shielding=e["kappa"], confine=e["confine"], gamma=e["gamma"]) as c:
+            c.upload(x, y, None, None, q, m)
+            c.rk4_steps(steps, 1e-3, 0.0)
+            nn, nt = c.pair_counts()
+        want_near += nn
+        want_tested += nt
+    assert 0 < near < tested
+    assert (near, tested) == (want_near, want_tested)
