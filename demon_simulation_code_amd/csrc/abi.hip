@@ -250,8 +250,7 @@ N3List n3_list(const demon_ctx *c) {
     l.plan = c->d_plan;
     l.target = (int32_t)(c->n3_slices * nsb);
     l.max_items = (int32_t)n3_max_items(c->npad);
-    // a persistent grid of four workgroups per CU (the kernel's occupancy)
-    l.grid = 4 * c->cus;
+    l.cus = c->cus;
     l.exptab = c->d_exptab;
     return l;
 }

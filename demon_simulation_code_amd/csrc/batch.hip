@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     int32_t n_items, unsigned int *__restrict__ counter, Scalars *__restrict__ sc, int32_t base_flags,
     pair::ExpCoef ec) {
     __shared__ n3::RedLds red;
-    __shared__ n3::ScrLds scr;
+    __shared__ __align__(16) n3::ScrLds<F32> scr;
     __shared__ n3::TBoxLds tbox;
     __shared__ int32_t slot;
     for (;;) {

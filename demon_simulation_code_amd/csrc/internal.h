@@ -93,7 +93,7 @@ struct N3List {
     N3Plan *plan;
     int32_t target;            // items wanted per launch: span = ceil(live tiles / target)
     int32_t max_items;         // >= target + nsb
-    int32_t grid;              // workgroups of the persistent pair kernel
+    int32_t cus;               // compute units (the persistent pair kernel fills each)
     const double *exptab;      // e^{-k/64}, k < 642: the pair kernel's exp table (copied to LDS)
 };
 

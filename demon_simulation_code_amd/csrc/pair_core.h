@@ -228,6 +228,17 @@ __device__ __forceinline__ Radii launch_radii(double rc2p, int32_t flags, unsign
     return r;
 }
 
+// A load at a wave-uniform address through the scalar data cache (the
+// constant address space), for the j particles of a chunk: every lane needs
+// the same values, and no kernel writes them while the pair kernels read
+// them.  Through a generic pointer the compiler proves this only when the
+// kernel's arguments are restrict-qualified pointers, and otherwise issues
+// a vector load per lane group.
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
+
 // Per-wave i-particle registers of a tile.
 template <int R>
 struct ITile {
@@ -314,7 +325,7 @@ __device__ __forceinline__ bool chunk_hit(const ITile<R> &t, const double (&cx)[
     uint64_t hit = 0;
 #pragma unroll
     for (int u = 0; u < C; ++u) {
-        const double cn = nc[j0 + u];
+        const double cn = ld_uniform(nc + j0 + u);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const double w = fma(t.xm[r], cx[u], fma(t.ym[r], cy[u], t.nT[r]));
@@ -436,12 +447,12 @@ __device__ __forceinline__ void gather_blocks(const ITile<R> &t, const double *_
             double cx[C], cy[C], cq[C];
 #pragma unroll
             for (int u = 0; u < C; ++u) {
-                cx[u] = xs[j0 + u];
-                cy[u] = ys[j0 + u];
+                cx[u] = ld_uniform(xs + j0 + u);
+                cy[u] = ld_uniform(ys + j0 + u);
             }
             if (cls != kDense && !chunk_hit<R, C>(t, cx, cy, nc, j0)) continue;
 #pragma unroll
-            for (int u = 0; u < C; ++u) cq[u] = q[j0 + u];
+            for (int u = 0; u < C; ++u) cq[u] = ld_uniform(q + j0 + u);
             const bool own = j0 < t.whi && j0 + C > t.wlo;
             if (own)
                 chunk_gather<R, C, F32, true, true, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);

@@ -103,10 +103,26 @@ __device__ __forceinline__ void store_wave_sums(const V (&f)[2 * kChunkN3], V *s
     if (p == 0) red[2 * (jj0 + (v & 3)) + (v >> 2)] = (double)t;
 }
 
-// LDS scratch per wave for the transposes: 8 rows of the FP64 row length
-// (which also holds the 8 FP32 rows).
+// LDS scratch per wave for the transposes: 8 rows of the row length of the
+// pair arithmetic's type (FP32 rows take half the space, which lets more
+// workgroups share a CU).
 constexpr int kScrDoubles = 2 * kChunkN3 * Scr<double>::kRow;
-static_assert(2 * kChunkN3 * Scr<float>::kRow <= 2 * kScrDoubles, "FP32 rows fit the scratch");
+template <bool F32>
+struct ScrSel {
+    using type = double[4][kScrDoubles];
+};
+template <>
+struct ScrSel<true> {
+    using type = float[4][2 * kChunkN3 * Scr<float>::kRow];
+};
+template <bool F32>
+using ScrLds = typename ScrSel<F32>::type;
+// one wave's rows, handed down as double * (the chunk code reads them as
+// its own type)
+template <bool F32>
+__device__ __forceinline__ double *scr_rows(ScrLds<F32> &scr, int wave) {
+    return reinterpret_cast<double *>(&scr[wave][0]);
+}
 
 // Newton-3 form over one chunk of 4 j: adds F_ij into ax/ay and stores the
 // wave's sum of F_ij for each j into red[2 jj], red[2 jj + 1] (the
@@ -125,6 +141,7 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
                                          double &ax, double &ay, double *scr, double *red, int32_t jj0) {
     using V = PairVal<F32>;
     V f[2 * kChunkN3];
+    uint64_t hits = 0;  // lanes with some pair of the chunk inside the cutoff (CHECK)
 #pragma unroll
     for (int u = 0; u < kChunkN3; ++u) {
         const double dx = t.xi[0] - cx[u];
@@ -132,7 +149,11 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
         const double r2 = fma(dx, dx, dy * dy);
         const bool in = pair_in<CHECK>(dx, dy, r2, e);
         // a dense block has 64 real i and every pair inside the cutoff
-        if (CHECK) n_near += 2 * (uint64_t)__builtin_popcountll(__ballot(in));
+        if (CHECK) {
+            const uint64_t b = __ballot(in);
+            n_near += 2 * (uint64_t)__builtin_popcountll(b);
+            hits |= b;
+        }
         V fx = 0, fy = 0;
         if (in) {
             if constexpr (F32) {
@@ -141,16 +162,18 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
                 const double s = pair_scale_env<TAB>(r2, t.cqi[0], cq[u], e);
                 fx = s * dx;
                 fy = s * dy;
+                ax = ax + fx;
+                ay = ay + fy;
             }
-        }
-        if constexpr (!F32) {
-            ax = ax + fx;
-            ay = ay + fy;
         }
         f[u] = fx;
         f[kChunkN3 + u] = fy;
         if (DTC) pair_dtcheck(t.xi[0], t.yi[0], cx[u], cy[u], r2, t.ibase, j0 + u, e, smin);
     }
+    // No pair of the chunk inside the cutoff: every f is zero, and the
+    // wave's sums of this chunk's j keep the zeros the tile stored in red
+    // (each chunk's slots are written once per tile).  hits is uniform.
+    if (CHECK && hits == 0) return;
     if constexpr (F32) {
         float sx = f[0], sy = f[kChunkN3];
 #pragma unroll
@@ -179,7 +202,6 @@ struct N3Row {
 };
 
 using RedLds = double[4][2 * kSuper];  // [wave][2 jj + component]
-using ScrLds = double[4][kScrDoubles]; // per-wave transpose scratch
 // Bounding boxes of the superblocks T a row visits, staged in LDS a chunk at
 // a time: one round trip to memory per chunk instead of one per offset,
 // which is what a launch of mostly culled tiles waits on.
@@ -210,10 +232,10 @@ __device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, cons
             PairVal<F32> cq[kChunkN3];
 #pragma unroll
             for (int u = 0; u < kChunkN3; ++u) {
-                cx[u] = xs[j0 + u];
-                cy[u] = ys[j0 + u];
-                if constexpr (F32) cq[u] = qf[j0 + u];
-                else cq[u] = q[j0 + u];
+                cx[u] = ld_uniform(xs + j0 + u);
+                cy[u] = ld_uniform(ys + j0 + u);
+                if constexpr (F32) cq[u] = ld_uniform(qf + j0 + u);
+                else cq[u] = ld_uniform(q + j0 + u);
             }
             chunk_n3<F32, false, DTC, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr, red, j0 - t0);
         }
@@ -224,14 +246,14 @@ __device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, cons
         PairVal<F32> cq[kChunkN3];
 #pragma unroll
         for (int u = 0; u < kChunkN3; ++u) {
-            cx[u] = xs[j0 + u];
-            cy[u] = ys[j0 + u];
+            cx[u] = ld_uniform(xs + j0 + u);
+            cy[u] = ld_uniform(ys + j0 + u);
         }
         if (!chunk_hit<1, kChunkN3>(t, cx, cy, nc, j0)) continue;
 #pragma unroll
         for (int u = 0; u < kChunkN3; ++u) {
-            if constexpr (F32) cq[u] = qf[j0 + u];
-            else cq[u] = q[j0 + u];
+            if constexpr (F32) cq[u] = ld_uniform(qf + j0 + u);
+            else cq[u] = ld_uniform(q + j0 + u);
         }
         chunk_n3<F32, true, DTC, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr, red, j0 - t0);
     }
@@ -245,7 +267,7 @@ template <bool F32, bool DTC, bool TAB = false>
 __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const ITile<1> &t, const Box &ib,
                                         bool ireal, int32_t T, int32_t d, double &ax, double &ay,
                                         uint64_t &n_near, uint64_t &n_tested, float &smin, RedLds &red,
-                                        ScrLds &scr) {
+                                        ScrLds<F32> &scr) {
     const int32_t npad = a.npad;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -255,7 +277,7 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
     for (int k = lane; k < 2 * kSuper; k += 64) r[k] = 0.0;
     for (int jb4 = 0; jb4 < 4; ++jb4)
         n3_jblock<F32, DTC, TAB>(a, e, t, ib, ireal, T * kSuper + jb4 * kBoxBlock, T * kSuper, ax, ay, n_near,
-                                 n_tested, smin, scr[wave], r);
+                                 n_tested, smin, scr_rows<F32>(scr, wave), r);
     __syncthreads();
     // the four waves' sums in a fixed order; the reaction is their negation
     const int jj = threadIdx.x;
@@ -275,7 +297,7 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
 template <bool F32, bool DTC, bool TAB>
 __device__ __forceinline__ void n3_diag(const N3Row &a, const PairEnv &e, const ITile<1> &t, const Box &ib,
                                         bool ireal, int32_t S, double &ax, double &ay, uint64_t &n_near,
-                                        uint64_t &n_tested, float &smin, RedLds &red, ScrLds &scr) {
+                                        uint64_t &n_tested, float &smin, RedLds &red, ScrLds<F32> &scr) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // the previous tile's sums have been read by all threads
@@ -289,10 +311,10 @@ __device__ __forceinline__ void n3_diag(const N3Row &a, const PairEnv &e, const 
     ax = ax + gx[0];
     ay = ay + gy[0];
     n3_jblock<F32, DTC, TAB>(a, e, t, ib, ireal, S * kSuper + ((wave + 1) & 3) * kBoxBlock, S * kSuper, ax, ay,
-                             n_near, n_tested, smin, scr[wave], r);
+                             n_near, n_tested, smin, scr_rows<F32>(scr, wave), r);
     if (wave < 2)
         n3_jblock<F32, DTC, TAB>(a, e, t, ib, ireal, S * kSuper + (wave + 2) * kBoxBlock, S * kSuper, ax, ay,
-                                 n_near, n_tested, smin, scr[wave], r);
+                                 n_near, n_tested, smin, scr_rows<F32>(scr, wave), r);
     __syncthreads();
     const int jj = threadIdx.x;
     ax = ax - (((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj]);
@@ -327,7 +349,7 @@ __device__ __forceinline__ void n3_commit_counts(Scalars *sc, int32_t slot_index
 // all 256 threads.
 template <bool F32, bool DTC>
 __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_t S, int32_t d0, int32_t d1,
-                                       int32_t slice, RedLds &red, ScrLds &scr, TBoxLds &tbox) {
+                                       int32_t slice, RedLds &red, ScrLds<F32> &scr, TBoxLds &tbox) {
     const double *__restrict__ box = a.box;
     const int32_t n = a.n, npad = a.npad;
     const Radii rad = launch_radii(a.rc2p, a.flags, a.sc->rmax2_bits);

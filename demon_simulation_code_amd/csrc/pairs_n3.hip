@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_n3_plan(N3Row a, N3List l) {
 template <bool F32, bool DTC>
 __device__ __forceinline__ void n3_item(const N3Row &a, const PairEnv &e, int32_t S, const int32_t *lst,
                                         int32_t k0, int32_t k1, int32_t it, uint64_t &n_near, uint64_t &n_tested,
-                                        float &smin, RedLds &red, ScrLds &scr) {
+                                        float &smin, RedLds &red, ScrLds<F32> &scr) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int32_t n = a.n, nsb = a.npad / kSuper;
@@ -186,7 +186,7 @@ template <bool F32, bool DTC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pairs_n3(N3Row a, N3List l,
                                                                                    ExpCoef ec) {
     __shared__ RedLds red;
-    __shared__ ScrLds scr;
+    __shared__ __align__(16) ScrLds<F32> scr;
     __shared__ double etab[kExpTabN];
     __shared__ int32_t slot;
     if ((a.flags & kFlagGated) && !a.sc->active) return;
@@ -236,7 +236,7 @@ hipError_t n3_row_args(const PairLaunch &a, N3Row *row) {
     const N3List &l = a.lst;
     if (a.npad % kPad || !a.box || !a.react || !a.rflag || a.n > a.npad || (a.fp32pair && !a.qf))
         return hipErrorInvalidValue;
-    if (!l.list || !l.cnt || !l.first || !l.itemrow || !l.plan || !l.exptab || l.target < 1 || l.grid < 1 ||
+    if (!l.list || !l.cnt || !l.first || !l.itemrow || !l.plan || !l.exptab || l.target < 1 || l.cus < 1 ||
         (int64_t)l.max_items < (int64_t)l.target + nsb || nsb / 2 > kMaxHalf)
         return hipErrorInvalidValue;
     int32_t flags = 0;
@@ -264,7 +264,17 @@ hipError_t launch_pairs_n3(const PairLaunch &a, hipStream_t s) {
     if (e != hipSuccess) return e;
     auto kern = a.fp32pair ? (a.dtcheck ? k_pairs_n3<true, true> : k_pairs_n3<true, false>)
                            : (a.dtcheck ? k_pairs_n3<false, true> : k_pairs_n3<false, false>);
-    hipLaunchKernelGGL(kern, dim3((unsigned)a.lst.grid), dim3(256), 0, s, row, a.lst, exp_tab_coef());
+    // a persistent grid: as many workgroups per CU as fit at once (registers
+    // and LDS of the variant), counted once per variant
+    static int per_cu[2][2] = {};
+    int &bpc = per_cu[a.fp32pair ? 1 : 0][a.dtcheck ? 1 : 0];
+    if (bpc == 0) {
+        int nb = 0;
+        const hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, 0);
+        if (oe != hipSuccess) return oe;
+        bpc = nb > 0 ? nb : 1;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)(bpc * a.lst.cus)), dim3(256), 0, s, row, a.lst, exp_tab_coef());
     return hipGetLastError();
 }
 
