@@ -7,7 +7,9 @@ particles on a hexagonal lattice (500 um spacing, +-50 um jitter), charge
 k = 5e-11 N/m, drag gamma = 2 /s, thermal off, RK4 with dt = 1e-3 s and the
 reference's dt check, FP64.  A "step" is one RK4 step (4 all-pairs force
 evaluations + the dt check) from the lattice; the timed `--steps K` are the
-first K steps of the 200-step trajectory (after a warm-up and a re-upload).
+first K steps of the 200-step trajectory (after a warm-up and a re-upload),
+each step replayed as the context's captured step graph (`--no-graphs`:
+plain launches).
 
 Prints ONE JSON line:
   value        pair interactions/s = 4 N (N-1) K / t   (ordered pairs)
@@ -16,8 +18,10 @@ Prints ONE JSON line:
                the pairs it actually evaluated -- 7 per ordered pair given a
                distance test, 20 per one inside the cutoff (counted exactly in
                the kernel) -- over its mean launch time (HIP events around
-               every launch on the kernel's stream), against the FP64 vector
-               peak of 78.6 TFLOP/s.  `effective` prices every ordered pair of
+               every launch on the kernel's stream, in a pass of the same
+               steps from the same state with plain launches, checked to end
+               in the timed run's state), against the FP64 vector peak of
+               78.6 TFLOP/s.  `effective` prices every ordered pair of
                N (N-1), the box-culled ones included, at 7 FLOP (the rate the
                culling buys, not a hardware throughput).
   update       the fused stage-update kernel: bytes per launch (its loads and
@@ -105,11 +109,16 @@ ZERO_PLUGINS = dict(confine_x=0.0, confine_y=0.0, void_decay=0.0, rot_inner=1e-3
                     heat_radius=1e-3, heat_scale=0.0, heat_offset=0.0)
 
 
-def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mode=None, plugins_zero=False):
+def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mode=None, plugins_zero=False,
+            graph=True):
     """Warm up on the workload, re-upload the initial state, then time exactly
-    `steps` RK4 steps from it (one demon_rk4_steps call); with `start`, from
-    the state after that many untimed steps instead.  With `snapshots` (step
-    numbers), a second, untimed pass records the state at those steps."""
+    `steps` RK4 steps from it (one demon_rk4_steps call, each step replayed
+    as the captured step graph when `graph`); with `start`, from the state
+    after that many untimed steps instead.  An instrumented pass then runs
+    the same steps from the same state with plain launches and HIP events
+    around every launch (the kernel times and pair counts of the roofline:
+    the same work, the steps being deterministic).  With `snapshots` (step
+    numbers), an untimed pass records the state at those steps."""
     import demon_simulation_code_amd as dm
 
     x, y, q, m = setup(wl)
@@ -129,17 +138,30 @@ def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mod
     if shape:
         c.set_pair_shape(*shape)
     c.upload(x, y, None, None, q, m)
+    c.use_graph(graph)
     if warmup:
         c.rk4_steps(warmup, wl.dt, 0.0)
     c.upload(x, y, None, None, q, m)
-    t = c.rk4_steps(start, wl.dt, 0.0) if start else 0.0
+    t_start = c.rk4_steps(start, wl.dt, 0.0) if start else 0.0
     n0, t0_ = c.pair_counts()
     tiles0, items0 = c.pair_tiles()
     c.synchronize()
     t0 = time.perf_counter()
-    t = c.rk4_steps(steps, wl.dt, t)
+    t = c.rk4_steps(steps, wl.dt, t_start)
     c.synchronize()
     wall = time.perf_counter() - t0
+    if graph:
+        # the instrumented pass, from the same state
+        timed_final = c.download()
+        c.use_graph(False)
+        c.upload(x, y, None, None, q, m)
+        if start:
+            c.rk4_steps(start, wl.dt, 0.0)
+        n0, t0_ = c.pair_counts()
+        tiles0, items0 = c.pair_tiles()
+        c.rk4_steps(steps, wl.dt, t_start)
+        if not all(np.array_equal(a, b) for a, b in zip(timed_final, c.download())):
+            raise RuntimeError("the instrumented pass did not reproduce the timed steps")
     pair_ms, total_ms = c.last_timing()
     kt = c.kernel_timing()
     near, tested = c.pair_counts()
@@ -161,7 +183,8 @@ def run_gpu(wl, steps, warmup, precision, shape=None, snapshots=(), start=0, mod
                 done = s
             snaps.append(c.download()[:2])
     c.close()
-    return dict(wall=wall, pair_ms=pair_ms, total_ms=total_ms, kt=kt, t=t, stats=st, near=near, tested=tested,
+    return dict(wall=wall, graph=graph, pair_ms=pair_ms, total_ms=total_ms, kt=kt, t=t, stats=st, near=near,
+                tested=tested,
                 tiles=tiles, items=items, final=(xs1, ys1, vx1, vy1), shape=shape_used, mode=mode_used,
                 slices=slices, snaps=snaps)
 
@@ -546,7 +569,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the N=1024 and N=16384 thermal legs")
     ap.add_argument("--no-graphs", action="store_true",
-                    help="plain launches instead of captured graphs in the N=1024/16384 and ensemble legs")
+                    help="time plain launches instead of replaying captured step graphs (every leg)")
     ap.add_argument("--plugins-zero", action="store_true",
                     help="add every per-particle force plugin at zero strength (same trajectory; "
                          "profiles what the plugins cost in the stage updates)")
@@ -570,7 +593,8 @@ def main():
     snap_steps = [] if args.no_cpu else [full // 2, 7 * full // 8]
 
     log(f"FP64 run: N={n} warmup={W} steps={K}")
-    r = run_gpu(wl, K, W, dm.FP64, shape, snapshots=[] if run_c3 else snap_steps, start=args.start_step,
+    graph = not args.no_graphs
+    r = run_gpu(wl, K, W, dm.FP64, shape, snapshots=[] if run_c3 else snap_steps, start=args.start_step, graph=graph,
                 mode=mode, plugins_zero=args.plugins_zero)
     pairs_total = 4.0 * n * (n - 1) * K
     value = pairs_total / r["wall"]
@@ -607,7 +631,12 @@ def main():
             "pair_shape": list(r["shape"]),
             "start_step": args.start_step,
             "plugins_zero": args.plugins_zero,
+            "step_graph": graph,
         },
+        "timing_note": ("the timed steps replay the captured step graph; kernel times and pair counts come "
+                        "from a pass of the same steps from the same state with plain launches and HIP events "
+                        "around each (checked to end in the same state)") if graph else
+                       "plain launches with HIP events around each",
         "rk4_steps_per_s": K / r["wall"],
         "roofline": rl,
         "update": update_traffic(r, n),
@@ -616,7 +645,7 @@ def main():
 
     if not args.no_fp32:
         log("FP32-pair run")
-        r32 = run_gpu(wl, K, W, dm.FP32_PAIR, shape, start=args.start_step, mode=mode)
+        r32 = run_gpu(wl, K, W, dm.FP32_PAIR, shape, start=args.start_step, mode=mode, graph=graph)
         x64, y64 = r["final"][0], r["final"][1]
         x32, y32 = r32["final"][0], r32["final"][1]
         perr = float(np.max(np.hypot(x64 - x32, y64 - y32)) / np.max(np.hypot(x64, y64)))
@@ -634,7 +663,7 @@ def main():
     c3 = r if not run_c3 else None
     if run_c3:
         log(f"whole trajectory: {full} steps, FP64")
-        c3 = run_gpu(wl, full, W, dm.FP64, shape, snapshots=snap_steps, mode=mode)
+        c3 = run_gpu(wl, full, W, dm.FP64, shape, snapshots=snap_steps, mode=mode, graph=graph)
     if not args.no_c3:
         v3 = 4.0 * n * (n - 1) * full / c3["wall"]
         out["c3_200"] = {
@@ -647,7 +676,7 @@ def main():
         log(f"whole trajectory done: {c3['wall']:.3f} s, {v3:.4g} pair interactions/s")
         if not args.no_fp32:
             log(f"whole trajectory: {full} steps, FP32-pair")
-            c32 = run_gpu(wl, full, W, dm.FP32_PAIR, shape, mode=mode)
+            c32 = run_gpu(wl, full, W, dm.FP32_PAIR, shape, mode=mode, graph=graph)
             x64, y64 = c3["final"][0], c3["final"][1]
             out["c3_200"]["fp32_pair"] = {
                 "value": 4.0 * n * (n - 1) * full / c32["wall"],
@@ -660,7 +689,6 @@ def main():
             }
             log(f"FP32-pair whole trajectory done: {c32['wall']:.3f} s")
 
-    graph = not args.no_graphs
     if not args.no_configs:
         from demon_simulation_code_amd.physics import PLAIN_1K, THERMAL_16K
         # the thermal magnitude for about 300 K with the reference's per-stage

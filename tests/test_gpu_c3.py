@@ -52,14 +52,18 @@ def _rel(a, b):
 
 
 @pytest.mark.timeout(300)
-@pytest.mark.parametrize("mode", [dm.PAIRS_NEWTON3, dm.PAIRS_GATHER])
-def test_trajectory_vs_oracle_full_size(mode):
+@pytest.mark.parametrize("mode,graph", [(dm.PAIRS_NEWTON3, False), (dm.PAIRS_NEWTON3, True),
+                                        (dm.PAIRS_GATHER, False)])
+def test_trajectory_vs_oracle_full_size(mode, graph):
+    """graph: every step replayed as the captured step graph (the bench's
+    timed mode)."""
     f = np.load(FIX)
     x, y, q, m = _lattice()
     assert _digest(x, y) == str(f["lattice_digest"])
     every, rows, dt = int(f["snap_every"]), f["rows"], float(f["dt"])
     times, dts, snaps = f["times"], f["dts"], f["snaps"]
     with _ctx(mode) as c:
+        c.use_graph(graph)
         c.upload(x, y, None, None, q, m)
         t = 0.0
         for k, want in enumerate(snaps):
