@@ -12,9 +12,12 @@
 // d = nsb/2 only for S < nsb/2, so that each unordered superblock pair is
 // taken once), plus itself (d = 0, gather form).  Every row does the same
 // amount of work.
-// A workgroup of four waves owns row S and a contiguous range of d (a
-// slice); wave w holds the 64 i-particles of block 4S + w in registers and
-// accumulates their forces across the slice (written once, to part[slice]).
+// A workgroup of four waves owns row S and a set of offsets d: a run of the
+// row's live tiles from the tile list (the single-cloud kernel,
+// pairs_n3.hip) or a contiguous range of d (a slice; the batched ensemble);
+// wave w holds the 64 i-particles of block 4S + w in registers and
+// accumulates their forces across its offsets (written once, to the item's
+// or the slice's partial sums).
 // For each pair tile (S, T) the reaction on every j of T is summed over the
 // wave's 64 lanes, then over the four waves in LDS in a fixed order, and
 // written to react[d - 1][.][j]; rflag[T nsb/2 + d - 1] says whether the

@@ -50,3 +50,40 @@ def force_rel_err(fx, fy, gx, gy, scale):
 def pos_rel_err(x, y, gx, gy):
     """max_i |dr_i| / max_i |r_i|."""
     return float(np.max(np.hypot(x - gx, y - gy)) / np.max(np.hypot(x, y)))
+
+
+KB = 1.380649e-23
+
+
+def kinetic_T(vx, vy, m=M0):
+    """Kinetic temperature m (var vx + var vy) / (2 k_B)."""
+    return m * (np.var(vx) + np.var(vy)) / (2 * KB)
+
+
+def excess_T(vx, vy, vx0, vy0, m=M0):
+    """m <|v - v0|^2> / (2 k_B): the kinetic energy a run's thermal kicks put
+    in, against the thermal-off run (v0) from the same start."""
+    return m * np.mean((vx - vx0) ** 2 + (vy - vy0) ** 2) / (2 * KB)
+
+
+def pair_hist_kdtree(x, y, rmax, bins):
+    """Histogram of the distances of the unordered pairs closer than rmax,
+    normalised to sum 1 (k-d tree neighbour counts at the bin edges)."""
+    from scipy.spatial import cKDTree
+
+    pts = np.column_stack([x, y])
+    edges = np.linspace(0.0, rmax, bins + 1)
+    tree = cKDTree(pts)
+    cum = tree.count_neighbors(tree, edges[1:], cumulative=True).astype(np.float64)
+    cum = (cum - len(x)) / 2.0  # drop the self pairs, count each pair once
+    h = np.diff(np.concatenate([[0.0], cum]))
+    return h / h.sum()
+
+
+def state_digest(*arrays):
+    import hashlib
+
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return h.hexdigest()
