@@ -228,21 +228,57 @@ __global__ __launch_bounds__(256) void k_bupdate(BUpdate a) {
             fy = fy + dr * vys;
         }
         if (a.forces & 4) {
+            // the slices' sums, four slices' loads issued before their adds
+            // (the same order of additions, more loads in flight)
             const double *pp = a.part + cl.part_off;
-            for (int32_t s = 0; s < cl.slices; ++s) {
+            int32_t s = 0;
+            for (; s + 4 <= cl.slices; s += 4) {
+                double px[4], py[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    px[k] = pp[(size_t)(2 * (s + k)) * cl.npad + i];
+                    py[k] = pp[(size_t)(2 * (s + k) + 1) * cl.npad + i];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    fx = fx + px[k];
+                    fy = fy + py[k];
+                }
+            }
+            for (; s < cl.slices; ++s) {
                 fx = fx + pp[(size_t)(2 * s) * cl.npad + i];
                 fy = fy + pp[(size_t)(2 * s + 1) * cl.npad + i];
             }
             if (n3c) {
-                // reactions in increasing superblock offset d (bit d - 1)
+                // reactions in increasing superblock offset d (bit d - 1), up
+                // to four offsets' loads in flight
                 const double *rr = a.react + cl.react_off;
                 for (int32_t w = 0; w < (half + 63) / 64; ++w) {
                     unsigned long long mask = fmask[w];
                     while (mask) {
-                        const int32_t d = 64 * w + __builtin_ctzll(mask) + 1;
-                        mask &= mask - 1;
-                        fx = fx + rr[(size_t)(d - 1) * 2 * cl.npad + i];
-                        fy = fy + rr[(size_t)(d - 1) * 2 * cl.npad + cl.npad + i];
+                        int32_t dd[4];
+                        int m = 0;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            dd[k] = 0;
+                            if (mask) {
+                                dd[k] = 64 * w + __builtin_ctzll(mask);
+                                mask &= mask - 1;
+                                m = k + 1;
+                            }
+                        }
+                        double rx[4], ry[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            rx[k] = k < m ? rr[(size_t)dd[k] * 2 * cl.npad + i] : 0.0;
+                            ry[k] = k < m ? rr[(size_t)dd[k] * 2 * cl.npad + cl.npad + i] : 0.0;
+                        }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (k < m) {
+                                fx = fx + rx[k];
+                                fy = fy + ry[k];
+                            }
                     }
                 }
             }

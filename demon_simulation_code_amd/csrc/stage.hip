@@ -120,9 +120,25 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
     }
     if (forces & 4) {
         if (ps.first) {
-            // Newton-3: the work items of superblock sb, in item order
+            // Newton-3: the work items of superblock sb, in item order; four
+            // items' loads are issued before their adds (the same order of
+            // additions, more loads in flight)
             const int32_t it1 = ps.first[sb + 1], li = (int32_t)(i - (int64_t)sb * kSuper);
-            for (int32_t it = ps.first[sb]; it < it1; ++it) {
+            int32_t it = ps.first[sb];
+            for (; it + 4 <= it1; it += 4) {
+                double px[4], py[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    px[k] = ps.part[(size_t)(it + k) * (2 * kSuper) + li];
+                    py[k] = ps.part[(size_t)(it + k) * (2 * kSuper) + kSuper + li];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    fx = fx + px[k];
+                    fy = fy + py[k];
+                }
+            }
+            for (; it < it1; ++it) {
                 fx = fx + ps.part[(size_t)it * (2 * kSuper) + li];
                 fy = fy + ps.part[(size_t)it * (2 * kSuper) + kSuper + li];
             }
@@ -137,12 +153,32 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
             const int32_t words = (ps.nsb / 2 + 63) / 64;
             for (int32_t w = 0; w < words; ++w) {
                 unsigned long long mask = fmask[w];
+                // up to four offsets' loads in flight, added in increasing d
                 while (mask) {
-                    const int32_t d = 64 * w + __builtin_ctzll(mask) + 1;
-                    mask &= mask - 1;
-                    const double *rr = ps.react + (size_t)(d - 1) * 2 * npad;
-                    fx = fx + rr[i];
-                    fy = fy + rr[npad + i];
+                    int32_t dd[4];
+                    int m = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        dd[k] = 0;
+                        if (mask) {
+                            dd[k] = 64 * w + __builtin_ctzll(mask);
+                            mask &= mask - 1;
+                            m = k + 1;
+                        }
+                    }
+                    double rx[4], ry[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const double *rr = ps.react + (size_t)dd[k] * 2 * npad;
+                        rx[k] = k < m ? rr[i] : 0.0;
+                        ry[k] = k < m ? rr[npad + i] : 0.0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (k < m) {
+                            fx = fx + rx[k];
+                            fy = fy + ry[k];
+                        }
                 }
             }
         }
