@@ -163,9 +163,11 @@ void choose_slices(demon_ctx *c) {
     c->jslice = (int32_t)js;
     // Newton-3: a launch is cut into about n3_slices * nsb work items of
     // consecutive live tiles (one workgroup of four waves each), i.e.
-    // n3_slices items per row when every one of its nsb/2 + 1 tiles is live
+    // n3_slices items per row when every one of its nsb/2 + 1 tiles is live;
+    // about 32768 items (32 per workgroup of the persistent grid), so that
+    // the last items of a dense launch are short
     const int64_t nsb = c->npad / kSuper, offsets = nsb / 2 + 1;
-    int64_t s3 = c->slices_req > 0 ? c->slices_req : (32768 + 4 * nsb - 1) / (4 * nsb);
+    int64_t s3 = c->slices_req > 0 ? c->slices_req : (32768 + nsb - 1) / nsb;
     c->n3_slices = (int32_t)std::max<int64_t>(1, std::min<int64_t>({s3, (int64_t)kMaxSlices, offsets}));
 }
 

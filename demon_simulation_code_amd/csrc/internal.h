@@ -75,10 +75,14 @@ struct PairSums {
 // Newton-3 work plan of one launch, written by the plan kernel: the live
 // tiles of every superblock row, cut into items of `span` consecutive live
 // tiles of one row, each item one workgroup's share.
+// Work queues of the Newton-3 pair kernel: queue 8 k + s holds the items of
+// rows (8 k + s) nsb/64 .. (8 k + s + 1) nsb/64 - 1, for XCD k.
+constexpr int kN3Queues = 64;
+
 struct N3Plan {
     int32_t span;              // live tiles per item (the last item of a row may hold fewer)
     int32_t nitems;            // items of the launch
-    uint32_t next[8];          // item counters of the pair kernel, one per XCD
+    uint32_t next[64];         // item counters of the pair kernel, one per queue (kN3Queues)
 };
 
 // Entries of the Newton-3 pair kernel's exp table, e^{-k/64} for k < 642

@@ -10,8 +10,8 @@
 // (four 64-particle box blocks).  With nsb superblocks, row S pairs with
 // superblocks T = S + d (mod nsb) for d = 1 .. floor(nsb/2) (for even nsb,
 // d = nsb/2 only for S < nsb/2, so that each unordered superblock pair is
-// taken once), plus itself (d = 0, gather form).  Every row does the same
-// amount of work.
+// taken once), plus itself (d = 0, each unordered pair of the superblock
+// once: n3_diag).  Every row does the same amount of work.
 // A workgroup of four waves owns row S and a set of offsets d: a run of the
 // row's live tiles from the tile list (the single-cloud kernel,
 // pairs_n3.hip) or a contiguous range of d (a slice; the batched ensemble);
@@ -87,8 +87,19 @@ __device__ __forceinline__ float dpp_mirror(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppHalfMirror, 0xF, 0xF, true));
 }
 
+// The j of a chunk of four: each one's particle index (the dt check), and
+// for this lane the slot in the tile's LDS reaction sums of the j whose sum
+// the lane stores (u = (lane / 8) mod 4 in store_wave_sums): j minus the
+// tile's first particle.
+struct ChunkJ {
+    int32_t j[kChunkN3];
+    int32_t lslot;
+};
+
+__device__ __forceinline__ int32_t lane_chunk_u() { return (int32_t)((threadIdx.x >> 3) & 3); }
+
 template <typename V>
-__device__ __forceinline__ void store_wave_sums(const V (&f)[2 * kChunkN3], V *scr, double *red, int32_t jj0) {
+__device__ __forceinline__ void store_wave_sums(const V (&f)[2 * kChunkN3], V *scr, double *red, int32_t lslot) {
     constexpr int kRow = Scr<V>::kRow;
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -103,7 +114,7 @@ __device__ __forceinline__ void store_wave_sums(const V (&f)[2 * kChunkN3], V *s
     t += dpp_swap1(t);
     t += dpp_swap2(t);
     t += dpp_mirror(t);
-    if (p == 0) red[2 * (jj0 + (v & 3)) + (v >> 2)] = (double)t;
+    if (p == 0) red[2 * lslot + (v >> 2)] = (double)t;
 }
 
 // LDS scratch per wave for the transposes: 8 rows of the row length of the
@@ -140,8 +151,8 @@ using PairVal = typename std::conditional<F32, float, double>::type;
 template <bool F32, bool CHECK, bool DTC, bool TAB = false>
 __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[kChunkN3],
                                          const double (&cy)[kChunkN3], const PairVal<F32> (&cq)[kChunkN3],
-                                         int32_t j0, const PairEnv &e, uint64_t &n_near, float &smin,
-                                         double &ax, double &ay, double *scr, double *red, int32_t jj0) {
+                                         const ChunkJ &cj, const PairEnv &e, uint64_t &n_near, float &smin,
+                                         double &ax, double &ay, double *scr, double *red) {
     using V = PairVal<F32>;
     V f[2 * kChunkN3];
     uint64_t hits = 0;  // lanes with some pair of the chunk inside the cutoff (CHECK)
@@ -171,7 +182,7 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
         }
         f[u] = fx;
         f[kChunkN3 + u] = fy;
-        if (DTC) pair_dtcheck(t.xi[0], t.yi[0], cx[u], cy[u], r2, t.ibase, j0 + u, e, smin);
+        if (DTC) pair_dtcheck(t.xi[0], t.yi[0], cx[u], cy[u], r2, t.ibase, cj.j[u], e, smin);
     }
     // No pair of the chunk inside the cutoff: every f is zero, and the
     // wave's sums of this chunk's j keep the zeros the tile stored in red
@@ -187,7 +198,7 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
         ax = ax + (double)sx;
         ay = ay + (double)sy;
     }
-    store_wave_sums<V>(f, reinterpret_cast<V *>(scr), red, jj0);
+    store_wave_sums<V>(f, reinterpret_cast<V *>(scr), red, cj.lslot);
 }
 
 // One cloud's arrays and launch constants for the row function.
@@ -227,8 +238,8 @@ __device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, cons
     const float *__restrict__ qf = a.qf;
     const BlockClass cls = classify(ib, a.box + 4 * (jb / kBoxBlock), ireal, jb + kBoxBlock <= a.n, e.rad, e.cut);
     if (cls == kSkip) return;
-    n_tested += 2 * 64 * kBoxBlock;
     if (cls == kDense) {
+        n_tested += 2 * 64 * kBoxBlock;
         n_near += 2 * 64 * kBoxBlock;
         for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
             double cx[kChunkN3], cy[kChunkN3];
@@ -240,10 +251,12 @@ __device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, cons
                 if constexpr (F32) cq[u] = ld_uniform(qf + j0 + u);
                 else cq[u] = ld_uniform(q + j0 + u);
             }
-            chunk_n3<F32, false, DTC, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr, red, j0 - t0);
+            const ChunkJ cj{{j0, j0 + 1, j0 + 2, j0 + 3}, j0 - t0 + lane_chunk_u()};
+            chunk_n3<F32, false, DTC, TAB>(t, cx, cy, cq, cj, e, n_near, smin, ax, ay, scr, red);
         }
         return;
     }
+    n_tested += 2 * 64 * kBoxBlock;
     for (int32_t j0 = jb; j0 < jb + kBoxBlock; j0 += kChunkN3) {
         double cx[kChunkN3], cy[kChunkN3];
         PairVal<F32> cq[kChunkN3];
@@ -258,7 +271,8 @@ __device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, cons
             if constexpr (F32) cq[u] = ld_uniform(qf + j0 + u);
             else cq[u] = ld_uniform(q + j0 + u);
         }
-        chunk_n3<F32, true, DTC, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay, scr, red, j0 - t0);
+        const ChunkJ cj{{j0, j0 + 1, j0 + 2, j0 + 3}, j0 - t0 + lane_chunk_u()};
+        chunk_n3<F32, true, DTC, TAB>(t, cx, cy, cq, cj, e, n_near, smin, ax, ay, scr, red);
     }
 }
 
@@ -374,9 +388,8 @@ __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_
     int32_t tb0 = 0;  // offset d of tbox[0]
     for (int32_t d = d0; d < d1; ++d) {
         if (d == 0) {
-            // the superblock with itself, every ordered pair, j == i skipped
-            gather_blocks<1, kChunkN3, F32>(t, a.xs, a.ys, a.nc, a.q, box, ib, n, S * kSuper, S * kSuper + kSuper,
-                                            e, ax, ay, n_near, n_tested, smin);
+            // the superblock with itself, each unordered pair once
+            n3_diag<F32, DTC, false>(a, e, t, ib, ireal, S, ax[0], ay[0], n_near, n_tested, smin, red, scr);
             continue;
         }
         if ((d - d0) % kBoxChunk == 0 || d == 1) {

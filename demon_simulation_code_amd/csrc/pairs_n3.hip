@@ -12,17 +12,16 @@
 //               consecutive live tiles, span = ceil(live tiles / target), and
 //               writes the item table (first item of each row, row of each
 //               item).
-//  k_pairs_n3   a persistent grid: every workgroup takes items from an
-//               atomic counter until none is left; one item = the row's i
+//  k_pairs_n3   a persistent grid: every workgroup takes items from atomic
+//               counters until none is left; one item = the row's i
 //               particles in registers against its span of live tiles, the
 //               i-side sums written once, to part[item].
 //
 // Only live tiles cost anything: on the benchmark lattice about 1% of the
 // superblock pairs are within the cutoff, and a launch takes as long as
-// those take.  Items are queued per XCD (rows k nsb/8 .. (k+1) nsb/8 - 1 on
-// XCD k, whose workgroups are those with blockIdx.x = k mod 8), which keeps
-// the j superblocks an XCD visits to 5/8 of the cloud, within its own L2;
-// a workgroup whose queue is empty takes items from the other XCDs' queues.
+// those take.  The items of rows k nsb/8 .. (k+1) nsb/8 - 1 go to XCD k,
+// whose workgroups are those with blockIdx.x = k mod 8, which keeps the j
+// superblocks an XCD visits to 5/8 of the cloud, within its own L2.
 #include "pairs_n3.h"
 
 namespace demon {
@@ -120,7 +119,7 @@ __global__ __launch_bounds__(256) void k_n3_plan(N3Row a, N3List l) {
     __shared__ int32_t scan[256];
     if ((a.flags & kFlagGated) && !a.sc->active) return;
     const int32_t nsb = a.npad / kSuper;
-    if (threadIdx.x < 8) l.plan->next[threadIdx.x] = 0u;
+    if (threadIdx.x < kN3Queues) l.plan->next[threadIdx.x] = 0u;
     const int32_t per = (nsb + 255) / 256;
     const int32_t r0 = min((int32_t)threadIdx.x * per, nsb), r1 = min(r0 + per, nsb);
     int32_t tiles = 0, rows = 0;
@@ -188,10 +187,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ RedLds red;
     __shared__ __align__(16) ScrLds<F32> scr;
     __shared__ double etab[kExpTabN];
-    __shared__ int32_t slot;
     if ((a.flags & kFlagGated) && !a.sc->active) return;
-    // The FP64 exp is exp_neg_tab on this table.  The item loop's first
-    // barrier orders these stores before any read.
+    // The FP64 exp is exp_neg_tab on this table.  The first tile's barrier
+    // orders these stores before any read.
     if (!F32)
         for (int k = (int)threadIdx.x; k < kExpTabN; k += 256) etab[k] = l.exptab[k];
     const int32_t nsb = a.npad / kSuper, half = nsb / 2;
@@ -200,16 +198,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int32_t span = __builtin_amdgcn_readfirstlane(l.plan->span);
     float smin = __builtin_inff();
     uint64_t n_near = 0, n_tested = 0;
-    const int32_t home = (int32_t)(blockIdx.x & 7);
-    for (int32_t qi = 0; qi < 8; ++qi) {
-        // this XCD's queue first, then the others'
-        const int32_t k = (home + qi) & 7;
-        const int32_t i0 = __builtin_amdgcn_readfirstlane(l.first[k * nsb / 8]);
-        const int32_t i1 = __builtin_amdgcn_readfirstlane(l.first[(k + 1) * nsb / 8]);
-        if (i0 >= i1) continue;
+    // Work queues: workgroup g runs on XCD k = g mod 8 and starts with queue
+    // 8 k + s, s = (g / 8) mod 8.  One item per atomic; sixty-four counters,
+    // since the requests to one counter serialise.  When its queue is
+    // exhausted, the workgroup reads all sixty-four counters at once (one
+    // per lane of wave 0) and moves to the first queue with items left, in
+    // the order: the other queues of its XCD, then those of the other XCDs.
+    __shared__ int32_t slot;
+    const int32_t home = (int32_t)(blockIdx.x & 7), sub = (int32_t)((blockIdx.x >> 3) & 7);
+    int32_t qk = 8 * home + sub;
+    for (;;) {
+        const int32_t i0 = __builtin_amdgcn_readfirstlane(l.first[qk * nsb / kN3Queues]);
+        const int32_t i1 = __builtin_amdgcn_readfirstlane(l.first[(qk + 1) * nsb / kN3Queues]);
         for (;;) {
             __syncthreads();  // the previous item is done with slot and the LDS sums
-            if (threadIdx.x == 0) slot = (int32_t)atomicAdd(&l.plan->next[k], 1u);
+            if (threadIdx.x == 0) slot = i0 < i1 ? (int32_t)atomicAdd(&l.plan->next[qk], 1u) : 0;
             __syncthreads();
             const int32_t it = i0 + __builtin_amdgcn_readfirstlane(slot);
             if (it >= i1) break;
@@ -220,6 +223,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             n3_item<F32, DTC>(a, e, S, l.list + (size_t)S * (half + 1), k0, k1, it, n_near, n_tested, smin, red,
                               scr);
         }
+        // the next queue with items left (a stale read only costs one more
+        // round); the choice is wave 0's, broadcast through LDS
+        __syncthreads();
+        if (threadIdx.x < kN3Queues) {
+            const int32_t t = (int32_t)threadIdx.x;
+            const int32_t q = 8 * ((home + (t >> 3)) & 7) + ((sub + t) & 7);
+            const int32_t q0 = l.first[q * nsb / kN3Queues], q1 = l.first[(q + 1) * nsb / kN3Queues];
+            const bool left = q0 < q1 &&
+                q0 + (int32_t)__hip_atomic_load(&l.plan->next[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < q1;
+            const uint64_t m = __ballot(left);
+            if (threadIdx.x == 0) slot = m ? 8 * ((home + (__builtin_ctzll(m) >> 3)) & 7) + ((sub + __builtin_ctzll(m)) & 7) : -1;
+        }
+        __syncthreads();
+        qk = __builtin_amdgcn_readfirstlane(slot);
+        if (qk < 0) break;
     }
     if (DTC) commit_dtmin(smin, &a.sc->dtmin_bits);
     n3_commit_counts(a.sc, (int32_t)blockIdx.x, n_near, n_tested, red);

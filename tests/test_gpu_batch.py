@@ -184,10 +184,11 @@ def test_batch_real_ensemble_sample_matches_oracle(oracle):
 @pytest.mark.parametrize("form", ["gather", "newton3"])
 def test_batch_pair_counts_and_kernel_timing(form):
     """The ensemble kernels' pair counts equal the single-cloud context's over
-    the same steps (the same box culling and contribution test on the same
-    stage states; the context's count of pairs inside the cutoff is the
-    oracle's exact count), and plain steps carry one event pair per all-pairs
-    pass and per stage update, captured steps none."""
+    the same steps, with the context's all-pairs kernel in the same form (the
+    same box culling and contribution test on the same stage states; the
+    context's count of pairs inside the cutoff is the oracle's exact count),
+    and plain steps carry one event pair per all-pairs pass and per stage
+    update, captured steps none."""
     b, ens, states = _make(5, seed=31, n_max=3000)
     steps = 3
     with b:
@@ -208,6 +209,7 @@ def test_batch_pair_counts_and_kernel_timing(form):
     want_near = want_tested = 0
     for e, (x, y, q, m) in zip(ens, states):
         with dm.Context(e["n"], shielding=e["kappa"], confine=e["confine"], gamma=e["gamma"]) as c:
+            c.set_pair_mode(dm.PAIRS_NEWTON3 if form == "newton3" else dm.PAIRS_GATHER)
             c.upload(x, y, None, None, q, m)
             c.rk4_steps(steps, 1e-3, 0.0)
             nn, nt = c.pair_counts()
