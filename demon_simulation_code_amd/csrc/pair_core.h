@@ -132,30 +132,34 @@ __device__ __forceinline__ double exp_neg(double u, const ExpCoef &ec) {
     return __builtin_ldexp(p, ni);
 }
 
-// e^-u for 0 <= u < 10.03 (callers pass u < 10) from a table in LDS,
-// tab[k] = e^{-k/64} (k < kExpTabN, each the double nearest to a
-// long-double evaluation on the host): n = rint(64 u) by the same
-// magic-number rounding, d = u - n/64 (exact: n/64 is exact and u - n/64
-// needs at most 43 significant bits), and e^-d by the degree-5 Chebyshev
-// interpolant of e^-d on |d| <= 1/128 (gen_exp_coef.py: 2.0e-17 relative
-// with the coefficients rounded to double); about 1 ulp in all.  Its
-// coefficients tc[0..4] (highest power first; the constant term is 1) and
-// the reduction's constants tc[6..8] = 64, 1.5 2^52, -1/64 arrive as kernel
-// arguments (SGPRs), as exp_neg's do: as literals the rounding constant
-// would be copied into a VGPR pair for every evaluation.
-// Three FMA-class operations and an add fewer than exp_neg, and no ldexp:
-// the table replaces the reduction's ln 2 split and the high-degree tail.
+// e^{-kappa r} for 0 <= kappa r < 10.03 (callers pass kappa r < 10) from
+// a table in LDS, tab[k] = e^{-k/64} (k < kExpTabN, each the double nearest
+// to a long-double evaluation on the host): n = rint(64 kappa r) by the same
+// magic-number rounding (m = 1.5 2^52 + n, with 64 kappa exact), -n/64 from
+// m exactly (fma(m, -1/64, 1.5 2^46): both products and the sum need at
+// most 53 bits), d = kappa r - n/64 with one rounding, and e^-d by the
+// degree-5 Chebyshev interpolant of e^-d on |d| <= 1/128 (gen_exp_coef.py:
+// 2.0e-17 relative with the coefficients rounded to double); about 1 ulp in
+// all.  Its coefficients tc[0..4] (highest power first; the constant term is
+// 1) and the reduction's constants tc[5..8] = 1.5 2^46, 64, 1.5 2^52, -1/64
+// arrive as kernel arguments (SGPRs), as exp_neg's do: as literals the
+// rounding constant would be copied into a VGPR pair for every evaluation.
+// The argument is r, not u = kappa r: the caller forms (1 + u)/r^3 without
+// u (pair_scale_noq, pair_scale_env).  Three FMA-class operations and an add fewer than
+// exp_neg, and no ldexp: the table replaces the reduction's ln 2 split and
+// the high-degree tail.
 constexpr double kExpTabScale = 64.0;
 
 inline ExpCoef exp_tab_coef() {
-    return ExpCoef{{-0.008333351498575732, 0.0416667938233981, -0.1666666666662509, 0.4999999999970896, -1.0, 0.0,
-                    kExpTabScale, 0x1.8p52, -1.0 / kExpTabScale, 0.0}};
+    return ExpCoef{{-0.008333351498575732, 0.0416667938233981, -0.1666666666662509, 0.4999999999970896, -1.0,
+                    0x1.8p52 / kExpTabScale, kExpTabScale, 0x1.8p52, -1.0 / kExpTabScale, 0.0}};
 }
 
-__device__ __forceinline__ double exp_neg_tab(double u, const double *__restrict__ tab, const ExpCoef &tc) {
-    const double m = fma(u, tc.c[6], tc.c[7]);
+__device__ __forceinline__ double exp_neg_tab(double r, double kappa, const double *__restrict__ tab,
+                                              const ExpCoef &tc) {
+    const double m = fma(r, kappa * tc.c[6], tc.c[7]);
     const int ni = __double2loint(m);
-    const double d = fma(m - tc.c[7], tc.c[8], u);
+    const double d = fma(r, kappa, fma(m, tc.c[8], tc.c[5]));
     double p = tc.c[0];
 #pragma unroll
     for (int k = 1; k < 5; ++k) p = fma(p, d, tc.c[k]);
@@ -346,15 +350,31 @@ struct PairEnv {
     const double *etab = nullptr;  // e^{-k/64} in LDS (kernels built with TAB)
 };
 
+// The pair scale without the factor q_j, C q_i (1 + u) e^-u / r^3, with
+// the launch's exp: exp_neg_tab on the LDS table (TAB), where (1 + u)/r^3
+// = y^3 + kappa r y^3 is formed as y^2 (y + kappa) (r y = 1 far below an
+// ulp after the refinement) and u itself is never needed, or exp_neg.
+template <bool TAB>
+__device__ __forceinline__ double pair_scale_noq(double r2, double cqi, const PairEnv &e) {
+    const InvSqrt s = inv_sqrt_fp64(r2);
+    if constexpr (TAB) {
+        const double g = (s.y * s.y) * (s.y + e.cut.kappa);
+        return (cqi * g) * exp_neg_tab(s.r, e.cut.kappa, e.etab, e.ec);
+    } else {
+        const double u = s.r * e.cut.kappa;
+        const double y3 = (s.y * s.y) * s.y;
+        return (cqi * fma(u, y3, y3)) * exp_neg(u, e.ec);
+    }
+}
+
 // The pair scale with the launch's exp: exp_neg_tab on the LDS table (TAB)
 // or exp_neg.
 template <bool TAB>
 __device__ __forceinline__ double pair_scale_env(double r2, double cqi, double qj, const PairEnv &e) {
     if constexpr (TAB) {
         const InvSqrt s = inv_sqrt_fp64(r2);
-        const double u = s.r * e.cut.kappa;
-        const double y3 = (s.y * s.y) * s.y;
-        return ((cqi * qj) * fma(u, y3, y3)) * exp_neg_tab(u, e.etab, e.ec);
+        const double g = (s.y * s.y) * (s.y + e.cut.kappa);
+        return ((cqi * qj) * g) * exp_neg_tab(s.r, e.cut.kappa, e.etab, e.ec);
     } else {
         return pair_scale_fp64(r2, cqi, qj, e.cut.kappa, e.ec);
     }

@@ -173,11 +173,13 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
             if constexpr (F32) {
                 pair_force_fp32f(dx, dy, t.cqf[0], cq[u], e.kl2_f, (float)e.cut.kappa, fx, fy);
             } else {
-                const double s = pair_scale_env<TAB>(r2, t.cqi[0], cq[u], e);
+                // the force without q_j: q_j multiplies the i-side term here
+                // and the j-side sums once per j (n3_tile, n3_diag)
+                const double s = pair_scale_noq<TAB>(r2, t.cqi[0], e);
                 fx = s * dx;
                 fy = s * dy;
-                ax = ax + fx;
-                ay = ay + fy;
+                ax = fma(cq[u], fx, ax);
+                ay = fma(cq[u], fy, ay);
             }
         }
         f[u] = fx;
@@ -297,9 +299,15 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
                                  n_tested, smin, scr_rows<F32>(scr, wave), r);
     __syncthreads();
     // the four waves' sums in a fixed order; the reaction is their negation
+    // (times q_j for FP64 pair arithmetic, whose sums leave it out)
     const int jj = threadIdx.x;
-    const double rx = ((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj];
-    const double ry = ((red[0][2 * jj + 1] + red[1][2 * jj + 1]) + red[2][2 * jj + 1]) + red[3][2 * jj + 1];
+    double rx = ((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj];
+    double ry = ((red[0][2 * jj + 1] + red[1][2 * jj + 1]) + red[2][2 * jj + 1]) + red[3][2 * jj + 1];
+    if constexpr (!F32) {
+        const double qj = a.q[T * kSuper + jj];
+        rx = rx * qj;
+        ry = ry * qj;
+    }
     double *rr = a.react + (size_t)(d - 1) * 2 * npad;
     rr[T * kSuper + jj] = -rx;
     rr[npad + T * kSuper + jj] = -ry;
@@ -334,8 +342,15 @@ __device__ __forceinline__ void n3_diag(const N3Row &a, const PairEnv &e, const 
                                  n_near, n_tested, smin, scr_rows<F32>(scr, wave), r);
     __syncthreads();
     const int jj = threadIdx.x;
-    ax = ax - (((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj]);
-    ay = ay - (((red[0][2 * jj + 1] + red[1][2 * jj + 1]) + red[2][2 * jj + 1]) + red[3][2 * jj + 1]);
+    double rx = ((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj];
+    double ry = ((red[0][2 * jj + 1] + red[1][2 * jj + 1]) + red[2][2 * jj + 1]) + red[3][2 * jj + 1];
+    if constexpr (!F32) {
+        const double qj = a.q[S * kSuper + jj];
+        rx = rx * qj;
+        ry = ry * qj;
+    }
+    ax = ax - rx;
+    ay = ay - ry;
 }
 
 // The pair counts of a workgroup's waves, added with one atomic each to one
