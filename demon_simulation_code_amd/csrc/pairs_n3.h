@@ -267,12 +267,29 @@ __device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, cons
             cx[u] = ld_uniform(xs + j0 + u);
             cy[u] = ld_uniform(ys + j0 + u);
         }
-        if (!chunk_hit<1, kChunkN3>(t, cx, cy, nc, j0)) continue;
+        // the norms and charges in the same batch of scalar loads: one
+        // round trip to memory per chunk, which is what a sparse launch's
+        // chunks wait on
+        double cn[kChunkN3];
 #pragma unroll
         for (int u = 0; u < kChunkN3; ++u) {
+            cn[u] = ld_uniform(nc + j0 + u);
             if constexpr (F32) cq[u] = ld_uniform(qf + j0 + u);
             else cq[u] = ld_uniform(q + j0 + u);
         }
+        // an empty statement that takes all sixteen values: the compiler
+        // must have every load back here, so it issues them together rather
+        // than sinking the charges behind the prefilter's branch
+        asm volatile("" ::"s"(cx[0]), "s"(cx[1]), "s"(cx[2]), "s"(cx[3]), "s"(cy[0]), "s"(cy[1]), "s"(cy[2]),
+                     "s"(cy[3]), "s"(cn[0]), "s"(cn[1]), "s"(cn[2]), "s"(cn[3]), "s"(cq[0]), "s"(cq[1]), "s"(cq[2]),
+                     "s"(cq[3]));
+        uint64_t hit = 0;
+#pragma unroll
+        for (int u = 0; u < kChunkN3; ++u) {
+            const double w = fma(t.xm[0], cx[u], fma(t.ym[0], cy[u], t.nT[0]));
+            hit |= __ballot(w < cn[u]);
+        }
+        if (!hit) continue;
         const ChunkJ cj{{j0, j0 + 1, j0 + 2, j0 + 3}, j0 - t0 + lane_chunk_u()};
         chunk_n3<F32, true, DTC, TAB>(t, cx, cy, cq, cj, e, n_near, smin, ax, ay, scr, red);
     }
