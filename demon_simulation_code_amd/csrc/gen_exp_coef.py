@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
 """Coefficients of the exp polynomials of pair_core.h: exp_neg's, and
-exp_neg_tab's (after its table e^{-k/64}: the degree-5 Chebyshev interpolant
+exp_neg_tab's (after its table e^{-k/64}: the degree-4 Chebyshev interpolant
 of e^-d on |d| <= 1/128, printed second).
 
 e^f = 1 + f + f^2 q(f) on [-ln2/2, ln2/2]; q is the degree-9 Chebyshev
@@ -38,7 +38,7 @@ def main():
 
 # --- exp_neg_tab: e^-d on |d| <= 1/128 after the table e^{-k/64} --------------
 TAB_HALF_STEP = mp.mpf(1) / 128
-DEGREE_TAB = 5
+DEGREE_TAB = 4
 
 
 def tab_main():

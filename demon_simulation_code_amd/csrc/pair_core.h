@@ -87,6 +87,16 @@ __device__ __forceinline__ InvSqrt inv_sqrt_fp64(double r2) {
 
 __device__ __forceinline__ double rsqrt_fp64(double r2) { return inv_sqrt_fp64(r2).y; }
 
+// The same with one Newton step, c = e/2: a relative error of about 3e^2/8
+// (3.5e-15 for the measured |e| <= 2^-23.2 of v_rsq_f64), one operation
+// fewer.  For the Newton-3 kernel's pair force, whose bar is 1e-12 relative.
+__device__ __forceinline__ InvSqrt inv_sqrt_fp64_newton(double r2) {
+    const double y0 = __builtin_amdgcn_rsq(r2);
+    const double t = r2 * y0;
+    const double c = 0.5 * fma(-t, y0, 1.0);
+    return InvSqrt{fma(y0, c, y0), fma(t, c, t)};
+}
+
 // Coefficients q_0 .. q_9 (highest power first) of the polynomial q with
 // e^f ~ 1 + f + f^2 q(f) on [-ln2/2, ln2/2]: the Chebyshev interpolant of
 // degree 9 of (e^f - 1 - f) / f^2 (near-minimax; 1.6e-17 relative from e^f with the
@@ -138,20 +148,20 @@ __device__ __forceinline__ double exp_neg(double u, const ExpCoef &ec) {
 // magic-number rounding (m = 1.5 2^52 + n, with 64 kappa exact), -n/64 from
 // m exactly (fma(m, -1/64, 1.5 2^46): both products and the sum need at
 // most 53 bits), d = kappa r - n/64 with one rounding, and e^-d by the
-// degree-5 Chebyshev interpolant of e^-d on |d| <= 1/128 (gen_exp_coef.py:
-// 2.0e-17 relative with the coefficients rounded to double); about 1 ulp in
-// all.  Its coefficients tc[0..4] (highest power first; the constant term is
-// 1) and the reduction's constants tc[5..8] = 1.5 2^46, 64, 1.5 2^52, -1/64
-// arrive as kernel arguments (SGPRs), as exp_neg's do: as literals the
-// rounding constant would be copied into a VGPR pair for every evaluation.
-// The argument is r, not u = kappa r: the caller forms (1 + u)/r^3 without
-// u (pair_scale_noq, pair_scale_env).  Three FMA-class operations and an add fewer than
-// exp_neg, and no ldexp: the table replaces the reduction's ln 2 split and
-// the high-degree tail.
+// degree-4 Chebyshev interpolant of e^-d on |d| <= 1/128 (gen_exp_coef.py:
+// 1.5e-14 relative with the coefficients rounded to double, the constant
+// term 1), well inside the pair force's 1e-12 bar.  Its coefficients
+// tc[0..3] (highest power first) and the reduction's constants tc[5..8] =
+// 1.5 2^46, 64, 1.5 2^52, -1/64 arrive as kernel arguments (SGPRs), as
+// exp_neg's do: as literals the rounding constant would be copied into a
+// VGPR pair for every evaluation.  The argument is r, not u = kappa r: the
+// caller forms (1 + u)/r^3 without u (pair_scale_noq, pair_scale_env).  Four
+// FMA-class operations and an add fewer than exp_neg, and no ldexp: the
+// table replaces the reduction's ln 2 split and the high-degree tail.
 constexpr double kExpTabScale = 64.0;
 
 inline ExpCoef exp_tab_coef() {
-    return ExpCoef{{-0.008333351498575732, 0.0416667938233981, -0.1666666666662509, 0.4999999999970896, -1.0,
+    return ExpCoef{{0.041666772630595095, -0.1666673024504682, 0.4999999999983831, -0.9999999999902988, 0.0,
                     0x1.8p52 / kExpTabScale, kExpTabScale, 0x1.8p52, -1.0 / kExpTabScale, 0.0}};
 }
 
@@ -162,7 +172,7 @@ __device__ __forceinline__ double exp_neg_tab(double r, double kappa, const doub
     const double d = fma(r, kappa, fma(m, tc.c[8], tc.c[5]));
     double p = tc.c[0];
 #pragma unroll
-    for (int k = 1; k < 5; ++k) p = fma(p, d, tc.c[k]);
+    for (int k = 1; k < 4; ++k) p = fma(p, d, tc.c[k]);
     p = fma(p, d, 1.0);
     return tab[ni] * p;
 }
@@ -351,16 +361,18 @@ struct PairEnv {
 };
 
 // The pair scale without the factor q_j, C q_i (1 + u) e^-u / r^3, with
-// the launch's exp: exp_neg_tab on the LDS table (TAB), where (1 + u)/r^3
-// = y^3 + kappa r y^3 is formed as y^2 (y + kappa) (r y = 1 far below an
-// ulp after the refinement) and u itself is never needed, or exp_neg.
+// the launch's exp: exp_neg_tab on the LDS table (TAB; with the Newton-step
+// reciprocal square root), where (1 + u)/r^3 = y^3 + kappa r y^3 is formed
+// as y^2 (y + kappa) (r y = 1 to about 1e-29 after the refinement) and u
+// itself is never needed, or exp_neg.
 template <bool TAB>
 __device__ __forceinline__ double pair_scale_noq(double r2, double cqi, const PairEnv &e) {
-    const InvSqrt s = inv_sqrt_fp64(r2);
     if constexpr (TAB) {
+        const InvSqrt s = inv_sqrt_fp64_newton(r2);
         const double g = (s.y * s.y) * (s.y + e.cut.kappa);
         return (cqi * g) * exp_neg_tab(s.r, e.cut.kappa, e.etab, e.ec);
     } else {
+        const InvSqrt s = inv_sqrt_fp64(r2);
         const double u = s.r * e.cut.kappa;
         const double y3 = (s.y * s.y) * s.y;
         return (cqi * fma(u, y3, y3)) * exp_neg(u, e.ec);
@@ -372,7 +384,7 @@ __device__ __forceinline__ double pair_scale_noq(double r2, double cqi, const Pa
 template <bool TAB>
 __device__ __forceinline__ double pair_scale_env(double r2, double cqi, double qj, const PairEnv &e) {
     if constexpr (TAB) {
-        const InvSqrt s = inv_sqrt_fp64(r2);
+        const InvSqrt s = inv_sqrt_fp64_newton(r2);
         const double g = (s.y * s.y) * (s.y + e.cut.kappa);
         return ((cqi * qj) * g) * exp_neg_tab(s.r, e.cut.kappa, e.etab, e.ec);
     } else {
