@@ -214,6 +214,21 @@ __host__ __device__ __forceinline__ float fp32_kl2(double kappa) {
     return (float)(-kappa * 1.4426950408889634074);
 }
 
+// The same without the factor q_j (the Newton-3 kernel applies it to the
+// i-side term and to the j-side sums).
+__device__ __forceinline__ void pair_force_fp32f_noq(double dx, double dy, float cqi, float kl2, float kf, float &fx,
+                                                     float &fy) {
+    const float gx = (float)dx, gy = (float)dy;
+    const float r2 = fmaf(gx, gx, gy * gy);
+    const float y = __builtin_amdgcn_rsqf(r2);
+    const float a = (r2 * y) * kl2;
+    const float e = __builtin_amdgcn_exp2f(a);
+    const float g = (y * y) * (y + kf);
+    const float s = (cqi * e) * g;
+    fx = s * gx;
+    fy = s * gy;
+}
+
 // The same, with the components returned in FP64 for FP64 accumulation.
 __device__ __forceinline__ void pair_force_fp32(double dx, double dy, float cqi, float qj, float kl2, float kf,
                                                 double &fx, double &fy) {
@@ -482,9 +497,34 @@ __device__ __forceinline__ void gather_blocks(const ITile<R> &t, const double *_
                 cx[u] = ld_uniform(xs + j0 + u);
                 cy[u] = ld_uniform(ys + j0 + u);
             }
-            if (cls != kDense && !chunk_hit<R, C>(t, cx, cy, nc, j0)) continue;
+            if constexpr (C == 4) {
+                // four j: positions, norms and charges in one batch of
+                // scalar loads (one round trip per chunk; the empty
+                // statement keeps the charge loads from being sunk behind
+                // the prefilter's branch)
+                double cn[C];
 #pragma unroll
-            for (int u = 0; u < C; ++u) cq[u] = ld_uniform(q + j0 + u);
+                for (int u = 0; u < C; ++u) {
+                    cn[u] = ld_uniform(nc + j0 + u);
+                    cq[u] = ld_uniform(q + j0 + u);
+                }
+                asm volatile("" ::"s"(cx[0]), "s"(cx[1]), "s"(cx[2]), "s"(cx[3]), "s"(cy[0]), "s"(cy[1]), "s"(cy[2]),
+                             "s"(cy[3]), "s"(cn[0]), "s"(cn[1]), "s"(cn[2]), "s"(cn[3]), "s"(cq[0]), "s"(cq[1]),
+                             "s"(cq[2]), "s"(cq[3]));
+                if (cls != kDense) {
+                    uint64_t hit = 0;
+#pragma unroll
+                    for (int u = 0; u < C; ++u)
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            hit |= __ballot(fma(t.xm[r], cx[u], fma(t.ym[r], cy[u], t.nT[r])) < cn[u]);
+                    if (!hit) continue;
+                }
+            } else {
+                if (cls != kDense && !chunk_hit<R, C>(t, cx, cy, nc, j0)) continue;
+#pragma unroll
+                for (int u = 0; u < C; ++u) cq[u] = ld_uniform(q + j0 + u);
+            }
             const bool own = j0 < t.whi && j0 + C > t.wlo;
             if (own)
                 chunk_gather<R, C, F32, true, true, TAB>(t, cx, cy, cq, j0, e, n_near, smin, ax, ay);

@@ -171,7 +171,8 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
         V fx = 0, fy = 0;
         if (in) {
             if constexpr (F32) {
-                pair_force_fp32f(dx, dy, t.cqf[0], cq[u], e.kl2_f, (float)e.cut.kappa, fx, fy);
+                // without q_j, as the FP64 form below
+                pair_force_fp32f_noq(dx, dy, t.cqf[0], e.kl2_f, (float)e.cut.kappa, fx, fy);
             } else {
                 // the force without q_j: q_j multiplies the i-side term here
                 // and the j-side sums once per j (n3_tile, n3_diag)
@@ -191,11 +192,11 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
     // (each chunk's slots are written once per tile).  hits is uniform.
     if (CHECK && hits == 0) return;
     if constexpr (F32) {
-        float sx = f[0], sy = f[kChunkN3];
+        float sx = cq[0] * f[0], sy = cq[0] * f[kChunkN3];
 #pragma unroll
         for (int u = 1; u < kChunkN3; ++u) {
-            sx += f[u];
-            sy += f[kChunkN3 + u];
+            sx = fmaf(cq[u], f[u], sx);
+            sy = fmaf(cq[u], f[kChunkN3 + u], sy);
         }
         ax = ax + (double)sx;
         ay = ay + (double)sy;
@@ -316,12 +317,12 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
                                  n_tested, smin, scr_rows<F32>(scr, wave), r);
     __syncthreads();
     // the four waves' sums in a fixed order; the reaction is their negation
-    // (times q_j for FP64 pair arithmetic, whose sums leave it out)
+    // times q_j (the pair sums leave it out; FP32 pairs: the rounded charge)
     const int jj = threadIdx.x;
     double rx = ((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj];
     double ry = ((red[0][2 * jj + 1] + red[1][2 * jj + 1]) + red[2][2 * jj + 1]) + red[3][2 * jj + 1];
-    if constexpr (!F32) {
-        const double qj = a.q[T * kSuper + jj];
+    {
+        const double qj = F32 ? (double)a.qf[T * kSuper + jj] : a.q[T * kSuper + jj];
         rx = rx * qj;
         ry = ry * qj;
     }
@@ -361,8 +362,8 @@ __device__ __forceinline__ void n3_diag(const N3Row &a, const PairEnv &e, const 
     const int jj = threadIdx.x;
     double rx = ((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj];
     double ry = ((red[0][2 * jj + 1] + red[1][2 * jj + 1]) + red[2][2 * jj + 1]) + red[3][2 * jj + 1];
-    if constexpr (!F32) {
-        const double qj = a.q[S * kSuper + jj];
+    {
+        const double qj = F32 ? (double)a.qf[S * kSuper + jj] : a.q[S * kSuper + jj];
         rx = rx * qj;
         ry = ry * qj;
     }
