@@ -39,21 +39,24 @@ __device__ __forceinline__ Box superblock_box(const double *__restrict__ box, in
 }
 
 // Block-wide exclusive prefix sum of one value per thread (256 threads);
-// *total receives the sum.  Ends with a barrier.
+// *total receives the sum.  A scan within each wave (six shuffle steps),
+// then the four wave totals through LDS: two barriers.  Ends with a barrier
+// (lds may be reused at once).
 __device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t *lds, int32_t *total) {
-    const int tid = threadIdx.x;
-    lds[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const int32_t add = tid >= off ? lds[tid - off] : 0;
-        __syncthreads();
-        lds[tid] += add;
-        __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
     }
-    const int32_t incl = lds[tid];
-    *total = lds[255];
+    if (lane == 63) lds[wave] = x;
     __syncthreads();
-    return incl - v;
+    int32_t base = 0;
+    for (int w = 0; w < wave; ++w) base += lds[w];
+    *total = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return base + x - v;
 }
 
 __global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {

@@ -120,20 +120,20 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
     }
     if (forces & 4) {
         if (ps.first) {
-            // Newton-3: the work items of superblock sb, in item order; four
+            // Newton-3: the work items of superblock sb, in item order; eight
             // items' loads are issued before their adds (the same order of
             // additions, more loads in flight)
             const int32_t it1 = ps.first[sb + 1], li = (int32_t)(i - (int64_t)sb * kSuper);
             int32_t it = ps.first[sb];
-            for (; it + 4 <= it1; it += 4) {
-                double px[4], py[4];
+            for (; it + 8 <= it1; it += 8) {
+                double px[8], py[8];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < 8; ++k) {
                     px[k] = ps.part[(size_t)(it + k) * (2 * kSuper) + li];
                     py[k] = ps.part[(size_t)(it + k) * (2 * kSuper) + kSuper + li];
                 }
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < 8; ++k) {
                     fx = fx + px[k];
                     fy = fy + py[k];
                 }
@@ -153,12 +153,12 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
             const int32_t words = (ps.nsb / 2 + 63) / 64;
             for (int32_t w = 0; w < words; ++w) {
                 unsigned long long mask = fmask[w];
-                // up to four offsets' loads in flight, added in increasing d
+                // up to eight offsets' loads in flight, added in increasing d
                 while (mask) {
-                    int32_t dd[4];
+                    int32_t dd[8];
                     int m = 0;
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
+                    for (int k = 0; k < 8; ++k) {
                         dd[k] = 0;
                         if (mask) {
                             dd[k] = 64 * w + __builtin_ctzll(mask);
@@ -166,15 +166,15 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
                             m = k + 1;
                         }
                     }
-                    double rx[4], ry[4];
+                    double rx[8], ry[8];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
+                    for (int k = 0; k < 8; ++k) {
                         const double *rr = ps.react + (size_t)dd[k] * 2 * npad;
                         rx[k] = k < m ? rr[i] : 0.0;
                         ry[k] = k < m ? rr[npad + i] : 0.0;
                     }
 #pragma unroll
-                    for (int k = 0; k < 4; ++k)
+                    for (int k = 0; k < 8; ++k)
                         if (k < m) {
                             fx = fx + rx[k];
                             fy = fy + ry[k];
