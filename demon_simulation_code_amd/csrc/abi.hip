@@ -234,9 +234,8 @@ int ensure_n3(demon_ctx *c) {
     HIP_TRY(c, hipMemsetAsync(c->d_rflag, 0, sizeof(uint32_t) * half * nsb, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->d_n3i, 0, sizeof(int32_t) * n3_ints(c->npad), c->stream));
     HIP_TRY(c, hipMemsetAsync(c->d_plan, 0, sizeof(N3Plan), c->stream));
-    // e^{-k/64}: the double nearest to a long-double evaluation
     double tab[kExpTabN];
-    for (int k = 0; k < kExpTabN; ++k) tab[k] = (double)std::exp(-(long double)k / 64.0L);
+    exp_table_host(tab);
     HIP_TRY(c, hipMemcpyAsync(c->d_exptab, tab, sizeof tab, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));  // tab is a local
     return DEMON_OK;

@@ -115,11 +115,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int32_t *__restrict__ orig, double *__restrict__ part, double *__restrict__ react,
     uint32_t *__restrict__ rflag, const BCloud *__restrict__ clouds, const BItem *__restrict__ items,
     int32_t n_items, unsigned int *__restrict__ counter, Scalars *__restrict__ sc, int32_t base_flags,
-    pair::ExpCoef ec) {
+    pair::ExpCoef ec, const double *__restrict__ exptab) {
     __shared__ n3::RedLds red;
     __shared__ __align__(16) n3::ScrLds<F32> scr;
     __shared__ n3::TBoxLds tbox;
+    __shared__ double etab[kExpTabN];
     __shared__ int32_t slot;
+    // the FP64 exp is exp_neg_tab on this table; the item loop's first
+    // barrier orders these stores before any read
+    if (!F32)
+        for (int k = (int)threadIdx.x; k < kExpTabN; k += 256) etab[k] = exptab[k];
     for (;;) {
         // the previous item is done with the LDS sums and with slot
         __syncthreads();
@@ -137,7 +142,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const n3::N3Row row{xs + cl.off, ys + cl.off, nc + cl.off, q + cl.off, qf + cl.off, orig + cl.off,
                             box + cl.off / 16, part + cl.part_off, react + cl.react_off,
                             rflag + cl.rflag_off, s, cl.n, cl.npad, base_flags, cl.kappa, cl.rc2p};
-        n3::n3_row<F32, DTC>(row, ec, item.ibase, d0, d1, item.slice, red, scr, tbox);
+        n3::n3_row<F32, DTC>(row, ec, etab, item.ibase, d0, d1, item.slice, red, scr, tbox);
     }
 }
 
@@ -419,6 +424,7 @@ struct demon_batch {
     BItem *d_items = nullptr;
     size_t items_cap = 0;
     unsigned int *d_counters = nullptr;
+    double *exptab = nullptr;    // e^{-k/64}, the Newton-3 form's exp table
     int32_t group_off[kGroups]{}, group_count[kGroups]{};
     double *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *q = nullptr, *m = nullptr;
     StageBuf sb[2]{};
@@ -587,14 +593,14 @@ int enqueue_bpairs(demon_batch *b, const StageBuf &st, int32_t flags, int only_g
         if (cnt == 0) continue;
         const BItem *it = items ? items : b->d_items + b->group_off[g];
         unsigned int *ctr = b->d_counters + g;
-        const pair::ExpCoef ec = pair::exp_coef();
+        const pair::ExpCoef ec = g == kGroupN3 ? pair::exp_tab_coef() : pair::exp_coef();
         if (g == kGroupN3) {
             const dim3 grid((unsigned)grid_for(cnt, 1)), blk(256);
             const bool dtc = (flags & pair::kFlagDtCheck) != 0;
             auto kern = f32 ? (dtc ? k_bpairs_n3<true, true> : k_bpairs_n3<true, false>)
                             : (dtc ? k_bpairs_n3<false, true> : k_bpairs_n3<false, false>);
             hipLaunchKernelGGL(kern, grid, blk, 0, b->stream, st.xs, st.ys, st.nc, st.box, b->q, b->qf, b->orig,
-                               b->part, b->react, b->rflag, b->d_clouds, it, cnt, ctr, b->d_sc, flags, ec);
+                               b->part, b->react, b->rflag, b->d_clouds, it, cnt, ctr, b->d_sc, flags, ec, b->exptab);
             BHIP(b, hipGetLastError());
             continue;
         }
@@ -745,7 +751,8 @@ int demon_batch_create(demon_batch **out, int32_t nclouds, const int64_t *n, con
         hipMalloc((void **)&b->d_block_cloud, sizeof(int32_t) * b->block_cloud.size()) != hipSuccess ||
         hipMalloc((void **)&b->d_clouds, sizeof(BCloud) * (size_t)nclouds) != hipSuccess ||
         hipMalloc((void **)&b->d_sc, sizeof(Scalars) * (size_t)nclouds) != hipSuccess ||
-        hipMalloc((void **)&b->d_counters, sizeof(unsigned int) * kGroups) != hipSuccess)
+        hipMalloc((void **)&b->d_counters, sizeof(unsigned int) * kGroups) != hipSuccess ||
+        hipMalloc((void **)&b->exptab, sizeof(double) * kExpTabN) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_ALLOC, "hipMalloc failed"));
     // every fill and copy goes on the batch's own stream: it is non-blocking,
     // so work on the null stream would not be ordered before its kernels
@@ -770,6 +777,13 @@ int demon_batch_create(demon_batch **out, int32_t nclouds, const int64_t *n, con
     if (hipMemcpyAsync(b->d_block_cloud, b->block_cloud.data(), sizeof(int32_t) * b->block_cloud.size(),
                        hipMemcpyHostToDevice, b->stream) != hipSuccess)
         return bail(bfail(b, DEMON_ERR_HIP, "hipMemcpy failed"));
+    {
+        double tab[kExpTabN];
+        exp_table_host(tab);
+        if (hipMemcpyAsync(b->exptab, tab, sizeof tab, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+            hipStreamSynchronize(b->stream) != hipSuccess)  // tab is a local
+            return bail(bfail(b, DEMON_ERR_HIP, "hipMemcpy failed"));
+    }
     for (int k = 0; k < 2; ++k)
         if (hipEventCreate(&b->ev[k]) != hipSuccess) return bail(bfail(b, DEMON_ERR_HIP, "hipEventCreate failed"));
     // padding: nc = -inf everywhere until clouds are uploaded
@@ -792,7 +806,8 @@ void demon_batch_destroy(demon_batch *b) {
     for (hipEvent_t e : b->tev) (void)hipEventDestroy(e);
     for (void *ptr : {(void *)b->arena, (void *)b->part, (void *)b->react, (void *)b->rflag, (void *)b->orig,
                       (void *)b->d_block_cloud,
-                      (void *)b->d_clouds, (void *)b->d_sc, (void *)b->d_items, (void *)b->d_counters})
+                      (void *)b->d_clouds, (void *)b->d_sc, (void *)b->d_items, (void *)b->d_counters,
+                      (void *)b->exptab})
         if (ptr) (void)hipFree(ptr);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 namespace demon {
 
 // 1/(4*pi*eps0) with eps0 = 8.8541878e-12, as ShieldedCoulombForce.cpp:21.
@@ -85,9 +87,15 @@ struct N3Plan {
     uint32_t next[64];         // item counters of the pair kernel, one per queue (kN3Queues)
 };
 
-// Entries of the Newton-3 pair kernel's exp table, e^{-k/64} for k < 642
+// Entries of the Newton-3 pair kernels' exp table, e^{-k/64} for k < 642
 // (u < 10.03).
 constexpr int kExpTabN = 642;
+
+// The table on the host: each entry the double nearest to a long-double
+// evaluation.
+inline void exp_table_host(double *tab) {
+    for (int k = 0; k < kExpTabN; ++k) tab[k] = (double)std::exp(-(long double)k / 64.0L);
+}
 
 struct N3List {
     int32_t *list;             // [nsb][nsb/2 + 1] live offsets d of each row, increasing (0: the row itself)

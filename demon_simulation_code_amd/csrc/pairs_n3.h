@@ -222,7 +222,7 @@ using RedLds = double[4][2 * kSuper];  // [wave][2 jj + component]
 // Bounding boxes of the superblocks T a row visits, staged in LDS a chunk at
 // a time: one round trip to memory per chunk instead of one per offset,
 // which is what a launch of mostly culled tiles waits on.
-constexpr int kBoxChunk = 128;
+constexpr int kBoxChunk = 32;
 using TBoxLds = Box[kBoxChunk];
 
 // The wave's i block against one 64-j block (first particle jb) in the
@@ -397,13 +397,15 @@ __device__ __forceinline__ void n3_commit_counts(Scalars *sc, int32_t slot_index
 // One workgroup's share in the batched ensemble: row S against superblocks
 // S + d for d in [d0, d1), its i-side sums written to part[slice].  Called by
 // all 256 threads.
+// etab: the exp table e^{-k/64} in LDS (FP64 pair arithmetic), with ec its
+// coefficients (exp_tab_coef).
 template <bool F32, bool DTC>
-__device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_t S, int32_t d0, int32_t d1,
-                                       int32_t slice, RedLds &red, ScrLds<F32> &scr, TBoxLds &tbox) {
+__device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, const double *etab, int32_t S, int32_t d0,
+                                       int32_t d1, int32_t slice, RedLds &red, ScrLds<F32> &scr, TBoxLds &tbox) {
     const double *__restrict__ box = a.box;
     const int32_t n = a.n, npad = a.npad;
     const Radii rad = launch_radii(a.rc2p, a.flags, a.sc->rmax2_bits);
-    const PairEnv e{make_cutoff(a.kappa, a.rc2p), rad, fp32_kl2(a.kappa), DTC, a.orig, ec};
+    const PairEnv e{make_cutoff(a.kappa, a.rc2p), rad, fp32_kl2(a.kappa), DTC, a.orig, ec, etab};
     float smin = __builtin_inff();
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -422,7 +424,7 @@ __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_
     for (int32_t d = d0; d < d1; ++d) {
         if (d == 0) {
             // the superblock with itself, each unordered pair once
-            n3_diag<F32, DTC, false>(a, e, t, ib, ireal, S, ax[0], ay[0], n_near, n_tested, smin, red, scr);
+            n3_diag<F32, DTC, true>(a, e, t, ib, ireal, S, ax[0], ay[0], n_near, n_tested, smin, red, scr);
             continue;
         }
         if ((d - d0) % kBoxChunk == 0 || d == 1) {
@@ -450,7 +452,7 @@ __device__ __forceinline__ void n3_row(const N3Row &a, const ExpCoef &ec, int32_
             if (threadIdx.x == 0) *fl = 0u;
             continue;
         }
-        n3_tile<F32, DTC>(a, e, t, ib, ireal, T, d, ax[0], ay[0], n_near, n_tested, smin, red, scr);
+        n3_tile<F32, DTC, true>(a, e, t, ib, ireal, T, d, ax[0], ay[0], n_near, n_tested, smin, red, scr);
         if (threadIdx.x == 0) *fl = 1u;
     }
     if (DTC) commit_dtmin(smin, &a.sc->dtmin_bits);
