@@ -185,7 +185,7 @@ __device__ __forceinline__ void n3_item(const N3Row &a, const PairEnv &e, int32_
 }
 
 template <bool F32, bool DTC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pairs_n3(N3Row a, N3List l,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F32 ? 6 : 4))) void k_pairs_n3(N3Row a, N3List l,
                                                                                    ExpCoef ec) {
     __shared__ RedLds red;
     __shared__ __align__(16) ScrLds<F32> scr;
