@@ -908,6 +908,11 @@ int demon_batch_download(demon_batch *b, int32_t cloud, double *x, double *y, do
     return DEMON_OK;
 }
 
+// Steps per captured graph: a call of s steps launches ceil(s / 4) graphs;
+// the steps beyond s are no-ops (k_bboundary leaves every cloud inactive once
+// its remaining steps are used up, and every kernel is gated on that).
+constexpr int kBatchGraphSteps = 4;
+
 int demon_batch_steps(demon_batch *b, int64_t steps, double dt) {
     if (!b || steps < 0 || !(dt > 0.0)) return bfail(b, DEMON_ERR_ARG, "bad steps/dt");
     if (!b->uploaded_all) return bfail(b, DEMON_ERR_ARG, "upload every cloud first");
@@ -924,7 +929,8 @@ int demon_batch_steps(demon_batch *b, int64_t steps, double dt) {
         if (!b->graph) {
             hipGraph_t g = nullptr;
             BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            const int rc = enqueue_bstep(b, false);
+            int rc = DEMON_OK;
+            for (int k = 0; k < kBatchGraphSteps && !rc; ++k) rc = enqueue_bstep(b, false);
             const hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) {
                 if (g) (void)hipGraphDestroy(g);
@@ -935,7 +941,7 @@ int demon_batch_steps(demon_batch *b, int64_t steps, double dt) {
             (void)hipGraphDestroy(g);
             if (e2 != hipSuccess) return bfail(b, DEMON_ERR_HIP, "graph instantiate failed");
         }
-        for (int64_t s = 0; s < steps; ++s) BHIP(b, hipGraphLaunch(b->graph, b->stream));
+        for (int64_t s = 0; s < steps; s += kBatchGraphSteps) BHIP(b, hipGraphLaunch(b->graph, b->stream));
     } else {
         for (int64_t s = 0; s < steps; ++s) {
             const int rc = enqueue_bstep(b, b->timing);
