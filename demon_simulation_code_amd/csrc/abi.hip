@@ -182,10 +182,10 @@ size_t part_doubles(int64_t npad) {
 }
 
 // int32 entries of the Newton-3 tile lists and item table: list
-// [nsb][nsb/2 + 1], cnt [nsb], first [nsb + 1], itemrow [max items].
+// [nsb][nsb/2 + 1], cnt [nsb], first [nsb + 1].
 size_t n3_ints(int64_t npad) {
     const size_t nsb = (size_t)(npad / kSuper);
-    return nsb * (nsb / 2 + 1) + nsb + (nsb + 1) + (size_t)n3_max_items(npad);
+    return nsb * (nsb / 2 + 1) + nsb + (nsb + 1);
 }
 
 // Device memory of the Newton-3 reaction buffers, tile lists and plan.
@@ -247,7 +247,6 @@ N3List n3_list(const demon_ctx *c) {
     l.list = c->d_n3i;
     l.cnt = l.list + nsb * (nsb / 2 + 1);
     l.first = l.cnt + nsb;
-    l.itemrow = l.first + nsb + 1;
     l.plan = c->d_plan;
     l.target = (int32_t)(c->n3_slices * nsb);
     l.max_items = (int32_t)n3_max_items(c->npad);

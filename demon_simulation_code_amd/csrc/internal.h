@@ -101,7 +101,6 @@ struct N3List {
     int32_t *list;             // [nsb][nsb/2 + 1] live offsets d of each row, increasing (0: the row itself)
     int32_t *cnt;              // [nsb] live tiles per row
     int32_t *first;            // [nsb + 1] first item of each row (first[nsb] = nitems)
-    int32_t *itemrow;          // [max_items] row of each item
     N3Plan *plan;
     int32_t target;            // items wanted per launch: span = ceil(live tiles / target)
     int32_t max_items;         // >= target + nsb

@@ -10,8 +10,7 @@
 //               itself, first), into list[S].
 //  k_n3_plan    one workgroup: cuts the rows into work items of `span`
 //               consecutive live tiles, span = ceil(live tiles / target), and
-//               writes the item table (first item of each row, row of each
-//               item).
+//               writes the first item of each row.
 //  k_pairs_n3   a persistent grid: every workgroup takes items from atomic
 //               counters until none is left; one item = the row's i
 //               particles in registers against its span of live tiles, the
@@ -116,34 +115,41 @@ __global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {
 }
 
 // One workgroup: cuts the rows' live tiles into items (span = ceil(live /
-// target) consecutive tiles of one row), writes the item table and resets
-// the pair kernel's queue counters.
+// target) consecutive tiles of one row), writes the first item of each row
+// (first[nsb] = items of the launch) and resets the pair kernel's queue
+// counters.
 __global__ __launch_bounds__(256) void k_n3_plan(N3Row a, N3List l) {
-    __shared__ int32_t scan[256];
+    constexpr int kRowsPer = 2 * kMaxHalf / 256;  // rows per thread at the largest nsb
+    __shared__ int32_t scan[4];
     if ((a.flags & kFlagGated) && !a.sc->active) return;
     const int32_t nsb = a.npad / kSuper;
     if (threadIdx.x < kN3Queues) l.plan->next[threadIdx.x] = 0u;
     const int32_t per = (nsb + 255) / 256;
-    const int32_t r0 = min((int32_t)threadIdx.x * per, nsb), r1 = min(r0 + per, nsb);
+    const int32_t r0 = min((int32_t)threadIdx.x * per, nsb);
+    int32_t c[kRowsPer];
     int32_t tiles = 0, rows = 0;
-    for (int32_t r = r0; r < r1; ++r) {
-        const int32_t c = l.cnt[r];
-        tiles += c;
-        rows += c > 0;
+#pragma unroll
+    for (int k = 0; k < kRowsPer; ++k) {
+        c[k] = k < per && r0 + k < nsb ? l.cnt[r0 + k] : 0;
+        tiles += c[k];
+        rows += c[k] > 0;
     }
     int32_t total = 0, real_rows = 0;
     (void)block_exclusive_scan(tiles, scan, &total);
     (void)block_exclusive_scan(rows, scan, &real_rows);
     const int32_t span = max(1, (total + l.target - 1) / l.target);
     int32_t items = 0;
-    for (int32_t r = r0; r < r1; ++r) items += (l.cnt[r] + span - 1) / span;
+#pragma unroll
+    for (int k = 0; k < kRowsPer; ++k) {
+        c[k] = (c[k] + span - 1) / span;
+        items += c[k];
+    }
     int32_t nitems = 0;
     int32_t f = block_exclusive_scan(items, scan, &nitems);
-    for (int32_t r = r0; r < r1; ++r) {
-        const int32_t ni = (l.cnt[r] + span - 1) / span;
-        l.first[r] = f;
-        for (int32_t k = 0; k < ni; ++k) l.itemrow[f + k] = r;
-        f += ni;
+#pragma unroll
+    for (int k = 0; k < kRowsPer; ++k) {
+        if (k < per && r0 + k < nsb) l.first[r0 + k] = f;
+        f += c[k];
     }
     if (threadIdx.x == 0) {
         l.first[nsb] = nitems;
@@ -211,17 +217,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F32 ? 6 : 4
     const int32_t home = (int32_t)(blockIdx.x & 7), sub = (int32_t)((blockIdx.x >> 3) & 7);
     int32_t qk = 8 * home + sub;
     for (;;) {
-        const int32_t i0 = __builtin_amdgcn_readfirstlane(l.first[qk * nsb / kN3Queues]);
-        const int32_t i1 = __builtin_amdgcn_readfirstlane(l.first[(qk + 1) * nsb / kN3Queues]);
+        const int32_t qr0 = qk * nsb / kN3Queues, nr = (qk + 1) * nsb / kN3Queues - qr0;
+        const int32_t i0 = __builtin_amdgcn_readfirstlane(l.first[qr0]);
+        const int32_t i1 = __builtin_amdgcn_readfirstlane(l.first[qr0 + nr]);
         for (;;) {
             __syncthreads();  // the previous item is done with slot and the LDS sums
             if (threadIdx.x == 0) slot = i0 < i1 ? (int32_t)atomicAdd(&l.plan->next[qk], 1u) : 0;
             __syncthreads();
             const int32_t it = i0 + __builtin_amdgcn_readfirstlane(slot);
             if (it >= i1) break;
-            const int32_t S = __builtin_amdgcn_readfirstlane(l.itemrow[it]);
-            const int32_t k0 = (it - __builtin_amdgcn_readfirstlane(l.first[S])) * span;
-            const int32_t nl = __builtin_amdgcn_readfirstlane(l.cnt[S]);
+            // the item's row: the last of the queue's rows (one per lane,
+            // t = rows: the queue's end) whose first item is at most it
+            // (first[] does not decrease); both loads in one round trip
+            const int32_t t = (int32_t)(threadIdx.x & 63);
+            const int32_t fl = t <= nr ? l.first[qr0 + t] : INT32_MAX;
+            const int32_t cl = t < nr ? l.cnt[qr0 + t] : 0;
+            const int32_t ri = __builtin_popcountll(__ballot(fl <= it)) - 1;
+            const int32_t S = qr0 + ri;
+            const int32_t k0 = (it - __builtin_amdgcn_readlane(fl, ri)) * span;
+            const int32_t nl = __builtin_amdgcn_readlane(cl, ri);
             const int32_t k1 = k0 + span < nl ? k0 + span : nl;
             n3_item<F32, DTC>(a, e, S, l.list + (size_t)S * (half + 1), k0, k1, it, n_near, n_tested, smin, red,
                               scr);
@@ -257,8 +271,9 @@ hipError_t n3_row_args(const PairLaunch &a, N3Row *row) {
     const N3List &l = a.lst;
     if (a.npad % kPad || !a.box || !a.react || !a.rflag || a.n > a.npad || (a.fp32pair && !a.qf))
         return hipErrorInvalidValue;
-    if (!l.list || !l.cnt || !l.first || !l.itemrow || !l.plan || !l.exptab || l.target < 1 || l.cus < 1 ||
-        (int64_t)l.max_items < (int64_t)l.target + nsb || nsb / 2 > kMaxHalf)
+    // (a queue's rows, nsb / 64 + 1 at most, are held one per lane)
+    if (!l.list || !l.cnt || !l.first || !l.plan || !l.exptab || l.target < 1 || l.cus < 1 ||
+        (int64_t)l.max_items < (int64_t)l.target + nsb || nsb / 2 > kMaxHalf || nsb / kN3Queues + 1 > 63)
         return hipErrorInvalidValue;
     int32_t flags = 0;
     if (a.dtcheck) flags |= kFlagDtCheck;
