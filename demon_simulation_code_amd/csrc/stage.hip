@@ -213,13 +213,24 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
 // must call it (it ends with a barrier).
 __device__ __forceinline__ void load_flag_mask(const PairSums &ps, unsigned long long *fmask) {
     if (ps.react && threadIdx.x < 64) {
-        const int32_t half = ps.nsb / 2;
+        const int32_t half = ps.nsb / 2, words = (half + 63) / 64;
         const uint32_t *fl = ps.rflag + (size_t)blockIdx.x * half;
-        for (int32_t w = 0; w < (half + 63) / 64; ++w) {
+        // every word's flags in one round trip, without branches (a flag
+        // beyond the row is read from the row's last slot and not
+        // counted), then the ballots
+        uint32_t f[kMaxHalf / 64];
+#pragma unroll
+        for (int w = 0; w < kMaxHalf / 64; ++w) {
             const int32_t k = 64 * w + (int32_t)threadIdx.x;
-            const unsigned long long m = __ballot(k < half && fl[k] != 0u);
-            if (threadIdx.x == 0) fmask[w] = m;
+            const uint32_t v = fl[k < half ? k : half - 1];
+            f[w] = k < half ? v : 0u;
         }
+#pragma unroll
+        for (int w = 0; w < kMaxHalf / 64; ++w)
+            if (w < words) {
+                const unsigned long long m = __ballot(f[w] != 0u);
+                if (threadIdx.x == 0) fmask[w] = m;
+            }
     }
     __syncthreads();
 }
