@@ -1105,21 +1105,26 @@ int demon_batch_tune(demon_batch *b, int32_t reps) {
             BHIP(b, hipMalloc((void **)&d_it, sizeof(BItem) * items.size()));
             BHIP(b, hipMemcpyAsync(d_it, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice,
                                    b->stream));
-            float tot = 0.0f;
+            // one warm-up pass, then `reps` passes back to back between two
+            // events: the GPU's time per pass, without the host's launch
+            // latency between an event and its kernel
+            float ms = 0.0f;
             int ok = DEMON_OK;
             for (int r = 0; r <= reps && !ok; ++r) {
+                if (r == 1) {
+                    if (hipStreamSynchronize(b->stream) != hipSuccess) ok = bfail(b, DEMON_ERR_HIP, "tune sync");
+                    hipEventRecord(b->ev[0], b->stream);
+                }
+                // the persistent kernels' item counters start at zero
                 hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * kGroups, b->stream);
-                hipEventRecord(b->ev[0], b->stream);
-                ok = enqueue_bpairs(b, b->sb[0], 0, shape_index(R), d_it, (int32_t)items.size());
-                hipEventRecord(b->ev[1], b->stream);
-                if (hipStreamSynchronize(b->stream) != hipSuccess) ok = bfail(b, DEMON_ERR_HIP, "tune sync");
-                float ms = 0.0f;
-                hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
-                if (r > 0) tot += ms;  // first run is a warm-up
+                if (!ok) ok = enqueue_bpairs(b, b->sb[0], 0, shape_index(R), d_it, (int32_t)items.size());
             }
+            hipEventRecord(b->ev[1], b->stream);
+            if (hipStreamSynchronize(b->stream) != hipSuccess && !ok) ok = bfail(b, DEMON_ERR_HIP, "tune sync");
+            if (!ok) hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
             (void)hipFree(d_it);
             if (ok) return ok;
-            const float us = tot / reps * 1000.0f;
+            const float us = ms / reps * 1000.0f;
             if (us < best) {
                 best = us;
                 bestR = R;
