@@ -296,15 +296,22 @@ __device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, cons
     }
 }
 
+// Work a caller may put where a tile's waves wait for each other (just
+// before the barrier that ends the tile's pair work): called by every wave
+// as it arrives there.  The default does nothing.
+struct NoHook {
+    __device__ __forceinline__ void operator()() const {}
+};
+
 // Tile (S, T): the four 64-j blocks of superblock T against the wave's i
 // block; adds the i-side forces into ax/ay and writes the reactions on T's
 // 256 particles to react[d - 1] (negated sums over the four waves in a fixed
 // order).  Called by all 256 threads; ib: the wave's i-block box.
-template <bool F32, bool DTC, bool TAB = false>
+template <bool F32, bool DTC, bool TAB = false, class HOOK = NoHook>
 __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const ITile<1> &t, const Box &ib,
                                         bool ireal, int32_t T, int32_t d, double &ax, double &ay,
                                         uint64_t &n_near, uint64_t &n_tested, float &smin, RedLds &red,
-                                        ScrLds<F32> &scr) {
+                                        ScrLds<F32> &scr, const HOOK &hook = HOOK()) {
     const int32_t npad = a.npad;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -315,6 +322,7 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
     for (int jb4 = 0; jb4 < 4; ++jb4)
         n3_jblock<F32, DTC, TAB>(a, e, t, ib, ireal, T * kSuper + jb4 * kBoxBlock, T * kSuper, ax, ay, n_near,
                                  n_tested, smin, scr_rows<F32>(scr, wave), r);
+    hook();
     __syncthreads();
     // the four waves' sums in a fixed order; the reaction is their negation
     // times q_j (the pair sums leave it out; FP32 pairs: the rounded charge)
@@ -337,10 +345,11 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
 // block pairs once.  The reactions meet in LDS and go to the thread that
 // holds the particle as its i (thread jj holds particle 256 S + jj), in a
 // fixed order.  Called by all 256 threads.
-template <bool F32, bool DTC, bool TAB>
+template <bool F32, bool DTC, bool TAB, class HOOK = NoHook>
 __device__ __forceinline__ void n3_diag(const N3Row &a, const PairEnv &e, const ITile<1> &t, const Box &ib,
                                         bool ireal, int32_t S, double &ax, double &ay, uint64_t &n_near,
-                                        uint64_t &n_tested, float &smin, RedLds &red, ScrLds<F32> &scr) {
+                                        uint64_t &n_tested, float &smin, RedLds &red, ScrLds<F32> &scr,
+                                        const HOOK &hook = HOOK()) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // the previous tile's sums have been read by all threads
@@ -358,6 +367,7 @@ __device__ __forceinline__ void n3_diag(const N3Row &a, const PairEnv &e, const 
     if (wave < 2)
         n3_jblock<F32, DTC, TAB>(a, e, t, ib, ireal, S * kSuper + (wave + 2) * kBoxBlock, S * kSuper, ax, ay,
                                  n_near, n_tested, smin, scr_rows<F32>(scr, wave), r);
+    hook();
     __syncthreads();
     const int jj = threadIdx.x;
     double rx = ((red[0][2 * jj] + red[1][2 * jj]) + red[2][2 * jj]) + red[3][2 * jj];

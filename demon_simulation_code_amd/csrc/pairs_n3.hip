@@ -161,12 +161,74 @@ __global__ __launch_bounds__(256) void k_n3_plan(N3Row a, N3List l) {
     }
 }
 
+// The next item of queue qk (rows qr0 .. qr0 + nr - 1, items i0 .. i1 - 1):
+// one atomic claims a slot, and the item's row is the last of the queue's
+// rows (one per lane, t = nr: the queue's end) whose first item is at most
+// it (first[] does not decrease).  Run by one whole wave; lane 0 leaves
+// (it, S, k0, k1) in nx (it >= i1: the queue is exhausted).
+struct QueueAt {
+    int32_t qk, qr0, nr, i0, i1, span;
+};
+
+struct QueueArrays {
+    const int32_t *first, *cnt;
+    uint32_t *next;
+};
+
+__device__ __forceinline__ void claim_item(const QueueArrays &qa, const QueueAt &q, int32_t *nx) {
+    const int lane = threadIdx.x & 63;
+    uint32_t s = 0u;
+    if (lane == 0 && q.i0 < q.i1) s = atomicAdd(&qa.next[q.qk], 1u);
+    const int32_t it = q.i0 + (int32_t)__builtin_amdgcn_readfirstlane(s);
+    int32_t S = 0, k0 = 0, k1 = 0;
+    if (it < q.i1) {
+        const int32_t fl = lane <= q.nr ? qa.first[q.qr0 + lane] : INT32_MAX;
+        const int32_t cl = lane < q.nr ? qa.cnt[q.qr0 + lane] : 0;
+        const int32_t ri = __builtin_popcountll(__ballot(fl <= it)) - 1;
+        S = q.qr0 + ri;
+        k0 = (it - __builtin_amdgcn_readlane(fl, ri)) * q.span;
+        const int32_t nl = __builtin_amdgcn_readlane(cl, ri);
+        k1 = k0 + q.span < nl ? k0 + q.span : nl;
+    }
+    if (lane == 0) {
+        nx[0] = it;
+        nx[1] = S;
+        nx[2] = k0;
+        nx[3] = k1;
+    }
+}
+
+// On an item's last tile, the first wave to finish its share claims the next
+// item while the others are still at work (the tile's shares differ: the
+// wait at the tile's barrier is where this costs nothing).  Used by the
+// FP32-pair kernel, where it made a lattice launch 5% faster; in the FP64
+// kernel, whose stage-1 variant is at its register limit, it cost 1%, and
+// the FP64 kernel claims at the item boundary instead (ClaimAtBoundary).
+struct ClaimAtBarrier {
+    QueueArrays qa;
+    QueueAt q;
+    int32_t *flag, *nx;
+    bool last;
+    __device__ __forceinline__ void operator()() const {
+        if (!last) return;
+        int32_t first = 0;
+        if ((threadIdx.x & 63) == 0) first = atomicAdd(flag, 1) == 0 ? 1 : 0;
+        if (__builtin_amdgcn_readfirstlane(first)) claim_item(qa, q, nx);
+    }
+};
+
+struct ClaimAtBoundary {
+    bool last;
+    __device__ __forceinline__ void operator()() const {}
+};
+
 // One item: row S's i particles (wave w: block 4 S + w) against the live
-// tiles lst[k0 .. k1); the i-side sums go to part[it].
-template <bool F32, bool DTC>
+// tiles lst[k0 .. k1); the i-side sums go to part[it].  claim runs at the
+// barrier of each tile's waves, with claim.last set on the last tile.
+template <bool F32, bool DTC, class CLAIM>
 __device__ __forceinline__ void n3_item(const N3Row &a, const PairEnv &e, int32_t S, const int32_t *lst,
                                         int32_t k0, int32_t k1, int32_t it, uint64_t &n_near, uint64_t &n_tested,
-                                        float &smin, RedLds &red, ScrLds<F32> &scr) {
+                                        float &smin, RedLds &red, ScrLds<F32> &scr, CLAIM claim) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int32_t n = a.n, nsb = a.npad / kSuper;
@@ -177,13 +239,14 @@ __device__ __forceinline__ void n3_item(const N3Row &a, const PairEnv &e, int32_
     double ax[1] = {0.0}, ay[1] = {0.0};
     for (int32_t k = k0; k < k1; ++k) {
         const int32_t d = __builtin_amdgcn_readfirstlane(lst[k]);
+        claim.last = k + 1 == k1;
         if (d == 0) {
             // the superblock with itself, each unordered pair once
-            n3_diag<F32, DTC, true>(a, e, t, ib, ireal, S, ax[0], ay[0], n_near, n_tested, smin, red, scr);
+            n3_diag<F32, DTC, true>(a, e, t, ib, ireal, S, ax[0], ay[0], n_near, n_tested, smin, red, scr, claim);
             continue;
         }
         const int32_t T = S + d < nsb ? S + d : S + d - nsb;
-        n3_tile<F32, DTC, true>(a, e, t, ib, ireal, T, d, ax[0], ay[0], n_near, n_tested, smin, red, scr);
+        n3_tile<F32, DTC, true>(a, e, t, ib, ireal, T, d, ax[0], ay[0], n_near, n_tested, smin, red, scr, claim);
     }
     double *px = a.part + (size_t)it * (2 * kSuper);
     px[wave * 64 + lane] = ax[0];
@@ -209,36 +272,65 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F32 ? 6 : 4
     uint64_t n_near = 0, n_tested = 0;
     // Work queues: workgroup g runs on XCD k = g mod 8 and starts with queue
     // 8 k + s, s = (g / 8) mod 8.  One item per atomic; sixty-four counters,
-    // since the requests to one counter serialise.  When its queue is
-    // exhausted, the workgroup reads all sixty-four counters at once (one
-    // per lane of wave 0) and moves to the first queue with items left, in
-    // the order: the other queues of its XCD, then those of the other XCDs.
+    // since the requests to one counter serialise.  FP64: thread 0 claims
+    // each item at the item boundary.  FP32 pairs: wave 0 claims a queue's first
+    // item, the first wave to finish its share of the current item's last
+    // tile every later one (the next item, its row and tile range wait in
+    // nx).  When its queue is exhausted, the
+    // workgroup reads all sixty-four counters at once (one per lane of wave
+    // 0) and moves to the first queue with items left, in the order: the
+    // other queues of its XCD, then those of the other XCDs.
     __shared__ int32_t slot;
+    __shared__ int32_t nx[4], claim_flag;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int32_t home = (int32_t)(blockIdx.x & 7), sub = (int32_t)((blockIdx.x >> 3) & 7);
     int32_t qk = 8 * home + sub;
+    const QueueArrays qarr{l.first, l.cnt, l.plan->next};
     for (;;) {
-        const int32_t qr0 = qk * nsb / kN3Queues, nr = (qk + 1) * nsb / kN3Queues - qr0;
-        const int32_t i0 = __builtin_amdgcn_readfirstlane(l.first[qr0]);
-        const int32_t i1 = __builtin_amdgcn_readfirstlane(l.first[qr0 + nr]);
-        for (;;) {
-            __syncthreads();  // the previous item is done with slot and the LDS sums
-            if (threadIdx.x == 0) slot = i0 < i1 ? (int32_t)atomicAdd(&l.plan->next[qk], 1u) : 0;
+        QueueAt qa;
+        qa.qk = qk;
+        qa.qr0 = qk * nsb / kN3Queues;
+        qa.nr = (qk + 1) * nsb / kN3Queues - qa.qr0;
+        qa.i0 = __builtin_amdgcn_readfirstlane(l.first[qa.qr0]);
+        qa.i1 = __builtin_amdgcn_readfirstlane(l.first[qa.qr0 + qa.nr]);
+        qa.span = span;
+        if constexpr (F32) {
+            __syncthreads();  // the previous item and the queue scan are done with nx and slot
+            if (wave == 0) claim_item(qarr, qa, nx);
             __syncthreads();
-            const int32_t it = i0 + __builtin_amdgcn_readfirstlane(slot);
-            if (it >= i1) break;
-            // the item's row: the last of the queue's rows (one per lane,
-            // t = rows: the queue's end) whose first item is at most it
-            // (first[] does not decrease); both loads in one round trip
-            const int32_t t = (int32_t)(threadIdx.x & 63);
-            const int32_t fl = t <= nr ? l.first[qr0 + t] : INT32_MAX;
-            const int32_t cl = t < nr ? l.cnt[qr0 + t] : 0;
-            const int32_t ri = __builtin_popcountll(__ballot(fl <= it)) - 1;
-            const int32_t S = qr0 + ri;
-            const int32_t k0 = (it - __builtin_amdgcn_readlane(fl, ri)) * span;
-            const int32_t nl = __builtin_amdgcn_readlane(cl, ri);
-            const int32_t k1 = k0 + span < nl ? k0 + span : nl;
-            n3_item<F32, DTC>(a, e, S, l.list + (size_t)S * (half + 1), k0, k1, it, n_near, n_tested, smin, red,
-                              scr);
+            for (;;) {
+                const int32_t it = __builtin_amdgcn_readfirstlane(nx[0]);
+                if (it >= qa.i1) break;
+                const int32_t S = __builtin_amdgcn_readfirstlane(nx[1]);
+                const int32_t k0 = __builtin_amdgcn_readfirstlane(nx[2]);
+                const int32_t k1 = __builtin_amdgcn_readfirstlane(nx[3]);
+                if (threadIdx.x == 0) claim_flag = 0;
+                // every wave has read nx (the claim during this item
+                // rewrites it), and the previous item is done with the LDS
+                // sums
+                __syncthreads();
+                n3_item<F32, DTC>(a, e, S, l.list + (size_t)S * (half + 1), k0, k1, it, n_near, n_tested, smin,
+                                  red, scr, ClaimAtBarrier{qarr, qa, &claim_flag, nx, false});
+            }
+        } else {
+            for (;;) {
+                __syncthreads();  // the previous item is done with slot and the LDS sums
+                if (threadIdx.x == 0) slot = qa.i0 < qa.i1 ? (int32_t)atomicAdd(&l.plan->next[qk], 1u) : 0;
+                __syncthreads();
+                const int32_t it = qa.i0 + __builtin_amdgcn_readfirstlane(slot);
+                if (it >= qa.i1) break;
+                // the item's row, as claim_item finds it
+                const int32_t t = (int32_t)(threadIdx.x & 63);
+                const int32_t fl = t <= qa.nr ? l.first[qa.qr0 + t] : INT32_MAX;
+                const int32_t cl = t < qa.nr ? l.cnt[qa.qr0 + t] : 0;
+                const int32_t ri = __builtin_popcountll(__ballot(fl <= it)) - 1;
+                const int32_t S = qa.qr0 + ri;
+                const int32_t k0 = (it - __builtin_amdgcn_readlane(fl, ri)) * span;
+                const int32_t nl = __builtin_amdgcn_readlane(cl, ri);
+                const int32_t k1 = k0 + span < nl ? k0 + span : nl;
+                n3_item<F32, DTC>(a, e, S, l.list + (size_t)S * (half + 1), k0, k1, it, n_near, n_tested, smin,
+                                  red, scr, ClaimAtBoundary{false});
+            }
         }
         // the next queue with items left (a stale read only costs one more
         // round); the choice is wave 0's, broadcast through LDS
