@@ -152,7 +152,7 @@ template <bool F32, bool CHECK, bool DTC, bool TAB = false>
 __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[kChunkN3],
                                          const double (&cy)[kChunkN3], const PairVal<F32> (&cq)[kChunkN3],
                                          const ChunkJ &cj, const PairEnv &e, uint64_t &n_near, float &smin,
-                                         double &ax, double &ay, double *scr, double *red) {
+                                         double &ax, double &ay, double *scr, double *red, bool dtc_blk) {
     using V = PairVal<F32>;
     V f[2 * kChunkN3];
     uint64_t hits = 0;  // lanes with some pair of the chunk inside the cutoff (CHECK)
@@ -185,7 +185,7 @@ __device__ __forceinline__ void chunk_n3(const ITile<1> &t, const double (&cx)[k
         }
         f[u] = fx;
         f[kChunkN3 + u] = fy;
-        if (DTC) pair_dtcheck(t.xi[0], t.yi[0], cx[u], cy[u], r2, t.ibase, cj.j[u], e, smin);
+        if (DTC && dtc_blk) pair_dtcheck(t.xi[0], t.yi[0], cx[u], cy[u], r2, t.ibase, cj.j[u], e, smin);
     }
     // No pair of the chunk inside the cutoff: every f is zero, and the
     // wave's sums of this chunk's j keep the zeros the tile stored in red
@@ -239,8 +239,12 @@ __device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, cons
     const double *__restrict__ nc = a.nc;
     const double *__restrict__ q = a.q;
     const float *__restrict__ qf = a.qf;
-    const BlockClass cls = classify(ib, a.box + 4 * (jb / kBoxBlock), ireal, jb + kBoxBlock <= a.n, e.rad, e.cut);
+    const double *jbx = a.box + 4 * (jb / kBoxBlock);
+    const BlockClass cls = classify(ib, jbx, ireal, jb + kBoxBlock <= a.n, e.rad, e.cut);
     if (cls == kSkip) return;
+    // the stage-1 dt check only where the boxes come within its radius (no
+    // pair of blocks farther apart can be that close)
+    const bool dtc_blk = DTC && box_gap2(ib, Box{jbx[0], jbx[1], jbx[2], jbx[3]}) <= e.rad.dt2;
     if (cls == kDense) {
         n_tested += 2 * 64 * kBoxBlock;
         n_near += 2 * 64 * kBoxBlock;
@@ -255,7 +259,7 @@ __device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, cons
                 else cq[u] = ld_uniform(q + j0 + u);
             }
             const ChunkJ cj{{j0, j0 + 1, j0 + 2, j0 + 3}, j0 - t0 + lane_chunk_u()};
-            chunk_n3<F32, false, DTC, TAB>(t, cx, cy, cq, cj, e, n_near, smin, ax, ay, scr, red);
+            chunk_n3<F32, false, DTC, TAB>(t, cx, cy, cq, cj, e, n_near, smin, ax, ay, scr, red, dtc_blk);
         }
         return;
     }
@@ -292,7 +296,7 @@ __device__ __forceinline__ void n3_jblock(const N3Row &a, const PairEnv &e, cons
         }
         if (!hit) continue;
         const ChunkJ cj{{j0, j0 + 1, j0 + 2, j0 + 3}, j0 - t0 + lane_chunk_u()};
-        chunk_n3<F32, true, DTC, TAB>(t, cx, cy, cq, cj, e, n_near, smin, ax, ay, scr, red);
+        chunk_n3<F32, true, DTC, TAB>(t, cx, cy, cq, cj, e, n_near, smin, ax, ay, scr, red, dtc_blk);
     }
 }
 
