@@ -80,6 +80,10 @@ struct PairSums {
 // Work queues of the Newton-3 pair kernel: queue 8 k + s holds the items of
 // rows (8 k + s) nsb/64 .. (8 k + s + 1) nsb/64 - 1, for XCD k.
 constexpr int kN3Queues = 64;
+// A row's live tile list holds the offsets d; kN3Swap set: the tile's i side
+// is superblock T's blocks (the pair kernel honours it for one-tile items).
+constexpr int32_t kN3Swap = 1 << 30;
+constexpr int32_t kN3Offset = kN3Swap - 1;
 
 struct N3Plan {
     int32_t span;              // live tiles per item (the last item of a row may hold fewer)

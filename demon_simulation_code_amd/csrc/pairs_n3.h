@@ -310,12 +310,14 @@ struct NoHook {
 // Tile (S, T): the four 64-j blocks of superblock T against the wave's i
 // block; adds the i-side forces into ax/ay and writes the reactions on T's
 // 256 particles to react[d - 1] (negated sums over the four waves in a fixed
-// order).  Called by all 256 threads; ib: the wave's i-block box.
+// order), or, given jdst, to jdst[jj] and jdst[jstr + jj].  Called by all
+// 256 threads; ib: the wave's i-block box.
 template <bool F32, bool DTC, bool TAB = false, class HOOK = NoHook>
 __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const ITile<1> &t, const Box &ib,
                                         bool ireal, int32_t T, int32_t d, double &ax, double &ay,
                                         uint64_t &n_near, uint64_t &n_tested, float &smin, RedLds &red,
-                                        ScrLds<F32> &scr, const HOOK &hook = HOOK()) {
+                                        ScrLds<F32> &scr, const HOOK &hook = HOOK(), double *jdst = nullptr,
+                                        int64_t jstr = 0) {
     const int32_t npad = a.npad;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -337,6 +339,11 @@ __device__ __forceinline__ void n3_tile(const N3Row &a, const PairEnv &e, const 
         const double qj = F32 ? (double)a.qf[T * kSuper + jj] : a.q[T * kSuper + jj];
         rx = rx * qj;
         ry = ry * qj;
+    }
+    if (jdst) {
+        jdst[jj] = -rx;
+        jdst[jstr + jj] = -ry;
+        return;
     }
     double *rr = a.react + (size_t)(d - 1) * 2 * npad;
     rr[T * kSuper + jj] = -rx;

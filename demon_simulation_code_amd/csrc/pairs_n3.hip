@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {
     __syncthreads();
     for (int32_t d0 = 1; d0 <= half; d0 += 256) {
         const int32_t d = d0 + (int32_t)threadIdx.x;
-        bool live = false;
+        bool live = false, swap = false;
         if (d <= half) {
             const int32_t T = S + d < nsb ? S + d : S + d - nsb;
             // for even nsb the pair (S, S + nsb/2) is reached from both rows
@@ -88,12 +88,26 @@ __global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {
             if (owned && sreal && box_gap2(sbS, superblock_box(a.box, T)) <= rad.rf2) {
                 // live when some block pair is within reach: otherwise the
                 // pair kernel would skip all sixteen (the same box test)
+                int32_t rowc[4] = {0, 0, 0, 0}, colmax = 0;
 #pragma unroll
                 for (int kj = 0; kj < 4; ++kj) {
                     const Box jb = box_union(empty_box(), a.box + 4 * (T * 4 + kj));
+                    int32_t colc = 0;
 #pragma unroll
-                    for (int ki = 0; ki < 4; ++ki) live = live || box_gap2(ibS[ki], jb) <= rad.rf2;
+                    for (int ki = 0; ki < 4; ++ki) {
+                        const int32_t m = box_gap2(ibS[ki], jb) <= rad.rf2 ? 1 : 0;
+                        rowc[ki] += m;
+                        colc += m;
+                    }
+                    colmax = max(colmax, colc);
                 }
+                const int32_t rowmax = max(max(rowc[0], rowc[1]), max(rowc[2], rowc[3]));
+                live = rowmax > 0;
+                // a tile takes as long as its busiest wave, and wave w takes
+                // block w of the i-side superblock: T's blocks are the i side
+                // when the busiest of them has fewer live block pairs than
+                // the busiest of S's
+                swap = colmax < rowmax;
             }
             a.rflag[(size_t)T * half + (d - 1)] = live ? 1u : 0u;
         }
@@ -105,7 +119,7 @@ __global__ __launch_bounds__(256) void k_n3_tiles(N3Row a, N3List l) {
         if (live) {
             const int32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            lst[off + below] = d;
+            lst[off + below] = d | (swap ? kN3Swap : 0);
         }
         __syncthreads();
         if (threadIdx.x == 0) base_s += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
@@ -165,14 +179,16 @@ __global__ __launch_bounds__(256) void k_n3_plan(N3Row a, N3List l) {
 // one atomic claims a slot, and the item's row is the last of the queue's
 // rows (one per lane, t = nr: the queue's end) whose first item is at most
 // it (first[] does not decrease).  Run by one whole wave; lane 0 leaves
-// (it, S, k0, k1) in nx (it >= i1: the queue is exhausted).
+// (it, S, k0, k1, e) in nx (it >= i1: the queue is exhausted), e the list
+// entry of an item of one swapped tile, else 0.
 struct QueueAt {
     int32_t qk, qr0, nr, i0, i1, span;
 };
 
 struct QueueArrays {
-    const int32_t *first, *cnt;
+    const int32_t *first, *cnt, *list;
     uint32_t *next;
+    int32_t stride;  // of list: nsb/2 + 1
 };
 
 __device__ __forceinline__ void claim_item(const QueueArrays &qa, const QueueAt &q, int32_t *nx) {
@@ -180,7 +196,7 @@ __device__ __forceinline__ void claim_item(const QueueArrays &qa, const QueueAt 
     uint32_t s = 0u;
     if (lane == 0 && q.i0 < q.i1) s = atomicAdd(&qa.next[q.qk], 1u);
     const int32_t it = q.i0 + (int32_t)__builtin_amdgcn_readfirstlane(s);
-    int32_t S = 0, k0 = 0, k1 = 0;
+    int32_t S = 0, k0 = 0, k1 = 0, ent = 0;
     if (it < q.i1) {
         const int32_t fl = lane <= q.nr ? qa.first[q.qr0 + lane] : INT32_MAX;
         const int32_t cl = lane < q.nr ? qa.cnt[q.qr0 + lane] : 0;
@@ -189,12 +205,17 @@ __device__ __forceinline__ void claim_item(const QueueArrays &qa, const QueueAt 
         k0 = (it - __builtin_amdgcn_readlane(fl, ri)) * q.span;
         const int32_t nl = __builtin_amdgcn_readlane(cl, ri);
         k1 = k0 + q.span < nl ? k0 + q.span : nl;
+        if (k1 == k0 + 1) {
+            ent = __builtin_amdgcn_readfirstlane(qa.list[(size_t)S * qa.stride + k0]);
+            if (!(ent & kN3Swap)) ent = 0;
+        }
     }
     if (lane == 0) {
         nx[0] = it;
         nx[1] = S;
         nx[2] = k0;
         nx[3] = k1;
+        nx[4] = ent;
     }
 }
 
@@ -225,20 +246,36 @@ struct ClaimAtBoundary {
 // One item: row S's i particles (wave w: block 4 S + w) against the live
 // tiles lst[k0 .. k1); the i-side sums go to part[it].  claim runs at the
 // barrier of each tile's waves, with claim.last set on the last tile.
+// swp: the item's one tile's list entry when it is swapped (else 0): then
+// T's particles are the i side, their sums go to the tile's reaction row
+// react[d - 1] and S's to part[it], where the stage kernel reads each
+// superblock's share in either orientation.
 template <bool F32, bool DTC, class CLAIM>
 __device__ __forceinline__ void n3_item(const N3Row &a, const PairEnv &e, int32_t S, const int32_t *lst,
-                                        int32_t k0, int32_t k1, int32_t it, uint64_t &n_near, uint64_t &n_tested,
-                                        float &smin, RedLds &red, ScrLds<F32> &scr, CLAIM claim) {
+                                        int32_t k0, int32_t k1, int32_t it, int32_t swp, uint64_t &n_near,
+                                        uint64_t &n_tested, float &smin, RedLds &red, ScrLds<F32> &scr,
+                                        CLAIM claim) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int32_t n = a.n, nsb = a.npad / kSuper;
-    const int32_t ibase = S * kSuper + wave * 64 + lane;
+    const int32_t dsw = swp & kN3Offset;
+    const int32_t isb = swp ? (S + dsw < nsb ? S + dsw : S + dsw - nsb) : S;
+    const int32_t ibase = isb * kSuper + wave * 64 + lane;
     const ITile<1> t = make_itile<1>(a.xs, a.ys, a.nc, a.q, n, ibase, e.rad.rf2);
     const Box ib = box_union(empty_box(), a.box + 4 * (t.wlo / kBoxBlock));
     const bool ireal = t.whi <= n;
     double ax[1] = {0.0}, ay[1] = {0.0};
+    if (swp) {
+        claim.last = true;
+        n3_tile<F32, DTC, true>(a, e, t, ib, ireal, S, dsw, ax[0], ay[0], n_near, n_tested, smin, red, scr, claim,
+                                a.part + (size_t)it * (2 * kSuper), kSuper);
+        double *rr = a.react + (size_t)(dsw - 1) * 2 * a.npad + (size_t)isb * kSuper;
+        rr[wave * 64 + lane] = ax[0];
+        rr[a.npad + wave * 64 + lane] = ay[0];
+        return;
+    }
     for (int32_t k = k0; k < k1; ++k) {
-        const int32_t d = __builtin_amdgcn_readfirstlane(lst[k]);
+        const int32_t d = __builtin_amdgcn_readfirstlane(lst[k]) & kN3Offset;
         claim.last = k + 1 == k1;
         if (d == 0) {
             // the superblock with itself, each unordered pair once
@@ -272,7 +309,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F32 ? 6 : 4
     uint64_t n_near = 0, n_tested = 0;
     // Work queues: workgroup g runs on XCD k = g mod 8 and starts with queue
     // 8 k + s, s = (g / 8) mod 8.  One item per atomic; sixty-four counters,
-    // since the requests to one counter serialise.  FP64: thread 0 claims
+    // since the requests to one counter serialise.  FP64: wave 0 claims
     // each item at the item boundary.  FP32 pairs: wave 0 claims a queue's first
     // item, the first wave to finish its share of the current item's last
     // tile every later one (the next item, its row and tile range wait in
@@ -281,11 +318,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F32 ? 6 : 4
     // 0) and moves to the first queue with items left, in the order: the
     // other queues of its XCD, then those of the other XCDs.
     __shared__ int32_t slot;
-    __shared__ int32_t nx[4], claim_flag;
+    __shared__ int32_t nx[5], claim_flag;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int32_t home = (int32_t)(blockIdx.x & 7), sub = (int32_t)((blockIdx.x >> 3) & 7);
     int32_t qk = 8 * home + sub;
-    const QueueArrays qarr{l.first, l.cnt, l.plan->next};
+    const QueueArrays qarr{l.first, l.cnt, l.list, l.plan->next, half + 1};
     for (;;) {
         QueueAt qa;
         qa.qk = qk;
@@ -304,32 +341,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F32 ? 6 : 4
                 const int32_t S = __builtin_amdgcn_readfirstlane(nx[1]);
                 const int32_t k0 = __builtin_amdgcn_readfirstlane(nx[2]);
                 const int32_t k1 = __builtin_amdgcn_readfirstlane(nx[3]);
+                const int32_t swp = __builtin_amdgcn_readfirstlane(nx[4]);
                 if (threadIdx.x == 0) claim_flag = 0;
                 // every wave has read nx (the claim during this item
                 // rewrites it), and the previous item is done with the LDS
                 // sums
                 __syncthreads();
-                n3_item<F32, DTC>(a, e, S, l.list + (size_t)S * (half + 1), k0, k1, it, n_near, n_tested, smin,
-                                  red, scr, ClaimAtBarrier{qarr, qa, &claim_flag, nx, false});
+                n3_item<F32, DTC>(a, e, S, l.list + (size_t)S * (half + 1), k0, k1, it, swp, n_near, n_tested,
+                                  smin, red, scr, ClaimAtBarrier{qarr, qa, &claim_flag, nx, false});
             }
         } else {
             for (;;) {
-                __syncthreads();  // the previous item is done with slot and the LDS sums
-                if (threadIdx.x == 0) slot = qa.i0 < qa.i1 ? (int32_t)atomicAdd(&l.plan->next[qk], 1u) : 0;
+                __syncthreads();  // the previous item is done with nx and the LDS sums
+                if (wave == 0) claim_item(qarr, qa, nx);
                 __syncthreads();
-                const int32_t it = qa.i0 + __builtin_amdgcn_readfirstlane(slot);
+                const int32_t it = __builtin_amdgcn_readfirstlane(nx[0]);
                 if (it >= qa.i1) break;
-                // the item's row, as claim_item finds it
-                const int32_t t = (int32_t)(threadIdx.x & 63);
-                const int32_t fl = t <= qa.nr ? l.first[qa.qr0 + t] : INT32_MAX;
-                const int32_t cl = t < qa.nr ? l.cnt[qa.qr0 + t] : 0;
-                const int32_t ri = __builtin_popcountll(__ballot(fl <= it)) - 1;
-                const int32_t S = qa.qr0 + ri;
-                const int32_t k0 = (it - __builtin_amdgcn_readlane(fl, ri)) * span;
-                const int32_t nl = __builtin_amdgcn_readlane(cl, ri);
-                const int32_t k1 = k0 + span < nl ? k0 + span : nl;
-                n3_item<F32, DTC>(a, e, S, l.list + (size_t)S * (half + 1), k0, k1, it, n_near, n_tested, smin,
-                                  red, scr, ClaimAtBoundary{false});
+                const int32_t S = __builtin_amdgcn_readfirstlane(nx[1]);
+                const int32_t k0 = __builtin_amdgcn_readfirstlane(nx[2]);
+                const int32_t k1 = __builtin_amdgcn_readfirstlane(nx[3]);
+                const int32_t swp = __builtin_amdgcn_readfirstlane(nx[4]);
+                n3_item<F32, DTC>(a, e, S, l.list + (size_t)S * (half + 1), k0, k1, it, swp, n_near, n_tested,
+                                  smin, red, scr, ClaimAtBoundary{false});
             }
         }
         // the next queue with items left (a stale read only costs one more
