@@ -375,6 +375,33 @@ struct PairEnv {
     const double *etab = nullptr;  // e^{-k/64} in LDS (kernels built with TAB)
 };
 
+// 1/sqrt(r2) after one Newton step and e^{-kappa r} from the LDS table
+// (exp_neg_tab), with the table read issued before the refinement: its index
+// n = rint(64 kappa t) comes from the first estimate t = r2 y0 (relative
+// error 2^-24.2, so n may differ from rint(64 kappa r) within 5e-7 of a
+// half-step), and d = kappa r - n/64 uses the refined r, so |d| stays within
+// 1/128 + 5.2e-7 (kappa r < 10), where the degree-4 interpolant keeps its
+// 1.5e-14 (an interpolant's error grows like the fifth power of |d| beyond
+// its interval: by 3e-4 relative here).
+struct InvSqrtExp {
+    double y, e;  // 1/sqrt(r2), e^{-kappa sqrt(r2)}
+};
+__device__ __forceinline__ InvSqrtExp inv_sqrt_exp_tab(double r2, double kappa, const double *__restrict__ tab,
+                                                       const ExpCoef &tc) {
+    const double y0 = __builtin_amdgcn_rsq(r2);
+    const double t = r2 * y0;
+    const double m = fma(t, kappa * tc.c[6], tc.c[7]);
+    const double te = tab[__double2loint(m)];
+    const double c = 0.5 * fma(-t, y0, 1.0);
+    const double y = fma(y0, c, y0), r = fma(t, c, t);
+    const double d = fma(r, kappa, fma(m, tc.c[8], tc.c[5]));
+    double p = tc.c[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) p = fma(p, d, tc.c[k]);
+    p = fma(p, d, 1.0);
+    return InvSqrtExp{y, te * p};
+}
+
 // The pair scale without the factor q_j, C q_i (1 + u) e^-u / r^3, with
 // the launch's exp: exp_neg_tab on the LDS table (TAB; with the Newton-step
 // reciprocal square root), where (1 + u)/r^3 = y^3 + kappa r y^3 is formed
@@ -383,9 +410,9 @@ struct PairEnv {
 template <bool TAB>
 __device__ __forceinline__ double pair_scale_noq(double r2, double cqi, const PairEnv &e) {
     if constexpr (TAB) {
-        const InvSqrt s = inv_sqrt_fp64_newton(r2);
+        const InvSqrtExp s = inv_sqrt_exp_tab(r2, e.cut.kappa, e.etab, e.ec);
         const double g = (s.y * s.y) * (s.y + e.cut.kappa);
-        return (cqi * g) * exp_neg_tab(s.r, e.cut.kappa, e.etab, e.ec);
+        return (cqi * g) * s.e;
     } else {
         const InvSqrt s = inv_sqrt_fp64(r2);
         const double u = s.r * e.cut.kappa;
@@ -399,9 +426,9 @@ __device__ __forceinline__ double pair_scale_noq(double r2, double cqi, const Pa
 template <bool TAB>
 __device__ __forceinline__ double pair_scale_env(double r2, double cqi, double qj, const PairEnv &e) {
     if constexpr (TAB) {
-        const InvSqrt s = inv_sqrt_fp64_newton(r2);
+        const InvSqrtExp s = inv_sqrt_exp_tab(r2, e.cut.kappa, e.etab, e.ec);
         const double g = (s.y * s.y) * (s.y + e.cut.kappa);
-        return ((cqi * qj) * g) * exp_neg_tab(s.r, e.cut.kappa, e.etab, e.ec);
+        return ((cqi * qj) * g) * s.e;
     } else {
         return pair_scale_fp64(r2, cqi, qj, e.cut.kappa, e.ec);
     }
