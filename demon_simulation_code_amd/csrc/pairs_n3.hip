@@ -176,13 +176,20 @@ __global__ __launch_bounds__(256) void k_n3_plan(N3Row a, N3List l) {
 }
 
 // The next item of queue qk (rows qr0 .. qr0 + nr - 1, items i0 .. i1 - 1):
-// one atomic claims a slot, and the item's row is the last of the queue's
+// one atomic claims a slot.  Slots go to items in row order except that the
+// first half of the last row's items come first: a row's items start with
+// its costliest tiles (the superblock with itself, then the nearest ones),
+// and the last row's were among the last items a launch waited on (0.6%
+// faster on the imploded cloud, no change on the lattice, where the last
+// items are also those taken from other queues).  The item's row is the last
+// of the queue's
 // rows (one per lane, t = nr: the queue's end) whose first item is at most
 // it (first[] does not decrease).  Run by one whole wave; lane 0 leaves
 // (it, S, k0, k1, e) in nx (it >= i1: the queue is exhausted), e the list
 // entry of an item of one swapped tile, else 0.
 struct QueueAt {
     int32_t qk, qr0, nr, i0, i1, span;
+    int32_t il;  // the first item of the queue's last row
 };
 
 struct QueueArrays {
@@ -195,7 +202,9 @@ __device__ __forceinline__ void claim_item(const QueueArrays &qa, const QueueAt 
     const int lane = threadIdx.x & 63;
     uint32_t s = 0u;
     if (lane == 0 && q.i0 < q.i1) s = atomicAdd(&qa.next[q.qk], 1u);
-    const int32_t it = q.i0 + (int32_t)__builtin_amdgcn_readfirstlane(s);
+    const int32_t sl = (int32_t)__builtin_amdgcn_readfirstlane(s);
+    const int32_t h = (q.i1 - q.il + 1) >> 1, mid = q.il - q.i0;
+    const int32_t it = sl < h ? q.il + sl : sl - h < mid ? q.i0 + (sl - h) : q.il + (sl - mid);
     int32_t S = 0, k0 = 0, k1 = 0, ent = 0;
     if (it < q.i1) {
         const int32_t fl = lane <= q.nr ? qa.first[q.qr0 + lane] : INT32_MAX;
@@ -330,6 +339,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F32 ? 6 : 4
         qa.nr = (qk + 1) * nsb / kN3Queues - qa.qr0;
         qa.i0 = __builtin_amdgcn_readfirstlane(l.first[qa.qr0]);
         qa.i1 = __builtin_amdgcn_readfirstlane(l.first[qa.qr0 + qa.nr]);
+        qa.il = __builtin_amdgcn_readfirstlane(l.first[qa.qr0 + (qa.nr > 0 ? qa.nr - 1 : 0)]);
         qa.span = span;
         if constexpr (F32) {
             __syncthreads();  // the previous item and the queue scan are done with nx and slot
