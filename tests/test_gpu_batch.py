@@ -158,8 +158,9 @@ def test_batch_newton3_bucket13_matches_oracle(oracle):
 
 def test_batch_real_ensemble_sample_matches_oracle(oracle):
     """The benchmark ensemble itself (256 clouds, N log-uniform in
-    [64, 8192], autotuned forms and shapes, one captured graph per step),
-    25 steps: the smallest, the largest and ten more clouds spread over the
+    [64, 8192], autotuned forms and shapes, steps replayed from a captured
+    graph of four), 25 steps (not a multiple of four: the surplus steps of
+    the last replay are no-ops): the smallest, the largest and ten more clouds spread over the
     sizes against the oracle (cell-list form)."""
     ens = ensemble(256, seed=4, multiple_of=4)
     b = dm.Batch([e["n"] for e in ens], [e["kappa"] for e in ens], [e["confine"] for e in ens],
