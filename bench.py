@@ -275,8 +275,9 @@ def update_traffic(r, n):
 def run_ensemble(steps, warmup, graph=True):
     """The batched ensemble: 256 clouds, N log-uniform in [64, 8192] (multiples
     of 4 for the dt check), lambda, charge and k per cloud, gamma = 2/s, hex
-    lattice (500 um, +-50 um), FP64, dt check on, 500 RK4 steps of 1 ms; one
-    graph-captured launch sequence per step, tile shapes from the autotuner."""
+    lattice (500 um, +-50 um), FP64, dt check on, 500 RK4 steps of 1 ms; the
+    launch sequence of four steps captured as one graph and replayed, tile
+    shapes from the autotuner."""
     import demon_simulation_code_amd as dm
     from demon_simulation_code_amd.physics import ensemble
 
