@@ -14,8 +14,9 @@
 // them, one workgroup per item.  Items are taken largest first; every wave
 // leaves the loop when the counter passes the item count.  The per-stage
 // update runs over all clouds in one launch and adds each Newton-3 cloud's
-// reactions as the single-cloud stage kernel does.  A step (boundary,
-// 4 x (pairs, update)) is captured once as a hipGraph and replayed.  The form
+// reactions as the single-cloud stage kernel does.  Four steps (each a
+// boundary and 4 x (pairs, update)) are captured once as a hipGraph and
+// replayed.  The form
 // and tile shape per N bucket come from the autotuner, demon_batch_tune.
 #include <algorithm>
 #include <cmath>
