@@ -16,8 +16,8 @@
 // update runs over all clouds in one launch and adds each Newton-3 cloud's
 // reactions as the single-cloud stage kernel does.  Four steps (each a
 // boundary and 4 x (pairs, update)) are captured once as a hipGraph and
-// replayed.  The form
-// and tile shape per N bucket come from the autotuner, demon_batch_tune.
+// replayed.  The form and tile shape per N bucket come from the autotuner,
+// demon_batch_tune.
 #include <algorithm>
 #include <cmath>
 #include <limits>

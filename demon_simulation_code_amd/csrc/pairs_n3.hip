@@ -182,9 +182,8 @@ __global__ __launch_bounds__(256) void k_n3_plan(N3Row a, N3List l) {
 // and the last row's were among the last items a launch waited on (0.6%
 // faster on the imploded cloud, no change on the lattice, where the last
 // items are also those taken from other queues).  The item's row is the last
-// of the queue's
-// rows (one per lane, t = nr: the queue's end) whose first item is at most
-// it (first[] does not decrease).  Run by one whole wave; lane 0 leaves
+// of the queue's rows (one per lane, t = nr: the queue's end) whose first
+// item is at most it (first[] does not decrease).  Run by one whole wave; lane 0 leaves
 // (it, S, k0, k1, e) in nx (it >= i1: the queue is exhausted), e the list
 // entry of an item of one swapped tile, else 0.
 struct QueueAt {
