@@ -51,7 +51,8 @@ struct demon_ctx {
     bool uploaded = false;
     int32_t i_per_lane = 1;
     int32_t slices_req = 0;      // 0 = automatic
-    int32_t slices = 1, jslice = 0;  // gather kernel (and the adjoint pair kernel)
+    int32_t slices = 1, jslice = 0;  // gather kernel
+    int32_t adj_slices = 1, adj_jslice = 0;  // adjoint pair kernel
     int32_t pair_mode = DEMON_PAIRS_GATHER;
     int32_t n3_slices = 1;       // Newton-3 kernel: work items per superblock row when every tile is live
     double *d_react = nullptr;   // Newton-3 reactions [nsb/2][2][npad]
@@ -149,18 +150,31 @@ int check_params(demon_ctx *c, const demon_params *p) {
     return DEMON_OK;
 }
 
+// j slices of at least min_len particles, about `want` of them: whole
+// 64-particle box blocks, or a quarter or a half of one (a slice never
+// straddles two blocks, whose boxes the kernels test one at a time).
+void slice_shape(int64_t npad, int64_t want, int64_t min_len, int32_t *slices, int32_t *jslice) {
+    const int64_t s = std::max<int64_t>(
+        1, std::min<int64_t>({want, (int64_t)kMaxSlices, std::max<int64_t>(1, npad / min_len)}));
+    const int64_t need = (npad + s - 1) / s;
+    const int64_t js = need <= kBoxBlock / 4 ? kBoxBlock / 4 : need <= kBoxBlock / 2 ? kBoxBlock / 2
+                                                                                      : round_up(need, kBoxBlock);
+    *slices = (int32_t)((npad + js - 1) / js);
+    *jslice = (int32_t)js;
+}
+
 // Number of j slices: about 32768 waves (32 per SIMD of the 256 CUs), so
-// that the uneven per-wave work left by the box culling evens out, while
-// keeping each slice at least 64 j particles long.
+// that the uneven per-wave work left by the box culling evens out.  The
+// gather kernel's slices go down to 16 j: in a cloud of few blocks a wave
+// walks its chunks one scalar-load round trip after another with its SIMD
+// to itself, and shorter slices put more waves on the chip (N = 1024: 27 us
+// per launch with 16 slices of 64 j, 43 us with 8 of 128).  The adjoint
+// pair kernel walks whole box blocks: its slices are at least 64 j long.
 void choose_slices(demon_ctx *c) {
     const int64_t waves_per_slice = c->npad / (64 * c->i_per_lane);
-    int64_t s = c->slices_req > 0 ? c->slices_req : (32768 + waves_per_slice - 1) / waves_per_slice;
-    const int64_t max_by_len = std::max<int64_t>(1, c->npad / 64);
-    s = std::max<int64_t>(1, std::min<int64_t>({s, (int64_t)kMaxSlices, max_by_len}));
-    int64_t js = round_up((c->npad + s - 1) / s, kBoxBlock);
-    s = (c->npad + js - 1) / js;
-    c->slices = (int32_t)s;
-    c->jslice = (int32_t)js;
+    const int64_t want = c->slices_req > 0 ? c->slices_req : (32768 + waves_per_slice - 1) / waves_per_slice;
+    slice_shape(c->npad, want, kBoxBlock / 4, &c->slices, &c->jslice);
+    slice_shape(c->npad, want, kBoxBlock, &c->adj_slices, &c->adj_jslice);
     // Newton-3: a launch is cut into about n3_slices * nsb work items of
     // consecutive live tiles (one workgroup of four waves each), i.e.
     // n3_slices items per row when every one of its nsb/2 + 1 tiles is live;
@@ -1043,7 +1057,7 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
     const int64_t store_steps = std::max(std::min(ckpt_every, steps), steps - kept0);
     const size_t off_st = off_lp + P, off_bar = off_st + (size_t)store_steps * 4 * SD;
     const size_t off_ib = off_bar + 4 * P, off_u = off_ib + 16 * P, off_g = off_u + 2 * P;
-    const size_t off_ap = off_g + 4 * P, off_dt = off_ap + (size_t)c->slices * 4 * P;
+    const size_t off_ap = off_g + 4 * P, off_dt = off_ap + (size_t)c->adj_slices * 4 * P;
     const size_t total = off_dt + (size_t)steps;
     int rc = fit_arena(c, total);
     if (rc) return rc;
@@ -1127,10 +1141,10 @@ int demon_fit_loss_grad(demon_ctx *c, int64_t steps, double dt, int64_t obs_ever
                     // timed with events when the context times its launches
                     bool on = false;
                     if ((rc = ev_open_if(c, timing, &on))) break;
-                    HIP_TRY(c, launch_adj_pairs(st, c->q, u, u + P, ap, c->n, c->npad, c->jslice, c->slices,
+                    HIP_TRY(c, launch_adj_pairs(st, c->q, u, u + P, ap, c->n, c->npad, c->adj_jslice, c->adj_slices,
                                                 c->p.shielding, rc2p, c->d_sc, c->stream));
                     if ((rc = ev_close(c, on, kTimedAdjPairs))) break;
-                    HIP_TRY(c, launch_adj_stage(bar, ibar, u, g, k, st, c->q, c->m, ap, c->slices,
+                    HIP_TRY(c, launch_adj_stage(bar, ibar, u, g, k, st, c->q, c->m, ap, c->adj_slices,
                                                 c->p.forces, dts[(size_t)s], c->p.confine,
                                                 c->p.drag_gamma, c->n, c->npad, c->stream));
                 }

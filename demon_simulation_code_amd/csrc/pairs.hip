@@ -98,7 +98,11 @@ hipError_t launch_shape(const PairLaunch &a, hipStream_t s) {
 }  // namespace
 
 hipError_t launch_pairs(const PairLaunch &a, hipStream_t s) {
-    if (a.npad % kPad || a.jslice % kBoxBlock || a.slices < 1 || a.n > a.npad) return hipErrorInvalidValue;
+    // a slice is whole box blocks or a quarter or half of one (never two
+    // blocks' parts: the tile tests one block's box for each 64 j)
+    const bool js_ok = a.jslice > 0 && (a.jslice % kBoxBlock == 0 || a.jslice == kBoxBlock / 4 ||
+                                        a.jslice == kBoxBlock / 2);
+    if (a.npad % kPad || !js_ok || a.slices < 1 || a.n > a.npad) return hipErrorInvalidValue;
     if ((int64_t)a.slices * a.jslice < a.npad) return hipErrorInvalidValue;
     switch (a.i_per_lane * 2 + (a.fp32pair ? 1 : 0)) {
         case 2: return launch_shape<1, false>(a, s);

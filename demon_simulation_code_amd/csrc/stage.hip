@@ -143,9 +143,27 @@ __device__ __forceinline__ void stage_force(int32_t forces, double xs, double ys
                 fy = fy + ps.part[(size_t)it * (2 * kSuper) + kSuper + li];
             }
         }
-        for (int32_t s = 0; s < ps.slices; ++s) {
-            fx = fx + ps.part[(size_t)(2 * s) * npad + i];
-            fy = fy + ps.part[(size_t)(2 * s + 1) * npad + i];
+        {
+            // gather kernel: its j slices in order, eight slices' loads issued
+            // before their adds
+            int32_t s = 0;
+            for (; s + 8 <= ps.slices; s += 8) {
+                double px[8], py[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    px[k] = ps.part[(size_t)(2 * (s + k)) * npad + i];
+                    py[k] = ps.part[(size_t)(2 * (s + k) + 1) * npad + i];
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    fx = fx + px[k];
+                    fy = fy + py[k];
+                }
+            }
+            for (; s < ps.slices; ++s) {
+                fx = fx + ps.part[(size_t)(2 * s) * npad + i];
+                fy = fy + ps.part[(size_t)(2 * s + 1) * npad + i];
+            }
         }
         if (ps.react) {
             // Newton-3 reactions of the evaluated tiles, in increasing

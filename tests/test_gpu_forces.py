@@ -70,6 +70,23 @@ def test_yukawa_pair_shapes(oracle, mode, shape):
     assert force_rel_err(fx, fy, gx, gy, pair_abs_sum(x, y, q, KAPPA)) <= FP64_TOL
 
 
+@pytest.mark.parametrize("n,shape,slices", [(1000, (1, 0), 64), (1000, (1, 40), 32), (1000, (2, 0), 64),
+                                            (1000, (4, 30), 16), (2000, (1, 0), 64), (2000, (2, 48), 32),
+                                            (2000, (1, 20), 16), (2000, (4, 0), 64)])
+@pytest.mark.parametrize("precision,tol", [(dm.FP64, FP64_TOL), (dm.FP32_PAIR, FP32_TOL)])
+def test_yukawa_short_j_slices(oracle, n, shape, slices, precision, tol):
+    """Gather kernel with j slices of a quarter, a half or a whole box block
+    (16, 32 or 64 j; N = 1000 pads to 1024, N = 2000 to 2048, the last block
+    partly padding): the requested slice count is rounded to such lengths."""
+    x, y = lattice(n, seed=7)
+    q = charges(n)
+    with _ctx(n, q, x=x, y=y, shape=shape, mode=dm.PAIRS_GATHER, precision=precision) as c:
+        assert c.pair_shape() == (shape[0], slices)
+        gx, gy = c.force_stage(dm.SHIELDED_COULOMB, 2, 0, x, y)
+    fx, fy = oracle.yukawa(x, y, q, KAPPA)
+    assert force_rel_err(fx, fy, gx, gy, pair_abs_sum(x, y, q, KAPPA)) <= tol
+
+
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("precision,tol", [(dm.FP64, FP64_TOL), (dm.FP32_PAIR, FP32_TOL)])
 def test_yukawa_dense_cluster(oracle, mode, precision, tol):
